@@ -1,0 +1,190 @@
+"""Benchmark: gene x target Mann-Whitney U tests/sec on one MI355X.
+
+Default workload: 150,000 cells x 8,000 genes of integer UMI counts (float32),
+reference 15,000 cells, 300 targets, zero_handling='min', tie + continuity
+correction, asymptotic p.  One step = one pass of the batched form (the
+persistent size-ordered kernel replayed from a hipGraph) over the resident
+matrix, producing all 2.4M (gene, target) U1 and p-values.
+
+Reported beside it (extra keys): the plain per-call form (one C-ABI call per
+job: upload, one-workgroup-per-column kernel, download) and its device-only
+kernel time; the roofline of the per-gene kernel; a CPU baseline (the oracle,
+a C restatement of the reference algorithm, timed on a bounded column sample
+with the host's cores).
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload histogram]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_workload(name):
+    from hpdex_amd import synthetic
+    if name == "histogram":
+        return synthetic.histogram_case()
+    if name == "small":
+        return synthetic.small_case()
+    if name == "float":
+        return synthetic.float_case()
+    if name == "exact":
+        return synthetic.exact_case()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def traffic_from_profiles(kernel_prefix, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), as recorded in profiles/ for this workload; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"gene_kernel_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def cpu_baseline(w, target_s=12.0):
+    import oracle
+    from hpdex_amd import synthetic
+    threads = min(16, os.cpu_count() or 1)
+    o = w["opts"]
+    C = w["n_cols"]
+    # calibrate on a small stride sample, then size the sample to ~target_s
+    stride = max(1, C // 100)
+    cols = np.arange(0, C, stride)
+    sub = synthetic.subset_columns(w, cols)
+    t0 = time.perf_counter()
+    oracle.mwu(sub["data"], sub["indices"], sub["indptr"], sub["group_id"], sub["n_targets"],
+               n_rows=sub["n_rows"], tie_correction=o["tie_correction"],
+               use_continuity=o["use_continuity"], zero_handling=o["zero_handling"],
+               method=o["method"], threads=threads)
+    dt = time.perf_counter() - t0
+    ncols = int(min(C, max(len(cols), len(cols) * target_s / max(dt, 1e-3))))
+    cols = np.unique(np.linspace(0, C - 1, ncols).astype(np.int64))
+    sub = synthetic.subset_columns(w, cols)
+    t0 = time.perf_counter()
+    oracle.mwu(sub["data"], sub["indices"], sub["indptr"], sub["group_id"], sub["n_targets"],
+               n_rows=sub["n_rows"], tie_correction=o["tie_correction"],
+               use_continuity=o["use_continuity"], zero_handling=o["zero_handling"],
+               method=o["method"], threads=threads)
+    dt = time.perf_counter() - t0
+    tests = len(cols) * w["n_targets"]
+    return {"value": tests / dt, "unit": "tests/s", "cores": threads, "kind": "port",
+            "sample": f"{len(cols)} of {C} genes (evenly spaced) x {w['n_targets']} targets, "
+                      f"{dt:.1f} s, oracle C restatement of the reference algorithm, OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="histogram")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(genes would shard by column; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    from hpdex_amd import _capi, synthetic
+    ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
+    log(f"generating workload {args.workload}")
+    t0 = time.perf_counter()
+    w = load_workload(args.workload)
+    o = w["opts"]
+    C, T, R = w["n_cols"], w["n_targets"], w["n_rows"]
+    nnz = int(w["indptr"][-1])
+    log(f"generated in {time.perf_counter() - t0:.1f}s: {R} cells x {C} genes, nnz={nnz}, T={T}")
+    info = _capi.device_info()
+    log(f"device {info}")
+    opts = _capi.make_opts(tie_correction=o["tie_correction"], use_continuity=o["use_continuity"],
+                           zero_handling=ZH[o["zero_handling"]],
+                           method=1 if o["method"] == "exact" else 2)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], R)
+    log(f"resident matrix: {m.device_bytes() / 1e9:.2f} GB")
+    pb = _capi.Plan(m, w["group_id"], T, opts, form=1)
+    log("warmup (batched)")
+    pb.run(args.warmup)
+    log(f"timing {args.steps} steps (batched)")
+    ms_total, kernel_ms = pb.run(args.steps)
+    ms_step = ms_total / args.steps
+    U1b, Pb = pb.results()
+    tests = C * T
+    value = tests / (ms_step / 1e3)
+    B = synthetic.algorithmic_bytes(w, 4 if w["data"].dtype == np.float32 else 8)
+    achieved = B / (kernel_ms / 1e3) / 1e9
+    log(f"batched: {ms_step:.3f} ms/step, kernel {kernel_ms:.3f} ms, {value:.3e} tests/s, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    extra = {}
+    if not args.no_plain:
+        pp = _capi.Plan(m, w["group_id"], T, opts, form=0)
+        pp.run(2)
+        pms, pker = pp.run(max(3, args.steps // 5))
+        U1p, Pp = pp.results()
+        same = bool(np.array_equal(U1p, U1b) and np.array_equal(Pp, Pb))
+        pp.close()
+        log(f"plain kernel (resident): {pker:.3f} ms; identical to batched: {same}")
+        t0 = time.perf_counter()
+        _capi.mwu_csc(w["data"], w["indices"], w["indptr"], w["group_id"], T, opts, n_rows=R)
+        e2e = time.perf_counter() - t0
+        log(f"plain per-call (host buffers, incl. transfers): {e2e * 1e3:.1f} ms")
+        extra["plain"] = {"per_call_ms": e2e * 1e3, "per_call_tests_per_s": tests / e2e,
+                          "resident_kernel_ms": pker, "resident_tests_per_s": tests / (pker / 1e3),
+                          "batched_over_plain_per_call": value / (tests / e2e),
+                          "batched_over_plain_resident": value / (tests / (pker / 1e3)),
+                          "bit_identical_to_batched": same}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle)")
+        cpu = cpu_baseline(w)
+        log(f"CPU baseline: {cpu['value']:.3e} tests/s on {cpu['cores']} threads")
+    pb.close()
+    m.close()
+    traffic = traffic_from_profiles("gene_persistent_kernel", args.workload)
+    line = {
+        "metric": "gene x target MWU tests/sec on 1 MI355X (batched persistent form)",
+        "value": value,
+        "unit": "tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-keys/int64-U/f64-p",
+        "data": "synthetic (seeded generator, hpdex_amd/synthetic.py)",
+        "config": {"workload": f"{args.workload}: {R} cells x {C} genes, nnz {nnz}, "
+                               f"reference {int((w['group_id'] == 0).sum())} cells, {T} targets, "
+                               f"zero_handling={o['zero_handling']}, method={o['method']}",
+                   "cells": R, "genes": C, "targets": T, "nnz": nnz,
+                   "tests_per_step": tests, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": B, "kernel_ms": kernel_ms},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

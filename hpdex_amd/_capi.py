@@ -1,0 +1,259 @@
+"""ctypes view of the C-ABI in include/hpdex_mi355x.h.
+
+Used by the plan/serving layer, the tests and bench.py to call the native
+library directly.  Loading fails loudly if the library has not been built:
+there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhpdex_mi355x.so")
+
+DTYPE_CODES = {
+    np.dtype(np.int8): 0, np.dtype(np.uint8): 1, np.dtype(np.int16): 2,
+    np.dtype(np.uint16): 3, np.dtype(np.int32): 4, np.dtype(np.uint32): 5,
+    np.dtype(np.int64): 6, np.dtype(np.uint64): 7, np.dtype(np.float32): 8,
+    np.dtype(np.float64): 9,
+}
+
+# Symbols declared in include/hpdex_mi355x.h (checked by the CPU tests).
+EXPORTED = (
+    "hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
+    "hpdex_matrix_create", "hpdex_matrix_destroy", "hpdex_matrix_device_bytes",
+    "hpdex_plan_create", "hpdex_plan_destroy", "hpdex_plan_run", "hpdex_plan_results",
+    "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
+)
+
+
+class MwuOpts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "ref_sorted", "tar_sorted", "tie_correction", "use_continuity", "fast_norm",
+        "zero_handling", "alternative", "method")]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "block_threads", "lds_bytes", "blocks_per_cu", "hist_max_bins")]
+
+
+class HpdexError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python -m hpdex_amd.build` "
+                "(the MI355X backend has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, ci, cp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_char_p
+        L.hpdex_mwu_csc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz, ctypes.POINTER(MwuOpts),
+                                    ci, vp, vp, cp, sz]
+        L.hpdex_mwu_csc_stats.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz,
+                                          ctypes.POINTER(MwuOpts), ci, vp, vp, vp, vp, vp, cp, sz]
+        L.hpdex_group_mean_csc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz, ci, ci, ci, vp,
+                                           cp, sz]
+        L.hpdex_matrix_create.argtypes = [ci, vp, vp, vp, sz, sz, sz, ctypes.POINTER(vp), cp, sz]
+        L.hpdex_matrix_destroy.argtypes = [vp]
+        L.hpdex_matrix_destroy.restype = None
+        L.hpdex_matrix_device_bytes.argtypes = [vp]
+        L.hpdex_matrix_device_bytes.restype = sz
+        L.hpdex_plan_create.argtypes = [vp, vp, sz, ctypes.POINTER(MwuOpts), ci,
+                                        ctypes.POINTER(vp), cp, sz]
+        L.hpdex_plan_destroy.argtypes = [vp]
+        L.hpdex_plan_destroy.restype = None
+        L.hpdex_plan_run.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_plan_results.argtypes = [vp, vp, vp, vp, vp, vp, cp, sz]
+        L.hpdex_get_tuning.argtypes = [ctypes.POINTER(Tuning)]
+        L.hpdex_set_tuning.argtypes = [ctypes.POINTER(Tuning), cp, sz]
+        L.hpdex_version.restype = cp
+        L.hpdex_device_info.argtypes = [ctypes.POINTER(ci), ctypes.POINTER(ci), cp, sz]
+        for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
+                     "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
+                     "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
+                     "hpdex_device_info"):
+            getattr(L, name).restype = ci
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def make_opts(tie_correction=True, use_continuity=False, zero_handling=3, alternative=2,
+              method=2, ref_sorted=False, tar_sorted=False, fast_norm=False) -> MwuOpts:
+    return MwuOpts(int(ref_sorted), int(tar_sorted), int(tie_correction), int(use_continuity),
+                   int(fast_norm), int(zero_handling), int(alternative), int(method))
+
+
+def _csc_args(data, indices, indptr):
+    data = np.ascontiguousarray(data)
+    if data.dtype not in DTYPE_CODES:
+        raise HpdexError("Unsupported dtype for data")
+    indices = np.ascontiguousarray(indices, dtype=np.int64)
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    return data, indices, indptr
+
+
+def mwu_csc(data, indices, indptr, group_id, n_targets, opts: MwuOpts, n_rows=None,
+            with_stats=False, threads=-1):
+    """Call hpdex_mwu_csc(_stats) on host arrays.  Returns (U1, P) or
+    (U1, P, tie, n1, n2)."""
+    data, indices, indptr = _csc_args(data, indices, indptr)
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    R = len(group_id) if n_rows is None else int(n_rows)
+    C = len(indptr) - 1
+    np_ = C * n_targets if n_targets > 0 else 0
+    outs = [np.empty(max(np_, 1)) for _ in range(5 if with_stats else 2)]
+    err = ctypes.create_string_buffer(1024)
+    if with_stats:
+        rc = lib().hpdex_mwu_csc_stats(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr),
+                                       R, C, len(data), _p(group_id), int(n_targets),
+                                       ctypes.byref(opts), threads, *[_p(o) for o in outs],
+                                       err, len(err))
+    else:
+        rc = lib().hpdex_mwu_csc(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr), R, C,
+                                 len(data), _p(group_id), int(n_targets), ctypes.byref(opts),
+                                 threads, _p(outs[0]), _p(outs[1]), err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return tuple(o[:np_] for o in outs)
+
+
+def group_mean_csc(data, indices, indptr, group_id, n_groups, include_zeros=True,
+                   use_kahan=False, threads=-1):
+    data, indices, indptr = _csc_args(data, indices, indptr)
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    C = len(indptr) - 1
+    out = np.empty(max(C * n_groups, 1))
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().hpdex_group_mean_csc(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr),
+                                    len(group_id), C, len(data), _p(group_id), int(n_groups),
+                                    int(include_zeros), int(use_kahan), threads, _p(out),
+                                    err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return out[:C * n_groups]
+
+
+class DeviceMatrix:
+    """A CSC matrix resident in HBM (hpdex_matrix_create)."""
+
+    def __init__(self, data, indices, indptr, n_rows):
+        data, indices, indptr = _csc_args(data, indices, indptr)
+        self.n_rows = int(n_rows)
+        self.n_cols = len(indptr) - 1
+        self.nnz = len(data)
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_matrix_create(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr),
+                                       self.n_rows, self.n_cols, self.nnz, ctypes.byref(h),
+                                       err, len(err))
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return int(lib().hpdex_matrix_device_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hpdex_matrix_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A job on a DeviceMatrix (hpdex_plan_create); form 0 plain, 1 batched."""
+
+    def __init__(self, matrix: DeviceMatrix, group_id, n_targets, opts: MwuOpts, form=1):
+        self.matrix = matrix
+        self.n_targets = int(n_targets)
+        gid = np.ascontiguousarray(group_id, dtype=np.int32)
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_plan_create(matrix.handle, _p(gid), self.n_targets, ctypes.byref(opts),
+                                     int(form), ctypes.byref(h), err, len(err))
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        self._h = h
+
+    def run(self, iters=1):
+        tot, ker = ctypes.c_double(), ctypes.c_double()
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_plan_run(self._h, int(iters), ctypes.byref(tot), ctypes.byref(ker),
+                                  err, len(err))
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        return tot.value, ker.value
+
+    def results(self, with_stats=False):
+        n = self.matrix.n_cols * self.n_targets
+        outs = [np.empty(max(n, 1)) for _ in range(5 if with_stats else 2)]
+        ptrs = [_p(o) for o in outs] + [None] * (5 - len(outs))
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_plan_results(self._h, *ptrs, err, len(err))
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        return tuple(o[:n] for o in outs)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hpdex_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_tuning() -> Tuning:
+    t = Tuning()
+    lib().hpdex_get_tuning(ctypes.byref(t))
+    return t
+
+
+def set_tuning(block_threads=None, lds_bytes=None, blocks_per_cu=None, hist_max_bins=None):
+    t = get_tuning()
+    if block_threads is not None:
+        t.block_threads = block_threads
+    if lds_bytes is not None:
+        t.lds_bytes = lds_bytes
+    if blocks_per_cu is not None:
+        t.blocks_per_cu = blocks_per_cu
+    if hist_max_bins is not None:
+        t.hist_max_bins = hist_max_bins
+    err = ctypes.create_string_buffer(1024)
+    if lib().hpdex_set_tuning(ctypes.byref(t), err, len(err)) != 0:
+        raise HpdexError(err.value.decode())
+
+
+def device_info():
+    cus, lds = ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(256)
+    if lib().hpdex_device_info(ctypes.byref(cus), ctypes.byref(lds), name, len(name)) != 0:
+        raise HpdexError("no HIP device available")
+    return {"cus": cus.value, "max_lds_per_block": lds.value, "name": name.value.decode()}

@@ -1,0 +1,727 @@
+// hpdex_api.cpp -- host side of the MI355X Mann-Whitney U backend: the C-ABI
+// declared in include/hpdex_mi355x.h, device memory, streams, the size-ordered
+// work queue, exact-null table planning and hipGraph capture.
+//
+// Host-side checks mirror the reference's:
+//   n_targets >= 1                 src/cpp/mannwhitneyu.cpp:1088-1090
+//   unsupported dtype              src/cpp/module.cpp:165-166
+//   per-column sample-size errors  src/cpp/mannwhitneyu.cpp:938-1038, raised
+//   after the pass (:1045); the device records the lowest (column, group).
+#include "hpdex_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hpdex_internal.h"
+
+using namespace hpdex;
+
+namespace {
+
+// Largest exact-null DP workspace per distinct (n1, n2), in doubles.
+constexpr size_t kExactMaxCells = (size_t)1 << 27;
+
+struct Tuning {
+    int block = 512;
+    int lds = 128 * 1024;
+    int blocks_per_cu = 1;
+    int hist_max = 8192;
+};
+
+std::mutex g_mu;
+Tuning g_tune;
+bool g_init = false;
+int g_cus = 0;
+int g_max_lds = 0;
+char g_name[256] = {0};
+
+int set_err(char* err, size_t len, const std::string& m) {
+    if (err && len) {
+        std::strncpy(err, m.c_str(), len - 1);
+        err[len - 1] = 0;
+    }
+    return 1;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                             " (" #x ")");                                 \
+    } while (0)
+
+int ensure_init(char* err, size_t err_len) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_init) return 0;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return set_err(err, err_len,
+                       "no HIP device available: the MI355X backend has no CPU fallback");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    HIPCHK(hipGetDeviceProperties(&pr, dev));
+    g_cus = pr.multiProcessorCount;
+    g_max_lds = (int)pr.sharedMemPerBlock;
+    std::snprintf(g_name, sizeof g_name, "%s (%s)", pr.name, pr.gcnArchName);
+    // ColState and friends take a little static LDS next to the dynamic part
+    const int cap = g_max_lds - 1024;
+    if (g_tune.lds > cap) g_tune.lds = cap;
+    HIPCHK(set_gene_kernel_lds_limit(g_tune.lds));
+    g_init = true;
+    return 0;
+}
+
+Tuning tuning() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_tune;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t bytes) {
+        reset();
+        n = bytes;
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+        if (s) hipStreamDestroy(s);
+    }
+    hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+};
+
+bool valid_dtype(int d) { return d >= HPDEX_INT8 && d <= HPDEX_FLOAT64; }
+size_t dtype_size(int d) {
+    switch (d) {
+    case HPDEX_INT8: case HPDEX_UINT8: return 1;
+    case HPDEX_INT16: case HPDEX_UINT16: return 2;
+    case HPDEX_INT32: case HPDEX_UINT32: case HPDEX_FLOAT32: return 4;
+    default: return 8;
+    }
+}
+
+// Column pointers must be a valid CSC index before any kernel reads through
+// them (the reference trusts them; a bad one would fault the device here).
+int check_indptr(const int64_t* indptr, size_t C, size_t nnz, char* err, size_t err_len) {
+    if (indptr[0] < 0) return set_err(err, err_len, "indptr[0] must be >= 0");
+    for (size_t c = 0; c < C; ++c)
+        if (indptr[c + 1] < indptr[c])
+            return set_err(err, err_len, "indptr must be non-decreasing");
+    if ((size_t)indptr[C] > nnz)
+        return set_err(err, err_len, "indptr[-1] exceeds the number of stored values");
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Resident matrix
+// ---------------------------------------------------------------------------
+struct hpdex_matrix {
+    int dtype = 0;
+    bool f64 = false;
+    size_t R = 0, C = 0, nnz = 0;
+    DevBuf vals, rows, indptr;
+    std::vector<int64_t> h_indptr;
+    std::vector<int32_t> order;   // columns, largest first
+};
+
+namespace {
+
+int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,
+                  const int64_t* indptr, size_t R, size_t C, size_t nnz, hipStream_t s,
+                  char* err, size_t err_len) {
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (R > (size_t)INT32_MAX) return set_err(err, err_len, "more than 2^31-1 rows is unsupported");
+    if (C > (size_t)INT32_MAX) return set_err(err, err_len, "more than 2^31-1 columns is unsupported");
+    if (check_indptr(indptr, C, nnz, err, err_len)) return 1;
+    m->dtype = dtype;
+    m->f64 = dtype_uses_double(dtype);
+    m->R = R;
+    m->C = C;
+    m->nnz = nnz;
+    const size_t vsz = m->f64 ? 8 : 4;
+    HIPCHK(m->vals.alloc(nnz * vsz));
+    HIPCHK(m->rows.alloc(nnz * 4));
+    HIPCHK(m->indptr.alloc((C + 1) * 8));
+    if (nnz) {
+        if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
+            HIPCHK(hipMemcpyAsync(m->vals.p, data, nnz * vsz, hipMemcpyHostToDevice, s));
+        } else {
+            DevBuf raw;
+            HIPCHK(raw.alloc(nnz * dtype_size(dtype)));
+            HIPCHK(hipMemcpyAsync(raw.p, data, nnz * dtype_size(dtype), hipMemcpyHostToDevice, s));
+            HIPCHK(launch_convert_values(dtype, raw.p, m->vals.p, nnz, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        DevBuf idx64;
+        HIPCHK(idx64.alloc(nnz * 8));
+        HIPCHK(hipMemcpyAsync(idx64.p, indices, nnz * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(launch_rows_i64_to_i32(idx64.as<int64_t>(), m->rows.as<int32_t>(), nnz,
+                                      (int64_t)R, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (C + 1) * 8, hipMemcpyHostToDevice, s));
+    m->h_indptr.assign(indptr, indptr + C + 1);
+    m->order.resize(C);
+    std::iota(m->order.begin(), m->order.end(), 0);
+    std::stable_sort(m->order.begin(), m->order.end(), [&](int32_t a, int32_t b) {
+        return (indptr[a + 1] - indptr[a]) > (indptr[b + 1] - indptr[b]);
+    });
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// Per-job device state shared by the one-shot call and the plan.
+struct JobState {
+    size_t T = 0, G = 0, C = 0;
+    hpdex_mwu_opts opts{};
+    std::vector<uint32_t> group_rows;
+    DevBuf gid, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    // exact null tables
+    std::vector<ExactTable> tabs;
+    std::vector<int32_t> tidx_by_g;
+    DevBuf d_tabs, tabbuf, tscratch, d_tidx;
+    size_t scratch_per_table = 0;
+    bool exact_by_pair = false;
+
+    unsigned long long* err_key() const { return flags.as<unsigned long long>(); }
+    unsigned long long* status_col() const { return flags.as<unsigned long long>() + 1; }
+    uint32_t* status() const { return (uint32_t*)(flags.as<unsigned long long>() + 2); }
+};
+
+int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
+              const hpdex_mwu_opts* o, hipStream_t s, char* err, size_t err_len) {
+    if (T < 1) return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    if (T + 1 > (size_t)INT32_MAX / 4) return set_err(err, err_len, "too many target groups");
+    js.T = T;
+    js.G = T + 1;
+    js.C = m->C;
+    js.opts = *o;
+    js.group_rows.assign(js.G, 0u);
+    for (size_t r = 0; r < m->R; ++r) {
+        const int g = group_id[r];
+        if (g >= 0 && (size_t)g < js.G) ++js.group_rows[g];
+    }
+    const size_t np = js.C * T;
+    HIPCHK(js.gid.alloc(m->R * 4));
+    if (m->R) HIPCHK(hipMemcpyAsync(js.gid.p, group_id, m->R * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(js.grows.alloc(js.G * 4));
+    HIPCHK(hipMemcpyAsync(js.grows.p, js.group_rows.data(), js.G * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(js.U1.alloc(np * 8));
+    HIPCHK(js.P.alloc(np * 8));
+    HIPCHK(js.tie.alloc(np * 8));
+    HIPCHK(js.n1.alloc(np * 8));
+    HIPCHK(js.n2.alloc(np * 8));
+    HIPCHK(js.flags.alloc(32));
+    HIPCHK(launch_init_job_state(js.err_key(), js.status(), js.status_col(), s));
+    return 0;
+}
+
+JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
+    JobDesc J{};
+    J.vals = m->vals.p;
+    J.rows = m->rows.as<int32_t>();
+    J.indptr = m->indptr.as<int64_t>();
+    J.gid = js.gid.as<int32_t>();
+    J.group_rows = js.grows.as<uint32_t>();
+    J.U1 = js.U1.as<double>();
+    J.P = js.P.as<double>();
+    J.tie = js.tie.as<double>();
+    J.n1o = js.n1.as<double>();
+    J.n2o = js.n2.as<double>();
+    J.err_key = js.err_key();
+    J.status = js.status();
+    J.status_col = js.status_col();
+    J.C = (int64_t)m->C;
+    J.R = (int32_t)m->R;
+    J.G = (int32_t)js.G;
+    J.T = (int32_t)js.T;
+    J.mode = js.opts.zero_handling;
+    J.alt = js.opts.alternative;
+    J.tie_corr = js.opts.tie_correction ? 1 : 0;
+    J.fuse_p = (js.opts.method == 2) ? 1 : 0;
+    J.hist_max = tu.hist_max;
+    J.cc = js.opts.use_continuity ? 0.5 : 0.0;
+    return J;
+}
+
+// Plan the exact-null tables.  min/max/mix: sizes are the group row counts,
+// known now.  none: sizes depend on the data; the caller passes n1/n2 per
+// pair read back after the U pass.
+int exact_plan(JobState& js, const double* n1pair, const double* n2pair, hipStream_t s,
+               char* err, size_t err_len) {
+    std::map<std::pair<int32_t, int32_t>, int32_t> idx;
+    js.tabs.clear();
+    int64_t off = 0;
+    size_t max_cells = 0;
+    auto add = [&](int32_t a, int32_t b) -> int32_t {
+        auto key = std::make_pair(a, b);
+        auto it = idx.find(key);
+        if (it != idx.end()) return it->second;
+        const int64_t U = (int64_t)a * b;
+        ExactTable t;
+        t.offset = off;
+        t.n1 = a;
+        t.n2 = b;
+        off += 1 + 2 * (U + 1);
+        const size_t s_ = (size_t)std::min(a, b);
+        const size_t cells = (s_ + 1) * (size_t)(U + 1);
+        max_cells = std::max(max_cells, cells);
+        const int32_t id = (int32_t)js.tabs.size();
+        js.tabs.push_back(t);
+        idx[key] = id;
+        return id;
+    };
+    std::vector<int32_t> tidx;
+    if (!n1pair) {
+        js.exact_by_pair = false;
+        js.tidx_by_g.resize(js.T);
+        for (size_t g = 1; g <= js.T; ++g)
+            js.tidx_by_g[g - 1] = add((int32_t)js.group_rows[0], (int32_t)js.group_rows[g]);
+        tidx = js.tidx_by_g;
+    } else {
+        js.exact_by_pair = true;
+        const size_t np = js.C * js.T;
+        tidx.resize(np);
+        for (size_t i = 0; i < np; ++i) tidx[i] = add((int32_t)n1pair[i], (int32_t)n2pair[i]);
+    }
+    if (max_cells > kExactMaxCells) {
+        for (const auto& t : js.tabs) {
+            const size_t s_ = (size_t)std::min(t.n1, t.n2);
+            if ((s_ + 1) * ((size_t)t.n1 * t.n2 + 1) > kExactMaxCells)
+                return set_err(err, err_len,
+                               "exact null table too large for n1=" + std::to_string(t.n1) +
+                                   ", n2=" + std::to_string(t.n2) + "; use method='asymptotic'");
+        }
+    }
+    js.scratch_per_table = 2 * max_cells;
+    HIPCHK(js.d_tabs.alloc(js.tabs.size() * sizeof(ExactTable)));
+    HIPCHK(hipMemcpyAsync(js.d_tabs.p, js.tabs.data(), js.tabs.size() * sizeof(ExactTable),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(js.tabbuf.alloc((size_t)off * 8));
+    HIPCHK(js.tscratch.alloc(js.tabs.size() * js.scratch_per_table * 8));
+    HIPCHK(js.d_tidx.alloc(tidx.size() * 4));
+    HIPCHK(hipMemcpyAsync(js.d_tidx.p, tidx.data(), tidx.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int exact_launch(const JobState& js, hipStream_t s, bool build_tables, char* err, size_t err_len) {
+    if (build_tables)
+        HIPCHK(launch_exact_tables(js.d_tabs.as<ExactTable>(), (int)js.tabs.size(),
+                                   js.tabbuf.as<double>(), js.tscratch.as<double>(),
+                                   js.scratch_per_table, s));
+    ExactJob E{};
+    E.U1 = js.U1.as<double>();
+    E.P = js.P.as<double>();
+    E.tables = js.d_tabs.as<ExactTable>();
+    E.tabbuf = js.tabbuf.as<double>();
+    E.tidx_by_g = js.exact_by_pair ? nullptr : js.d_tidx.as<int32_t>();
+    E.tidx_by_pair = js.exact_by_pair ? js.d_tidx.as<int32_t>() : nullptr;
+    E.n_pairs = (int64_t)(js.C * js.T);
+    E.T = (int32_t)js.T;
+    E.alt = js.opts.alternative;
+    HIPCHK(launch_exact_p(E, s));
+    return 0;
+}
+
+// Read the job's error state and turn it into the reference's message.
+int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
+    unsigned long long h[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, js.flags.p, 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const unsigned long long key = h[0];
+    const uint32_t status = (uint32_t)h[2];
+    if (key != ~0ull) {
+        const unsigned long long span = 2ull * js.G + 1ull;
+        const unsigned long long col = key / span, rank = key % span;
+        if (rank < js.G)
+            return set_err(err, err_len,
+                           "Non-canonical CSC: column " + std::to_string(col) +
+                               " holds more entries for group " + std::to_string(rank) +
+                               " than the group has rows");
+        if (rank == js.G)
+            return set_err(err, err_len,
+                           "Sample too small for reference at column " + std::to_string(col));
+        return set_err(err, err_len,
+                       "Sample too small for group " + std::to_string(rank - js.G) +
+                           " at column " + std::to_string(col));
+    }
+    if (status) {
+        const std::string why = (status & kStatusGroups)
+                                    ? "too many target groups for the on-chip group counters"
+                                    : "a reference or target slice exceeds the on-chip capacity";
+        return set_err(err, err_len,
+                       "column " + std::to_string(h[1]) + ": " + why +
+                           " (raise lds_bytes with hpdex_set_tuning)");
+    }
+    return 0;
+}
+
+int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1, double* n2,
+             hipStream_t s, char* err, size_t err_len) {
+    const size_t b = js.C * js.T * 8;
+    if (U1 && b) HIPCHK(hipMemcpyAsync(U1, js.U1.p, b, hipMemcpyDeviceToHost, s));
+    if (P && b) HIPCHK(hipMemcpyAsync(P, js.P.p, b, hipMemcpyDeviceToHost, s));
+    if (tie && b) HIPCHK(hipMemcpyAsync(tie, js.tie.p, b, hipMemcpyDeviceToHost, s));
+    if (n1 && b) HIPCHK(hipMemcpyAsync(n1, js.n1.p, b, hipMemcpyDeviceToHost, s));
+    if (n2 && b) HIPCHK(hipMemcpyAsync(n2, js.n2.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+bool opts_ok(const hpdex_mwu_opts* o) {
+    return o && o->zero_handling >= 0 && o->zero_handling <= 3 && o->alternative >= 0 &&
+           o->alternative <= 2 && (o->method == 1 || o->method == 2);
+}
+
+// Exact p for zero_handling none needs the data-dependent sample sizes.
+int run_exact_none(JobState& js, hipStream_t s, char* err, size_t err_len) {
+    const size_t np = js.C * js.T;
+    std::vector<double> n1(np), n2(np);
+    if (np) {
+        HIPCHK(hipMemcpyAsync(n1.data(), js.n1.p, np * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(n2.data(), js.n2.p, np * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (exact_plan(js, n1.data(), n2.data(), s, err, err_len)) return 1;
+    return exact_launch(js, s, true, err, err_len);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* hpdex_version(void) { return "hpdex-mi355x 0.1 (gfx950)"; }
+
+int hpdex_device_info(int* n_cus, int* max_lds_per_block, char* name, size_t name_len) {
+    char err[256];
+    if (ensure_init(err, sizeof err)) return 1;
+    if (n_cus) *n_cus = g_cus;
+    if (max_lds_per_block) *max_lds_per_block = g_max_lds;
+    if (name && name_len) {
+        std::strncpy(name, g_name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return 0;
+}
+
+int hpdex_get_tuning(hpdex_tuning* out) {
+    if (!out) return 1;
+    const Tuning t = tuning();
+    out->block_threads = t.block;
+    out->lds_bytes = t.lds;
+    out->blocks_per_cu = t.blocks_per_cu;
+    out->hist_max_bins = t.hist_max;
+    return 0;
+}
+
+int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
+    if (!t) return set_err(err, err_len, "null tuning");
+    if (ensure_init(err, err_len)) return 1;
+    if (t->block_threads < 64 || t->block_threads > 512 || t->block_threads % 64)
+        return set_err(err, err_len, "block_threads must be a multiple of 64 in [64, 512]");
+    if (t->lds_bytes < 16384 || t->lds_bytes > g_max_lds - 1024)
+        return set_err(err, err_len, "lds_bytes out of range for this device");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8)
+        return set_err(err, err_len, "blocks_per_cu must be in [1, 8]");
+    if (t->hist_max_bins < 0 || t->hist_max_bins > 32768)
+        return set_err(err, err_len, "hist_max_bins must be in [0, 32768]");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tune.block = t->block_threads;
+    g_tune.lds = t->lds_bytes;
+    g_tune.blocks_per_cu = t->blocks_per_cu;
+    g_tune.hist_max = t->hist_max_bins;
+    HIPCHK(set_gene_kernel_lds_limit(g_tune.lds));
+    return 0;
+}
+
+int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
+                        const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                        const int32_t* group_id, size_t n_targets, const hpdex_mwu_opts* opts,
+                        int threads, double* U1_out, double* P_out, double* tie_out,
+                        double* n1_out, double* n2_out, char* err, size_t err_len) {
+    (void)threads;
+    if (n_targets < 1)
+        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (!opts_ok(opts)) return set_err(err, err_len, "invalid option block");
+    if (n_cols == 0) return 0;
+    if (ensure_init(err, err_len)) return 1;
+    const Tuning tu = tuning();
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix m;
+    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
+        return 1;
+    JobState js;
+    if (job_setup(js, &m, group_id, n_targets, opts, st.s, err, err_len)) return 1;
+    const JobDesc J = make_desc(js, &m, tu);
+    HIPCHK(launch_gene_plain(J, m.f64, tu.block, tu.lds, st.s));
+    if (job_errors(js, st.s, err, err_len)) return 1;
+    if (opts->method == 1) {
+        if (opts->zero_handling == 0) {
+            if (run_exact_none(js, st.s, err, err_len)) return 1;
+        } else {
+            if (exact_plan(js, nullptr, nullptr, st.s, err, err_len)) return 1;
+            if (exact_launch(js, st.s, true, err, err_len)) return 1;
+        }
+    }
+    return copy_out(js, U1_out, P_out, tie_out, n1_out, n2_out, st.s, err, err_len);
+}
+
+int hpdex_mwu_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                  size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                  size_t n_targets, const hpdex_mwu_opts* opts, int threads, double* U1_out,
+                  double* P_out, char* err, size_t err_len) {
+    return hpdex_mwu_csc_stats(dtype, data, indices, indptr, n_rows, n_cols, nnz, group_id,
+                               n_targets, opts, threads, U1_out, P_out, nullptr, nullptr,
+                               nullptr, err, err_len);
+}
+
+int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
+                         const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                         const int32_t* group_id, size_t n_groups, int include_zeros,
+                         int use_kahan, int threads, double* out, char* err, size_t err_len) {
+    (void)threads;
+    (void)use_kahan;   // the device sums in FP64 either way
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (n_groups == 0 || n_cols == 0) return 0;
+    if (ensure_init(err, err_len)) return 1;
+    const Tuning tu = tuning();
+    if (16 * n_groups > (size_t)tu.lds)
+        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators");
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix m;
+    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
+        return 1;
+    std::vector<uint32_t> grows(n_groups, 0u);
+    for (size_t r = 0; r < n_rows; ++r) {
+        const int g = group_id[r];
+        if (g >= 0 && (size_t)g < n_groups) ++grows[g];
+    }
+    DevBuf gid, dg, dout;
+    HIPCHK(gid.alloc(n_rows * 4));
+    if (n_rows) HIPCHK(hipMemcpyAsync(gid.p, group_id, n_rows * 4, hipMemcpyHostToDevice, st.s));
+    HIPCHK(dg.alloc(n_groups * 4));
+    HIPCHK(hipMemcpyAsync(dg.p, grows.data(), n_groups * 4, hipMemcpyHostToDevice, st.s));
+    HIPCHK(dout.alloc(n_cols * n_groups * 8));
+    HIPCHK(launch_group_mean(m.vals.p, m.f64, m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
+                             gid.as<int32_t>(), dg.as<uint32_t>(), (int64_t)n_cols,
+                             (int32_t)n_rows, (int32_t)n_groups, include_zeros, dout.as<double>(),
+                             nullptr, tu.block, tu.lds, st.s));
+    HIPCHK(hipMemcpyAsync(out, dout.p, n_cols * n_groups * 8, hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
+int hpdex_matrix_create(int dtype, const void* data, const int64_t* indices,
+                        const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                        hpdex_matrix** out, char* err, size_t err_len) {
+    if (!out) return set_err(err, err_len, "null output handle");
+    *out = nullptr;
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix* m = new hpdex_matrix();
+    if (upload_matrix(m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len)) {
+        delete m;
+        return 1;
+    }
+    *out = m;
+    return 0;
+}
+
+void hpdex_matrix_destroy(hpdex_matrix* m) { delete m; }
+
+size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
+    if (!m) return 0;
+    return m->vals.n + m->rows.n + m->indptr.n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Plans (resident matrix, plain or batched form)
+// ---------------------------------------------------------------------------
+struct hpdex_plan {
+    hpdex_matrix* m = nullptr;
+    int form = 1;
+    Tuning tu;
+    Stream st;
+    JobState js;
+    JobDesc J{};
+    DevBuf d_job, d_items, d_ctl;   // ctl: head, done
+    int n_items = 0;
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool tables_built = false;
+    ~hpdex_plan() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+// The per-job launch sequence (everything after upload): the per-gene
+// kernel, then the exact p lookup when asked for (tables of min/max/mix
+// depend only on group sizes and are built once per plan).
+int plan_enqueue(hpdex_plan* p, char* err, size_t err_len) {
+    const hipStream_t s = p->st.s;
+    if (p->form == 1) {
+        HIPCHK(launch_gene_persistent(p->d_job.as<JobDesc>(), p->d_items.as<WorkItem>(),
+                                      p->n_items, p->d_ctl.as<unsigned>(),
+                                      p->d_ctl.as<unsigned>() + 1, p->m->f64, p->grid,
+                                      p->tu.block, p->tu.lds, s));
+    } else {
+        HIPCHK(launch_gene_plain(p->J, p->m->f64, p->tu.block, p->tu.lds, s));
+    }
+    if (p->js.opts.method == 1 && p->js.opts.zero_handling != 0)
+        if (exact_launch(p->js, s, false, err, err_len)) return 1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                      const hpdex_mwu_opts* opts, int form, hpdex_plan** out, char* err,
+                      size_t err_len) {
+    if (!out || !m) return set_err(err, err_len, "null handle");
+    *out = nullptr;
+    if (!opts_ok(opts)) return set_err(err, err_len, "invalid option block");
+    if (form != 0 && form != 1) return set_err(err, err_len, "form must be 0 (plain) or 1 (batched)");
+    if (ensure_init(err, err_len)) return 1;
+    hpdex_plan* p = new hpdex_plan();
+    p->m = m;
+    p->form = form;
+    p->tu = tuning();
+    auto bail = [&](int rc) { delete p; return rc; };
+    if (p->st.create() != hipSuccess) return bail(set_err(err, err_len, "stream creation failed"));
+    if (job_setup(p->js, m, group_id, n_targets, opts, p->st.s, err, err_len)) return bail(1);
+    p->J = make_desc(p->js, m, p->tu);
+    if (p->d_job.alloc(sizeof(JobDesc)) != hipSuccess) return bail(set_err(err, err_len, "alloc failed"));
+    hipMemcpyAsync(p->d_job.p, &p->J, sizeof(JobDesc), hipMemcpyHostToDevice, p->st.s);
+    std::vector<WorkItem> items(m->C);
+    for (size_t k = 0; k < m->C; ++k) items[k] = WorkItem{0, m->order[k]};
+    p->n_items = (int)m->C;
+    if (p->d_items.alloc(items.size() * sizeof(WorkItem)) != hipSuccess)
+        return bail(set_err(err, err_len, "alloc failed"));
+    if (!items.empty())
+        hipMemcpyAsync(p->d_items.p, items.data(), items.size() * sizeof(WorkItem),
+                       hipMemcpyHostToDevice, p->st.s);
+    unsigned ctl[2] = {0u, 0u};
+    if (p->d_ctl.alloc(sizeof ctl) != hipSuccess) return bail(set_err(err, err_len, "alloc failed"));
+    hipMemcpyAsync(p->d_ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, p->st.s);
+    p->grid = std::max(1, std::min((int)m->C, g_cus * p->tu.blocks_per_cu));
+    if (opts->method == 1 && opts->zero_handling != 0) {
+        if (exact_plan(p->js, nullptr, nullptr, p->st.s, err, err_len)) return bail(1);
+        const hipError_t e = launch_exact_tables(p->js.d_tabs.as<ExactTable>(),
+                                                 (int)p->js.tabs.size(), p->js.tabbuf.as<double>(),
+                                                 p->js.tscratch.as<double>(),
+                                                 p->js.scratch_per_table, p->st.s);
+        if (e != hipSuccess) return bail(set_err(err, err_len, hipGetErrorString(e)));
+    }
+    if (hipStreamSynchronize(p->st.s) != hipSuccess)
+        return bail(set_err(err, err_len, "plan setup failed"));
+    if (form == 1 && m->C > 0) {
+        if (hipStreamBeginCapture(p->st.s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return bail(set_err(err, err_len, "graph capture failed to start"));
+        const int rc = plan_enqueue(p, err, err_len);
+        const hipError_t e = hipStreamEndCapture(p->st.s, &p->graph);
+        if (rc) return bail(1);
+        if (e != hipSuccess) return bail(set_err(err, err_len, hipGetErrorString(e)));
+        if (hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+            return bail(set_err(err, err_len, "graph instantiation failed"));
+    }
+    *out = p;
+    return 0;
+}
+
+void hpdex_plan_destroy(hpdex_plan* p) { delete p; }
+
+int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms, char* err,
+                   size_t err_len) {
+    if (!p) return set_err(err, err_len, "null plan");
+    if (iters < 1) iters = 1;
+    if (p->m->C == 0) {
+        if (ms_total) *ms_total = 0;
+        if (kernel_ms) *kernel_ms = 0;
+        return 0;
+    }
+    const hipStream_t s = p->st.s;
+    std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = 0;
+    for (int it = 0; it < iters && !rc; ++it) {
+        hipEventRecord(ev[2 * it], s);
+        if (p->form == 1) {
+            const hipError_t e = hipGraphLaunch(p->exec, s);
+            if (e != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(e));
+        } else {
+            rc = plan_enqueue(p, err, err_len);
+        }
+        hipEventRecord(ev[2 * it + 1], s);
+    }
+    const hipError_t se = hipStreamSynchronize(s);
+    if (!rc && se != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(se));
+    if (!rc) {
+        float tot = 0.f, sum = 0.f;
+        hipEventElapsedTime(&tot, ev[0], ev[2 * iters - 1]);
+        for (int it = 0; it < iters; ++it) {
+            float x = 0.f;
+            hipEventElapsedTime(&x, ev[2 * it], ev[2 * it + 1]);
+            sum += x;
+        }
+        if (ms_total) *ms_total = tot;
+        if (kernel_ms) *kernel_ms = sum / iters;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (!rc && p->js.opts.method == 1 && p->js.opts.zero_handling == 0) {
+        if (job_errors(p->js, s, err, err_len)) return 1;
+        rc = run_exact_none(p->js, s, err, err_len);
+    }
+    return rc;
+}
+
+int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie_out,
+                       double* n1_out, double* n2_out, char* err, size_t err_len) {
+    if (!p) return set_err(err, err_len, "null plan");
+    if (job_errors(p->js, p->st.s, err, err_len)) return 1;
+    return copy_out(p->js, U1_out, P_out, tie_out, n1_out, n2_out, p->st.s, err, err_len);
+}
+
+}  // extern "C"

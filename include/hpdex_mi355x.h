@@ -1,0 +1,140 @@
+/*
+ * hpdex_mi355x.h -- C-ABI of the MI355X Mann-Whitney U differential-expression
+ * backend.  Plain pointers and sizes only; every function returns 0 on success
+ * or nonzero with a NUL-terminated message copied into err (err may be NULL).
+ *
+ * Host entry points (hpdex_mwu_csc, hpdex_group_mean_csc) take host buffers,
+ * borrow them for the duration of the call, and write caller-allocated host
+ * outputs.  They are reentrant: each call owns its stream and device memory.
+ *
+ * Device-resident entry points (hpdex_matrix_*, hpdex_plan_*) keep a CSC matrix
+ * in HBM across calls for the batched persistent-kernel form and for serving.
+ * The serving session (hpdex_session_*) admits jobs by HBM footprint and
+ * interleaves their work in one persistent kernel.
+ */
+#ifndef HPDEX_MI355X_H
+#define HPDEX_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types of the CSC data array.  The reference dispatches the same ten
+ * types (src/cpp/include/macro.hpp:4-14, src/cpp/module.cpp:144-166) and
+ * promotes every value to double before comparing (src/cpp/mannwhitneyu.cpp:920). */
+enum hpdex_dtype {
+    HPDEX_INT8 = 0, HPDEX_UINT8 = 1, HPDEX_INT16 = 2, HPDEX_UINT16 = 3,
+    HPDEX_INT32 = 4, HPDEX_UINT32 = 5, HPDEX_INT64 = 6, HPDEX_UINT64 = 7,
+    HPDEX_FLOAT32 = 8, HPDEX_FLOAT64 = 9
+};
+
+/* Option block.  Enum values are those of MannWhitneyuOption
+ * (src/cpp/include/mannwhitneyu.hpp:16-32):
+ *   zero_handling: 0 none, 1 min, 2 max, 3 mix
+ *   alternative:   0 less, 1 greater, 2 two-sided
+ *   method:        1 exact, 2 asymptotic
+ * ref_sorted / tar_sorted are accepted hints (results are defined only when
+ * they hold; the device path sorts regardless).  fast_norm is accepted and the
+ * precise erfc path is used for both values. */
+typedef struct hpdex_mwu_opts {
+    int32_t ref_sorted;
+    int32_t tar_sorted;
+    int32_t tie_correction;
+    int32_t use_continuity;
+    int32_t fast_norm;
+    int32_t zero_handling;
+    int32_t alternative;
+    int32_t method;
+} hpdex_mwu_opts;
+
+/* Mann-Whitney U of every (gene column c, target group g) pair of a CSC matrix
+ * against reference group 0.  Replaces hpdex::mannwhitneyu<T>
+ * (src/cpp/include/mannwhitneyu.hpp:34-42; src/cpp/mannwhitneyu.cpp:1067-1111)
+ * as bound by kernel.mannwhitneyu (src/cpp/module.cpp:35-107, 196-228).
+ *   data/indices/indptr : CSC triple (nnz values of type dtype, nnz int64 row
+ *                         indices, n_cols+1 int64 column pointers)
+ *   group_id            : n_rows int32; 0 = reference, 1..n_targets = targets,
+ *                         anything else = ignored row
+ *   U1_out, P_out       : n_cols*n_targets doubles each, gene-major
+ *                         (index c*n_targets + g-1)
+ *   threads             : accepted for signature parity; the device path
+ *                         ignores it.                                        */
+int hpdex_mwu_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                  size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                  size_t n_targets, const hpdex_mwu_opts* opts, int threads,
+                  double* U1_out, double* P_out, char* err, size_t err_len);
+
+/* hpdex_mwu_csc plus the per-pair statistics the reference keeps internally
+ * (tie_arr, n1_arr, n2_arr of src/cpp/mannwhitneyu.cpp:859-861): tie_out is
+ * the tie term sum(t^3 - t) (0 when tie_correction is off), n1_out/n2_out the
+ * sample sizes.  Any of tie_out/n1_out/n2_out may be NULL.                   */
+int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
+                        const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                        const int32_t* group_id, size_t n_targets, const hpdex_mwu_opts* opts,
+                        int threads, double* U1_out, double* P_out, double* tie_out,
+                        double* n1_out, double* n2_out, char* err, size_t err_len);
+
+/* Per-(column, group) mean.  Replaces hpdex::group_mean<T>
+ * (src/cpp/include/mannwhitneyu.hpp:45-53; src/cpp/mannwhitneyu.cpp:1113-1304)
+ * as bound by kernel.group_mean (src/cpp/module.cpp:110-141, 230-242).
+ * out: n_cols*n_groups doubles, index c*n_groups + g.                        */
+int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
+                         const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                         const int32_t* group_id, size_t n_groups, int include_zeros,
+                         int use_kahan, int threads, double* out, char* err, size_t err_len);
+
+/* ---------------- device-resident matrices and batched plans ----------------
+ * No counterpart in the reference (its API is one call per job).  They let a
+ * caller keep a CSC matrix in HBM and run several target sets against it.   */
+typedef struct hpdex_matrix hpdex_matrix;
+typedef struct hpdex_plan hpdex_plan;
+
+/* Upload a host CSC matrix (same argument meaning as hpdex_mwu_csc). */
+int hpdex_matrix_create(int dtype, const void* data, const int64_t* indices,
+                        const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                        hpdex_matrix** out, char* err, size_t err_len);
+void hpdex_matrix_destroy(hpdex_matrix* m);
+/* Bytes of HBM the resident matrix occupies. */
+size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
+
+/* A job on a resident matrix: group labels and options fixed, outputs held on
+ * the device.  form: 0 = plain (one workgroup per column, column order),
+ * 1 = batched (persistent grid over a size-ordered work queue, replayed from a
+ * captured hipGraph).                                                        */
+int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                      const hpdex_mwu_opts* opts, int form, hpdex_plan** out,
+                      char* err, size_t err_len);
+void hpdex_plan_destroy(hpdex_plan* p);
+/* Run the plan `iters` times back to back on its stream and wait.  If ms_total
+ * is non-NULL it receives the elapsed milliseconds (HIP events on the plan's
+ * stream); if kernel_ms is non-NULL it receives the mean duration of the
+ * per-gene kernel alone (HIP events around that launch).                    */
+int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms,
+                   char* err, size_t err_len);
+/* Copy the last run's results to host buffers (n_cols*n_targets doubles each;
+ * any pointer may be NULL). Returns the run's error, if it had one.          */
+int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie_out,
+                       double* n1_out, double* n2_out, char* err, size_t err_len);
+
+/* Launch geometry used by the per-gene kernels (the autotuner's choice). */
+typedef struct hpdex_tuning {
+    int32_t block_threads;     /* threads per workgroup (multiple of 64, <= 1024) */
+    int32_t lds_bytes;         /* dynamic LDS per workgroup                        */
+    int32_t blocks_per_cu;     /* persistent grid = CUs * blocks_per_cu            */
+    int32_t hist_max_bins;     /* largest integer range sent to the histogram path */
+} hpdex_tuning;
+int hpdex_get_tuning(hpdex_tuning* out);
+int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);
+
+/* Library version string and device summary, for logs. */
+const char* hpdex_version(void);
+int hpdex_device_info(int* n_cus, int* max_lds_per_block, char* name, size_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPDEX_MI355X_H */

@@ -1,0 +1,264 @@
+"""Write the golden fixtures in this folder from scipy/numpy, independent of
+both the oracle and the GPU path.
+
+Each fixture holds a CSC input (float values incl. explicit zeros, negatives,
+ties, NaN/Inf), group labels, options, and the expected per-pair U1, tie term
+and p-value.  The expectations come from closed-form reductions of hpdex's
+zero modes (src/cpp/mannwhitneyu.cpp:935-1039) to plain two-sample tests:
+
+  none : the stored finite values of each group (explicit zeros included)
+  min  : every group row; stored finite nonzero values as they are, explicit
+         zeros, absent entries and non-finite values replaced by -inf
+  max  : the same with +inf
+  mix  : stored finite values as they are, absent and non-finite entries = 0
+
+scipy.stats.mannwhitneyu(ref, tar).statistic is U1 (the reference group's U,
+src/cpp/mannwhitneyu.cpp:960) and its asymptotic p-value is the tie-corrected
+normal approximation with optional continuity correction; with
+tie_correction=False the expected p comes from scipy.stats.norm on the
+untied variance.  The tie term is sum(t^3 - t) over np.unique counts.
+Exact p-values come from scipy's exact null (method="exact").
+group_mean expectations come from numpy means with the include_zeros
+denominators of src/cpp/mannwhitneyu.cpp:1183-1186.
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import scipy.stats as st
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def random_csc(rng, R, C, density, values, p_explicit_zero=0.05, p_nonfinite=0.0):
+    """CSC arrays whose stored entries may include explicit zeros and NaN/Inf."""
+    data, indices, indptr = [], [], [0]
+    for _ in range(C):
+        rows = np.nonzero(rng.random(R) < density)[0]
+        v = values(len(rows)).astype(np.float64)
+        v[rng.random(len(rows)) < p_explicit_zero] = 0.0
+        nf = rng.random(len(rows)) < p_nonfinite
+        v[nf] = rng.choice([np.nan, np.inf, -np.inf], size=int(nf.sum()))
+        data.append(v)
+        indices.append(rows)
+        indptr.append(indptr[-1] + len(rows))
+    return (np.concatenate(data) if data else np.zeros(0), np.concatenate(indices).astype(np.int64),
+            np.asarray(indptr, dtype=np.int64))
+
+
+def column_samples(data, indices, indptr, R, gid, c, g, mode):
+    lo, hi = indptr[c], indptr[c + 1]
+    rows, vals = indices[lo:hi], data[lo:hi]
+    stored = np.full(R, np.nan)
+    present = np.zeros(R, bool)
+    stored[rows] = vals
+    present[rows] = True
+    sel = gid == g
+    v, pr = stored[sel], present[sel]
+    fin = pr & np.isfinite(v)
+    if mode == "none":
+        return v[fin]
+    out = np.where(fin, v, 0.0)
+    if mode == "min":
+        out = np.where(out == 0.0, -np.inf, out)
+    elif mode == "max":
+        out = np.where(out == 0.0, np.inf, out)
+    return out
+
+
+def tie_term(a, b):
+    _, cnt = np.unique(np.concatenate([a, b]), return_counts=True)
+    cnt = cnt.astype(np.float64)
+    return float(np.sum(cnt ** 3 - cnt))
+
+
+def expected(data, indices, indptr, R, gid, T, mode, tie_correction, use_continuity,
+             alternative, method):
+    C = len(indptr) - 1
+    U1 = np.zeros(C * T)
+    P = np.zeros(C * T)
+    TIE = np.zeros(C * T)
+    for c in range(C):
+        a = column_samples(data, indices, indptr, R, gid, c, 0, mode)
+        for g in range(1, T + 1):
+            b = column_samples(data, indices, indptr, R, gid, c, g, mode)
+            i = c * T + g - 1
+            r = st.mannwhitneyu(a, b, use_continuity=use_continuity, alternative=alternative,
+                                method=method)
+            U1[i] = r.statistic
+            t = tie_term(a, b)
+            TIE[i] = t if tie_correction else 0.0
+            if method == "asymptotic" and not tie_correction:
+                n1, n2 = len(a), len(b)
+                mu = n1 * n2 / 2.0
+                sd = np.sqrt(n1 * n2 * (n1 + n2 + 1) / 12.0)
+                cc = 0.5 if use_continuity else 0.0
+                if alternative == "two-sided":
+                    z = (max(r.statistic, n1 * n2 - r.statistic) - mu - cc) / sd
+                    P[i] = min(1.0, 2.0 * st.norm.sf(z))
+                elif alternative == "greater":
+                    P[i] = st.norm.sf((r.statistic - mu - cc) / sd)
+                else:
+                    P[i] = st.norm.sf((n1 * n2 - r.statistic - mu - cc) / sd)
+            else:
+                P[i] = r.pvalue
+    return U1, P, TIE
+
+
+def group_means(data, indices, indptr, R, gid, G, include_zeros):
+    C = len(indptr) - 1
+    out = np.zeros(C * G)
+    for c in range(C):
+        lo, hi = indptr[c], indptr[c + 1]
+        rows, vals = indices[lo:hi], data[lo:hi]
+        g_of = gid[rows]
+        for g in range(G):
+            m = g_of == g
+            v = vals[m]
+            fin = np.isfinite(v)
+            s = float(np.sum(v[fin]))
+            denom = (np.sum(gid == g) - np.sum(~fin)) if include_zeros else np.sum(fin)
+            out[c * G + g] = s / denom if denom > 0 else 0.0
+    return out
+
+
+def cases():
+    rng = np.random.default_rng(20261203)
+    ints = lambda n: rng.integers(-4, 6, n)                      # heavy ties, negatives
+    counts = lambda n: rng.negative_binomial(0.7, 0.3, n) + 1     # UMI-like
+    floats = lambda n: rng.normal(0.0, 2.0, n)                    # continuous
+    out = []
+    k = 0
+    for mode in ("none", "min", "max", "mix"):
+        for vals, dens, nonfin in ((ints, 0.5, 0.05), (counts, 0.3, 0.0), (floats, 0.6, 0.03)):
+            for alt in ("two-sided", "less", "greater"):
+                R, C, T = int(rng.integers(30, 90)), int(rng.integers(3, 9)), int(rng.integers(1, 5))
+                gid = rng.integers(-1, T + 1, R).astype(np.int32)
+                gid[:2] = 0                      # at least two rows in every group
+                for g in range(1, T + 1):
+                    gid[2 * g:2 * g + 2] = g
+                data, indices, indptr = random_csc(rng, R, C, dens, vals, 0.05, nonfin)
+                if mode == "none":
+                    # every group needs >= 2 stored finite values in every column
+                    data, indices, indptr = _ensure_none_sizes(rng, data, indices, indptr, R, gid, T)
+                tc = bool(k % 3 != 2)
+                cc = bool(k % 2 == 0)
+                U1, P, TIE = expected(data, indices, indptr, R, gid, T, mode, tc, cc, alt,
+                                      "asymptotic")
+                out.append(dict(name=f"asym_{mode}_{k:02d}", data=data, indices=indices,
+                                indptr=indptr, n_rows=R, group_id=gid, n_targets=T,
+                                zero_handling=mode, tie_correction=tc, use_continuity=cc,
+                                alternative=alt, method="asymptotic", U1=U1, P=P, tie=TIE))
+                k += 1
+    # exact null, small groups, no ties (scipy's exact distribution)
+    for alt in ("two-sided", "less", "greater"):
+        for mode in ("min", "none"):
+            R, C, T = 60, 5, 3
+            gid = np.repeat(np.arange(-1, T + 1), [6, 14, 12, 15, 13]).astype(np.int32)
+            rng.shuffle(gid)
+            dense = rng.normal(0, 1, (R, C))
+            rows = np.tile(np.arange(R), C)
+            data = dense.T.reshape(-1)
+            indptr = np.arange(C + 1, dtype=np.int64) * R
+            U1, P, TIE = expected(data, rows.astype(np.int64), indptr, R, gid, T, mode, True,
+                                  False, alt, "exact")
+            out.append(dict(name=f"exact_{mode}_{alt}", data=data, indices=rows.astype(np.int64),
+                            indptr=indptr, n_rows=R, group_id=gid, n_targets=T,
+                            zero_handling=mode, tie_correction=True, use_continuity=False,
+                            alternative=alt, method="exact", U1=U1, P=P, tie=TIE))
+    # exact with ties (scipy floors U like the reference, src/cpp/mannwhitneyu.cpp:104-105)
+    R, C, T = 40, 4, 2
+    gid = np.repeat(np.arange(-1, T + 1), [4, 12, 10, 14]).astype(np.int32)
+    rng.shuffle(gid)
+    data, indices, indptr = random_csc(rng, R, C, 0.7, ints, 0.0, 0.0)
+    U1, P, TIE = expected(data, indices, indptr, R, gid, T, "mix", True, False, "two-sided",
+                          "exact")
+    out.append(dict(name="exact_mix_ties", data=data, indices=indices, indptr=indptr, n_rows=R,
+                    group_id=gid, n_targets=T, zero_handling="mix", tie_correction=True,
+                    use_continuity=False, alternative="two-sided", method="exact", U1=U1, P=P,
+                    tie=TIE))
+    return out
+
+
+def _ensure_none_sizes(rng, data, indices, indptr, R, gid, T):
+    """Add stored values so that every group has >= 2 finite entries per column."""
+    C = len(indptr) - 1
+    nd, ni, nip = [], [], [0]
+    for c in range(C):
+        lo, hi = indptr[c], indptr[c + 1]
+        rows = list(indices[lo:hi])
+        vals = list(data[lo:hi])
+        for g in range(T + 1):
+            members = np.nonzero(gid == g)[0]
+            have = [r for r, v in zip(rows, vals) if gid[r] == g and np.isfinite(v)]
+            for r in members:
+                if len(have) >= 2:
+                    break
+                if r not in rows:
+                    rows.append(r)
+                    vals.append(float(rng.integers(1, 5)))
+                    have.append(r)
+                else:
+                    j = rows.index(r)
+                    if not np.isfinite(vals[j]):
+                        vals[j] = float(rng.integers(1, 5))
+                        have.append(r)
+        o = np.argsort(rows)
+        nd.append(np.asarray(vals)[o])
+        ni.append(np.asarray(rows, dtype=np.int64)[o])
+        nip.append(nip[-1] + len(rows))
+    return np.concatenate(nd), np.concatenate(ni), np.asarray(nip, dtype=np.int64)
+
+
+def small_config_fixture():
+    """The 2,000 x 500 CPU-runnable workload (tie+continuity, min, asymptotic,
+    group means of 5 groups with implicit zeros)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from hpdex_amd.synthetic import small_case
+    w = small_case()
+    data = w["data"].astype(np.float64)
+    U1, P, TIE = expected(data, w["indices"], w["indptr"], w["n_rows"], w["group_id"], 4, "min",
+                          True, True, "two-sided", "asymptotic")
+    GM = group_means(data, w["indices"], w["indptr"], w["n_rows"], w["group_id"], 5, True)
+    return dict(U1=U1, P=P, tie=TIE, group_mean=GM)
+
+
+def main():
+    cs = cases()
+    arrays = {}
+    names = []
+    for c in cs:
+        names.append(c["name"])
+        for key, v in c.items():
+            if key == "name":
+                continue
+            arrays[f"{c['name']}__{key}"] = np.asarray(v)
+    arrays["__names__"] = np.asarray(names)
+    np.savez_compressed(os.path.join(HERE, "mwu_cases.npz"), **arrays)
+    # group_mean cases
+    rng = np.random.default_rng(7)
+    gm = {}
+    for k, (inc, nonfin) in enumerate(((True, 0.0), (False, 0.0), (True, 0.05), (False, 0.05))):
+        R, C, G = 70, 6, 4
+        gid = rng.integers(-1, G, R).astype(np.int32)
+        data, indices, indptr = random_csc(rng, R, C, 0.4, lambda n: rng.normal(1, 3, n), 0.05,
+                                           nonfin)
+        gm[f"gm{k}__data"] = data
+        gm[f"gm{k}__indices"] = indices
+        gm[f"gm{k}__indptr"] = indptr
+        gm[f"gm{k}__group_id"] = gid
+        gm[f"gm{k}__n_groups"] = np.asarray(G)
+        gm[f"gm{k}__include_zeros"] = np.asarray(inc)
+        gm[f"gm{k}__expected"] = group_means(data, indices, indptr, R, gid, G, inc)
+    np.savez_compressed(os.path.join(HERE, "group_mean_cases.npz"), **gm)
+    sc = small_config_fixture()
+    np.savez_compressed(os.path.join(HERE, "small_config_expected.npz"), **sc)
+    print(f"wrote {len(cs)} MWU cases, 4 group_mean cases, small-config expectations")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+"""GPU parity: the HIP path, called through the C-ABI, against the oracle and
+the golden fixtures.  U1 and the tie term must be bit-identical; asymptotic
+and exact p within 1e-12 relative of the FP64 oracle; group means within
+1e-6 relative."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from fixtures import group_mean_cases, mwu_cases, rel_close, small_config_expected
+from hpdex_amd import _capi, synthetic
+
+pytestmark = pytest.mark.gpu
+
+ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
+ALT = {"less": 0, "greater": 1, "two-sided": 2}
+METH = {"exact": 1, "asymptotic": 2}
+P_RTOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_stats(w, mode, alt="two-sided", method="asymptotic", tie=True, cc=True):
+    opts = _capi.make_opts(tie_correction=tie, use_continuity=cc, zero_handling=ZH[mode],
+                           alternative=ALT[alt], method=METH[method])
+    return _capi.mwu_csc(w["data"], w["indices"], w["indptr"], w["group_id"], w["n_targets"],
+                         opts, n_rows=w["n_rows"], with_stats=True)
+
+
+def oracle_stats(w, mode, alt="two-sided", method="asymptotic", tie=True, cc=True):
+    return oracle.mwu(w["data"], w["indices"], w["indptr"], w["group_id"], w["n_targets"],
+                      n_rows=w["n_rows"], tie_correction=tie, use_continuity=cc,
+                      zero_handling=mode, alternative=alt, method=method, with_stats=True)
+
+
+def assert_parity(g, o, p_rtol=P_RTOL):
+    U1g, Pg, Tg, n1g, n2g = g
+    U1o, Po, To, n1o, n2o = o
+    assert np.array_equal(n1g, n1o) and np.array_equal(n2g, n2o), "sample sizes differ"
+    bad = np.nonzero(U1g != U1o)[0]
+    assert len(bad) == 0, f"U1 differs at {bad[:5]}: {U1g[bad[:5]]} vs {U1o[bad[:5]]}"
+    bad = np.nonzero(Tg != To)[0]
+    assert len(bad) == 0, f"tie differs at {bad[:5]}: {Tg[bad[:5]]} vs {To[bad[:5]]}"
+    assert rel_close(Pg, Po, rtol=p_rtol), float(np.max(np.abs(Pg - Po) / np.maximum(Po, 1e-300)))
+
+
+CASES = mwu_cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_golden_cases(case):
+    say(f"golden {case['name']}")
+    w = dict(data=case["data"], indices=case["indices"], indptr=case["indptr"],
+             group_id=case["group_id"], n_targets=case["n_targets"], n_rows=case["n_rows"])
+    g = gpu_stats(w, case["zero_handling"], case["alternative"], case["method"],
+                  case["tie_correction"], case["use_continuity"])
+    assert np.array_equal(g[0], case["U1"])
+    assert np.array_equal(g[2], case["tie"])
+    assert rel_close(g[1], case["P"], rtol=1e-12)
+    o = oracle_stats(w, case["zero_handling"], case["alternative"], case["method"],
+                     case["tie_correction"], case["use_continuity"])
+    assert_parity(g, o)
+
+
+def test_small_config_and_wrapper():
+    say("small config")
+    w = synthetic.small_case()
+    exp = small_config_expected()
+    g = gpu_stats(w, "min")
+    assert np.array_equal(g[0], exp["U1"]) and np.array_equal(g[2], exp["tie"])
+    assert_parity(g, oracle_stats(w, "min"))
+    from hpdex_amd.backend import group_mean, mannwhitneyu
+    M = sp.csc_matrix((w["data"], w["indices"], w["indptr"]), shape=(w["n_rows"], w["n_cols"]))
+    U1, P = mannwhitneyu(M, w["group_id"], 4, use_continuity=True, tie_correction=True,
+                         zero_handling="min")
+    assert not U1.flags.writeable and not P.flags.writeable and U1.dtype == np.float64
+    assert np.array_equal(U1, exp["U1"]) and rel_close(P, exp["P"])
+    gm = group_mean(M, w["group_id"], 5, include_zeros=True)
+    assert rel_close(gm, exp["group_mean"], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_group_mean_golden(k):
+    say(f"group_mean {k}")
+    c = group_mean_cases()[k]
+    out = _capi.group_mean_csc(c["data"], c["indices"], c["indptr"], c["group_id"],
+                               c["n_groups"], include_zeros=c["include_zeros"])
+    assert rel_close(out, c["expected"], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32,
+                                   np.int64, np.uint64, np.float32, np.float64])
+def test_all_dtypes(dtype):
+    say(f"dtype {np.dtype(dtype).name}")
+    rng = np.random.default_rng(11)
+    R, C, T = 300, 40, 3
+    dense = rng.integers(0, 9, (R, C)) * (rng.random((R, C)) < 0.4)
+    if np.dtype(dtype).kind == "i":
+        dense = dense - 3 * (rng.random((R, C)) < 0.2)
+    M = sp.csc_matrix(dense.astype(dtype))
+    gid = rng.integers(-1, T + 1, R).astype(np.int32)
+    w = dict(data=M.data, indices=M.indices, indptr=M.indptr, group_id=gid, n_targets=T,
+             n_rows=R)
+    for mode in ("min", "mix", "none"):
+        assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+    gm = _capi.group_mean_csc(M.data, M.indices, M.indptr, gid, T + 1)
+    ref = oracle.group_mean(M.data, M.indices, M.indptr, gid, T + 1)
+    assert rel_close(gm, ref, rtol=1e-6, atol=1e-12)
+
+
+def test_error_messages_match_oracle():
+    say("errors")
+    cases = []
+    gid = np.array([0, 1, 1, 1], dtype=np.int32)
+    M = sp.csc_matrix(np.arange(1, 9, dtype=np.float32).reshape(4, 2))
+    cases.append((M, gid, 1, "min"))
+    gid = np.array([0, 0, 0, 1, 1, 1, 2], dtype=np.int32)
+    M = sp.csc_matrix(np.arange(1, 15, dtype=np.float32).reshape(7, 2))
+    cases.append((M, gid, 2, "mix"))
+    dense = np.array([[1, 1], [2, 2], [3, 0], [4, 5], [5, 0]], dtype=np.float32)
+    cases.append((sp.csc_matrix(dense), np.array([0, 0, 1, 1, 1], dtype=np.int32), 1, "none"))
+    dense = np.array([[1, 0], [0, 2], [3, 1], [4, 5], [5, 6]], dtype=np.float32)
+    cases.append((sp.csc_matrix(dense), np.array([0, 0, 1, 1, 1], dtype=np.int32), 1, "none"))
+    for M, gid, T, mode in cases:
+        with pytest.raises(oracle.OracleError) as eo:
+            oracle.mwu(M.data, M.indices, M.indptr, gid, T, zero_handling=mode)
+        with pytest.raises(_capi.HpdexError) as eg:
+            _capi.mwu_csc(M.data, M.indices, M.indptr, gid, T,
+                          _capi.make_opts(zero_handling=ZH[mode]))
+        assert str(eg.value) == str(eo.value)
+    with pytest.raises(_capi.HpdexError, match="n_targets must be >= 1"):
+        _capi.mwu_csc(M.data, M.indices, M.indptr, gid, 0, _capi.make_opts())
+
+
+def _random_workload(seed, R, C, T, dens, values, ref_frac=0.3):
+    rng = np.random.default_rng(seed)
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.full(C, dens))
+    data = values(rng, len(rows)).astype(np.float32)
+    gid = np.full(R, -1, dtype=np.int32)
+    perm = rng.permutation(R)
+    nref = int(R * ref_frac)
+    gid[perm[:nref]] = 0
+    gid[perm[nref:]] = rng.integers(1, T + 1, R - nref)
+    return dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R,
+                n_cols=C)
+
+
+@pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
+def test_float_sort_path(mode):
+    """Continuous values with negatives: the sort path, large reference slices
+    (multi-chunk LDS sort) and target slices above 1024 (chunked ties)."""
+    say(f"float sort path {mode}")
+    w = _random_workload(5, 12000, 24, 3, 0.5,
+                         lambda r, n: np.round(r.normal(0, 2, n), 2))
+    assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+
+
+@pytest.mark.parametrize("mode", ["min", "mix", "none"])
+def test_integer_hist_path(mode):
+    say(f"integer hist path {mode}")
+    w = _random_workload(6, 20000, 30, 25, 0.3,
+                         lambda r, n: r.negative_binomial(0.8, 0.2, n) - (mode != "min") * 3)
+    assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+
+
+def test_wide_integer_range_uses_sort_path():
+    say("wide integer range")
+    w = _random_workload(7, 5000, 20, 6, 0.4, lambda r, n: r.integers(-100000, 100000, n))
+    assert_parity(gpu_stats(w, "mix"), oracle_stats(w, "mix"))
+
+
+@pytest.mark.parametrize("mode", ["min", "none"])
+@pytest.mark.parametrize("alt", ["two-sided", "less", "greater"])
+def test_exact_path(mode, alt):
+    say(f"exact {mode} {alt}")
+    w = synthetic.exact_case(n_cells=3000, n_genes=40, n_ref=60, n_targets=60)
+    g = gpu_stats(w, mode, alt, "exact", True, False)
+    o = oracle_stats(w, mode, alt, "exact", True, False)
+    assert_parity(g, o)
+
+
+def test_plans_plain_batched_bit_identical():
+    say("plans")
+    w = _random_workload(8, 8000, 200, 12, 0.15, lambda r, n: r.negative_binomial(1, 0.3, n) + 1)
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    oneshot = _capi.mwu_csc(w["data"], w["indices"], w["indptr"], w["group_id"], 12, opts,
+                            with_stats=True)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    for form in (0, 1):
+        p = _capi.Plan(m, w["group_id"], 12, opts, form=form)
+        p.run(3)
+        res = p.results(with_stats=True)
+        for a, b in zip(res, oneshot):
+            assert np.array_equal(a, b)
+        p.close()
+    m.close()
+
+
+def test_histogram_workload_full_size():
+    """The 150,000 x 8,000 integer-count workload: batched == plain bit for
+    bit, structural properties everywhere, oracle parity on a column sample
+    that includes the densest genes."""
+    say("histogram workload generate")
+    w = synthetic.histogram_case()
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    say("histogram workload batched")
+    pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1)
+    pb.run(2)
+    U1, P, tie, n1, n2 = pb.results(with_stats=True)
+    say("histogram workload plain")
+    pp = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=0)
+    pp.run(1)
+    res_plain = pp.results(with_stats=True)
+    for a, b in zip((U1, P, tie, n1, n2), res_plain):
+        assert np.array_equal(a, b)
+    assert np.all(U1 >= 0) and np.all(U1 <= n1 * n2)
+    assert np.array_equal(2 * U1, np.round(2 * U1))
+    assert np.all((P >= 0) & (P <= 1))
+    assert np.all(n1 == 15000)
+    sizes = np.bincount(w["group_id"][w["group_id"] > 0], minlength=w["n_targets"] + 1)[1:]
+    assert np.array_equal(n2.reshape(-1, w["n_targets"])[0], sizes.astype(float))
+    nnz = np.diff(w["indptr"])
+    cols = np.unique(np.concatenate([np.argsort(nnz)[-40:], np.arange(0, w["n_cols"], 97)]))
+    sub = synthetic.subset_columns(w, cols)
+    say(f"histogram workload oracle on {len(cols)} columns")
+    o = oracle_stats(sub, "min")
+    T = w["n_targets"]
+    g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
+    assert_parity(g, o)
+    pb.close(); pp.close(); m.close()
