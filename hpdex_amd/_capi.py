@@ -27,6 +27,7 @@ EXPORTED = (
     "hpdex_matrix_create", "hpdex_matrix_destroy", "hpdex_matrix_device_bytes",
     "hpdex_plan_create", "hpdex_plan_destroy", "hpdex_plan_run", "hpdex_plan_results",
     "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
+    "hpdex_bandwidth_probe",
 )
 
 
@@ -37,8 +38,7 @@ class MwuOpts(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in (
-        "block_threads", "lds_bytes", "blocks_per_cu", "hist_max_bins")]
+    _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered")]
 
 
 class HpdexError(RuntimeError):
@@ -79,10 +79,11 @@ def lib() -> ctypes.CDLL:
         L.hpdex_set_tuning.argtypes = [ctypes.POINTER(Tuning), cp, sz]
         L.hpdex_version.restype = cp
         L.hpdex_device_info.argtypes = [ctypes.POINTER(ci), ctypes.POINTER(ci), cp, sz]
+        L.hpdex_bandwidth_probe.argtypes = [sz, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
                      "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
-                     "hpdex_device_info"):
+                     "hpdex_device_info", "hpdex_bandwidth_probe"):
             getattr(L, name).restype = ci
         _lib = L
     return _lib
@@ -236,16 +237,12 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(block_threads=None, lds_bytes=None, blocks_per_cu=None, hist_max_bins=None):
+def set_tuning(hist_max_bins=None, size_ordered=None):
     t = get_tuning()
-    if block_threads is not None:
-        t.block_threads = block_threads
-    if lds_bytes is not None:
-        t.lds_bytes = lds_bytes
-    if blocks_per_cu is not None:
-        t.blocks_per_cu = blocks_per_cu
     if hist_max_bins is not None:
         t.hist_max_bins = hist_max_bins
+    if size_ordered is not None:
+        t.size_ordered = size_ordered
     err = ctypes.create_string_buffer(1024)
     if lib().hpdex_set_tuning(ctypes.byref(t), err, len(err)) != 0:
         raise HpdexError(err.value.decode())
@@ -257,3 +254,12 @@ def device_info():
     if lib().hpdex_device_info(ctypes.byref(cus), ctypes.byref(lds), name, len(name)) != 0:
         raise HpdexError("no HIP device available")
     return {"cus": cus.value, "max_lds_per_block": lds.value, "name": name.value.decode()}
+
+
+def bandwidth_probe(nbytes=1 << 30, iters=10) -> float:
+    """GB/s of a streaming copy (read + write bytes) on the device."""
+    g = ctypes.c_double()
+    err = ctypes.create_string_buffer(1024)
+    if lib().hpdex_bandwidth_probe(int(nbytes), int(iters), ctypes.byref(g), err, len(err)) != 0:
+        raise HpdexError(err.value.decode())
+    return g.value

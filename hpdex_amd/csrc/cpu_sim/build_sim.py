@@ -1,0 +1,56 @@
+"""Build the host simulator of the gfx950 kernels (test/debug tool).
+
+The device part of hpdex_kernels.hip (everything above its launcher section)
+is copied, with the two device-only constructs replaced -- the dynamic LDS
+declaration becomes the simulator's buffer and the empty register-pinning
+asm statements are dropped -- and compiled with g++ and AddressSanitizer
+together with sim_main.cpp into _build/cpu_sim/libhpdex_cpusim.so.
+"""
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.dirname(HERE)
+ROOT = os.path.dirname(os.path.dirname(CSRC))
+OUT = os.path.join(ROOT, "_build", "cpu_sim")
+LIB = os.path.join(OUT, "libhpdex_cpusim.so")
+
+
+def _device_text(path, cut_marker=None):
+    s = open(path).read()
+    if cut_marker:
+        s = s.split(cut_marker)[0]
+    s = s.replace("extern __shared__ __align__(16) unsigned char smem[];",
+                  "unsigned char* smem = sim::g_lds;")
+    s = re.sub(r'asm volatile\(""\s*:\s*"\+v"\((\w+)\)\);', r"(void)\1;", s)
+    # workgroup-static LDS objects live in a simulator buffer that is filled
+    # with a garbage pattern before every workgroup, as device LDS is
+    s = re.sub(r"__shared__ (\w+) (\w+);",
+               r"\1& \2 = *reinterpret_cast<\1*>(sim::static_lds(sizeof(\1)));", s)
+    return s
+
+
+def build(sanitize=True, compiler="g++", opt="-O1", lib=None):
+    os.makedirs(OUT, exist_ok=True)
+    dev = _device_text(os.path.join(CSRC, "hpdex_device.h"))
+    ker = _device_text(os.path.join(CSRC, "hpdex_kernels.hip"),
+                       "// ---------------------------------------------------------------------------\n// launchers")
+    ker = ker.replace('#include "hpdex_device.h"', dev) + "\n}  // namespace hpdex\n"
+    with open(os.path.join(OUT, "hpdex_kernels_device.inc"), "w") as f:
+        f.write(ker)
+    flags = [opt, "-g", "-std=c++20", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
+             "-Wno-unknown-pragmas", "-Wno-attributes"]
+    if sanitize:
+        flags += ["-fsanitize=address,undefined", "-fno-omit-frame-pointer"]
+    cmd = [compiler, *flags, "-I", HERE, "-I", OUT, "-I", CSRC,
+           os.path.join(HERE, "sim_main.cpp"), "-o", lib or LIB]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-6000:])
+    return lib or LIB
+
+
+if __name__ == "__main__":
+    print(build(sanitize="--no-asan" not in sys.argv))

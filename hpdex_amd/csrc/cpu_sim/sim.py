@@ -1,0 +1,49 @@
+"""ctypes driver of the host kernel simulator (see build_sim.py)."""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build_sim
+
+_lib = None
+# Plain build for in-process use (AddressSanitizer needs its runtime preloaded:
+# run build_sim.py directly and LD_PRELOAD libasan for that).
+NOSAN = os.path.join(build_sim.OUT, "libhpdex_cpusim_nosan.so")
+
+
+def lib(path=None):
+    global _lib
+    if _lib is None:
+        path = path or NOSAN
+        srcs = [os.path.join(build_sim.CSRC, f) for f in ("hpdex_kernels.hip", "hpdex_device.h",
+                                                          "hpdex_internal.h")]
+        srcs += [os.path.join(build_sim.HERE, f) for f in ("sim_main.cpp", "sim_runtime.h")]
+        if not os.path.exists(path) or any(os.path.getmtime(x) > os.path.getmtime(path)
+                                           for x in srcs):
+            build_sim.build(sanitize=False, opt="-O2", lib=path)
+        L = ctypes.CDLL(path)
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.sim_mwu.argtypes = [ci, vp, vp, vp, ci, ci, vp, ci, ci, ci, ci, ci, ci, ci, ci,
+                              vp, vp, vp, vp, vp, vp, vp, ctypes.c_double]
+        L.sim_mwu.restype = ci
+        _lib = L
+    return _lib
+
+
+def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
+        lds=76 * 1024, block=512, hist_max=8192, f64=False, timeout_s=30.0):
+    vals = np.ascontiguousarray(data, dtype=np.float64 if f64 else np.float32)
+    rows = np.where((indices >= 0) & (indices < n_rows), indices, -1).astype(np.int32)
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    gid = np.ascontiguousarray(group_id, dtype=np.int32)
+    C = len(indptr) - 1
+    n = C * n_targets
+    outs = [np.zeros(max(n, 1)) for _ in range(5)]
+    err = np.zeros(1, dtype=np.uint64)
+    status = np.zeros(1, dtype=np.uint32)
+    lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
+                  gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), lds, block, hist_max,
+                  *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
+                  float(timeout_s))
+    return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

@@ -1,0 +1,124 @@
+// sim_main.cpp -- run the per-gene kernels of hpdex_kernels.hip on the host
+// (see sim_runtime.h).  Exposes a C entry point that mirrors the plain
+// one-workgroup-per-column launch, for CPU tests and fault hunting.  Built
+// with AddressSanitizer by build_sim.py; never shipped.
+#include <chrono>
+#include <string>
+#include <thread>
+
+#include "sim_runtime.h"
+
+thread_local SimDim threadIdx;
+SimDim blockIdx, blockDim, gridDim;
+namespace sim {
+std::vector<std::unique_ptr<Wave>> g_waves;
+std::unique_ptr<std::barrier<>> g_block_bar;
+unsigned char* g_lds = nullptr;
+size_t g_lds_bytes = 0;
+thread_local const char* t_last_op = "";
+alignas(64) unsigned char g_static_lds[4096];
+}  // namespace sim
+
+#include "hpdex_kernels_device.inc"
+
+using namespace hpdex;
+
+namespace {
+
+struct ThreadInfo { std::atomic<const char*>* last; };
+
+// Run one workgroup: nt host threads execute `body` with threadIdx set.
+// A watchdog aborts with a per-wave summary if the block does not finish.
+bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void()>& body,
+               double timeout_s) {
+    blockIdx.x = b;
+    blockDim.x = nt;
+    gridDim.x = grid;
+    sim::g_waves.clear();
+    for (unsigned w = 0; w < nt / 64; ++w) sim::g_waves.emplace_back(new sim::Wave());
+    sim::g_block_bar.reset(new std::barrier<>((std::ptrdiff_t)nt));
+    // device LDS is not cleared between workgroups: start from garbage
+    std::memset(sim::g_lds, 0xA5, sim::g_lds_bytes);
+    std::memset(sim::g_static_lds, 0x5A, sizeof(sim::g_static_lds));
+    std::vector<std::atomic<const char*>> last(nt);
+    std::atomic<unsigned> finished{0};
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t] {
+            threadIdx.x = t;
+            body();
+            last[t].store("done");
+            finished.fetch_add(1);
+        });
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    while (finished.load() < nt) {
+        std::this_thread::sleep_for(std::chrono::milliseconds(5));
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (el > timeout_s) {
+            std::fprintf(stderr, "sim: block %u stalled after %.1f s; %u of %u threads finished\n",
+                         b, el, finished.load(), nt);
+            std::fflush(stderr);
+            std::_Exit(3);
+        }
+    }
+    for (auto& x : th) x.join();
+    return true;
+}
+
+}  // namespace
+
+template <typename V>
+static void run_pipeline(const JobDesc& J, const std::vector<WorkItem>& items, double timeout_s) {
+    const JobDesc* jobs = &J;
+    const WorkItem* it = items.data();
+    const unsigned n = (unsigned)items.size();
+    std::vector<unsigned char> buf(64 * 1024);
+    sim::g_lds = buf.data();
+    sim::g_lds_bytes = buf.size();
+    for (unsigned b = 0; b < n; ++b)
+        run_block(b, kColBlock, n, [&] { split_kernel<V>(jobs, it); }, timeout_s);
+    for (unsigned b = 0; b < n; ++b)
+        run_block(b, kColBlock, n, [&] { ref_kernel<V>(jobs, it); }, timeout_s);
+    for (unsigned b = 0; b < n; ++b)
+        run_block(b, kColBlock, n, [&] { target_kernel<V>(jobs, it); }, timeout_s);
+    const int64_t np = J.C * J.T;
+    const unsigned pg = (unsigned)((np + 255) / 256);
+    for (unsigned b = 0; b < pg; ++b)
+        run_block(b, 256, pg, [&] { pair_kernel(J); }, timeout_s);
+}
+
+extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int64_t* indptr,
+                       int R, int C, const int32_t* gid, int T, int mode, int alt, int tie_corr,
+                       int cc, int lds, int block, int hist_max, double* U1, double* P,
+                       double* tie, double* n1, double* n2, unsigned long long* err_key,
+                       unsigned* status, double timeout_s) {
+    (void)lds;
+    (void)block;
+    const int G = T + 1;
+    std::vector<uint32_t> grows(G, 0u);
+    for (int r = 0; r < R; ++r)
+        if (gid[r] >= 0 && gid[r] < G) ++grows[gid[r]];
+    unsigned long long status_col = ~0ull;
+    *err_key = ~0ull;
+    *status = 0u;
+    const size_t nnz = (size_t)indptr[C];
+    std::vector<uint64_t> keys(nnz + 1);
+    std::vector<uint32_t> cnt((size_t)C * G), zcnt((size_t)C * G), goff((size_t)C * G);
+    std::vector<ColPlan> plan(C);
+    std::vector<uint32_t> reftab((size_t)C * kTabStride);
+    JobDesc J{};
+    J.vals = vals; J.rows = rows; J.indptr = indptr; J.gid = gid; J.group_rows = grows.data();
+    J.keys = keys.data(); J.cnt = cnt.data(); J.zcnt = zcnt.data(); J.goff = goff.data();
+    J.plan = plan.data(); J.reftab = reftab.data();
+    J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;
+    J.err_key = err_key; J.status = status; J.status_col = &status_col;
+    J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
+    J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
+    std::vector<WorkItem> items(C);
+    for (int c = 0; c < C; ++c) items[c] = WorkItem{0, C - 1 - c};   // reversed order
+    if (f64) run_pipeline<double>(J, items, timeout_s);
+    else run_pipeline<float>(J, items, timeout_s);
+    return 0;
+}

@@ -30,10 +30,8 @@ namespace {
 constexpr size_t kExactMaxCells = (size_t)1 << 27;
 
 struct Tuning {
-    int block = 512;
-    int lds = 128 * 1024;
-    int blocks_per_cu = 1;
-    int hist_max = 8192;
+    int hist_max = kHistMaxBins;
+    int size_ordered = 1;
 };
 
 std::mutex g_mu;
@@ -74,10 +72,6 @@ int ensure_init(char* err, size_t err_len) {
     g_cus = pr.multiProcessorCount;
     g_max_lds = (int)pr.sharedMemPerBlock;
     std::snprintf(g_name, sizeof g_name, "%s (%s)", pr.name, pr.gcnArchName);
-    // ColState and friends take a little static LDS next to the dynamic part
-    const int cap = g_max_lds - 1024;
-    if (g_tune.lds > cap) g_tune.lds = cap;
-    HIPCHK(set_gene_kernel_lds_limit(g_tune.lds));
     g_init = true;
     return 0;
 }
@@ -203,6 +197,8 @@ struct JobState {
     hpdex_mwu_opts opts{};
     std::vector<uint32_t> group_rows;
     DevBuf gid, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
+    DevBuf d_job, d_items;                          // descriptor and column queue
     // exact null tables
     std::vector<ExactTable> tabs;
     std::vector<int32_t> tidx_by_g;
@@ -239,6 +235,12 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     HIPCHK(js.n1.alloc(np * 8));
     HIPCHK(js.n2.alloc(np * 8));
     HIPCHK(js.flags.alloc(32));
+    HIPCHK(js.keys.alloc(m->nnz * (m->f64 ? 8 : 4)));
+    HIPCHK(js.cnt.alloc(js.C * js.G * 4));
+    HIPCHK(js.zcnt.alloc(js.C * js.G * 4));
+    HIPCHK(js.goff.alloc(js.C * js.G * 4));
+    HIPCHK(js.plan.alloc(js.C * sizeof(ColPlan)));
+    HIPCHK(js.reftab.alloc(js.C * (size_t)kTabStride * 4));
     HIPCHK(launch_init_job_state(js.err_key(), js.status(), js.status_col(), s));
     return 0;
 }
@@ -250,6 +252,12 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.indptr = m->indptr.as<int64_t>();
     J.gid = js.gid.as<int32_t>();
     J.group_rows = js.grows.as<uint32_t>();
+    J.keys = js.keys.p;
+    J.cnt = js.cnt.as<uint32_t>();
+    J.zcnt = js.zcnt.as<uint32_t>();
+    J.goff = js.goff.as<uint32_t>();
+    J.plan = js.plan.as<ColPlan>();
+    J.reftab = js.reftab.as<uint32_t>();
     J.U1 = js.U1.as<double>();
     J.P = js.P.as<double>();
     J.tie = js.tie.as<double>();
@@ -265,10 +273,27 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.mode = js.opts.zero_handling;
     J.alt = js.opts.alternative;
     J.tie_corr = js.opts.tie_correction ? 1 : 0;
-    J.fuse_p = (js.opts.method == 2) ? 1 : 0;
+    J.asym = (js.opts.method == 2) ? 1 : 0;
     J.hist_max = tu.hist_max;
     J.cc = js.opts.use_continuity ? 0.5 : 0.0;
     return J;
+}
+
+// Put the job descriptor and its column queue on the device: columns in
+// order (plain form) or largest first (batched form).
+int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, bool size_ordered,
+                    hipStream_t s, char* err, size_t err_len) {
+    HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
+    HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
+    std::vector<WorkItem> items(m.C);
+    for (size_t k = 0; k < m.C; ++k)
+        items[k] = WorkItem{0, size_ordered ? m.order[k] : (int32_t)k};
+    HIPCHK(js.d_items.alloc(items.size() * sizeof(WorkItem)));
+    if (!items.empty())
+        HIPCHK(hipMemcpyAsync(js.d_items.p, items.data(), items.size() * sizeof(WorkItem),
+                              hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));   // items is a host temporary
+    return 0;
 }
 
 // Plan the exact-null tables.  min/max/mix: sizes are the group row counts,
@@ -374,12 +399,12 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
                            " at column " + std::to_string(col));
     }
     if (status) {
-        const std::string why = (status & kStatusGroups)
-                                    ? "too many target groups for the on-chip group counters"
-                                    : "a reference or target slice exceeds the on-chip capacity";
-        return set_err(err, err_len,
-                       "column " + std::to_string(h[1]) + ": " + why +
-                           " (raise lds_bytes with hpdex_set_tuning)");
+        const std::string why =
+            (status & kStatusGroups)
+                ? "more target groups than the on-chip group counters hold (5,460)"
+                : "the reference slice holds more distinct-valued entries than the on-chip "
+                  "sort takes (8,192 for 4-byte values, 4,096 for 8-byte values)";
+        return set_err(err, err_len, "column " + std::to_string(h[1]) + ": " + why);
     }
     return 0;
 }
@@ -435,33 +460,48 @@ int hpdex_device_info(int* n_cus, int* max_lds_per_block, char* name, size_t nam
     return 0;
 }
 
+int hpdex_bandwidth_probe(size_t bytes, int iters, double* gbs, char* err, size_t err_len) {
+    if (ensure_init(err, err_len)) return 1;
+    if (bytes < (1u << 20) || iters < 1) return set_err(err, err_len, "probe needs >= 1 MiB and >= 1 iteration");
+    bytes &= ~(size_t)15;
+    Stream st;
+    HIPCHK(st.create());
+    DevBuf a, b;
+    HIPCHK(a.alloc(bytes));
+    HIPCHK(b.alloc(bytes));
+    HIPCHK(launch_stream_copy(a.p, b.p, bytes, g_cus * 8, st.s));   // first touch
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipEventRecord(e0, st.s);
+    for (int i = 0; i < iters; ++i) HIPCHK(launch_stream_copy(a.p, b.p, bytes, g_cus * 8, st.s));
+    hipEventRecord(e1, st.s);
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (gbs) *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    return 0;
+}
+
 int hpdex_get_tuning(hpdex_tuning* out) {
     if (!out) return 1;
     const Tuning t = tuning();
-    out->block_threads = t.block;
-    out->lds_bytes = t.lds;
-    out->blocks_per_cu = t.blocks_per_cu;
     out->hist_max_bins = t.hist_max;
+    out->size_ordered = t.size_ordered;
     return 0;
 }
 
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     if (!t) return set_err(err, err_len, "null tuning");
-    if (ensure_init(err, err_len)) return 1;
-    if (t->block_threads < 64 || t->block_threads > 512 || t->block_threads % 64)
-        return set_err(err, err_len, "block_threads must be a multiple of 64 in [64, 512]");
-    if (t->lds_bytes < 16384 || t->lds_bytes > g_max_lds - 1024)
-        return set_err(err, err_len, "lds_bytes out of range for this device");
-    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8)
-        return set_err(err, err_len, "blocks_per_cu must be in [1, 8]");
-    if (t->hist_max_bins < 0 || t->hist_max_bins > 32768)
-        return set_err(err, err_len, "hist_max_bins must be in [0, 32768]");
+    if (t->hist_max_bins < 0 || t->hist_max_bins > kHistMaxBins)
+        return set_err(err, err_len, "hist_max_bins must be in [0, 8192]");
+    if (t->size_ordered != 0 && t->size_ordered != 1)
+        return set_err(err, err_len, "size_ordered must be 0 or 1");
     std::lock_guard<std::mutex> lk(g_mu);
-    g_tune.block = t->block_threads;
-    g_tune.lds = t->lds_bytes;
-    g_tune.blocks_per_cu = t->blocks_per_cu;
     g_tune.hist_max = t->hist_max_bins;
-    HIPCHK(set_gene_kernel_lds_limit(g_tune.lds));
+    g_tune.size_ordered = t->size_ordered;
     return 0;
 }
 
@@ -486,7 +526,9 @@ int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
     JobState js;
     if (job_setup(js, &m, group_id, n_targets, opts, st.s, err, err_len)) return 1;
     const JobDesc J = make_desc(js, &m, tu);
-    HIPCHK(launch_gene_plain(J, m.f64, tu.block, tu.lds, st.s));
+    if (job_upload_desc(js, J, m, false, st.s, err, err_len)) return 1;
+    HIPCHK(launch_mwu_pipeline(js.d_job.as<JobDesc>(), js.d_items.as<WorkItem>(), (int)m.C, J,
+                               m.f64, st.s));
     if (job_errors(js, st.s, err, err_len)) return 1;
     if (opts->method == 1) {
         if (opts->zero_handling == 0) {
@@ -518,8 +560,8 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     if (n_groups == 0 || n_cols == 0) return 0;
     if (ensure_init(err, err_len)) return 1;
     const Tuning tu = tuning();
-    if (16 * n_groups > (size_t)tu.lds)
-        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators");
+    if (16 * n_groups > (size_t)kLdsBudget)
+        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
@@ -539,7 +581,7 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     HIPCHK(launch_group_mean(m.vals.p, m.f64, m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
                              gid.as<int32_t>(), dg.as<uint32_t>(), (int64_t)n_cols,
                              (int32_t)n_rows, (int32_t)n_groups, include_zeros, dout.as<double>(),
-                             nullptr, tu.block, tu.lds, st.s));
+                             st.s));
     HIPCHK(hipMemcpyAsync(out, dout.p, n_cols * n_groups * 8, hipMemcpyDeviceToHost, st.s));
     HIPCHK(hipStreamSynchronize(st.s));
     return 0;
@@ -581,12 +623,8 @@ struct hpdex_plan {
     Stream st;
     JobState js;
     JobDesc J{};
-    DevBuf d_job, d_items, d_ctl;   // ctl: head, done
-    int n_items = 0;
-    int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool tables_built = false;
     ~hpdex_plan() {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
@@ -595,19 +633,13 @@ struct hpdex_plan {
 
 namespace {
 
-// The per-job launch sequence (everything after upload): the per-gene
-// kernel, then the exact p lookup when asked for (tables of min/max/mix
-// depend only on group sizes and are built once per plan).
+// The per-job launch sequence (everything after upload): split, reference
+// tables, targets and pairs, then the exact p lookup when asked for (tables
+// of min/max/mix depend only on group sizes and are built once per plan).
 int plan_enqueue(hpdex_plan* p, char* err, size_t err_len) {
     const hipStream_t s = p->st.s;
-    if (p->form == 1) {
-        HIPCHK(launch_gene_persistent(p->d_job.as<JobDesc>(), p->d_items.as<WorkItem>(),
-                                      p->n_items, p->d_ctl.as<unsigned>(),
-                                      p->d_ctl.as<unsigned>() + 1, p->m->f64, p->grid,
-                                      p->tu.block, p->tu.lds, s));
-    } else {
-        HIPCHK(launch_gene_plain(p->J, p->m->f64, p->tu.block, p->tu.lds, s));
-    }
+    HIPCHK(launch_mwu_pipeline(p->js.d_job.as<JobDesc>(), p->js.d_items.as<WorkItem>(),
+                               (int)p->m->C, p->J, p->m->f64, s));
     if (p->js.opts.method == 1 && p->js.opts.zero_handling != 0)
         if (exact_launch(p->js, s, false, err, err_len)) return 1;
     return 0;
@@ -633,20 +665,8 @@ int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets
     if (p->st.create() != hipSuccess) return bail(set_err(err, err_len, "stream creation failed"));
     if (job_setup(p->js, m, group_id, n_targets, opts, p->st.s, err, err_len)) return bail(1);
     p->J = make_desc(p->js, m, p->tu);
-    if (p->d_job.alloc(sizeof(JobDesc)) != hipSuccess) return bail(set_err(err, err_len, "alloc failed"));
-    hipMemcpyAsync(p->d_job.p, &p->J, sizeof(JobDesc), hipMemcpyHostToDevice, p->st.s);
-    std::vector<WorkItem> items(m->C);
-    for (size_t k = 0; k < m->C; ++k) items[k] = WorkItem{0, m->order[k]};
-    p->n_items = (int)m->C;
-    if (p->d_items.alloc(items.size() * sizeof(WorkItem)) != hipSuccess)
-        return bail(set_err(err, err_len, "alloc failed"));
-    if (!items.empty())
-        hipMemcpyAsync(p->d_items.p, items.data(), items.size() * sizeof(WorkItem),
-                       hipMemcpyHostToDevice, p->st.s);
-    unsigned ctl[2] = {0u, 0u};
-    if (p->d_ctl.alloc(sizeof ctl) != hipSuccess) return bail(set_err(err, err_len, "alloc failed"));
-    hipMemcpyAsync(p->d_ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, p->st.s);
-    p->grid = std::max(1, std::min((int)m->C, g_cus * p->tu.blocks_per_cu));
+    const bool ordered = form == 1 && p->tu.size_ordered;
+    if (job_upload_desc(p->js, p->J, *m, ordered, p->st.s, err, err_len)) return bail(1);
     if (opts->method == 1 && opts->zero_handling != 0) {
         if (exact_plan(p->js, nullptr, nullptr, p->st.s, err, err_len)) return bail(1);
         const hipError_t e = launch_exact_tables(p->js.d_tabs.as<ExactTable>(),

@@ -55,7 +55,19 @@ template <> struct KeyOf<double> {
 // ---------------------------------------------------------------------------
 // Wave primitives (wave64).
 // ---------------------------------------------------------------------------
-__device__ inline int lane_id() { return __lane_id(); }
+// Lane and thread indices pass through an empty volatile asm so that values
+// derived from them are recomputed where used instead of being hoisted out of
+// the persistent kernel's work loop (hoisting them pins ~80 extra VGPRs).
+__device__ inline int lane_id() {
+    int l = __lane_id();
+    asm volatile("" : "+v"(l));
+    return l;
+}
+__device__ inline int tid_x() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
 
 __device__ inline uint32_t shfl_xor_u32(uint32_t v, int m) {
     return (uint32_t)__shfl_xor((int)v, m, kWave);
@@ -199,7 +211,7 @@ __device__ inline void wave_merge_tail(Key (&v)[K], int k, int base) {
 // every thread.  scratch: >= 17 uint32 of LDS.  Must be called by all threads.
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t block_excl_scan(uint32_t* a, int n, uint32_t* scratch) {
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = tid_x(), nt = blockDim.x;
     const int chunk = (n + nt - 1) / nt;
     const int b = tid * chunk;
     const int e = min(b + chunk, n);
@@ -225,7 +237,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t* a, int n, uint32_t* scratch
 
 // Workgroup sum of a u64 value (all threads receive it).  scratch: >= 16 u64.
 __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* scratch) {
-    const int tid = threadIdx.x, nw = blockDim.x / kWave;
+    const int tid = tid_x(), nw = blockDim.x / kWave;
     v = wave_sum_u64(v);
     if (lane_id() == 0) scratch[tid / kWave] = v;
     __syncthreads();

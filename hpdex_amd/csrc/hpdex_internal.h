@@ -12,24 +12,53 @@ enum ZeroMode : int32_t { kZeroNone = 0, kZeroMin = 1, kZeroMax = 2, kZeroMix = 
 
 // Status bits a kernel raises when a column does not fit its on-chip plan.
 enum : uint32_t {
-    kStatusCapacity = 1u,   // a slice exceeded the workgroup's LDS budget
+    kStatusCapacity = 1u,   // a reference slice exceeded the workgroup's LDS budget
     kStatusGroups = 2u,     // too many groups for the LDS counters
 };
 
+// Launch geometry shared by the per-column kernels.
+constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
+constexpr int kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup, at most
+constexpr int kHistMaxBins = 8192;          // histogram path: widest integer range
+constexpr int kMaxBuckets = 4096;           // sort path: bucket index entries (pow2)
+constexpr int kRefKeysF32 = 8192;           // sort path: reference keys kept on chip (4 B keys)
+constexpr int kRefKeysF64 = 4096;           // sort path: reference keys kept on chip (8 B keys)
+constexpr int kTabStride = (kHistMaxBins > kMaxBuckets ? kHistMaxBins : kMaxBuckets) + 1;
+
+// Per-column plan: written by the split kernel (and TA by the reference-table
+// kernel), read by the later kernels of the same job.
+struct ColPlan {
+    uint64_t kmin, kmax;   // reference key range (sort path)
+    uint64_t TA;           // reference-only tie sum over nonzero values
+    uint32_t na;           // reference nonzero values
+    uint32_t refneg;       // reference nonzero values below zero
+    int32_t vmin;          // histogram path: bin = value - vmin
+    uint32_t hist;         // 1 = histogram path, 0 = sort path
+    uint32_t nb;           // histogram bins, or reference buckets (sort path)
+    uint32_t shift;        // sort path: bucket = (key - kmin) >> shift
+};
+
 // One Mann-Whitney job: a device-resident CSC matrix, its group labels and
-// options, and where its per-pair results go.  Lives in device memory for the
-// persistent kernel (one entry per job in flight) or is passed by value.
+// options, the per-job workspace of the kernel pipeline, and its outputs.
 struct JobDesc {
     const void* vals;            // float* (f32 and small-int inputs) or double*
     const int32_t* rows;         // nnz row indices, -1 = out of range
     const int64_t* indptr;       // C+1
     const int32_t* gid;          // R group labels
     const uint32_t* group_rows;  // G rows per group
-    double* U1;                  // C*T
-    double* P;                   // C*T (written when fuse_p)
-    double* tie;                 // C*T or null
-    double* n1o;                 // C*T or null
-    double* n2o;                 // C*T or null
+    // workspace
+    void* keys;                  // nnz keys (uint32 for float values, uint64 for double)
+    uint32_t* cnt;               // C*G stored nonzero finite values per (column, group)
+    uint32_t* zcnt;              // C*G stored explicit zeros per (column, group)
+    uint32_t* goff;              // C*G start of each group's keys, from indptr[c]
+    ColPlan* plan;               // C
+    uint32_t* reftab;            // C*kTabStride reference tables
+    // outputs (gene-major, c*T + g-1)
+    double* U1;
+    double* P;
+    double* tie;
+    double* n1o;
+    double* n2o;
     unsigned long long* err_key; // min over (column, rank) of recorded errors
     uint32_t* status;            // kStatus* bits
     unsigned long long* status_col;  // lowest column that raised a status bit
@@ -40,7 +69,7 @@ struct JobDesc {
     int32_t mode;                // ZeroMode
     int32_t alt;                 // 0 less, 1 greater, 2 two-sided
     int32_t tie_corr;
-    int32_t fuse_p;              // asymptotic p computed in the epilogue
+    int32_t asym;                // 1: asymptotic p from the pair kernel
     int32_t hist_max;            // largest integer range binned (0 = off)
     double cc;                   // continuity correction, 0.5 or 0
 };
@@ -77,10 +106,11 @@ hipError_t launch_rows_i64_to_i32(const int64_t* in, int32_t* out, size_t n, int
 hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n, hipStream_t s);
 bool dtype_uses_double(int dtype);
 
-hipError_t launch_gene_plain(const JobDesc& J, bool f64, int block, int lds, hipStream_t s);
-hipError_t launch_gene_persistent(const JobDesc* jobs, const WorkItem* items, int n_items,
-                                  unsigned* head, unsigned* done, bool f64, int grid, int block,
-                                  int lds, hipStream_t s);
+// The per-job pipeline over the columns listed in `items` (in that order):
+// split -> reference tables -> targets -> per-pair sizes and asymptotic p.
+// J0 supplies the sizes of the pair pass (one job per launch sequence).
+hipError_t launch_mwu_pipeline(const JobDesc* jobs, const WorkItem* items, int n_items,
+                               const JobDesc& J0, bool f64, hipStream_t s);
 hipError_t launch_init_job_state(unsigned long long* err_key, uint32_t* status,
                                  unsigned long long* status_col, hipStream_t s);
 hipError_t launch_exact_tables(const ExactTable* tabs, int n_tabs, double* tabbuf,
@@ -89,8 +119,7 @@ hipError_t launch_exact_p(const ExactJob& E, hipStream_t s);
 hipError_t launch_group_mean(const void* vals, bool f64, const int32_t* rows,
                              const int64_t* indptr, const int32_t* gid,
                              const uint32_t* group_rows, int64_t C, int32_t R, int32_t G,
-                             int include_zeros, double* out, uint32_t* status, int block,
-                             int lds, hipStream_t s);
-hipError_t set_gene_kernel_lds_limit(int lds);
+                             int include_zeros, double* out, hipStream_t s);
+hipError_t launch_stream_copy(const void* in, void* out, size_t bytes, int grid, hipStream_t s);
 
 }  // namespace hpdex

@@ -1,28 +1,40 @@
 // hpdex_kernels.hip -- gfx950 kernels of the Mann-Whitney U backend.
 //
-// The per-gene kernel fuses what the reference does per column
-// (src/cpp/mannwhitneyu.cpp:903-1043): group split, sort of the reference
-// slice once per gene, U and tie term of every target against it, and the
-// asymptotic p-value (:191-240).  It never materialises a merge: for each
-// target value v it needs only lt(v) = #reference values below v and
-// eq(v) = #reference values equal to v, so
+// What the reference does per gene column (src/cpp/mannwhitneyu.cpp:903-1043)
+// runs here as a pipeline of small kernels over the columns of a job:
+//
+//   split      count stored values per group, exclusive scan, scatter each
+//              group's nonzero values (as order-preserving keys, or as
+//              histogram bins for small-integer columns) into a per-column
+//              key range in HBM                       (north-star subsystem a)
+//   reference  sort the reference slice once per gene (register bitonic
+//              runs + LDS merges) and build its rank table -- a bucket index
+//              over the sorted keys, or the histogram prefix for integer
+//              counts -- kept in HBM for every target        (b, d)
+//   target     load the column's rank table into LDS; each wave sorts one
+//              target slice in registers and ranks it against the table (c)
+//   pair       sample sizes and the FP64 asymptotic p-value per pair    (e)
+//
+// Ranking never materialises a merge: a target value v needs only
+// lt(v) = #reference values below v and eq(v) = #reference values equal to v:
 //
 //   2*U_target = sum_b (2*lt(b) + eq(b))            over all target values b
-//   U1         = n1*n2 - U_target                    (reference-group U)
+//   U1         = n1*n2 - U_target                    (reference-group U, :960)
 //   tie        = sum over distinct values of t^3 - t, built per element as
 //                3*x*(x-1) with x = eq(b) + j(b), j(b) = 1-based position of b
 //                inside its run of equal target values, plus the reference's
-//                own runs and the zero block.
+//                own runs and the zero block
 //
-// since f(a+b) - f(a) = sum_{j=1..b} 3(a+j)(a+j-1) for f(t) = t^3 - t.  All of
-// it is exact integer arithmetic (U held as 2U in 64 bits, ties in 64 bits),
-// which is what makes U1 and the tie term bit-identical to the reference's
-// midrank merge (:618-840).
+// since f(a+b) - f(a) = sum_{j=1..b} 3(a+j)(a+j-1) for f(t) = t^3 - t.  It is
+// exact integer arithmetic (2U and the tie term in 64 bits), which makes U1 and
+// the tie term bit-identical to the reference's midrank merge (:618-840).
 //
 // Zero handling (:935-1039): zeros are never stored.  Explicit zeros, and in
-// min/max/mix also absent and non-finite entries, form one block whose
-// position is below every value (min), above every value (max) or at 0
-// (mix, and none -- where only explicit zeros count).
+// min/max/mix also absent and non-finite entries, form one block placed below
+// every value (min), above every value (max) or at 0 (mix, and none -- where
+// only explicit zeros count).
+//
+// Every per-column kernel uses 256-thread workgroups and at most 48 KB of LDS.
 #include <climits>
 
 #include "hpdex_device.h"
@@ -30,31 +42,38 @@
 
 namespace hpdex {
 
-constexpr int kChunk = 1024;          // largest target run sorted in registers
-constexpr int kMaxBuckets = 16384;    // bucket index over the reference keys
-
-struct ColState {
-    uint64_t kmin, kmax;   // reference key range (sort path)
-    uint64_t TA;           // reference-only tie sum over nonzero values
-    uint32_t all_int;      // every in-group nonzero value is a small integer
-    int32_t vmin, vmax;    // their range
-    uint32_t refneg;       // reference nonzero values below zero
-    uint32_t refcur;
-    uint32_t next_t;
-    uint32_t hist;         // 1 = histogram path, 0 = sort path
-    uint32_t nb;           // histogram bins or reference buckets
-    uint32_t shift;        // bucket = (key - kmin) >> shift
-    uint32_t P;            // padded reference length (sort path)
-    uint32_t tgt_cap;      // target keys that fit the LDS region
-    uint32_t glo, ghi, off0;
-    uint32_t fail;
-    uint32_t ttotal;       // reused as the "a target is too large" flag
-    int32_t item;
-    uint32_t scan[20];
-    uint64_t red[16];
-};
+constexpr int kChunkK = 8;            // keys per lane in a register sort
+constexpr int kChunk = 64 * kChunkK;  // longest run sorted in registers
 
 __device__ inline uint64_t tie_f(uint64_t t) { return t * t * t - t; }
+
+// Visit the in-group entries of CSC range [p0, p1).  Each thread keeps
+// kUnroll row/value loads and kUnroll group_id gathers in flight so the
+// stream is not bound by one load latency per entry.
+constexpr int kUnroll = 8;
+template <typename V, typename F>
+__device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
+                                               const V* __restrict__ vals,
+                                               const int32_t* __restrict__ gid, uint32_t R,
+                                               uint32_t G, int64_t p0, int64_t p1, F&& f) {
+    const int tid = tid_x(), nt = blockDim.x;
+    for (int64_t pb = p0; pb < p1; pb += (int64_t)nt * kUnroll) {
+        uint32_t r[kUnroll];
+        V v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int64_t p = pb + (int64_t)u * nt + tid;
+            if (p < p1) { r[u] = (uint32_t)rows[p]; v[u] = vals[p]; }
+            else { r[u] = 0xFFFFFFFFu; v[u] = V(0); }
+        }
+        uint32_t g[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) g[u] = r[u] < R ? (uint32_t)gid[r[u]] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            if (g[u] < G) f(g[u], v[u]);
+    }
+}
 
 __device__ inline void raise_status(const JobDesc& J, uint32_t bit, int64_t c) {
     atomicOr(J.status, bit);
@@ -86,7 +105,7 @@ __device__ inline int wave_max_i(int v) {
 
 // first index in a[lo, hi) with a[i] >= v
 template <typename K>
-__device__ inline uint32_t lower_bound_lds(const K* a, uint32_t lo, uint32_t hi, K v) {
+__device__ inline uint32_t lower_bound_k(const K* a, uint32_t lo, uint32_t hi, K v) {
     uint32_t n = hi - lo;
     while (n > 0) {
         const uint32_t h = n >> 1;
@@ -96,7 +115,7 @@ __device__ inline uint32_t lower_bound_lds(const K* a, uint32_t lo, uint32_t hi,
 }
 // first index in a[lo, hi) with a[i] > v
 template <typename K>
-__device__ inline uint32_t upper_bound_lds(const K* a, uint32_t lo, uint32_t hi, K v) {
+__device__ inline uint32_t upper_bound_k(const K* a, uint32_t lo, uint32_t hi, K v) {
     uint32_t n = hi - lo;
     while (n > 0) {
         const uint32_t h = n >> 1;
@@ -105,7 +124,172 @@ __device__ inline uint32_t upper_bound_lds(const K* a, uint32_t lo, uint32_t hi,
     return lo;
 }
 
-// Reference table seen by the target phase.
+// The work item of this workgroup, uniform across it (kept in SGPRs).
+__device__ inline void work_item(const WorkItem* items, int& job, int64_t& col) {
+    job = __builtin_amdgcn_readfirstlane(items[blockIdx.x].job);
+    col = (int64_t)__builtin_amdgcn_readfirstlane(items[blockIdx.x].col);
+}
+
+// Zero-block sizes and sample sizes of pair (reference, g) in column c
+// (mannwhitneyu.cpp:936-1037): none counts stored finite values only; the
+// other modes count every group row, absent and non-finite entries as zeros.
+__device__ inline void pair_sizes(const JobDesc& J, int64_t c, uint32_t g, uint32_t& za,
+                                  uint32_t& zb, uint32_t& n1, uint32_t& n2) {
+    const int64_t b = c * J.G;
+    const uint32_t nza = J.cnt[b], nzb = J.cnt[b + g];
+    if (J.mode == kZeroNone) {
+        za = J.zcnt[b]; zb = J.zcnt[b + g];
+        n1 = nza + za; n2 = nzb + zb;
+    } else {
+        const uint32_t gr0 = J.group_rows[0], grg = J.group_rows[g];
+        za = gr0 > nza ? gr0 - nza : 0u;
+        zb = grg > nzb ? grg - nzb : 0u;
+        n1 = gr0; n2 = grg;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// split: per-group counts, value statistics, errors, offsets and the scatter
+// of nonzero values into the column's key range (mannwhitneyu.cpp:913-926).
+// ---------------------------------------------------------------------------
+struct SplitState {
+    uint64_t kmin, kmax;
+    uint32_t all_int;
+    int32_t vmin, vmax;
+    uint32_t refneg;
+    uint32_t hist;
+    uint32_t scan[20];
+};
+
+template <typename V>
+__global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs,
+                                                          const WorkItem* items) {
+    using KT = KeyOf<V>;
+    using K = typename KT::K;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ SplitState st;
+    int job;
+    int64_t c;
+    work_item(items, job, c);
+    const JobDesc J = jobs[job];
+    const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
+    const int G = J.G;
+    const uint32_t R = (uint32_t)J.R;
+    const V* vals = (const V*)J.vals;
+    const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
+    if (12 * G > kLdsBudget) {
+        if (tid == 0) raise_status(J, kStatusGroups, c);
+        return;
+    }
+    uint32_t* cnt_nz = (uint32_t*)smem;
+    uint32_t* cnt_ez = cnt_nz + G;
+    uint32_t* cur = cnt_ez + G;
+
+    for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; }
+    if (tid == 0) {
+        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
+        st.kmin = ~0ull; st.kmax = 0;
+    }
+    __syncthreads();
+
+    // counts and value statistics
+    {
+        bool li = true;
+        int lmin = INT_MAX, lmax = INT_MIN;
+        K kmn = KT::kSentinel, kmx = 0;
+        uint32_t neg = 0;
+        for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, [&](uint32_t g, V v) {
+            if (!KT::finite(v)) return;
+            if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
+            atomicAdd(&cnt_nz[g], 1u);
+            int iv;
+            if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+            else li = false;
+            if (g == 0) {
+                const K k = KT::key(v);
+                kmn = k < kmn ? k : kmn;
+                kmx = k > kmx ? k : kmx;
+                neg += (v < V(0)) ? 1u : 0u;
+            }
+        });
+        const bool wall = __all(li ? 1 : 0);
+        lmin = wave_min_i(lmin);
+        lmax = wave_max_i(lmax);
+        kmn = wave_min_k<K>(kmn);
+        kmx = wave_max_k<K>(kmx);
+        neg = wave_sum_u32(neg);
+        if (lane == 0) {
+            if (!wall) atomicAnd(&st.all_int, 0u);
+            atomicMin(&st.vmin, lmin);
+            atomicMax(&st.vmax, lmax);
+            atomicAdd(&st.refneg, neg);
+            atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
+            atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
+        }
+    }
+    __syncthreads();
+
+    // errors of this column (mannwhitneyu.cpp:936-1038), counts out
+    {
+        const uint64_t ebase = (uint64_t)c * (uint64_t)(2 * G + 1);
+        const int64_t b = c * G;
+        for (int g = tid; g < G; g += nt) {
+            const uint32_t nz = cnt_nz[g], ez = cnt_ez[g], gr = J.group_rows[g];
+            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
+            const uint32_t n = (J.mode == kZeroNone) ? nz + ez : gr;
+            if (n < 2) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
+            J.cnt[b + g] = nz;
+            J.zcnt[b + g] = ez;
+            cur[g] = nz;
+        }
+    }
+    __syncthreads();
+    block_excl_scan(cur, G, st.scan);
+    for (int g = tid; g < G; g += nt) J.goff[c * G + g] = cur[g];
+
+    // plan of the column
+    if (tid == 0) {
+        ColPlan pl;
+        pl.na = cnt_nz[0];
+        pl.refneg = st.refneg;
+        pl.kmin = st.kmin;
+        pl.kmax = st.kmax;
+        pl.TA = 0;
+        const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
+        if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
+            pl.hist = 1;
+            pl.nb = (uint32_t)range;
+            pl.vmin = (st.vmax >= st.vmin) ? st.vmin : 0;
+            pl.shift = 0;
+        } else {
+            pl.hist = 0;
+            uint32_t nbk = 64;
+            while (nbk < pl.na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
+            const uint64_t rng = (pl.na > 0) ? pl.kmax - pl.kmin : 0ull;
+            const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
+            const uint32_t lg = 31u - (uint32_t)__clz(nbk);
+            pl.nb = nbk;
+            pl.shift = bits > lg ? bits - lg : 0u;
+            pl.vmin = 0;
+        }
+        J.plan[c] = pl;
+        st.hist = pl.hist;
+        st.vmin = pl.vmin;
+    }
+    __syncthreads();
+
+    // scatter nonzero values into the groups' key ranges
+    const uint32_t hist = st.hist;
+    const int32_t vmin = st.vmin;
+    K* keys = (K*)J.keys + p0;
+    for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, [&](uint32_t g, V v) {
+        if (!KT::finite(v) || v == V(0)) return;
+        const uint32_t pos = atomicAdd(&cur[g], 1u);
+        keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
+    });
+}
+
+// Rank table of the reference slice seen by the target phase.
 template <typename K>
 struct RefView {
     const uint32_t* cum;   // histogram path: exclusive prefix of bin counts
@@ -124,12 +308,141 @@ struct RefView {
         if ((uint64_t)v > kmax) { lt = na; eq = 0; return; }
         const uint32_t bkt = (uint32_t)(((uint64_t)v - kmin) >> shift);
         const uint32_t lo = bidx[bkt], hi = bidx[bkt + 1];
-        const uint32_t l = lower_bound_lds<K>(keys, lo, hi, v);
-        const uint32_t u = upper_bound_lds<K>(keys, l, hi, v);
+        const uint32_t l = lower_bound_k<K>(keys, lo, hi, v);
+        const uint32_t u = upper_bound_k<K>(keys, l, hi, v);
         lt = l; eq = u - l;
     }
 };
 
+// Sort reference keys rk[0..P) (P a power of two >= 64, padded with the
+// sentinel) in LDS: registers for runs of up to kChunk, LDS merge stages above.
+template <typename K>
+__device__ __forceinline__ void sort_ref(K* rk, uint32_t P) {
+    const int tid = tid_x(), nt = blockDim.x;
+    const int wid = tid / kWave, nw = nt / kWave, lane = lane_id();
+    if (P <= (uint32_t)kChunk) {
+        if (wid == 0) {
+            const int Kk = (int)(P / kWave);
+#define HPDEX_SORT_SMALL(KK)                                                      \
+    {                                                                              \
+        K v[KK];                                                                   \
+        for (int i = 0; i < KK; ++i) v[i] = rk[i * kWave + lane];                  \
+        wave_sort<KK, K>(v);                                                       \
+        for (int i = 0; i < KK; ++i) rk[i * kWave + lane] = v[i];                  \
+    }
+            if (Kk == 1) HPDEX_SORT_SMALL(1)
+            else if (Kk == 2) HPDEX_SORT_SMALL(2)
+            else if (Kk == 4) HPDEX_SORT_SMALL(4)
+            else HPDEX_SORT_SMALL(8)
+#undef HPDEX_SORT_SMALL
+        }
+        __syncthreads();
+        return;
+    }
+    const uint32_t nchunk = P / kChunk;
+    for (uint32_t ch = wid; ch < nchunk; ch += nw) {
+        K v[kChunkK];
+        K* b = rk + (size_t)ch * kChunk;
+        for (int i = 0; i < kChunkK; ++i) v[i] = b[i * kWave + lane];
+        wave_sort<kChunkK, K>(v, (int)(ch * kChunk));
+        for (int i = 0; i < kChunkK; ++i) b[i * kWave + lane] = v[i];
+    }
+    __syncthreads();
+    for (uint32_t k = 2 * kChunk; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j >= (uint32_t)kChunk; j >>= 1) {
+            for (uint32_t x = tid; x < P / 2; x += nt) {
+                const uint32_t lo = (x / j) * 2 * j + (x % j);
+                const uint32_t hi = lo + j;
+                const bool asc = (lo & k) == 0;
+                const K a = rk[lo], b = rk[hi];
+                if (asc ? (b < a) : (a < b)) { rk[lo] = b; rk[hi] = a; }
+            }
+            __syncthreads();
+        }
+        for (uint32_t ch = wid; ch < nchunk; ch += nw) {
+            K v[kChunkK];
+            K* b = rk + (size_t)ch * kChunk;
+            for (int i = 0; i < kChunkK; ++i) v[i] = b[i * kWave + lane];
+            wave_merge_tail<kChunkK, K>(v, (int)k, (int)(ch * kChunk));
+            for (int i = 0; i < kChunkK; ++i) b[i * kWave + lane] = v[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reference: sort the reference slice once per gene and build its rank table
+// (histogram prefix or bucket index) plus its own tie sum.
+// ---------------------------------------------------------------------------
+struct RefState {
+    uint32_t scan[20];
+    uint64_t red[16];
+};
+
+template <typename V>
+__global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs,
+                                                        const WorkItem* items) {
+    using KT = KeyOf<V>;
+    using K = typename KT::K;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ RefState st;
+    int job;
+    int64_t c;
+    work_item(items, job, c);
+    const JobDesc J = jobs[job];
+    const int tid = tid_x(), nt = blockDim.x;
+    const ColPlan pl = J.plan[c];
+    K* base = (K*)J.keys + J.indptr[c];   // group 0's keys start the column
+    uint32_t* tab = J.reftab + (size_t)c * kTabStride;
+    // uniform decisions (kept in SGPRs so no barrier below sits under an exec mask)
+    const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
+    const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
+    const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
+    uint64_t ta = 0;
+    if (!hist && na > cap) {
+        if (tid == 0) raise_status(J, kStatusCapacity, c);
+    } else if (hist) {
+        uint32_t* cum = (uint32_t*)smem;
+        for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = 0u;
+        __syncthreads();
+        for (uint32_t x = tid; x < na; x += nt) atomicAdd(&cum[(uint32_t)base[x]], 1u);
+        __syncthreads();
+        for (uint32_t b = tid; b < pl.nb; b += nt) ta += tie_f(cum[b]);
+        ta = block_sum_u64(ta, st.red);
+        block_excl_scan(cum, (int)pl.nb + 1, st.scan);
+        for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = cum[b];
+    } else {
+        uint32_t P = 64;
+        while (P < na) P <<= 1;
+        K* rk = (K*)smem;
+        uint32_t* bidx = (uint32_t*)(rk + P);
+        for (uint32_t x = tid; x < P; x += nt) rk[x] = x < na ? base[x] : KT::kSentinel;
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
+        __syncthreads();
+        sort_ref<K>(rk, P);
+        for (uint32_t x = tid; x < na; x += nt) {
+            base[x] = rk[x];
+            atomicAdd(&bidx[(uint32_t)(((uint64_t)rk[x] - pl.kmin) >> pl.shift)], 1u);
+        }
+        __syncthreads();
+        block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
+        for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
+        RefView<K> rv;
+        rv.cum = nullptr; rv.keys = rk; rv.bidx = bidx; rv.kmin = pl.kmin; rv.kmax = pl.kmax;
+        rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        for (uint32_t x = tid; x < na; x += nt) {
+            uint32_t lt, eq;
+            rv.lookup(rk[x], lt, eq);
+            if (lt == x) ta += tie_f(eq);
+        }
+        ta = block_sum_u64(ta, st.red);
+    }
+    if (tid == 0 && (hist || na <= cap)) J.plan[c].TA = ta;
+}
+
+// ---------------------------------------------------------------------------
+// target: every target of a column against its rank table
+// ---------------------------------------------------------------------------
 template <typename K>
 struct TargetCtx {
     RefView<K> ref;
@@ -142,7 +455,8 @@ struct TargetCtx {
 
 // Is the zero block strictly below value key v?
 template <typename V>
-__device__ inline bool zero_below(const TargetCtx<typename KeyOf<V>::K>& t, typename KeyOf<V>::K v) {
+__device__ inline bool zero_below(const TargetCtx<typename KeyOf<V>::K>& t,
+                                  typename KeyOf<V>::K v) {
     if (t.mode == kZeroMin) return true;
     if (t.mode == kZeroMax) return false;
     if (t.ref.hist) return (int32_t)v + t.vmin > 0;
@@ -150,12 +464,13 @@ __device__ inline bool zero_below(const TargetCtx<typename KeyOf<V>::K>& t, type
 }
 
 // Sort one run of up to 64*KK target keys in registers and accumulate its
-// contribution.  seg: the target's keys in LDS; q: chunk number; earlier
-// chunks (already sorted in place) are searched for equal keys so that j
-// counts position inside the whole run of equal values.
+// contribution.  seg: the target's keys in HBM; q: chunk number; earlier
+// chunks (already sorted and written back) are searched for equal keys so
+// that j counts the position inside the whole run of equal values.
 template <typename V, int KK>
-__device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t, typename KeyOf<V>::K* seg,
-                             uint32_t m, int q, bool store) {
+__device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
+                                             typename KeyOf<V>::K* seg, uint32_t m, int q,
+                                             bool store) {
     using K = typename KeyOf<V>::K;
     const int lane = lane_id();
     K* base = seg + (size_t)q * kChunk;
@@ -172,7 +487,7 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
             const uint32_t e = (uint32_t)(i * kWave + lane);
             if (e < m) base[e] = v[i];
         }
-        __threadfence_block();
+        __threadfence();
     }
     uint32_t carry = 0;
 #pragma unroll
@@ -195,8 +510,8 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
             uint32_t prior = 0;
             for (int qq = 0; qq < q; ++qq) {
                 const K* pc = seg + (size_t)qq * kChunk;
-                const uint32_t l = lower_bound_lds<K>(pc, 0, kChunk, v[i]);
-                const uint32_t u = upper_bound_lds<K>(pc, l, kChunk, v[i]);
+                const uint32_t l = lower_bound_k<K>(pc, 0, kChunk, v[i]);
+                const uint32_t u = upper_bound_k<K>(pc, l, kChunk, v[i]);
                 prior += u - l;
             }
             const uint64_t jj = (uint64_t)(e - h + 1) + prior;
@@ -209,396 +524,111 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
 }
 
 template <typename V>
-__device__ __forceinline__ void process_target(const JobDesc& J, int64_t c, uint32_t g, ColState& st,
-                               const uint32_t* cnt_nz, const uint32_t* cnt_ez, const uint32_t* seg,
-                               typename KeyOf<V>::K* tgt, const RefView<typename KeyOf<V>::K>& ref) {
+__device__ __forceinline__ void process_target(const JobDesc& J, int64_t c, uint32_t g,
+                                               const ColPlan& pl,
+                                               const RefView<typename KeyOf<V>::K>& ref) {
     using K = typename KeyOf<V>::K;
     const int lane = lane_id();
-    const uint32_t nzb = cnt_nz[g];
-    const uint32_t start = seg[g] - nzb - st.off0;
-    const uint32_t nza = cnt_nz[0];
-    const uint32_t gr0 = J.group_rows[0], grg = J.group_rows[g];
+    const int64_t b = c * J.G;
+    const uint32_t nzb = J.cnt[b + g];
+    K* seg = (K*)J.keys + J.indptr[c] + J.goff[b + g];
     uint32_t za, zb, n1, n2;
-    if (J.mode == kZeroNone) {
-        za = cnt_ez[0]; zb = cnt_ez[g];
-        n1 = nza + za; n2 = nzb + zb;
-    } else {
-        za = gr0 > nza ? gr0 - nza : 0u;
-        zb = grg > nzb ? grg - nzb : 0u;
-        n1 = gr0; n2 = grg;
-    }
+    pair_sizes(J, c, g, za, zb, n1, n2);
     TargetCtx<K> t;
     t.ref = ref;
     t.mode = J.mode;
-    t.vmin = st.vmin;
+    t.vmin = pl.vmin;
     t.za = za;
     t.S = 0;
     t.TT = 0;
-    K* sp = tgt + start;
     const uint32_t nch = (nzb + kChunk - 1) / kChunk;
     for (uint32_t q = 0; q < nch; ++q) {
         const uint32_t m = min((uint32_t)kChunk, nzb - q * kChunk);
         const bool store = nch > 1;
-        if (m <= 64) target_chunk<V, 1>(t, sp, m, q, store);
-        else if (m <= 128) target_chunk<V, 2>(t, sp, m, q, store);
-        else if (m <= 256) target_chunk<V, 4>(t, sp, m, q, store);
-        else if (m <= 512) target_chunk<V, 8>(t, sp, m, q, store);
-        else target_chunk<V, 16>(t, sp, m, q, store);
+        if (m <= 64) target_chunk<V, 1>(t, seg, m, q, store);
+        else if (m <= 128) target_chunk<V, 2>(t, seg, m, q, store);
+        else if (m <= 256) target_chunk<V, 4>(t, seg, m, q, store);
+        else target_chunk<V, 8>(t, seg, m, q, store);
     }
     const uint64_t S = wave_sum_u64(t.S);
     const uint64_t TT = wave_sum_u64(t.TT);
     if (lane == 0) {
         uint32_t ltz;
         if (J.mode == kZeroMin) ltz = 0;
-        else if (J.mode == kZeroMax) ltz = nza;
-        else ltz = st.refneg;
+        else if (J.mode == kZeroMax) ltz = pl.na;
+        else ltz = pl.refneg;
         const uint64_t S2 = S + (uint64_t)zb * (2ull * ltz + za);
-        const uint64_t tie = TT + st.TA + tie_f((uint64_t)za + zb);
+        const uint64_t tie = TT + pl.TA + tie_f((uint64_t)za + zb);
         const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
         const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
-        const double U1 = (double)u1x2 * 0.5;
-        const double tied = J.tie_corr ? (double)tie : 0.0;
         const int64_t idx = c * J.T + (int64_t)(g - 1);
-        J.U1[idx] = U1;
-        if (J.tie) J.tie[idx] = tied;
-        if (J.n1o) J.n1o[idx] = (double)n1;
-        if (J.n2o) J.n2o[idx] = (double)n2;
-        if (J.fuse_p) J.P[idx] = p_asymptotic(U1, (double)n1, (double)n2, tied, J.cc, J.alt);
+        J.U1[idx] = (double)u1x2 * 0.5;
+        J.tie[idx] = J.tie_corr ? (double)tie : 0.0;
     }
 }
 
-// Sort reference keys rk[0..P) (P a power of two >= 64, padded with the
-// sentinel) in LDS: registers for runs of up to 1024, LDS merge stages above.
-template <typename K>
-__device__ __forceinline__ void sort_ref(K* rk, uint32_t P) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int wid = tid / kWave, nw = nt / kWave, lane = lane_id();
-    if (P <= (uint32_t)kChunk) {
-        if (wid == 0) {
-            const int Kk = (int)(P / kWave);
-#define HPDEX_SORT_SMALL(KK)                                                      \
-    {                                                                              \
-        K v[KK];                                                                   \
-        for (int i = 0; i < KK; ++i) v[i] = rk[i * kWave + lane];                  \
-        wave_sort<KK, K>(v);                                                       \
-        for (int i = 0; i < KK; ++i) rk[i * kWave + lane] = v[i];                  \
-    }
-            if (Kk == 1) HPDEX_SORT_SMALL(1)
-            else if (Kk == 2) HPDEX_SORT_SMALL(2)
-            else if (Kk == 4) HPDEX_SORT_SMALL(4)
-            else if (Kk == 8) HPDEX_SORT_SMALL(8)
-            else HPDEX_SORT_SMALL(16)
-#undef HPDEX_SORT_SMALL
-        }
-        __syncthreads();
-        return;
-    }
-    const uint32_t nchunk = P / kChunk;
-    for (uint32_t ch = wid; ch < nchunk; ch += nw) {
-        K v[16];
-        K* b = rk + (size_t)ch * kChunk;
-        for (int i = 0; i < 16; ++i) v[i] = b[i * kWave + lane];
-        wave_sort<16, K>(v, (int)(ch * kChunk));
-        for (int i = 0; i < 16; ++i) b[i * kWave + lane] = v[i];
-    }
-    __syncthreads();
-    for (uint32_t k = 2 * kChunk; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j >= (uint32_t)kChunk; j >>= 1) {
-            for (uint32_t x = tid; x < P / 2; x += nt) {
-                const uint32_t lo = (x / j) * 2 * j + (x % j);
-                const uint32_t hi = lo + j;
-                const bool asc = (lo & k) == 0;
-                const K a = rk[lo], b = rk[hi];
-                if (asc ? (b < a) : (a < b)) { rk[lo] = b; rk[hi] = a; }
-            }
-            __syncthreads();
-        }
-        for (uint32_t ch = wid; ch < nchunk; ch += nw) {
-            K v[16];
-            K* b = rk + (size_t)ch * kChunk;
-            for (int i = 0; i < 16; ++i) v[i] = b[i * kWave + lane];
-            wave_merge_tail<16, K>(v, (int)k, (int)(ch * kChunk));
-            for (int i = 0; i < 16; ++i) b[i * kWave + lane] = v[i];
-        }
-        __syncthreads();
-    }
-}
+struct TargetState {
+    uint32_t next_t;
+};
 
 template <typename V>
-__device__ __forceinline__ void process_column(const JobDesc& J, int64_t c, unsigned char* smem, int lds,
-                               ColState& st) {
-    using KT = KeyOf<V>;
-    using K = typename KT::K;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = lane_id();
-    const int G = J.G;
-    const V* vals = (const V*)J.vals;
-    const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
-    const uint32_t R = (uint32_t)J.R;
-
-    uint32_t* cnt_nz = (uint32_t*)smem;
-    uint32_t* cnt_ez = cnt_nz + G;
-    uint32_t* seg = cnt_ez + G;
-    const int hdr = (int)(((size_t)3 * G * 4 + 15) & ~(size_t)15);
-    unsigned char* rb = smem + hdr;
-    const int avail = lds - hdr;
-    if (avail < 8192) {
-        if (tid == 0) raise_status(J, kStatusGroups, c);
-        return;
-    }
-
-    for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; }
-    if (tid == 0) {
-        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0; st.refcur = 0;
-        st.kmin = ~0ull; st.kmax = 0; st.fail = 0; st.TA = 0;
-    }
-    __syncthreads();
-
-    // ---- A: per-group counts and value statistics ----
-    {
-        bool li = true;
-        int lmin = INT_MAX, lmax = INT_MIN;
-        K kmn = KT::kSentinel, kmx = 0;
-        uint32_t neg = 0;
-        for (int64_t p = p0 + tid; p < p1; p += nt) {
-            const uint32_t r = (uint32_t)J.rows[p];
-            if (r >= R) continue;
-            const uint32_t g = (uint32_t)J.gid[r];
-            if (g >= (uint32_t)G) continue;
-            const V v = vals[p];
-            if (!KT::finite(v)) continue;
-            if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); continue; }
-            atomicAdd(&cnt_nz[g], 1u);
-            int iv;
-            if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
-            else li = false;
-            if (g == 0) {
-                const K k = KT::key(v);
-                kmn = k < kmn ? k : kmn;
-                kmx = k > kmx ? k : kmx;
-                neg += (v < V(0)) ? 1u : 0u;
-            }
-        }
-        const bool wall = __all(li ? 1 : 0);
-        lmin = wave_min_i(lmin);
-        lmax = wave_max_i(lmax);
-        kmn = wave_min_k<K>(kmn);
-        kmx = wave_max_k<K>(kmx);
-        neg = wave_sum_u32(neg);
-        if (lane == 0) {
-            if (!wall) atomicAnd(&st.all_int, 0u);
-            atomicMin(&st.vmin, lmin);
-            atomicMax(&st.vmax, lmax);
-            atomicAdd(&st.refneg, neg);
-            atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
-            atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
-        }
-    }
-    __syncthreads();
-
-    // ---- errors of this column (mannwhitneyu.cpp:936-1038) ----
-    {
-        const uint64_t ebase = (uint64_t)c * (uint64_t)(2 * G + 1);
-        for (int g = tid; g < G; g += nt) {
-            const uint32_t nz = cnt_nz[g], ez = cnt_ez[g], gr = J.group_rows[g];
-            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
-            const uint32_t n = (J.mode == kZeroNone) ? nz + ez : gr;
-            if (n < 2) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
-        }
-    }
-
-    // ---- choose the reference plan ----
-    if (tid == 0) {
-        const uint32_t na = cnt_nz[0];
-        const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
-        uint32_t refb;
-        if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
-            st.hist = 1;
-            st.nb = (uint32_t)range;
-            if (st.vmax < st.vmin) st.vmin = 0;
-            refb = 4u * (st.nb + 1);
-        } else {
-            st.hist = 0;
-            uint32_t P = 64;
-            while (P < na) P <<= 1;
-            uint32_t nbk = 64;
-            while (nbk < na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
-            st.P = P;
-            st.nb = nbk;
-            const uint64_t rng = (na > 0) ? st.kmax - st.kmin : 0ull;
-            const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
-            const uint32_t lg = 31u - (uint32_t)__clz(nbk);
-            st.shift = bits > lg ? bits - lg : 0u;
-            refb = (uint32_t)sizeof(K) * P + 4u * (nbk + 1);
-        }
-        refb = (refb + 15u) & ~15u;
-        if ((int64_t)refb + 64 * (int64_t)sizeof(K) > avail) {
-            st.fail = 1;
-        } else {
-            st.tgt_cap = (uint32_t)((avail - (int)refb) / (int)sizeof(K));
-        }
-        st.scan[19] = refb;
-    }
-    __syncthreads();
-    if (st.fail) {
-        if (tid == 0) raise_status(J, kStatusCapacity, c);
-        return;
-    }
-    const uint32_t refb = st.scan[19];
-    K* tgt = (K*)(rb + refb);
-    uint32_t* cum = (uint32_t*)rb;
-    K* rk = (K*)rb;
-    uint32_t* bidx = (uint32_t*)(rb + sizeof(K) * (size_t)st.P);
-    const uint32_t hist = st.hist;
-
-    // target segment starts: exclusive prefix over targets 1..G-1
-    for (int g = tid; g < G; g += nt) seg[g] = (g == 0) ? 0u : cnt_nz[g];
-    __syncthreads();
-    const uint32_t ttotal = block_excl_scan(seg + 1, G - 1, st.scan);
-    (void)ttotal;
-    // any single target too large for the LDS region?  (st.fail was read by
-    // every thread before the scan's barriers; a second flag avoids a race)
-    if (tid == 0) st.ttotal = 0u;
-    __syncthreads();
-    for (int g = 1 + tid; g < G; g += nt)
-        if (cnt_nz[g] > st.tgt_cap) st.ttotal = 1u;
+__global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
+                                                           const WorkItem* items) {
+    using K = typename KeyOf<V>::K;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TargetState st;
+    int job;
+    int64_t c;
+    work_item(items, job, c);
+    const JobDesc J = jobs[job];
+    const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
+    const ColPlan pl = J.plan[c];
+    const uint32_t* tab = J.reftab + (size_t)c * kTabStride;
+    // uniform decisions (kept in SGPRs so the barrier below is not under an exec mask)
+    const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
+    const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
+    const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
+    // a reference too large for the chip: the reference kernel raised the status
+    if (!hist && na > cap) return;
+    RefView<K> rv;
+    rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = hist;
     if (hist) {
-        for (uint32_t b = tid; b <= st.nb; b += nt) cum[b] = 0u;
+        uint32_t* cum = (uint32_t*)smem;
+        for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = tab[b];
+        rv.cum = cum; rv.keys = nullptr; rv.bidx = nullptr;
+    } else {
+        K* rk = (K*)smem;
+        uint32_t* bidx = (uint32_t*)(rk + cap);
+        const K* base = (const K*)J.keys + J.indptr[c];
+        for (uint32_t x = tid; x < na; x += nt) rk[x] = base[x];
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = tab[b];
+        rv.cum = nullptr; rv.keys = rk; rv.bidx = bidx;
     }
+    if (tid == 0) st.next_t = 1u;
     __syncthreads();
-    if (st.ttotal) {
-        if (tid == 0) raise_status(J, kStatusCapacity, c);
-        return;
-    }
-
-    const int32_t vmin = st.vmin;
-    uint32_t glo = 1;
-    bool first = true;
-    while (glo < (uint32_t)G) {
-        // batch [glo, ghi): targets whose keys fit from seg[glo] on
-        if (tid == 0) {
-            const uint32_t off0 = seg[glo];
-            uint32_t lo = glo, hi = (uint32_t)G;   // last g with seg[g]+nz[g]-off0 <= cap
-            // binary search for the first g > glo whose end exceeds the cap
-            uint32_t a = glo + 1, b = (uint32_t)G;
-            while (a < b) {
-                const uint32_t mid = (a + b) >> 1;
-                if (seg[mid] + cnt_nz[mid] - off0 <= st.tgt_cap) a = mid + 1; else b = mid;
-            }
-            hi = a;
-            (void)lo;
-            st.glo = glo; st.ghi = hi; st.off0 = off0;
-            st.next_t = glo;
-        }
-        __syncthreads();
-        const uint32_t ghi = st.ghi, off0 = st.off0;
-
-        // ---- C: scatter keys (reference once, targets of this batch) ----
-        for (int64_t p = p0 + tid; p < p1; p += nt) {
-            const uint32_t r = (uint32_t)J.rows[p];
-            if (r >= R) continue;
-            const uint32_t g = (uint32_t)J.gid[r];
-            if (g >= (uint32_t)G) continue;
-            const V v = vals[p];
-            if (!KT::finite(v) || v == V(0)) continue;
-            if (g == 0) {
-                if (first) {
-                    if (hist) atomicAdd(&cum[(uint32_t)((int)v - vmin)], 1u);
-                    else rk[atomicAdd(&st.refcur, 1u)] = KT::key(v);
-                }
-            } else if (g >= glo && g < ghi) {
-                const uint32_t pos = atomicAdd(&seg[g], 1u) - off0;
-                tgt[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
-            }
-        }
-        __syncthreads();
-
-        // ---- B: reference table ----
-        if (first) {
-            const uint32_t na = cnt_nz[0];
-            uint64_t ta = 0;
-            if (hist) {
-                for (uint32_t b = tid; b < st.nb; b += nt) ta += tie_f(cum[b]);
-                ta = block_sum_u64(ta, st.red);
-                block_excl_scan(cum, (int)st.nb + 1, st.scan);
-            } else {
-                const uint32_t P = st.P;
-                for (uint32_t x = na + tid; x < P; x += nt) rk[x] = KT::kSentinel;
-                __syncthreads();
-                sort_ref<K>(rk, P);
-                for (uint32_t b = tid; b <= st.nb; b += nt) bidx[b] = 0u;
-                __syncthreads();
-                const uint64_t kmin = st.kmin;
-                const uint32_t sh = st.shift;
-                for (uint32_t x = tid; x < na; x += nt)
-                    atomicAdd(&bidx[(uint32_t)(((uint64_t)rk[x] - kmin) >> sh)], 1u);
-                __syncthreads();
-                block_excl_scan(bidx, (int)st.nb + 1, st.scan);
-                RefView<K> rv;
-                rv.cum = cum; rv.keys = rk; rv.bidx = bidx; rv.kmin = st.kmin; rv.kmax = st.kmax;
-                rv.shift = st.shift; rv.na = na; rv.hist = 0;
-                for (uint32_t x = tid; x < na; x += nt) {
-                    uint32_t lt, eq;
-                    rv.lookup(rk[x], lt, eq);
-                    if (lt == x) ta += tie_f(eq);
-                }
-                ta = block_sum_u64(ta, st.red);
-            }
-            if (tid == 0) st.TA = ta;
-            __syncthreads();
-            first = false;
-        }
-
-        // ---- D: targets of this batch, one wave per target ----
-        {
-            RefView<K> rv;
-            rv.cum = cum; rv.keys = rk; rv.bidx = bidx; rv.kmin = st.kmin; rv.kmax = st.kmax;
-            rv.shift = st.shift; rv.na = cnt_nz[0]; rv.hist = hist;
-            for (;;) {
-                uint32_t g = 0;
-                if (lane == 0) g = atomicAdd(&st.next_t, 1u);
-                g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
-                if (g >= ghi) break;
-                process_target<V>(J, c, g, st, cnt_nz, cnt_ez, seg, tgt, rv);
-            }
-        }
-        __syncthreads();
-        glo = ghi;
-    }
-}
-
-template <typename V>
-__global__ __launch_bounds__(512) void gene_plain_kernel(JobDesc J, int lds) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ColState st;
-    process_column<V>(J, (int64_t)blockIdx.x, smem, lds, st);
-}
-
-template <typename V>
-__global__ __launch_bounds__(512) void gene_persistent_kernel(const JobDesc* jobs,
-                                                               const WorkItem* items,
-                                                               int n_items, unsigned* head,
-                                                               unsigned* done, int lds) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ColState st;
     for (;;) {
-        if (threadIdx.x == 0) st.item = (int32_t)atomicAdd(head, 1u);
-        __syncthreads();
-        const int it = st.item;
-        if (it >= n_items) break;
-        const WorkItem w = items[it];
-        const JobDesc J = jobs[w.job];
-        process_column<V>(J, (int64_t)w.col, smem, lds, st);
-        __syncthreads();
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(&st.next_t, 1u);
+        g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
+        if (g >= (uint32_t)J.G) break;
+        process_target<V>(J, c, g, pl, rv);
     }
-    // The last workgroup to leave rewinds the queue for the next launch.
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(done, 1u) == gridDim.x - 1) {
-            atomicExch(head, 0u);
-            atomicExch(done, 0u);
-        }
-    }
+}
+
+// ---------------------------------------------------------------------------
+// pair: sample sizes and, for the asymptotic method, the FP64 p-value of
+// every (gene, target) pair (mannwhitneyu.cpp:191-240).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pair_kernel(JobDesc J) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= J.C * J.T) return;
+    const int64_t c = i / J.T;
+    const uint32_t g = (uint32_t)(i - c * J.T) + 1u;
+    uint32_t za, zb, n1, n2;
+    pair_sizes(J, c, g, za, zb, n1, n2);
+    J.n1o[i] = (double)n1;
+    J.n2o[i] = (double)n2;
+    if (J.asym) J.P[i] = p_asymptotic(J.U1[i], (double)n1, (double)n2, J.tie[i], J.cc, J.alt);
 }
 
 __global__ void init_job_state_kernel(unsigned long long* err_key, uint32_t* status,
@@ -626,7 +656,7 @@ __global__ __launch_bounds__(1024) void exact_table_kernel(const ExactTable* tab
     double* fp = scratch + (size_t)blockIdx.x * scratch_per_table;
     double* fc = fp + (size_t)(s + 1) * W;
     double* out = tabbuf + t.offset;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = tid_x(), nt = blockDim.x;
     for (int64_t x = tid; x < (s + 1) * W; x += nt) fp[x] = (x % W == 0) ? 1.0 : 0.0;
     __syncthreads();
     for (int64_t i = 1; i <= l; ++i) {
@@ -711,19 +741,15 @@ __global__ __launch_bounds__(1024) void group_mean_kernel(const V* vals, const i
     uint32_t* vc = (uint32_t*)(sum + G);
     uint32_t* ic = vc + G;
     const int64_t c = blockIdx.x;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = tid_x(), nt = blockDim.x;
     for (int g = tid; g < G; g += nt) { sum[g] = 0.0; vc[g] = 0u; ic[g] = 0u; }
     __syncthreads();
     const int64_t p0 = indptr[c], p1 = indptr[c + 1];
-    for (int64_t p = p0 + tid; p < p1; p += nt) {
-        const uint32_t r = (uint32_t)rows[p];
-        if (r >= (uint32_t)R) continue;
-        const uint32_t g = (uint32_t)gid[r];
-        if (g >= (uint32_t)G) continue;
-        const double v = (double)vals[p];
+    for_each_entry<V>(rows, vals, gid, (uint32_t)R, (uint32_t)G, p0, p1, [&](uint32_t g, V x) {
+        const double v = (double)x;
         if (__builtin_isfinite(v)) { atomicAdd(&sum[g], v); atomicAdd(&vc[g], 1u); }
         else atomicAdd(&ic[g], 1u);
-    }
+    });
     __syncthreads();
     for (int g = tid; g < G; g += nt) {
         const uint64_t denom = include_zeros ? ((uint64_t)group_rows[g] - (uint64_t)ic[g])
@@ -748,6 +774,15 @@ __global__ void convert_values_kernel(const Tin* in, Tout* out, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         out[i] = (Tout)in[i];
+}
+
+// Streaming copy used to calibrate HBM counters and measure attainable
+// bandwidth: 16-byte loads and stores, grid-stride.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ in,
+                                                          uint4* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -788,40 +823,44 @@ hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n,
     return hipGetLastError();
 }
 
-hipError_t set_gene_kernel_lds_limit(int lds) {
-    hipError_t e;
-    e = hipFuncSetAttribute((const void*)gene_plain_kernel<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)gene_plain_kernel<double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)gene_persistent_kernel<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)gene_persistent_kernel<double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)group_mean_kernel<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)group_mean_kernel<double>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+static inline unsigned align16(size_t b) { return (unsigned)((b + 15) & ~(size_t)15); }
+
+// LDS of the reference-table and target kernels: the larger of a full
+// histogram table and the on-chip sorted reference plus its bucket index.
+static unsigned table_lds_bytes(bool f64) {
+    const size_t hist = 4u * (size_t)(kHistMaxBins + 1);
+    const size_t keys = f64 ? 8u * (size_t)kRefKeysF64 : 4u * (size_t)kRefKeysF32;
+    const size_t sort = keys + 4u * (size_t)(kMaxBuckets + 1);
+    return align16(hist > sort ? hist : sort);
 }
 
-hipError_t launch_gene_plain(const JobDesc& J, bool f64, int block, int lds, hipStream_t s) {
-    if (J.C <= 0) return hipSuccess;
-    if (f64) gene_plain_kernel<double><<<(unsigned)J.C, block, lds, s>>>(J, lds);
-    else gene_plain_kernel<float><<<(unsigned)J.C, block, lds, s>>>(J, lds);
+hipError_t launch_mwu_pipeline(const JobDesc* jobs, const WorkItem* items, int n_items,
+                               const JobDesc& J0, bool f64, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    size_t split = 12u * (size_t)J0.G;
+    if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
+    const unsigned split_lds = align16(split);
+    const unsigned tab_lds = table_lds_bytes(f64);
+    if (f64) {
+        split_kernel<double><<<n_items, kColBlock, split_lds, s>>>(jobs, items);
+        ref_kernel<double><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+        target_kernel<double><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+    } else {
+        split_kernel<float><<<n_items, kColBlock, split_lds, s>>>(jobs, items);
+        ref_kernel<float><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+        target_kernel<float><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t np = J0.C * (int64_t)J0.T;
+    if (np > 0) pair_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(J0);
     return hipGetLastError();
 }
 
-hipError_t launch_gene_persistent(const JobDesc* jobs, const WorkItem* items, int n_items,
-                                  unsigned* head, unsigned* done, bool f64, int grid, int block,
-                                  int lds, hipStream_t s) {
-    if (n_items <= 0) return hipSuccess;
-    if (f64) gene_persistent_kernel<double><<<grid, block, lds, s>>>(jobs, items, n_items, head, done, lds);
-    else gene_persistent_kernel<float><<<grid, block, lds, s>>>(jobs, items, n_items, head, done, lds);
+hipError_t launch_stream_copy(const void* in, void* out, size_t bytes, int grid, hipStream_t s) {
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    stream_copy_kernel<<<grid, 256, 0, s>>>((const uint4*)in, (uint4*)out, n);
     return hipGetLastError();
 }
 
@@ -848,16 +887,16 @@ hipError_t launch_exact_p(const ExactJob& E, hipStream_t s) {
 hipError_t launch_group_mean(const void* vals, bool f64, const int32_t* rows,
                              const int64_t* indptr, const int32_t* gid,
                              const uint32_t* group_rows, int64_t C, int32_t R, int32_t G,
-                             int include_zeros, double* out, uint32_t* status, int block,
-                             int lds, hipStream_t s) {
-    (void)status;
+                             int include_zeros, double* out, hipStream_t s) {
     if (C <= 0 || G <= 0) return hipSuccess;
-    const int need = 16 * G;
-    if (need > lds) return hipErrorInvalidValue;
+    const size_t need = 16u * (size_t)G;
+    if (need > (size_t)kLdsBudget) return hipErrorInvalidValue;
     if (f64)
-        group_mean_kernel<double><<<(unsigned)C, block, need, s>>>((const double*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
+        group_mean_kernel<double><<<(unsigned)C, kColBlock, align16(need), s>>>(
+            (const double*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
     else
-        group_mean_kernel<float><<<(unsigned)C, block, need, s>>>((const float*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
+        group_mean_kernel<float><<<(unsigned)C, kColBlock, align16(need), s>>>(
+            (const float*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
     return hipGetLastError();
 }
 

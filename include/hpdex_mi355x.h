@@ -8,9 +8,7 @@
  * outputs.  They are reentrant: each call owns its stream and device memory.
  *
  * Device-resident entry points (hpdex_matrix_*, hpdex_plan_*) keep a CSC matrix
- * in HBM across calls for the batched persistent-kernel form and for serving.
- * The serving session (hpdex_session_*) admits jobs by HBM footprint and
- * interleaves their work in one persistent kernel.
+ * in HBM across calls for the batched form and for serving.
  */
 #ifndef HPDEX_MI355X_H
 #define HPDEX_MI355X_H
@@ -101,17 +99,17 @@ void hpdex_matrix_destroy(hpdex_matrix* m);
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
 
 /* A job on a resident matrix: group labels and options fixed, outputs held on
- * the device.  form: 0 = plain (one workgroup per column, column order),
- * 1 = batched (persistent grid over a size-ordered work queue, replayed from a
- * captured hipGraph).                                                        */
+ * the device.  form: 0 = plain (the kernel sequence launched eagerly over the
+ * columns in order), 1 = batched (columns queued largest first, the whole
+ * launch sequence captured once in a hipGraph and replayed).                */
 int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
                       const hpdex_mwu_opts* opts, int form, hpdex_plan** out,
                       char* err, size_t err_len);
 void hpdex_plan_destroy(hpdex_plan* p);
 /* Run the plan `iters` times back to back on its stream and wait.  If ms_total
  * is non-NULL it receives the elapsed milliseconds (HIP events on the plan's
- * stream); if kernel_ms is non-NULL it receives the mean duration of the
- * per-gene kernel alone (HIP events around that launch).                    */
+ * stream); if kernel_ms is non-NULL it receives the mean duration of one run
+ * of the device work (HIP events around each run).                          */
 int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms,
                    char* err, size_t err_len);
 /* Copy the last run's results to host buffers (n_cols*n_targets doubles each;
@@ -119,15 +117,19 @@ int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms
 int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie_out,
                        double* n1_out, double* n2_out, char* err, size_t err_len);
 
-/* Launch geometry used by the per-gene kernels (the autotuner's choice). */
+/* Tunable choices of the device path (set by the autotuner). */
 typedef struct hpdex_tuning {
-    int32_t block_threads;     /* threads per workgroup (multiple of 64, <= 1024) */
-    int32_t lds_bytes;         /* dynamic LDS per workgroup                        */
-    int32_t blocks_per_cu;     /* persistent grid = CUs * blocks_per_cu            */
-    int32_t hist_max_bins;     /* largest integer range sent to the histogram path */
+    int32_t hist_max_bins;     /* widest integer value range a column may have to use
+                                  the histogram path (0 = never, at most 8192)        */
+    int32_t size_ordered;      /* batched form: 1 = queue the largest columns first  */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);
+
+/* Attainable HBM bandwidth: a 16-byte streaming copy of `bytes` (read + write
+ * counted), run `iters` times; gbs receives GB/s.  Also the calibration kernel
+ * for HBM byte counters (its traffic is exactly 2*bytes per launch). */
+int hpdex_bandwidth_probe(size_t bytes, int iters, double* gbs, char* err, size_t err_len);
 
 /* Library version string and device summary, for logs. */
 const char* hpdex_version(void);
