@@ -97,6 +97,10 @@ def main():
     ap.add_argument("--workload", default="histogram")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the streaming copy kernel over this many MiB (3 launches "
+                         "after one warm launch) and print its GB/s: the calibration run for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -104,6 +108,10 @@ def main():
         sys.exit(2)
 
     from hpdex_amd import _capi, synthetic
+    if args.copy_probe_mib:
+        gbs = _capi.bandwidth_probe(args.copy_probe_mib << 20, iters=3)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20, "gbs": gbs}), flush=True)
+        return
     ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
     log(f"generating workload {args.workload}")
     t0 = time.perf_counter()
