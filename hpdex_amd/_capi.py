@@ -27,7 +27,7 @@ EXPORTED = (
     "hpdex_matrix_create", "hpdex_matrix_destroy", "hpdex_matrix_device_bytes",
     "hpdex_plan_create", "hpdex_plan_destroy", "hpdex_plan_run", "hpdex_plan_results",
     "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
-    "hpdex_bandwidth_probe",
+    "hpdex_bandwidth_probe", "hpdex_plan_get_info", "hpdex_plan_stage_times",
 )
 
 
@@ -38,7 +38,12 @@ class MwuOpts(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered")]
+    _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes")]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "count_columns", "count_nnz", "split_columns", "handed_over")]
 
 
 class HpdexError(RuntimeError):
@@ -80,10 +85,13 @@ def lib() -> ctypes.CDLL:
         L.hpdex_version.restype = cp
         L.hpdex_device_info.argtypes = [ctypes.POINTER(ci), ctypes.POINTER(ci), cp, sz]
         L.hpdex_bandwidth_probe.argtypes = [sz, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo), cp, sz]
+        L.hpdex_plan_stage_times.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
                      "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
-                     "hpdex_device_info", "hpdex_bandwidth_probe"):
+                     "hpdex_device_info", "hpdex_bandwidth_probe", "hpdex_plan_get_info",
+                     "hpdex_plan_stage_times"):
             getattr(L, name).restype = ci
         _lib = L
     return _lib
@@ -185,12 +193,19 @@ class DeviceMatrix:
             pass
 
 
-class Plan:
-    """A job on a DeviceMatrix (hpdex_plan_create); form 0 plain, 1 batched."""
+PLAN_PLAIN, PLAN_BATCHED, PLAN_STATS = 0, 1, 2
 
-    def __init__(self, matrix: DeviceMatrix, group_id, n_targets, opts: MwuOpts, form=1):
+
+class Plan:
+    """A job on a DeviceMatrix (hpdex_plan_create); form 0 plain, 1 batched.
+    stats=True keeps the tie term and sample sizes for results(with_stats=True)."""
+
+    def __init__(self, matrix: DeviceMatrix, group_id, n_targets, opts: MwuOpts, form=1,
+                 stats=False):
         self.matrix = matrix
         self.n_targets = int(n_targets)
+        self.stats = bool(stats)
+        form = int(form) | (PLAN_STATS if stats else 0)
         gid = np.ascontiguousarray(group_id, dtype=np.int32)
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(1024)
@@ -219,6 +234,23 @@ class Plan:
             raise HpdexError(err.value.decode())
         return tuple(o[:n] for o in outs)
 
+    def info(self) -> dict:
+        """Column routing of the plan and hand-overs of its last run."""
+        i = PlanInfo()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_plan_get_info(self._h, ctypes.byref(i), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        return {n: int(getattr(i, n)) for n, _ in PlanInfo._fields_}
+
+    def stage_times(self, iters=5) -> dict:
+        """Mean ms of the count, split, reference and target kernels (HIP
+        events between the kernels of the plain launch sequence)."""
+        ms = (ctypes.c_double * 4)()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_plan_stage_times(self._h, int(iters), ms, err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        return dict(zip(("count", "split", "reference", "target"), list(ms)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().hpdex_plan_destroy(self._h)
@@ -237,12 +269,14 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(hist_max_bins=None, size_ordered=None):
+def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None):
     t = get_tuning()
     if hist_max_bins is not None:
         t.hist_max_bins = hist_max_bins
     if size_ordered is not None:
         t.size_ordered = size_ordered
+    if count_lds_bytes is not None:
+        t.count_lds_bytes = count_lds_bytes
     err = ctypes.create_string_buffer(1024)
     if lib().hpdex_set_tuning(ctypes.byref(t), err, len(err)) != 0:
         raise HpdexError(err.value.decode())

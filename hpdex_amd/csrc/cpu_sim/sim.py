@@ -32,7 +32,9 @@ def lib(path=None):
 
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
-        lds=76 * 1024, block=512, hist_max=8192, f64=False, timeout_s=30.0):
+        count_lds=76 * 1024, block=512, hist_max=8192, f64=False, timeout_s=30.0):
+    """Run the launch sequence on the host.  count_lds: dynamic LDS of the
+    count kernel (0 = every column takes the split/reference/target kernels)."""
     vals = np.ascontiguousarray(data, dtype=np.float64 if f64 else np.float32)
     rows = np.where((indices >= 0) & (indices < n_rows), indices, -1).astype(np.int32)
     indptr = np.ascontiguousarray(indptr, dtype=np.int64)
@@ -42,8 +44,9 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     outs = [np.zeros(max(n, 1)) for _ in range(5)]
     err = np.zeros(1, dtype=np.uint64)
     status = np.zeros(1, dtype=np.uint32)
-    lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
-                  gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), lds, block, hist_max,
+    counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
+                  gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
                   float(timeout_s))
+    mwu.last_counted = counted   # columns the count kernel finished itself
     return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

@@ -2,6 +2,7 @@
 // (see sim_runtime.h).  Exposes a C entry point that mirrors the plain
 // one-workgroup-per-column launch, for CPU tests and fault hunting.  Built
 // with AddressSanitizer by build_sim.py; never shipped.
+#include <algorithm>
 #include <chrono>
 #include <string>
 #include <thread>
@@ -69,32 +70,55 @@ bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void(
 
 }  // namespace
 
+// The launch sequence of launch_mwu_pipeline: column classification, the
+// count kernel over the integer-valued columns, then the split / reference /
+// target kernels over the queue (host-routed columns plus hand-overs).
+// Returns the number of columns the count kernel finished itself.
 template <typename V>
-static void run_pipeline(const JobDesc& J, const std::vector<WorkItem>& items, double timeout_s) {
+static int run_pipeline(const JobDesc& J, const std::vector<int32_t>& order, uint32_t count_lds,
+                        double timeout_s) {
     const JobDesc* jobs = &J;
-    const WorkItem* it = items.data();
-    const unsigned n = (unsigned)items.size();
-    std::vector<unsigned char> buf(64 * 1024);
+    const int C = (int)J.C;
+    std::vector<unsigned char> buf(std::max<size_t>(64 * 1024, count_lds));
     sim::g_lds = buf.data();
     sim::g_lds_bytes = buf.size();
-    for (unsigned b = 0; b < n; ++b)
-        run_block(b, kColBlock, n, [&] { split_kernel<V>(jobs, it); }, timeout_s);
-    for (unsigned b = 0; b < n; ++b)
-        run_block(b, kColBlock, n, [&] { ref_kernel<V>(jobs, it); }, timeout_s);
-    for (unsigned b = 0; b < n; ++b)
-        run_block(b, kColBlock, n, [&] { target_kernel<V>(jobs, it); }, timeout_s);
-    const int64_t np = J.C * J.T;
-    const unsigned pg = (unsigned)((np + 255) / 256);
-    for (unsigned b = 0; b < pg; ++b)
-        run_block(b, 256, pg, [&] { pair_kernel(J); }, timeout_s);
+    std::vector<uint8_t> col_int(C, 0);
+    for (int c = 0; c < C; ++c)
+        run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
+                                                              col_int.data()); }, timeout_s);
+    const bool count_ok = count_lds > 0 && (uint32_t)J.G <= kCountMaxGroups &&
+                          24u * (uint32_t)J.G + 4u * kCountReserveWords <= count_lds;
+    std::vector<WorkItem> count_items, gen_items(C + 1);
+    uint32_t n_static = 0;
+    for (int32_t c : order) {
+        if (count_ok && col_int[c]) count_items.push_back(WorkItem{0, c});
+        else gen_items[n_static++] = WorkItem{0, c};
+    }
+    uint32_t gen_n = n_static;
+    const unsigned nc = (unsigned)count_items.size();
+    for (unsigned b = 0; b < nc; ++b)
+        run_block(b, kCountBlock, nc, [&] {
+            count_kernel<V>(jobs, count_items.data(), gen_items.data(), &gen_n, count_lds);
+        }, timeout_s);
+    const unsigned ng = n_static + nc;
+    const WorkItem* it = gen_items.data();
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kColBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+    return (int)(nc - (gen_n - n_static));
 }
 
+// count_lds: dynamic LDS of the count kernel (0 = every column takes the
+// split/reference/target kernels).  Columns are visited in reversed order.
+// Returns the number of columns the count kernel finished itself.
 extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int64_t* indptr,
                        int R, int C, const int32_t* gid, int T, int mode, int alt, int tie_corr,
-                       int cc, int lds, int block, int hist_max, double* U1, double* P,
+                       int cc, int count_lds, int block, int hist_max, double* U1, double* P,
                        double* tie, double* n1, double* n2, unsigned long long* err_key,
                        unsigned* status, double timeout_s) {
-    (void)lds;
     (void)block;
     const int G = T + 1;
     std::vector<uint32_t> grows(G, 0u);
@@ -116,9 +140,8 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
-    std::vector<WorkItem> items(C);
-    for (int c = 0; c < C; ++c) items[c] = WorkItem{0, C - 1 - c};   // reversed order
-    if (f64) run_pipeline<double>(J, items, timeout_s);
-    else run_pipeline<float>(J, items, timeout_s);
-    return 0;
+    std::vector<int32_t> order(C);
+    for (int c = 0; c < C; ++c) order[c] = C - 1 - c;   // reversed order
+    if (f64) return run_pipeline<double>(J, order, (uint32_t)count_lds, timeout_s);
+    return run_pipeline<float>(J, order, (uint32_t)count_lds, timeout_s);
 }

@@ -32,6 +32,7 @@ constexpr size_t kExactMaxCells = (size_t)1 << 27;
 struct Tuning {
     int hist_max = kHistMaxBins;
     int size_ordered = 1;
+    int count_lds = kCountLdsDefault;
 };
 
 std::mutex g_mu;
@@ -39,6 +40,7 @@ Tuning g_tune;
 bool g_init = false;
 int g_cus = 0;
 int g_max_lds = 0;
+bool g_count_ready = false;   // the count kernel may use up to kCountLdsMax of LDS
 char g_name[256] = {0};
 
 int set_err(char* err, size_t len, const std::string& m) {
@@ -72,6 +74,8 @@ int ensure_init(char* err, size_t err_len) {
     g_cus = pr.multiProcessorCount;
     g_max_lds = (int)pr.sharedMemPerBlock;
     std::snprintf(g_name, sizeof g_name, "%s (%s)", pr.name, pr.gcnArchName);
+    g_count_ready = g_max_lds >= kCountLdsMax && prepare_count_kernel(kCountLdsMax) == hipSuccess;
+    if (!g_count_ready) g_tune.count_lds = 0;
     g_init = true;
     return 0;
 }
@@ -142,7 +146,9 @@ struct hpdex_matrix {
     size_t R = 0, C = 0, nnz = 0;
     DevBuf vals, rows, indptr;
     std::vector<int64_t> h_indptr;
-    std::vector<int32_t> order;   // columns, largest first
+    std::vector<int32_t> order;     // columns, largest first
+    std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
+                                    // non-finite) -- the column is sent to the count kernel
 };
 
 namespace {
@@ -182,6 +188,15 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     }
     HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (C + 1) * 8, hipMemcpyHostToDevice, s));
     m->h_indptr.assign(indptr, indptr + C + 1);
+    m->col_int.assign(C, 0);
+    if (C) {
+        DevBuf flags;
+        HIPCHK(flags.alloc(C));
+        HIPCHK(launch_classify_columns(m->vals.p, m->f64, m->indptr.as<int64_t>(), (int64_t)C,
+                                       flags.as<uint8_t>(), s));
+        HIPCHK(hipMemcpyAsync(m->col_int.data(), flags.p, C, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     m->order.resize(C);
     std::iota(m->order.begin(), m->order.end(), 0);
     std::stable_sort(m->order.begin(), m->order.end(), [&](int32_t a, int32_t b) {
@@ -194,11 +209,16 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
 // Per-job device state shared by the one-shot call and the plan.
 struct JobState {
     size_t T = 0, G = 0, C = 0;
+    bool stats = false;          // tie and sample sizes kept per pair
     hpdex_mwu_opts opts{};
     std::vector<uint32_t> group_rows;
     DevBuf gid, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
-    DevBuf d_job, d_items;                          // descriptor and column queue
+    DevBuf d_job;                                   // descriptor
+    DevBuf d_count_items, d_gen_items, d_gen_n;     // column queues of the two paths
+    int n_count = 0, n_gen_static = 0;
+    int64_t count_nnz = 0;
+    uint32_t count_lds = 0;
     // exact null tables
     std::vector<ExactTable> tabs;
     std::vector<int32_t> tidx_by_g;
@@ -212,13 +232,14 @@ struct JobState {
 };
 
 int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
-              const hpdex_mwu_opts* o, hipStream_t s, char* err, size_t err_len) {
+              const hpdex_mwu_opts* o, bool stats, hipStream_t s, char* err, size_t err_len) {
     if (T < 1) return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
     if (T + 1 > (size_t)INT32_MAX / 4) return set_err(err, err_len, "too many target groups");
     js.T = T;
     js.G = T + 1;
     js.C = m->C;
     js.opts = *o;
+    js.stats = stats;
     js.group_rows.assign(js.G, 0u);
     for (size_t r = 0; r < m->R; ++r) {
         const int g = group_id[r];
@@ -231,9 +252,13 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     HIPCHK(hipMemcpyAsync(js.grows.p, js.group_rows.data(), js.G * 4, hipMemcpyHostToDevice, s));
     HIPCHK(js.U1.alloc(np * 8));
     HIPCHK(js.P.alloc(np * 8));
-    HIPCHK(js.tie.alloc(np * 8));
-    HIPCHK(js.n1.alloc(np * 8));
-    HIPCHK(js.n2.alloc(np * 8));
+    // sample sizes are also needed by the exact p of zero_handling none
+    const bool sizes = stats || (o->method == 1 && o->zero_handling == 0);
+    if (stats) HIPCHK(js.tie.alloc(np * 8));
+    if (sizes) {
+        HIPCHK(js.n1.alloc(np * 8));
+        HIPCHK(js.n2.alloc(np * 8));
+    }
     HIPCHK(js.flags.alloc(32));
     HIPCHK(js.keys.alloc(m->nnz * (m->f64 ? 8 : 4)));
     HIPCHK(js.cnt.alloc(js.C * js.G * 4));
@@ -260,7 +285,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.reftab = js.reftab.as<uint32_t>();
     J.U1 = js.U1.as<double>();
     J.P = js.P.as<double>();
-    J.tie = js.tie.as<double>();
+    J.tie = js.tie.as<double>();   // null when not kept
     J.n1o = js.n1.as<double>();
     J.n2o = js.n2.as<double>();
     J.err_key = js.err_key();
@@ -279,21 +304,57 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     return J;
 }
 
-// Put the job descriptor and its column queue on the device: columns in
-// order (plain form) or largest first (batched form).
-int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, bool size_ordered,
-                    hipStream_t s, char* err, size_t err_len) {
+// Put the job descriptor and its column queues on the device: integer-valued
+// columns to the count kernel, the rest to the split/reference/target kernels;
+// each queue in column order (plain form) or largest first (batched form).
+int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const Tuning& tu,
+                    bool size_ordered, hipStream_t s, char* err, size_t err_len) {
     HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
     HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
-    std::vector<WorkItem> items(m.C);
-    for (size_t k = 0; k < m.C; ++k)
-        items[k] = WorkItem{0, size_ordered ? m.order[k] : (int32_t)k};
-    HIPCHK(js.d_items.alloc(items.size() * sizeof(WorkItem)));
-    if (!items.empty())
-        HIPCHK(hipMemcpyAsync(js.d_items.p, items.data(), items.size() * sizeof(WorkItem),
-                              hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));   // items is a host temporary
+    // the count kernel takes the job if its per-group LDS state leaves room
+    const bool count_ok = tu.count_lds > 0 && js.G <= kCountMaxGroups &&
+                          24u * js.G + 4u * kCountReserveWords <= (size_t)tu.count_lds;
+    std::vector<WorkItem> count_items, gen_items;
+    count_items.reserve(m.C);
+    gen_items.reserve(m.C);
+    for (size_t k = 0; k < m.C; ++k) {
+        const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
+        const int64_t len = m.h_indptr[c + 1] - m.h_indptr[c];
+        if (count_ok && m.col_int[c] && len < ((int64_t)1 << 31)) {
+            count_items.push_back(WorkItem{0, c});
+            js.count_nnz += len;
+        } else
+            gen_items.push_back(WorkItem{0, c});
+    }
+    js.n_count = (int)count_items.size();
+    js.n_gen_static = (int)gen_items.size();
+    js.count_lds = (uint32_t)tu.count_lds;
+    HIPCHK(js.d_count_items.alloc(count_items.size() * sizeof(WorkItem)));
+    if (!count_items.empty())
+        HIPCHK(hipMemcpyAsync(js.d_count_items.p, count_items.data(),
+                              count_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    // room for every column: the count kernel appends those it hands over
+    HIPCHK(js.d_gen_items.alloc(m.C * sizeof(WorkItem)));
+    if (!gen_items.empty())
+        HIPCHK(hipMemcpyAsync(js.d_gen_items.p, gen_items.data(),
+                              gen_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    HIPCHK(js.d_gen_n.alloc(16));
+    HIPCHK(hipStreamSynchronize(s));   // the item vectors are host temporaries
     return 0;
+}
+
+PipelineLaunch pipeline_of(const JobState& js, bool f64) {
+    PipelineLaunch L{};
+    L.jobs = js.d_job.as<JobDesc>();
+    L.count_items = js.d_count_items.as<WorkItem>();
+    L.n_count = js.n_count;
+    L.gen_items = js.d_gen_items.as<WorkItem>();
+    L.gen_n = js.d_gen_n.as<uint32_t>();
+    L.n_gen_static = js.n_gen_static;
+    L.G = (int)js.G;
+    L.f64 = f64;
+    L.count_lds = js.count_lds;
+    return L;
 }
 
 // Plan the exact-null tables.  min/max/mix: sizes are the group row counts,
@@ -412,6 +473,10 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
 int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1, double* n2,
              hipStream_t s, char* err, size_t err_len) {
     const size_t b = js.C * js.T * 8;
+    if ((tie && !js.tie.p) || (n1 && !js.n1.p) || (n2 && !js.n2.p))
+        return set_err(err, err_len,
+                       "the tie term and sample sizes are kept only by plans created with "
+                       "HPDEX_PLAN_STATS");
     if (U1 && b) HIPCHK(hipMemcpyAsync(U1, js.U1.p, b, hipMemcpyDeviceToHost, s));
     if (P && b) HIPCHK(hipMemcpyAsync(P, js.P.p, b, hipMemcpyDeviceToHost, s));
     if (tie && b) HIPCHK(hipMemcpyAsync(tie, js.tie.p, b, hipMemcpyDeviceToHost, s));
@@ -490,6 +555,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     const Tuning t = tuning();
     out->hist_max_bins = t.hist_max;
     out->size_ordered = t.size_ordered;
+    out->count_lds_bytes = t.count_lds;
     return 0;
 }
 
@@ -499,9 +565,17 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
         return set_err(err, err_len, "hist_max_bins must be in [0, 8192]");
     if (t->size_ordered != 0 && t->size_ordered != 1)
         return set_err(err, err_len, "size_ordered must be 0 or 1");
+    if (t->count_lds_bytes != 0 &&
+        (t->count_lds_bytes < 16 * 1024 || t->count_lds_bytes > kCountLdsMax ||
+         t->count_lds_bytes % 16 != 0))
+        return set_err(err, err_len,
+                       "count_lds_bytes must be 0 or a multiple of 16 in [16384, 155648]");
     std::lock_guard<std::mutex> lk(g_mu);
+    if (g_init && !g_count_ready && t->count_lds_bytes != 0)
+        return set_err(err, err_len, "this device does not give the count kernel its LDS");
     g_tune.hist_max = t->hist_max_bins;
     g_tune.size_ordered = t->size_ordered;
+    g_tune.count_lds = t->count_lds_bytes;
     return 0;
 }
 
@@ -524,11 +598,11 @@ int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
     JobState js;
-    if (job_setup(js, &m, group_id, n_targets, opts, st.s, err, err_len)) return 1;
+    const bool stats = tie_out || n1_out || n2_out;
+    if (job_setup(js, &m, group_id, n_targets, opts, stats, st.s, err, err_len)) return 1;
     const JobDesc J = make_desc(js, &m, tu);
-    if (job_upload_desc(js, J, m, false, st.s, err, err_len)) return 1;
-    HIPCHK(launch_mwu_pipeline(js.d_job.as<JobDesc>(), js.d_items.as<WorkItem>(), (int)m.C, J,
-                               m.f64, st.s));
+    if (job_upload_desc(js, J, m, tu, false, st.s, err, err_len)) return 1;
+    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), st.s));
     if (job_errors(js, st.s, err, err_len)) return 1;
     if (opts->method == 1) {
         if (opts->zero_handling == 0) {
@@ -638,8 +712,7 @@ namespace {
 // of min/max/mix depend only on group sizes and are built once per plan).
 int plan_enqueue(hpdex_plan* p, char* err, size_t err_len) {
     const hipStream_t s = p->st.s;
-    HIPCHK(launch_mwu_pipeline(p->js.d_job.as<JobDesc>(), p->js.d_items.as<WorkItem>(),
-                               (int)p->m->C, p->J, p->m->f64, s));
+    HIPCHK(launch_mwu_pipeline(pipeline_of(p->js, p->m->f64), s));
     if (p->js.opts.method == 1 && p->js.opts.zero_handling != 0)
         if (exact_launch(p->js, s, false, err, err_len)) return 1;
     return 0;
@@ -655,7 +728,12 @@ int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets
     if (!out || !m) return set_err(err, err_len, "null handle");
     *out = nullptr;
     if (!opts_ok(opts)) return set_err(err, err_len, "invalid option block");
-    if (form != 0 && form != 1) return set_err(err, err_len, "form must be 0 (plain) or 1 (batched)");
+    if (form < 0 || form > (HPDEX_PLAN_BATCHED | HPDEX_PLAN_STATS))
+        return set_err(err, err_len,
+                       "form must be HPDEX_PLAN_PLAIN or HPDEX_PLAN_BATCHED, optionally with "
+                       "HPDEX_PLAN_STATS");
+    const bool stats = (form & HPDEX_PLAN_STATS) != 0;
+    form &= HPDEX_PLAN_BATCHED;
     if (ensure_init(err, err_len)) return 1;
     hpdex_plan* p = new hpdex_plan();
     p->m = m;
@@ -663,10 +741,11 @@ int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets
     p->tu = tuning();
     auto bail = [&](int rc) { delete p; return rc; };
     if (p->st.create() != hipSuccess) return bail(set_err(err, err_len, "stream creation failed"));
-    if (job_setup(p->js, m, group_id, n_targets, opts, p->st.s, err, err_len)) return bail(1);
+    if (job_setup(p->js, m, group_id, n_targets, opts, stats, p->st.s, err, err_len))
+        return bail(1);
     p->J = make_desc(p->js, m, p->tu);
     const bool ordered = form == 1 && p->tu.size_ordered;
-    if (job_upload_desc(p->js, p->J, *m, ordered, p->st.s, err, err_len)) return bail(1);
+    if (job_upload_desc(p->js, p->J, *m, p->tu, ordered, p->st.s, err, err_len)) return bail(1);
     if (opts->method == 1 && opts->zero_handling != 0) {
         if (exact_plan(p->js, nullptr, nullptr, p->st.s, err, err_len)) return bail(1);
         const hipError_t e = launch_exact_tables(p->js.d_tabs.as<ExactTable>(),
@@ -692,6 +771,50 @@ int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets
 }
 
 void hpdex_plan_destroy(hpdex_plan* p) { delete p; }
+
+int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len) {
+    if (!p || !out) return set_err(err, err_len, "null handle");
+    out->count_columns = p->js.n_count;
+    out->count_nnz = p->js.count_nnz;
+    out->split_columns = p->js.n_gen_static;
+    out->handed_over = 0;
+    if (p->js.d_gen_n.p && p->m->C > 0) {
+        uint32_t n = 0;
+        HIPCHK(hipMemcpyAsync(&n, p->js.d_gen_n.p, 4, hipMemcpyDeviceToHost, p->st.s));
+        HIPCHK(hipStreamSynchronize(p->st.s));
+        out->handed_over = n >= (uint32_t)p->js.n_gen_static ? n - p->js.n_gen_static : 0;
+    }
+    return 0;
+}
+
+int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size_t err_len) {
+    if (!p || !ms) return set_err(err, err_len, "null handle");
+    if (iters < 1) iters = 1;
+    for (int k = 0; k < 4; ++k) ms[k] = 0.0;
+    if (p->m->C == 0) return 0;
+    const hipStream_t s = p->st.s;
+    std::vector<hipEvent_t> ev(5 * (size_t)iters, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = 0;
+    for (int it = 0; it < iters && !rc; ++it) {
+        const hipError_t e = launch_mwu_pipeline(pipeline_of(p->js, p->m->f64), s, &ev[5 * it]);
+        if (e != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(e));
+        if (!rc && p->js.opts.method == 1 && p->js.opts.zero_handling != 0)
+            rc = exact_launch(p->js, s, false, err, err_len);
+    }
+    const hipError_t se = hipStreamSynchronize(s);
+    if (!rc && se != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(se));
+    if (!rc) {
+        for (int it = 0; it < iters; ++it)
+            for (int k = 0; k < 4; ++k) {
+                float x = 0.f;
+                hipEventElapsedTime(&x, ev[5 * it + k], ev[5 * it + k + 1]);
+                ms[k] += x / iters;
+            }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return rc;
+}
 
 int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms, char* err,
                    size_t err_len) {

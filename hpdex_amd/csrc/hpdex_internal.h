@@ -25,6 +25,14 @@ constexpr int kRefKeysF32 = 8192;           // sort path: reference keys kept on
 constexpr int kRefKeysF64 = 4096;           // sort path: reference keys kept on chip (8 B keys)
 constexpr int kTabStride = (kHistMaxBins > kMaxBuckets ? kHistMaxBins : kMaxBuckets) + 1;
 
+// Count kernel (integer-valued columns in one pass).
+constexpr int kCountBlock = 512;            // threads per workgroup (8 waves)
+constexpr int kCountLdsDefault = 76 * 1024; // dynamic LDS per workgroup: two per CU
+constexpr int kCountLdsMax = 152 * 1024;    // largest dynamic LDS the tuning accepts
+constexpr uint32_t kCountReserveWords = 1024;  // LDS words kept for tables beside a staged column
+constexpr int kMaxWindows = 64;             // value windows per column before handing it over
+constexpr uint32_t kCountMaxGroups = 65535; // group ids are packed in 16 bits
+
 // Per-column plan: written by the split kernel (and TA by the reference-table
 // kernel), read by the later kernels of the same job.
 struct ColPlan {
@@ -53,7 +61,7 @@ struct JobDesc {
     uint32_t* goff;              // C*G start of each group's keys, from indptr[c]
     ColPlan* plan;               // C
     uint32_t* reftab;            // C*kTabStride reference tables
-    // outputs (gene-major, c*T + g-1)
+    // outputs (gene-major, c*T + g-1); tie, n1o and n2o may be null (not kept)
     double* U1;
     double* P;
     double* tie;
@@ -69,7 +77,7 @@ struct JobDesc {
     int32_t mode;                // ZeroMode
     int32_t alt;                 // 0 less, 1 greater, 2 two-sided
     int32_t tie_corr;
-    int32_t asym;                // 1: asymptotic p from the pair kernel
+    int32_t asym;                // 1: asymptotic p written with U1
     int32_t hist_max;            // largest integer range binned (0 = off)
     double cc;                   // continuity correction, 0.5 or 0
 };
@@ -106,11 +114,32 @@ hipError_t launch_rows_i64_to_i32(const int64_t* in, int32_t* out, size_t n, int
 hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n, hipStream_t s);
 bool dtype_uses_double(int dtype);
 
-// The per-job pipeline over the columns listed in `items` (in that order):
-// split -> reference tables -> targets -> per-pair sizes and asymptotic p.
-// J0 supplies the sizes of the pair pass (one job per launch sequence).
-hipError_t launch_mwu_pipeline(const JobDesc* jobs, const WorkItem* items, int n_items,
-                               const JobDesc& J0, bool f64, hipStream_t s);
+// One launch sequence of the Mann-Whitney pass.  Integer-valued columns go to
+// the count kernel (count_items); the rest, and every column the count kernel
+// hands over at run time, to the split -> reference -> target kernels, which
+// read their items from gen_items[0 .. *gen_n).  gen_items holds the
+// n_gen_static host-routed columns first and has room for n_count more.
+struct PipelineLaunch {
+    const JobDesc* jobs;
+    const WorkItem* count_items;
+    int n_count;
+    WorkItem* gen_items;
+    uint32_t* gen_n;
+    int n_gen_static;
+    int G;                 // largest group count over the jobs (LDS sizing)
+    bool f64;
+    uint32_t count_lds;    // dynamic LDS bytes of the count kernel
+};
+// ev (optional, 5 events): recorded before the count kernel and after the
+// count, split, reference and target kernels, for per-kernel timing.
+hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,
+                               hipEvent_t* ev = nullptr);
+// Allow the count kernel `bytes` of dynamic LDS (once per process).
+hipError_t prepare_count_kernel(uint32_t bytes);
+// Per-column routing hint of a resident matrix: out[c] = 1 when every stored
+// value of column c is non-finite or an integer in [-32768, 32767].
+hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
+                                   uint8_t* out, hipStream_t s);
 hipError_t launch_init_job_state(unsigned long long* err_key, uint32_t* status,
                                  unsigned long long* status_col, hipStream_t s);
 hipError_t launch_exact_tables(const ExactTable* tabs, int n_tabs, double* tabbuf,

@@ -1,19 +1,24 @@
 // hpdex_kernels.hip -- gfx950 kernels of the Mann-Whitney U backend.
 //
 // What the reference does per gene column (src/cpp/mannwhitneyu.cpp:903-1043)
-// runs here as a pipeline of small kernels over the columns of a job:
+// runs here as one of two launch paths over the columns of a job:
 //
-//   split      count stored values per group, exclusive scan, scatter each
-//              group's nonzero values (as order-preserving keys, or as
-//              histogram bins for small-integer columns) into a per-column
-//              key range in HBM                       (north-star subsystem a)
+//   count      integer-valued columns (UMI counts): one workgroup reads the
+//              column from HBM once, keeps it in LDS, bins the reference
+//              slice into a histogram and ranks every target entry against
+//              its prefix, then writes U1, the tie term and the asymptotic
+//              p-value                           (north-star subsystems a, c-e)
+//
+//   split      every other column (and any the count kernel hands over):
+//              count stored values per group, exclusive scan, scatter each
+//              group's nonzero values as order-preserving keys (or histogram
+//              bins) into a per-column key range in HBM           (a)
 //   reference  sort the reference slice once per gene (register bitonic
 //              runs + LDS merges) and build its rank table -- a bucket index
-//              over the sorted keys, or the histogram prefix for integer
-//              counts -- kept in HBM for every target        (b, d)
+//              over the sorted keys, or the histogram prefix -- in HBM (b, d)
 //   target     load the column's rank table into LDS; each wave sorts one
-//              target slice in registers and ranks it against the table (c)
-//   pair       sample sizes and the FP64 asymptotic p-value per pair    (e)
+//              target slice in registers, ranks it against the table and
+//              writes U1, sizes and the asymptotic p          (c, e)
 //
 // Ranking never materialises a merge: a target value v needs only
 // lt(v) = #reference values below v and eq(v) = #reference values equal to v:
@@ -34,7 +39,8 @@
 // every value (min), above every value (max) or at 0 (mix, and none -- where
 // only explicit zeros count).
 //
-// Every per-column kernel uses 256-thread workgroups and at most 48 KB of LDS.
+// The split/reference/target kernels use 256-thread workgroups and at most
+// 48 KB of LDS; the count kernel 512 threads and a tuned LDS size (76 KB).
 #include <climits>
 
 #include "hpdex_device.h"
@@ -163,11 +169,14 @@ struct SplitState {
 
 template <typename V>
 __global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs,
-                                                          const WorkItem* items) {
+                                                          const WorkItem* items,
+                                                          const uint32_t* n_items) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ SplitState st;
+    // items past the queue's length (the grid is sized for its capacity)
+    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
     work_item(items, job, c);
@@ -381,11 +390,14 @@ struct RefState {
 
 template <typename V>
 __global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs,
-                                                        const WorkItem* items) {
+                                                        const WorkItem* items,
+                                                        const uint32_t* n_items) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ RefState st;
+    // items past the queue's length (the grid is sized for its capacity)
+    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
     work_item(items, job, c);
@@ -562,8 +574,12 @@ __device__ __forceinline__ void process_target(const JobDesc& J, int64_t c, uint
         const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
         const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
         const int64_t idx = c * J.T + (int64_t)(g - 1);
-        J.U1[idx] = (double)u1x2 * 0.5;
-        J.tie[idx] = J.tie_corr ? (double)tie : 0.0;
+        const double U1 = (double)u1x2 * 0.5;
+        const double td = J.tie_corr ? (double)tie : 0.0;
+        J.U1[idx] = U1;
+        if (J.tie) J.tie[idx] = td;
+        if (J.n1o) { J.n1o[idx] = (double)n1; J.n2o[idx] = (double)n2; }
+        if (J.asym) J.P[idx] = p_asymptotic(U1, (double)n1, (double)n2, td, J.cc, J.alt);
     }
 }
 
@@ -573,10 +589,13 @@ struct TargetState {
 
 template <typename V>
 __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
-                                                           const WorkItem* items) {
+                                                           const WorkItem* items,
+                                                           const uint32_t* n_items) {
     using K = typename KeyOf<V>::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TargetState st;
+    // items past the queue's length (the grid is sized for its capacity)
+    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
     work_item(items, job, c);
@@ -616,19 +635,297 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
 }
 
 // ---------------------------------------------------------------------------
-// pair: sample sizes and, for the asymptotic method, the FP64 p-value of
-// every (gene, target) pair (mannwhitneyu.cpp:191-240).
+// count: the whole pass of an integer-valued column in one workgroup (the
+// UMI-count case of mannwhitneyu.cpp:903-1043).
+//
+// The column is read from HBM once.  Each stored entry becomes one LDS word at
+// its CSC position: (group << 16) | (value & 0xFFFF) for an in-group nonzero
+// finite integer in [-32768, 32767], kEmptySlot otherwise.  The reference
+// values are binned into a histogram whose exclusive prefix gives lt/eq of any
+// value.  Target entries are then ranked one by one: j, the 1-based position
+// of an entry inside the run of equal values of its target, is taken from an
+// LDS counter per (target, value).  The order in which entries draw their j
+// does not matter -- a run of b equal values adds
+// sum_{j=1..b} 3(eq+j)(eq+j-1) whichever entry draws which j -- so no sort is
+// needed.  The counters are dense over a window of VS values; a wider value
+// range takes several windows.  A column too long to stage is re-read from
+// HBM (it is in L2 or the MALL by then) instead of from LDS.  Columns that do
+// not fit the kernel (a non-integer or out-of-range value, a value range over
+// the histogram limit, too many windows) are appended to the job's overflow
+// queue for the split/reference/target kernels.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void pair_kernel(JobDesc J) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= J.C * J.T) return;
-    const int64_t c = i / J.T;
-    const uint32_t g = (uint32_t)(i - c * J.T) + 1u;
-    uint32_t za, zb, n1, n2;
-    pair_sizes(J, c, g, za, zb, n1, n2);
-    J.n1o[i] = (double)n1;
-    J.n2o[i] = (double)n2;
-    if (J.asym) J.P[i] = p_asymptotic(J.U1[i], (double)n1, (double)n2, J.tie[i], J.cc, J.alt);
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
+
+struct CountState {
+    uint32_t all_int;
+    int32_t vmin, vmax;
+    uint32_t pad;
+    uint32_t scan[20];
+    uint64_t red[16];
+};
+
+template <typename V>
+__device__ __forceinline__ uint32_t slot_word(uint32_t g, V v, uint32_t G) {
+    using KT = KeyOf<V>;
+    if (g >= G || !KT::finite(v) || v == V(0)) return kEmptySlot;
+    int iv;
+    if (!KT::small_int(v, iv) || iv < -32768 || iv > 32767) return kEmptySlot;
+    return (g << 16) | ((uint32_t)iv & 0xFFFFu);
+}
+
+__device__ __forceinline__ int32_t slot_value(uint32_t w) {
+    return (int32_t)(int16_t)(uint16_t)(w & 0xFFFFu);
+}
+
+// Visit the LDS word of every stored entry of [p0, p1): from the staged copy,
+// or recomputed from HBM when the column was too long to stage.
+template <typename V, typename F>
+__device__ __forceinline__ void for_each_word(bool staged, const uint32_t* stage,
+                                              const JobDesc& J, uint32_t G, int64_t p0,
+                                              int64_t p1, F&& f) {
+    const int tid = tid_x(), nt = blockDim.x;
+    if (staged) {
+        const uint32_t n = (uint32_t)(p1 - p0);
+        for (uint32_t x = tid; x < n; x += nt) f(stage[x]);
+    } else {
+        for_each_entry<V>(J.rows, (const V*)J.vals, J.gid, (uint32_t)J.R, G, p0, p1,
+                          [&](uint32_t g, V v) { f(slot_word<V>(g, v, G)); });
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
+                                                            const WorkItem* items,
+                                                            WorkItem* overflow,
+                                                            uint32_t* n_overflow,
+                                                            uint32_t lds_bytes) {
+    using KT = KeyOf<V>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ CountState st;
+    int job;
+    int64_t c;
+    work_item(items, job, c);
+    const JobDesc J = jobs[job];
+    const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
+    const uint32_t G = (uint32_t)J.G;
+    const uint32_t R = (uint32_t)J.R;
+    const V* vals = (const V*)J.vals;
+    const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
+    const uint32_t n = (uint32_t)(p1 - p0);
+    // LDS: acc[2G] u64 | cnz[G] | cez[G] | staged words | histogram | counters
+    uint64_t* acc = (uint64_t*)smem;   // [2g] 2*U_target part, [2g+1] tie part
+    uint32_t* cnz = (uint32_t*)(acc + 2u * G);
+    uint32_t* cez = cnz + G;
+    uint32_t* rest = cez + G;
+    const uint32_t rest_words = (lds_bytes - 24u * G) / 4u;
+    const bool staged = (uint64_t)n + kCountReserveWords <= (uint64_t)rest_words;
+    uint32_t* stage = rest;
+
+    for (uint32_t x = tid; x < 2u * G; x += nt) acc[x] = 0ull;
+    for (uint32_t g = tid; g < G; g += nt) { cnz[g] = 0u; cez[g] = 0u; }
+    if (tid == 0) { st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; }
+    __syncthreads();
+
+    // the one read of the column from HBM: counts, value range, staging
+    {
+        bool li = true;
+        int lmin = INT_MAX, lmax = INT_MIN;
+        for (int64_t pb = p0; pb < p1; pb += (int64_t)nt * kUnroll) {
+            uint32_t r[kUnroll];
+            V v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int64_t p = pb + (int64_t)u * nt + tid;
+                if (p < p1) { r[u] = (uint32_t)J.rows[p]; v[u] = vals[p]; }
+                else { r[u] = 0xFFFFFFFFu; v[u] = V(0); }
+            }
+            uint32_t g[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) g[u] = r[u] < R ? (uint32_t)J.gid[r[u]] : 0xFFFFFFFFu;
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int64_t p = pb + (int64_t)u * nt + tid;
+                uint32_t w = kEmptySlot;
+                if (g[u] < G && KT::finite(v[u])) {
+                    if (v[u] == V(0)) {
+                        atomicAdd(&cez[g[u]], 1u);
+                    } else {
+                        atomicAdd(&cnz[g[u]], 1u);
+                        int iv;
+                        if (KT::small_int(v[u], iv) && iv >= -32768 && iv <= 32767) {
+                            lmin = min(lmin, iv);
+                            lmax = max(lmax, iv);
+                            w = (g[u] << 16) | ((uint32_t)iv & 0xFFFFu);
+                        } else {
+                            li = false;
+                        }
+                    }
+                }
+                if (staged && p < p1) stage[(uint32_t)(p - p0)] = w;
+            }
+        }
+        const bool wall = __all(li ? 1 : 0);
+        lmin = wave_min_i(lmin);
+        lmax = wave_max_i(lmax);
+        if (lane == 0) {
+            if (!wall) atomicAnd(&st.all_int, 0u);
+            atomicMin(&st.vmin, lmin);
+            atomicMax(&st.vmax, lmax);
+        }
+    }
+    __syncthreads();
+
+    // layout of the rest of LDS, and whether this column fits the kernel
+    // (every value below is uniform: read once from LDS into SGPRs)
+    const uint32_t all_int = (uint32_t)__builtin_amdgcn_readfirstlane((int)st.all_int);
+    const int vlo = __builtin_amdgcn_readfirstlane(st.vmin);
+    const int vhi = __builtin_amdgcn_readfirstlane(st.vmax);
+    const bool anyv = vhi >= vlo;
+    const int32_t vmin = anyv ? vlo : 0;
+    const uint32_t NB = anyv ? (uint32_t)(vhi - vlo + 1) : 1u;   // histogram bins
+    const uint32_t stage_words = staged ? ((n + 3u) & ~3u) : 0u;
+    const uint32_t tab_words = (NB + 1u + 3u) & ~3u;
+    const uint32_t used = stage_words + tab_words;
+    const uint32_t cell_words = rest_words > used ? rest_words - used : 0u;
+    const uint32_t nT = G - 1u;
+    // 16-bit counters when staged (a column that fits LDS has < 2^16 entries)
+    const uint32_t cap_cells = staged ? 2u * cell_words : cell_words;
+    // the divisions run on the vector unit: take their (uniform) results back
+    // into SGPRs so the window loop and the hand-over branch stay scalar
+    uint32_t VS = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap_cells / nT));
+    if (VS > NB) VS = NB;
+    const uint32_t nwin =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(VS ? (NB + VS - 1u) / VS : 0u));
+    const int handover = __builtin_amdgcn_readfirstlane(
+        (!all_int || (int64_t)NB > (int64_t)J.hist_max || VS == 0u ||
+         nwin > (uint32_t)kMaxWindows) ? 1 : 0);
+    if (handover) {
+        if (tid == 0) {
+            const uint32_t k = atomicAdd(n_overflow, 1u);
+            overflow[k] = WorkItem{job, (int32_t)c};
+        }
+        return;
+    }
+
+    // errors of this column (mannwhitneyu.cpp:936-1038)
+    {
+        const uint64_t ebase = (uint64_t)c * (uint64_t)(2u * G + 1u);
+        for (uint32_t g = tid; g < G; g += nt) {
+            const uint32_t nz = cnz[g], ez = cez[g], gr = J.group_rows[g];
+            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
+            const uint32_t ns = (J.mode == kZeroNone) ? nz + ez : gr;
+            if (ns < 2u)
+                atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
+        }
+    }
+
+    // reference histogram -> exclusive prefix: tab[b] = #reference values
+    // below bin b, tab[NB] = #reference nonzero values
+    uint32_t* tab = rest + stage_words;
+    uint32_t* cells = tab + tab_words;
+    for (uint32_t b = tid; b <= NB; b += nt) tab[b] = 0u;
+    __syncthreads();
+    for_each_word<V>(staged, stage, J, G, p0, p1, [&](uint32_t w) {
+        if ((w >> 16) == 0u) atomicAdd(&tab[(uint32_t)(slot_value(w) - vmin)], 1u);
+    });
+    __syncthreads();
+    uint64_t ta = 0;
+    for (uint32_t b = tid; b < NB; b += nt) ta += tie_f(tab[b]);
+    ta = block_sum_u64(ta, st.red);
+    block_excl_scan(tab, (int)NB + 1, st.scan);
+
+    // rank the target entries, one value window at a time
+    const int32_t mode = J.mode;
+    const uint32_t gr0 = J.group_rows[0];
+    const uint32_t na = cnz[0];
+    const uint32_t za = (mode == kZeroNone) ? cez[0] : (gr0 > na ? gr0 - na : 0u);
+    const uint32_t ncw = staged ? (nT * VS + 1u) / 2u : nT * VS;
+    for (uint32_t w0 = 0; w0 < NB; w0 += VS) {
+        for (uint32_t x = tid; x < ncw; x += nt) cells[x] = 0u;
+        __syncthreads();
+        for_each_word<V>(staged, stage, J, G, p0, p1, [&](uint32_t w) {
+            const uint32_t g = w >> 16;
+            if (g == 0u || g >= G) return;
+            const uint32_t b = (uint32_t)(slot_value(w) - vmin);
+            const uint32_t d = b - w0;
+            if (d >= VS) return;
+            const uint32_t q = (g - 1u) * VS + d;
+            uint32_t j;
+            if (staged) {
+                const uint32_t sh = (q & 1u) << 4;
+                j = ((atomicAdd(&cells[q >> 1], 1u << sh) >> sh) & 0xFFFFu) + 1u;
+            } else {
+                j = atomicAdd(&cells[q], 1u) + 1u;
+            }
+            const uint32_t lt = tab[b], eq = tab[b + 1u] - lt;
+            const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
+            const uint64_t ltz = (uint64_t)lt + (zbelow ? za : 0u);
+            const uint64_t x = (uint64_t)eq + j;
+            atomicAdd((unsigned long long*)&acc[2u * g], (unsigned long long)(2ull * ltz + eq));
+            atomicAdd((unsigned long long*)&acc[2u * g + 1u],
+                      (unsigned long long)(3ull * x * (x - 1ull)));
+        });
+        __syncthreads();
+    }
+
+    // per target: zero block, U1, tie, sizes, asymptotic p
+    uint32_t ltz0;   // reference values below the zero block
+    if (mode == kZeroMin) ltz0 = 0u;
+    else if (mode == kZeroMax) ltz0 = na;
+    else ltz0 = vmin < 0 ? tab[min((uint32_t)(-vmin), NB)] : 0u;
+    for (uint32_t g = 1u + tid; g < G; g += nt) {
+        const uint32_t nzb = cnz[g];
+        uint32_t zb, n1, n2;
+        if (mode == kZeroNone) {
+            zb = cez[g];
+            n1 = na + za;
+            n2 = nzb + zb;
+        } else {
+            const uint32_t grg = J.group_rows[g];
+            zb = grg > nzb ? grg - nzb : 0u;
+            n1 = gr0;
+            n2 = grg;
+        }
+        const uint64_t S2 = acc[2u * g] + (uint64_t)zb * (2ull * ltz0 + za);
+        const uint64_t tie = acc[2u * g + 1u] + ta + tie_f((uint64_t)za + zb);
+        const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
+        const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
+        const int64_t idx = c * J.T + (int64_t)(g - 1u);
+        const double U1 = (double)u1x2 * 0.5;
+        const double td = J.tie_corr ? (double)tie : 0.0;
+        J.U1[idx] = U1;
+        if (J.tie) J.tie[idx] = td;
+        if (J.n1o) { J.n1o[idx] = (double)n1; J.n2o[idx] = (double)n2; }
+        if (J.asym) J.P[idx] = p_asymptotic(U1, (double)n1, (double)n2, td, J.cc, J.alt);
+    }
+}
+
+// Routing hint computed once per resident matrix: out[c] = 1 when every
+// stored value of column c is non-finite or an integer in [-32768, 32767].
+template <typename V>
+__global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
+                                                               const int64_t* indptr,
+                                                               uint8_t* out) {
+    __shared__ uint32_t ok;
+    const int64_t c = blockIdx.x;
+    if (threadIdx.x == 0) ok = 1u;
+    __syncthreads();
+    bool li = true;
+    for (int64_t p = indptr[c] + threadIdx.x; p < indptr[c + 1]; p += blockDim.x) {
+        const V v = vals[p];
+        int iv;
+        if (KeyOf<V>::finite(v) && !(KeyOf<V>::small_int(v, iv) && iv >= -32768 && iv <= 32767))
+            li = false;
+    }
+    if (!li) atomicAnd(&ok, 0u);
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = (uint8_t)ok;
+}
+
+// Start of every launch sequence: the overflow queue holds the host-routed
+// columns only.
+__global__ void reset_queue_kernel(uint32_t* n, uint32_t v) {
+    if (threadIdx.x == 0) *n = v;
 }
 
 __global__ void init_job_state_kernel(unsigned long long* err_key, uint32_t* status,
@@ -834,26 +1131,57 @@ static unsigned table_lds_bytes(bool f64) {
     return align16(hist > sort ? hist : sort);
 }
 
-hipError_t launch_mwu_pipeline(const JobDesc* jobs, const WorkItem* items, int n_items,
-                               const JobDesc& J0, bool f64, hipStream_t s) {
-    if (n_items <= 0) return hipSuccess;
-    size_t split = 12u * (size_t)J0.G;
+hipError_t prepare_count_kernel(uint32_t bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)count_kernel<float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)count_kernel<double>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
+    const int n_gen = L.n_gen_static + L.n_count;
+    if (n_gen <= 0) return hipSuccess;
+    auto mark = [&](int i) { if (ev) hipEventRecord(ev[i], s); };
+    reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
+    mark(0);
+    if (L.n_count > 0) {
+        if (L.f64)
+            count_kernel<double><<<L.n_count, kCountBlock, L.count_lds, s>>>(
+                L.jobs, L.count_items, L.gen_items, L.gen_n, L.count_lds);
+        else
+            count_kernel<float><<<L.n_count, kCountBlock, L.count_lds, s>>>(
+                L.jobs, L.count_items, L.gen_items, L.gen_n, L.count_lds);
+    }
+    mark(1);
+    size_t split = 12u * (size_t)L.G;
     if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
     const unsigned split_lds = align16(split);
-    const unsigned tab_lds = table_lds_bytes(f64);
-    if (f64) {
-        split_kernel<double><<<n_items, kColBlock, split_lds, s>>>(jobs, items);
-        ref_kernel<double><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
-        target_kernel<double><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+    const unsigned tab_lds = table_lds_bytes(L.f64);
+    if (L.f64) {
+        split_kernel<double><<<n_gen, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(2);
+        ref_kernel<double><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(3);
+        target_kernel<double><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
     } else {
-        split_kernel<float><<<n_items, kColBlock, split_lds, s>>>(jobs, items);
-        ref_kernel<float><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
-        target_kernel<float><<<n_items, kColBlock, tab_lds, s>>>(jobs, items);
+        split_kernel<float><<<n_gen, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(2);
+        ref_kernel<float><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(3);
+        target_kernel<float><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int64_t np = J0.C * (int64_t)J0.T;
-    if (np > 0) pair_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(J0);
+    mark(4);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
+                                   uint8_t* out, hipStream_t s) {
+    if (C <= 0) return hipSuccess;
+    if (f64)
+        classify_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, indptr, out);
+    else
+        classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, indptr, out);
     return hipGetLastError();
 }
 

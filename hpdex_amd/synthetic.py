@@ -189,4 +189,11 @@ def algorithmic_bytes(w: dict, value_bytes: int = 4) -> int:
     pass over the job must move (inputs once, U1 and P out)."""
     nnz = int(w["indptr"][-1])
     C, R, T = w["n_cols"], w["n_rows"], w["n_targets"]
-    return nnz * (value_bytes + 4) + (C + 1) * 8 + R * 4 + C * T * 16
+    return pass_bytes(nnz, C, R, T, value_bytes)
+
+
+def pass_bytes(nnz: int, C: int, R: int, T: int, value_bytes: int = 4) -> int:
+    """Algorithmic bytes of one pass over C columns holding nnz stored values:
+    values and row indices once, the column pointers, the group label of every
+    row, and U1 and P (8 bytes each) of every (column, target) pair."""
+    return int(nnz) * (value_bytes + 4) + (int(C) + 1) * 8 + int(R) * 4 + int(C) * int(T) * 16

@@ -99,9 +99,15 @@ void hpdex_matrix_destroy(hpdex_matrix* m);
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
 
 /* A job on a resident matrix: group labels and options fixed, outputs held on
- * the device.  form: 0 = plain (the kernel sequence launched eagerly over the
- * columns in order), 1 = batched (columns queued largest first, the whole
- * launch sequence captured once in a hipGraph and replayed).                */
+ * the device.  form: HPDEX_PLAN_PLAIN (the kernel sequence launched eagerly
+ * over the columns in order) or HPDEX_PLAN_BATCHED (columns queued largest
+ * first, the whole launch sequence captured once in a hipGraph and replayed),
+ * optionally or-ed with HPDEX_PLAN_STATS to keep the tie term and sample
+ * sizes of every pair for hpdex_plan_results (otherwise only U1 and P are
+ * written, which saves 24 bytes of HBM writes per pair).                    */
+#define HPDEX_PLAN_PLAIN 0
+#define HPDEX_PLAN_BATCHED 1
+#define HPDEX_PLAN_STATS 2
 int hpdex_plan_create(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
                       const hpdex_mwu_opts* opts, int form, hpdex_plan** out,
                       char* err, size_t err_len);
@@ -113,15 +119,37 @@ void hpdex_plan_destroy(hpdex_plan* p);
 int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms,
                    char* err, size_t err_len);
 /* Copy the last run's results to host buffers (n_cols*n_targets doubles each;
- * any pointer may be NULL). Returns the run's error, if it had one.          */
+ * any pointer may be NULL; tie_out, n1_out and n2_out need a plan created with
+ * HPDEX_PLAN_STATS). Returns the run's error, if it had one.                 */
 int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie_out,
                        double* n1_out, double* n2_out, char* err, size_t err_len);
+
+/* Where the plan's columns go: the one-pass count kernel (integer-valued
+ * columns) or the split/reference/target kernels; handed_over counts the
+ * count-kernel columns that the last run passed on to the split kernels
+ * (value range too wide for the on-chip histogram).                         */
+typedef struct hpdex_plan_info {
+    int64_t count_columns;
+    int64_t count_nnz;
+    int64_t split_columns;
+    int64_t handed_over;
+} hpdex_plan_info;
+int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len);
+/* Per-kernel timing: runs the plain launch sequence `iters` times with HIP
+ * events between its kernels (on the plan's stream) and writes the mean
+ * milliseconds of the count, split, reference and target kernels to ms[0..3]
+ * (a kernel with no columns times as an empty launch).  The results are
+ * those of a normal run.                                                    */
+int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size_t err_len);
 
 /* Tunable choices of the device path (set by the autotuner). */
 typedef struct hpdex_tuning {
     int32_t hist_max_bins;     /* widest integer value range a column may have to use
                                   the histogram path (0 = never, at most 8192)        */
     int32_t size_ordered;      /* batched form: 1 = queue the largest columns first  */
+    int32_t count_lds_bytes;   /* LDS per workgroup of the one-pass kernel for integer-
+                                  valued columns (0 = off: every column takes the
+                                  split/reference/target kernels; else 16 KiB..152 KiB) */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

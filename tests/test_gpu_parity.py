@@ -184,6 +184,66 @@ def test_exact_path(mode, alt):
     assert_parity(g, o)
 
 
+@pytest.fixture
+def restore_tuning():
+    t = _capi.get_tuning()
+    yield
+    _capi.set_tuning(t.hist_max_bins, t.size_ordered, t.count_lds_bytes)
+
+
+def _plan_of(w, mode, form=1, stats=True):
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=ZH[mode])
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    return m, _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=form, stats=stats)
+
+
+@pytest.mark.parametrize("lds", [0, 16 * 1024, 24 * 1024, 152 * 1024])
+def test_count_kernel_lds_sizes(lds, restore_tuning):
+    """The one-pass count kernel at several LDS sizes: 0 sends every column to
+    the split path; 16 KiB re-reads columns too long to stage and needs
+    several value windows; 152 KiB stages every column in one window."""
+    _capi.set_tuning(count_lds_bytes=lds)
+    w = _random_workload(11, 9000, 16, 40, 0.6,
+                         lambda r, n: r.negative_binomial(0.8, 0.1, n) - 2)
+    for mode in ("min", "max", "mix", "none"):
+        say(f"count kernel lds={lds} mode={mode}")
+        m, p = _plan_of(w, mode)
+        p.run(1)
+        info = p.info()
+        assert info["count_columns"] == (16 if lds else 0)
+        assert info["handed_over"] == 0
+        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+        p.close(); m.close()
+
+
+def test_count_kernel_hands_over_wide_ranges():
+    """Integer columns whose value range exceeds the histogram limit are handed
+    to the split path at run time; results are unchanged."""
+    say("count kernel hand-over")
+    w = _random_workload(12, 6000, 10, 6, 0.4, lambda r, n: r.integers(-20000, 20000, n))
+    m, p = _plan_of(w, "mix")
+    p.run(2)
+    info = p.info()
+    assert info["count_columns"] == 10 and info["handed_over"] == 10
+    assert_parity(p.results(with_stats=True), oracle_stats(w, "mix"))
+    p.close(); m.close()
+
+
+def test_plan_without_stats_keeps_u1_and_p_only():
+    say("plan without stats")
+    w = _random_workload(13, 3000, 6, 4, 0.3, lambda r, n: r.integers(1, 20, n))
+    m, p = _plan_of(w, "min", stats=False)
+    p.run(1)
+    U1, P = p.results()
+    o = oracle_stats(w, "min")
+    assert np.array_equal(U1, o[0]) and rel_close(P, o[1], rtol=P_RTOL)
+    with pytest.raises(_capi.HpdexError, match="HPDEX_PLAN_STATS"):
+        p.results(with_stats=True)
+    st = p.stage_times(2)
+    assert st["count"] > 0
+    p.close(); m.close()
+
+
 def test_plans_plain_batched_bit_identical():
     say("plans")
     w = _random_workload(8, 8000, 200, 12, 0.15, lambda r, n: r.negative_binomial(1, 0.3, n) + 1)
@@ -192,7 +252,7 @@ def test_plans_plain_batched_bit_identical():
                             with_stats=True)
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
     for form in (0, 1):
-        p = _capi.Plan(m, w["group_id"], 12, opts, form=form)
+        p = _capi.Plan(m, w["group_id"], 12, opts, form=form, stats=True)
         p.run(3)
         res = p.results(with_stats=True)
         for a, b in zip(res, oneshot):
@@ -210,11 +270,11 @@ def test_histogram_workload_full_size():
     opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
     say("histogram workload batched")
-    pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1)
+    pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1, stats=True)
     pb.run(2)
     U1, P, tie, n1, n2 = pb.results(with_stats=True)
     say("histogram workload plain")
-    pp = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=0)
+    pp = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=0, stats=True)
     pp.run(1)
     res_plain = pp.results(with_stats=True)
     for a, b in zip((U1, P, tie, n1, n2), res_plain):

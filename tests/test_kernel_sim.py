@@ -68,3 +68,69 @@ def test_sim_eight_byte_keys():
 def test_sim_histogram_path():
     d, r, ip, g = _rw(6, 4000, 4, 5, 0.3, lambda rg, n: rg.negative_binomial(0.8, 0.2, n) - 3)
     run_both(d, r, ip, g, 5, 4000, "mix")
+
+
+# ---- the count kernel (integer-valued columns in one pass) ----
+
+def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist_max=8192,
+              expect_counted=None):
+    U1, P, TIE, n1, n2, err, status = sim.mwu(data, indices, indptr, gid, T, R, mode=ZH[mode],
+                                              count_lds=count_lds, f64=f64, hist_max=hist_max,
+                                              timeout_s=120)
+    assert err == 2 ** 64 - 1 and status == 0
+    oU1, oP, oT, on1, on2 = oracle.mwu(data, indices, indptr, gid, T, n_rows=R,
+                                       zero_handling=mode, use_continuity=True, with_stats=True)
+    assert np.array_equal(U1, oU1)
+    assert np.array_equal(TIE, oT)
+    assert np.array_equal(n1, on1) and np.array_equal(n2, on2)
+    assert np.array_equal(P, oP)
+    if expect_counted is not None:
+        assert sim.mwu.last_counted == expect_counted
+
+
+@pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
+def test_sim_count_kernel_modes(mode):
+    # small signed integers with explicit zeros: staged columns, one window
+    d, r, ip, g = _rw(21, 3000, 5, 7, 0.4, lambda rg, n: rg.integers(-4, 9, n))
+    run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5)
+
+
+@pytest.mark.parametrize("lds", [16 * 1024, 24 * 1024, 152 * 1024])
+def test_sim_count_kernel_lds_sizes(lds):
+    # 16 KiB: columns too long to stage (~3,600 entries; re-read from memory,
+    # 32-bit counters) and several value windows; 24 KiB: staged, 16-bit
+    # counters, several windows; 152 KiB: staged, one window
+    d, r, ip, g = _rw(22, 6000, 4, 40, 0.6, lambda rg, n: rg.negative_binomial(0.8, 0.1, n) + 1)
+    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4)
+
+
+def test_sim_count_kernel_hands_over_wide_ranges():
+    # values span more than hist_max bins: every column goes to the split path
+    d, r, ip, g = _rw(23, 3000, 3, 5, 0.4, lambda rg, n: rg.integers(-20000, 20000, n))
+    run_count(d, r, ip, g, 5, 3000, "mix", 76 * 1024, expect_counted=0)
+    # a low histogram limit hands over a narrow range too
+    d, r, ip, g = _rw(24, 3000, 3, 5, 0.4, lambda rg, n: rg.integers(1, 300, n))
+    run_count(d, r, ip, g, 5, 3000, "min", 76 * 1024, hist_max=128, expect_counted=0)
+
+
+def test_sim_count_kernel_hands_over_too_many_windows():
+    # 300 targets in 16 KiB leave 5-value windows: a 1,000-value range needs 200
+    d, r, ip, g = _rw(25, 4000, 2, 300, 0.5, lambda rg, n: rg.integers(1, 1001, n))
+    run_count(d, r, ip, g, 300, 4000, "min", 16 * 1024, expect_counted=0)
+
+
+def test_sim_count_kernel_nonfinite_and_mixed_columns():
+    d, r, ip, g = _rw(26, 2500, 4, 4, 0.4, lambda rg, n: rg.integers(0, 6, n))
+    d = d.copy()
+    c0 = slice(ip[0], ip[1])
+    d[c0][::7] = np.nan          # column 0: NaNs are skipped (zero block outside none)
+    d[ip[1]:ip[2]][::11] = np.inf
+    d[ip[2]] = 0.5               # column 2: one non-integer -> split path
+    for mode in ("min", "none"):
+        run_count(d, r, ip, g, 4, 2500, mode, 76 * 1024, expect_counted=3)
+
+
+def test_sim_count_kernel_eight_byte_values():
+    d, r, ip, g = _rw(27, 2000, 3, 4, 0.4, lambda rg, n: rg.integers(-30, 200, n))
+    run_count(d.astype(np.float64), r, ip, g, 4, 2000, "mix", 76 * 1024, f64=True,
+              expect_counted=3)
