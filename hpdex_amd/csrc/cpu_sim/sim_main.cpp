@@ -75,8 +75,8 @@ bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void(
 // target kernels over the queue (host-routed columns plus hand-overs).
 // Returns the number of columns the count kernel finished itself.
 template <typename V>
-static int run_pipeline(const JobDesc& J, const std::vector<int32_t>& order, uint32_t count_lds,
-                        double timeout_s) {
+static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int32_t>& order,
+                        uint32_t count_lds, double timeout_s) {
     const JobDesc* jobs = &J;
     const int C = (int)J.C;
     std::vector<unsigned char> buf(std::max<size_t>(64 * 1024, count_lds));
@@ -85,13 +85,14 @@ static int run_pipeline(const JobDesc& J, const std::vector<int32_t>& order, uin
     std::vector<uint8_t> col_int(C, 0);
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
-                                                              col_int.data()); }, timeout_s);
+                                                              col_int.data(), range); }, timeout_s);
     const bool count_ok = count_lds > 0 && (uint32_t)J.G <= kCountMaxGroups &&
                           24u * (uint32_t)J.G + 4u * kCountReserveWords <= count_lds;
     std::vector<WorkItem> count_items, gen_items(C + 1);
     uint32_t n_static = 0;
     for (int32_t c : order) {
-        if (count_ok && col_int[c]) count_items.push_back(WorkItem{0, c});
+        const int64_t bins = (int64_t)range[2 * c + 1] - range[2 * c] + 1;
+        if (count_ok && col_int[c] && bins <= J.hist_max) count_items.push_back(WorkItem{0, c});
         else gen_items[n_static++] = WorkItem{0, c};
     }
     uint32_t gen_n = n_static;
@@ -140,8 +141,10 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
+    std::vector<int32_t> range(2 * (size_t)C + 2);
+    J.colrange = range.data();
     std::vector<int32_t> order(C);
     for (int c = 0; c < C; ++c) order[c] = C - 1 - c;   // reversed order
-    if (f64) return run_pipeline<double>(J, order, (uint32_t)count_lds, timeout_s);
-    return run_pipeline<float>(J, order, (uint32_t)count_lds, timeout_s);
+    if (f64) return run_pipeline<double>(J, range.data(), order, (uint32_t)count_lds, timeout_s);
+    return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, timeout_s);
 }

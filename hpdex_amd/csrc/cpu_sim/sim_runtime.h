@@ -32,6 +32,7 @@
 
 using hipError_t = int;
 using hipStream_t = void*;
+using hipEvent_t = void*;
 constexpr int hipSuccess = 0;
 constexpr int hipErrorInvalidValue = 1;
 struct uint4 { unsigned x, y, z, w; };
@@ -100,6 +101,7 @@ inline unsigned long long __ballot(int p) {
     return m;
 }
 inline int __all(int p) { return __ballot(p) == ~0ull; }
+inline int __any(int p) { return __ballot(p) != 0ull; }
 inline int __builtin_amdgcn_readfirstlane(int v) {
     sim::t_last_op = "readfirstlane";
     const int first = sim::exchange(v, 0u);

@@ -145,10 +145,12 @@ struct hpdex_matrix {
     bool f64 = false;
     size_t R = 0, C = 0, nnz = 0;
     DevBuf vals, rows, indptr;
+    DevBuf colrange;                // 2C int32: value range of integer-valued columns
     std::vector<int64_t> h_indptr;
     std::vector<int32_t> order;     // columns, largest first
     std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
-                                    // non-finite) -- the column is sent to the count kernel
+                                    // non-finite) -- the column may go to the count kernel
+    std::vector<int32_t> h_range;   // host copy of colrange
 };
 
 namespace {
@@ -188,13 +190,19 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     }
     HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (C + 1) * 8, hipMemcpyHostToDevice, s));
     m->h_indptr.assign(indptr, indptr + C + 1);
+    // column metadata: integer-valued columns and their value ranges, which
+    // let the count kernel bin values during its one read of a column
     m->col_int.assign(C, 0);
+    m->h_range.assign(2 * C, 0);
+    HIPCHK(m->colrange.alloc(2 * C * 4));
     if (C) {
         DevBuf flags;
         HIPCHK(flags.alloc(C));
         HIPCHK(launch_classify_columns(m->vals.p, m->f64, m->indptr.as<int64_t>(), (int64_t)C,
-                                       flags.as<uint8_t>(), s));
+                                       flags.as<uint8_t>(), m->colrange.as<int32_t>(), s));
         HIPCHK(hipMemcpyAsync(m->col_int.data(), flags.p, C, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(m->h_range.data(), m->colrange.p, 2 * C * 4, hipMemcpyDeviceToHost,
+                              s));
         HIPCHK(hipStreamSynchronize(s));
     }
     m->order.resize(C);
@@ -277,6 +285,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.indptr = m->indptr.as<int64_t>();
     J.gid = js.gid.as<int32_t>();
     J.group_rows = js.grows.as<uint32_t>();
+    J.colrange = m->colrange.as<int32_t>();
     J.keys = js.keys.p;
     J.cnt = js.cnt.as<uint32_t>();
     J.zcnt = js.zcnt.as<uint32_t>();
@@ -320,7 +329,8 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     for (size_t k = 0; k < m.C; ++k) {
         const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
         const int64_t len = m.h_indptr[c + 1] - m.h_indptr[c];
-        if (count_ok && m.col_int[c] && len < ((int64_t)1 << 31)) {
+        const int64_t bins = (int64_t)m.h_range[2 * c + 1] - m.h_range[2 * c] + 1;
+        if (count_ok && m.col_int[c] && bins <= tu.hist_max && len < ((int64_t)1 << 31)) {
             count_items.push_back(WorkItem{0, c});
             js.count_nnz += len;
         } else
@@ -682,7 +692,7 @@ void hpdex_matrix_destroy(hpdex_matrix* m) { delete m; }
 
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
     if (!m) return 0;
-    return m->vals.n + m->rows.n + m->indptr.n;
+    return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n;
 }
 
 }  // extern "C"

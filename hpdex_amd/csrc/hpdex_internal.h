@@ -54,6 +54,7 @@ struct JobDesc {
     const int64_t* indptr;       // C+1
     const int32_t* gid;          // R group labels
     const uint32_t* group_rows;  // G rows per group
+    const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
     // workspace
     void* keys;                  // nnz keys (uint32 for float values, uint64 for double)
     uint32_t* cnt;               // C*G stored nonzero finite values per (column, group)
@@ -136,10 +137,11 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,
                                hipEvent_t* ev = nullptr);
 // Allow the count kernel `bytes` of dynamic LDS (once per process).
 hipError_t prepare_count_kernel(uint32_t bytes);
-// Per-column routing hint of a resident matrix: out[c] = 1 when every stored
-// value of column c is non-finite or an integer in [-32768, 32767].
+// Column metadata of a resident matrix: ok[c] = 1 when every stored value of
+// column c is non-finite or an integer in [-32768, 32767]; range[2c..2c+1] =
+// smallest and largest stored nonzero finite value of such a column.
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
-                                   uint8_t* out, hipStream_t s);
+                                   uint8_t* ok, int32_t* range, hipStream_t s);
 hipError_t launch_init_job_state(unsigned long long* err_key, uint32_t* status,
                                  unsigned long long* status_col, hipStream_t s);
 hipError_t launch_exact_tables(const ExactTable* tabs, int n_tabs, double* tabbuf,

@@ -136,6 +136,16 @@ __device__ inline void work_item(const WorkItem* items, int& job, int64_t& col) 
     col = (int64_t)__builtin_amdgcn_readfirstlane(items[blockIdx.x].col);
 }
 
+// A workgroup-uniform value held in an SGPR.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = uniform_u32((uint32_t)u), hi = uniform_u32((uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Zero-block sizes and sample sizes of pair (reference, g) in column c
 // (mannwhitneyu.cpp:936-1037): none counts stored finite values only; the
 // other modes count every group row, absent and non-finite entries as zeros.
@@ -638,28 +648,30 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
 // count: the whole pass of an integer-valued column in one workgroup (the
 // UMI-count case of mannwhitneyu.cpp:903-1043).
 //
-// The column is read from HBM once.  Each stored entry becomes one LDS word at
-// its CSC position: (group << 16) | (value & 0xFFFF) for an in-group nonzero
-// finite integer in [-32768, 32767], kEmptySlot otherwise.  The reference
-// values are binned into a histogram whose exclusive prefix gives lt/eq of any
-// value.  Target entries are then ranked one by one: j, the 1-based position
-// of an entry inside the run of equal values of its target, is taken from an
-// LDS counter per (target, value).  The order in which entries draw their j
-// does not matter -- a run of b equal values adds
-// sum_{j=1..b} 3(eq+j)(eq+j-1) whichever entry draws which j -- so no sort is
-// needed.  The counters are dense over a window of VS values; a wider value
-// range takes several windows.  A column too long to stage is re-read from
-// HBM (it is in L2 or the MALL by then) instead of from LDS.  Columns that do
-// not fit the kernel (a non-integer or out-of-range value, a value range over
-// the histogram limit, too many windows) are appended to the job's overflow
-// queue for the split/reference/target kernels.
+// The resident matrix carries each column's value range [vmin, vmax] (stored
+// nonzero finite values), so histogram bins b = value - vmin are known before
+// the column is read.  One read of the column from HBM then does everything
+// with one LDS atomic per stored entry: a reference value increments its bin
+// of the reference histogram, a target value the counter of its (target,
+// bin) pair.  Afterwards, with lt(b)/eq(b) from the histogram's exclusive
+// prefix and b_gv from the counters, each target's sums follow per bin:
+//
+//   2*U_target(g) += b_gv * (2*lt(v) + eq(v))      (zero block added below)
+//   tie(g)        += f(eq(v) + b_gv) - f(eq(v))      f(t) = t^3 - t
+//
+// so no entry needs to know its position inside a run of equal values.  When
+// the counters of every (target, bin) do not fit beside the histogram, the
+// bins are taken in windows of VS: the first window during the read, the rest
+// in further passes over a staged copy of the column in LDS (or over HBM
+// again when the column is too long to stage).  Columns that do not fit at
+// all (more windows than kMaxWindows, values outside the recorded range) are
+// appended to the overflow queue of the split/reference/target kernels.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 
 struct CountState {
-    uint32_t all_int;
-    int32_t vmin, vmax;
-    uint32_t pad;
+    uint32_t bad;
+    uint32_t pad[3];
     uint32_t scan[20];
     uint64_t red[16];
 };
@@ -693,6 +705,25 @@ __device__ __forceinline__ void for_each_word(bool staged, const uint32_t* stage
     }
 }
 
+// Counters of (target, bin) pairs: 16-bit halves of LDS words when no count
+// can reach 2^16 (a column of fewer than 2^16 entries), else whole words.
+// Rows are `rs` counters apart, with an odd word stride so that the per-target
+// epilogue (one thread per row) reads without bank conflicts.
+struct PairCounters {
+    uint32_t* w;
+    uint32_t rs;
+    bool wide;
+    __device__ __forceinline__ void inc(uint32_t g, uint32_t d) const {
+        const uint32_t q = (g - 1u) * rs + d;
+        if (wide) atomicAdd(&w[q], 1u);
+        else atomicAdd(&w[q >> 1], 1u << ((q & 1u) << 4));
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t g, uint32_t d) const {
+        const uint32_t q = (g - 1u) * rs + d;
+        return wide ? w[q] : ((w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu);
+    }
+};
+
 template <typename V>
 __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
                                                             const WorkItem* items,
@@ -706,30 +737,85 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     int64_t c;
     work_item(items, job, c);
     const JobDesc J = jobs[job];
-    const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
-    const uint32_t G = (uint32_t)J.G;
-    const uint32_t R = (uint32_t)J.R;
+    const int tid = tid_x(), nt = blockDim.x;
+    // every value that steers control flow around a barrier is read into an
+    // SGPR: the kernel writes global memory, so these loads may be vector
+    // loads, and a branch on a VGPR would put the barriers under exec masks
+    const uint32_t G = uniform_u32((uint32_t)J.G), nT = G - 1u;
+    const uint32_t R = uniform_u32((uint32_t)J.R);
     const V* vals = (const V*)J.vals;
-    const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
+    const int64_t p0 = uniform_i64(J.indptr[c]), p1 = uniform_i64(J.indptr[c + 1]);
     const uint32_t n = (uint32_t)(p1 - p0);
-    // LDS: acc[2G] u64 | cnz[G] | cez[G] | staged words | histogram | counters
-    uint64_t* acc = (uint64_t*)smem;   // [2g] 2*U_target part, [2g+1] tie part
-    uint32_t* cnz = (uint32_t*)(acc + 2u * G);
-    uint32_t* cez = cnz + G;
-    uint32_t* rest = cez + G;
-    const uint32_t rest_words = (lds_bytes - 24u * G) / 4u;
-    const bool staged = (uint64_t)n + kCountReserveWords <= (uint64_t)rest_words;
-    uint32_t* stage = rest;
+    const int32_t vmin0 = (int32_t)uniform_u32((uint32_t)J.colrange[2 * c]);
+    const int32_t vmax0 = (int32_t)uniform_u32((uint32_t)J.colrange[2 * c + 1]);
+    const bool anyv = vmax0 >= vmin0;
+    const int32_t vmin = anyv ? vmin0 : 0;
+    const uint32_t NB = anyv ? (uint32_t)(vmax0 - vmin0 + 1) : 1u;   // histogram bins
+
+    // LDS: acc[2G] u64 | cez[G] | cnz[G] | tab[NB+1] | counters | staged words
+    // (every size below is uniform and kept in SGPRs; divisions run on the
+    // vector unit, so their results are read back with readfirstlane)
+    uint64_t* acc = (uint64_t*)smem;   // per target: [2g] 2*U_target part, [2g+1] tie part
+    uint32_t* cez = (uint32_t*)(acc + 2u * G);   // explicit zeros per group
+    uint32_t* cnz = cez + G;                     // nonzero values per group
+    uint32_t* tab = cnz + G;
+    const uint32_t tab_words = (NB + 1u + 3u) & ~3u;
+    const uint32_t avail = (lds_bytes - 24u * G) / 4u;
+    const uint32_t free0 = avail > tab_words ? avail - tab_words : 0u;
+    const bool wide = n >= 65536u;
+    const uint32_t per_word = wide ? 1u : 2u;
+    // row stride in counters: an odd number of words per row
+    auto row_stride = [&](uint32_t vs) -> uint32_t {
+        if (wide) return vs | 1u;
+        uint32_t r = (vs + 1u) & ~1u;
+        if (((r >> 1) & 1u) == 0u) r += 2u;
+        return r;
+    };
+    // (sizes below stay far under 2^32: nT < 2^16, rs <= 8194, free0 < 2^16)
+    const uint32_t cap1 = free0 * per_word;   // counters that fit without staging
+    const bool single =
+        __builtin_amdgcn_readfirstlane(NB <= 8192u && nT * row_stride(NB) <= cap1 ? 1 : 0) != 0;
+    const uint32_t stage_words = (n + 3u) & ~3u;
+    const bool staged = !single && __builtin_amdgcn_readfirstlane(
+        (uint64_t)stage_words + kCountReserveWords <= (uint64_t)free0 ? 1 : 0) != 0;
+    const uint32_t cap = (staged ? free0 - stage_words : free0) * per_word;
+    uint32_t VS = NB;
+    if (!single) {
+        VS = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap / nT));
+        if (VS > NB) VS = NB;
+        while (VS > 0u && nT * row_stride(VS) > cap) --VS;
+    }
+    const uint32_t rs = (uint32_t)__builtin_amdgcn_readfirstlane((int)row_stride(VS));
+    const uint32_t nwin =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(VS ? (NB + VS - 1u) / VS : 0u));
+    const int handover = __builtin_amdgcn_readfirstlane(
+        ((int64_t)NB > (int64_t)uniform_u32((uint32_t)J.hist_max) || free0 == 0u || VS == 0u ||
+         nwin > (uint32_t)kMaxWindows) ? 1 : 0);
+    if (handover) {
+        if (tid == 0) {
+            const uint32_t k = atomicAdd(n_overflow, 1u);
+            overflow[k] = WorkItem{job, (int32_t)c};
+        }
+        return;
+    }
+    PairCounters pc;
+    pc.w = tab + tab_words;
+    pc.rs = rs;
+    pc.wide = wide;
+    const uint32_t cell_words = wide ? nT * rs : (nT * rs + 1u) / 2u;
+    uint32_t* stage = pc.w + ((cell_words + 3u) & ~3u);
 
     for (uint32_t x = tid; x < 2u * G; x += nt) acc[x] = 0ull;
-    for (uint32_t g = tid; g < G; g += nt) { cnz[g] = 0u; cez[g] = 0u; }
-    if (tid == 0) { st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; }
+    for (uint32_t g = tid; g < G; g += nt) { cez[g] = 0u; cnz[g] = 0u; }
+    for (uint32_t b = tid; b <= NB; b += nt) tab[b] = 0u;
+    for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
+    if (tid == 0) st.bad = 0u;
     __syncthreads();
 
-    // the one read of the column from HBM: counts, value range, staging
+    // the one read of the column from HBM: reference histogram, counters of
+    // the first window, explicit zeros, staging for later windows
     {
-        bool li = true;
-        int lmin = INT_MAX, lmax = INT_MIN;
+        bool bad = false;
         for (int64_t pb = p0; pb < p1; pb += (int64_t)nt * kUnroll) {
             uint32_t r[kUnroll];
             V v[kUnroll];
@@ -750,56 +836,26 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
                     if (v[u] == V(0)) {
                         atomicAdd(&cez[g[u]], 1u);
                     } else {
-                        atomicAdd(&cnz[g[u]], 1u);
                         int iv;
-                        if (KT::small_int(v[u], iv) && iv >= -32768 && iv <= 32767) {
-                            lmin = min(lmin, iv);
-                            lmax = max(lmax, iv);
-                            w = (g[u] << 16) | ((uint32_t)iv & 0xFFFFu);
+                        if (KT::small_int(v[u], iv) && iv >= vmin && iv <= vmax0) {
+                            const uint32_t b = (uint32_t)(iv - vmin);
+                            if (g[u] == 0u) atomicAdd(&tab[b], 1u);
+                            else if (b < VS) pc.inc(g[u], b);
+                            else w = (g[u] << 16) | ((uint32_t)iv & 0xFFFFu);
                         } else {
-                            li = false;
+                            bad = true;
                         }
                     }
                 }
                 if (staged && p < p1) stage[(uint32_t)(p - p0)] = w;
             }
         }
-        const bool wall = __all(li ? 1 : 0);
-        lmin = wave_min_i(lmin);
-        lmax = wave_max_i(lmax);
-        if (lane == 0) {
-            if (!wall) atomicAnd(&st.all_int, 0u);
-            atomicMin(&st.vmin, lmin);
-            atomicMax(&st.vmax, lmax);
-        }
+        if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
     }
     __syncthreads();
-
-    // layout of the rest of LDS, and whether this column fits the kernel
-    // (every value below is uniform: read once from LDS into SGPRs)
-    const uint32_t all_int = (uint32_t)__builtin_amdgcn_readfirstlane((int)st.all_int);
-    const int vlo = __builtin_amdgcn_readfirstlane(st.vmin);
-    const int vhi = __builtin_amdgcn_readfirstlane(st.vmax);
-    const bool anyv = vhi >= vlo;
-    const int32_t vmin = anyv ? vlo : 0;
-    const uint32_t NB = anyv ? (uint32_t)(vhi - vlo + 1) : 1u;   // histogram bins
-    const uint32_t stage_words = staged ? ((n + 3u) & ~3u) : 0u;
-    const uint32_t tab_words = (NB + 1u + 3u) & ~3u;
-    const uint32_t used = stage_words + tab_words;
-    const uint32_t cell_words = rest_words > used ? rest_words - used : 0u;
-    const uint32_t nT = G - 1u;
-    // 16-bit counters when staged (a column that fits LDS has < 2^16 entries)
-    const uint32_t cap_cells = staged ? 2u * cell_words : cell_words;
-    // the divisions run on the vector unit: take their (uniform) results back
-    // into SGPRs so the window loop and the hand-over branch stay scalar
-    uint32_t VS = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap_cells / nT));
-    if (VS > NB) VS = NB;
-    const uint32_t nwin =
-        (uint32_t)__builtin_amdgcn_readfirstlane((int)(VS ? (NB + VS - 1u) / VS : 0u));
-    const int handover = __builtin_amdgcn_readfirstlane(
-        (!all_int || (int64_t)NB > (int64_t)J.hist_max || VS == 0u ||
-         nwin > (uint32_t)kMaxWindows) ? 1 : 0);
-    if (handover) {
+    // a value outside the recorded range (the range no longer matches the
+    // data): leave the column to the split path
+    if (__builtin_amdgcn_readfirstlane((int)st.bad)) {
         if (tid == 0) {
             const uint32_t k = atomicAdd(n_overflow, 1u);
             overflow[k] = WorkItem{job, (int32_t)c};
@@ -807,65 +863,68 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
         return;
     }
 
-    // errors of this column (mannwhitneyu.cpp:936-1038)
-    {
-        const uint64_t ebase = (uint64_t)c * (uint64_t)(2u * G + 1u);
-        for (uint32_t g = tid; g < G; g += nt) {
-            const uint32_t nz = cnz[g], ez = cez[g], gr = J.group_rows[g];
-            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
-            const uint32_t ns = (J.mode == kZeroNone) ? nz + ez : gr;
-            if (ns < 2u)
-                atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
-        }
-    }
-
-    // reference histogram -> exclusive prefix: tab[b] = #reference values
-    // below bin b, tab[NB] = #reference nonzero values
-    uint32_t* tab = rest + stage_words;
-    uint32_t* cells = tab + tab_words;
-    for (uint32_t b = tid; b <= NB; b += nt) tab[b] = 0u;
-    __syncthreads();
-    for_each_word<V>(staged, stage, J, G, p0, p1, [&](uint32_t w) {
-        if ((w >> 16) == 0u) atomicAdd(&tab[(uint32_t)(slot_value(w) - vmin)], 1u);
-    });
-    __syncthreads();
+    // reference tie sum, exclusive prefix: tab[b] = #reference values below
+    // bin b, tab[NB] = #reference nonzero values
     uint64_t ta = 0;
     for (uint32_t b = tid; b < NB; b += nt) ta += tie_f(tab[b]);
     ta = block_sum_u64(ta, st.red);
     block_excl_scan(tab, (int)NB + 1, st.scan);
 
-    // rank the target entries, one value window at a time
-    const int32_t mode = J.mode;
+    const int32_t mode = (int32_t)uniform_u32((uint32_t)J.mode);
     const uint32_t gr0 = J.group_rows[0];
-    const uint32_t na = cnz[0];
+    const uint32_t na = tab[NB];
     const uint32_t za = (mode == kZeroNone) ? cez[0] : (gr0 > na ? gr0 - na : 0u);
-    const uint32_t ncw = staged ? (nT * VS + 1u) / 2u : nT * VS;
-    for (uint32_t w0 = 0; w0 < NB; w0 += VS) {
-        for (uint32_t x = tid; x < ncw; x += nt) cells[x] = 0u;
+
+    // per-target sums over the bins of one window [w0, w0 + VS)
+    auto window_sums = [&](uint32_t w0) {
+        const uint32_t wn = min(VS, NB - w0);
+        for (uint32_t g = 1u + tid; g < G; g += nt) {
+            uint64_t S = 0, T = 0;
+            uint32_t k = 0;
+            for (uint32_t d = 0; d < wn; ++d) {
+                const uint32_t bgv = pc.get(g, d);
+                if (bgv == 0u) continue;
+                const uint32_t b = w0 + d;
+                const uint32_t lt = tab[b], eq = tab[b + 1u] - lt;
+                const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
+                const uint64_t ltz = (uint64_t)lt + (zbelow ? za : 0u);
+                S += (uint64_t)bgv * (2ull * ltz + eq);
+                T += tie_f((uint64_t)eq + bgv) - tie_f(eq);
+                k += bgv;
+            }
+            acc[2u * g] += S;
+            acc[2u * g + 1u] += T;
+            cnz[g] += k;
+        }
+    };
+    window_sums(0u);
+    for (uint32_t w0 = VS; w0 < NB; w0 += VS) {
+        __syncthreads();
+        for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
         __syncthreads();
         for_each_word<V>(staged, stage, J, G, p0, p1, [&](uint32_t w) {
             const uint32_t g = w >> 16;
             if (g == 0u || g >= G) return;
-            const uint32_t b = (uint32_t)(slot_value(w) - vmin);
-            const uint32_t d = b - w0;
-            if (d >= VS) return;
-            const uint32_t q = (g - 1u) * VS + d;
-            uint32_t j;
-            if (staged) {
-                const uint32_t sh = (q & 1u) << 4;
-                j = ((atomicAdd(&cells[q >> 1], 1u << sh) >> sh) & 0xFFFFu) + 1u;
-            } else {
-                j = atomicAdd(&cells[q], 1u) + 1u;
-            }
-            const uint32_t lt = tab[b], eq = tab[b + 1u] - lt;
-            const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
-            const uint64_t ltz = (uint64_t)lt + (zbelow ? za : 0u);
-            const uint64_t x = (uint64_t)eq + j;
-            atomicAdd((unsigned long long*)&acc[2u * g], (unsigned long long)(2ull * ltz + eq));
-            atomicAdd((unsigned long long*)&acc[2u * g + 1u],
-                      (unsigned long long)(3ull * x * (x - 1ull)));
+            const uint32_t d = (uint32_t)(slot_value(w) - vmin) - w0;
+            if (d < VS) pc.inc(g, d);
         });
         __syncthreads();
+        window_sums(w0);
+    }
+    __syncthreads();
+
+    // errors of this column (mannwhitneyu.cpp:936-1038)
+    if (tid == 0) cnz[0] = na;
+    __syncthreads();
+    {
+        const uint64_t ebase = (uint64_t)c * (uint64_t)(2u * G + 1u);
+        for (uint32_t g = tid; g < G; g += nt) {
+            const uint32_t nz = cnz[g], ez = cez[g], gr = J.group_rows[g];
+            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
+            const uint32_t ns = (mode == kZeroNone) ? nz + ez : gr;
+            if (ns < 2u)
+                atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
+        }
     }
 
     // per target: zero block, U1, tie, sizes, asymptotic p
@@ -900,26 +959,41 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     }
 }
 
-// Routing hint computed once per resident matrix: out[c] = 1 when every
-// stored value of column c is non-finite or an integer in [-32768, 32767].
+// Metadata of a resident matrix, computed once at upload: ok[c] = 1 when every
+// stored value of column c is non-finite or an integer in [-32768, 32767];
+// range[2c], range[2c+1] = smallest and largest stored nonzero finite value
+// (range[2c] > range[2c+1] when there is none).
 template <typename V>
 __global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
                                                                const int64_t* indptr,
-                                                               uint8_t* out) {
+                                                               uint8_t* ok_out, int32_t* range) {
     __shared__ uint32_t ok;
+    __shared__ int32_t lo, hi;
     const int64_t c = blockIdx.x;
-    if (threadIdx.x == 0) ok = 1u;
+    if (threadIdx.x == 0) { ok = 1u; lo = INT_MAX; hi = INT_MIN; }
     __syncthreads();
     bool li = true;
+    int mn = INT_MAX, mx = INT_MIN;
     for (int64_t p = indptr[c] + threadIdx.x; p < indptr[c + 1]; p += blockDim.x) {
         const V v = vals[p];
+        if (!KeyOf<V>::finite(v) || v == V(0)) continue;
         int iv;
-        if (KeyOf<V>::finite(v) && !(KeyOf<V>::small_int(v, iv) && iv >= -32768 && iv <= 32767))
+        if (KeyOf<V>::small_int(v, iv) && iv >= -32768 && iv <= 32767) {
+            mn = min(mn, iv);
+            mx = max(mx, iv);
+        } else {
             li = false;
+        }
     }
     if (!li) atomicAnd(&ok, 0u);
+    atomicMin(&lo, mn);
+    atomicMax(&hi, mx);
     __syncthreads();
-    if (threadIdx.x == 0) out[c] = (uint8_t)ok;
+    if (threadIdx.x == 0) {
+        ok_out[c] = (uint8_t)ok;
+        range[2 * c] = ok ? lo : 1;
+        range[2 * c + 1] = ok ? hi : 0;
+    }
 }
 
 // Start of every launch sequence: the overflow queue holds the host-routed
@@ -1142,7 +1216,7 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
     const int n_gen = L.n_gen_static + L.n_count;
     if (n_gen <= 0) return hipSuccess;
-    auto mark = [&](int i) { if (ev) hipEventRecord(ev[i], s); };
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
     mark(0);
     if (L.n_count > 0) {
@@ -1176,12 +1250,14 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
 }
 
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
-                                   uint8_t* out, hipStream_t s) {
+                                   uint8_t* ok, int32_t* range, hipStream_t s) {
     if (C <= 0) return hipSuccess;
     if (f64)
-        classify_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, indptr, out);
+        classify_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, indptr,
+                                                                     ok, range);
     else
-        classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, indptr, out);
+        classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, indptr,
+                                                                    ok, range);
     return hipGetLastError();
 }
 

@@ -216,17 +216,46 @@ def test_count_kernel_lds_sizes(lds, restore_tuning):
         p.close(); m.close()
 
 
-def test_count_kernel_hands_over_wide_ranges():
-    """Integer columns whose value range exceeds the histogram limit are handed
-    to the split path at run time; results are unchanged."""
-    say("count kernel hand-over")
+def test_count_kernel_routes_wide_ranges_to_split():
+    """Integer columns whose value range exceeds the histogram limit go to the
+    split path when the plan is made; results are unchanged."""
+    say("count kernel routing")
     w = _random_workload(12, 6000, 10, 6, 0.4, lambda r, n: r.integers(-20000, 20000, n))
     m, p = _plan_of(w, "mix")
     p.run(2)
     info = p.info()
-    assert info["count_columns"] == 10 and info["handed_over"] == 10
+    assert info["count_columns"] == 0 and info["split_columns"] == 10
     assert_parity(p.results(with_stats=True), oracle_stats(w, "mix"))
     p.close(); m.close()
+
+
+def test_count_kernel_hands_over_at_run_time(restore_tuning):
+    """300 targets x 1,000 values in 16 KiB would need 200 value windows: the
+    count kernel hands the columns to the split path while running."""
+    say("count kernel hand-over")
+    _capi.set_tuning(count_lds_bytes=16 * 1024)
+    w = _random_workload(14, 6000, 4, 300, 0.5, lambda r, n: r.integers(1, 1001, n))
+    m, p = _plan_of(w, "min")
+    p.run(2)
+    info = p.info()
+    assert info["count_columns"] == 4 and info["handed_over"] == 4
+    assert_parity(p.results(with_stats=True), oracle_stats(w, "min"))
+    p.close(); m.close()
+
+
+@pytest.mark.parametrize("lds", [76 * 1024, 24 * 1024])
+def test_count_kernel_value_windows(lds, restore_tuning):
+    """200 targets x 400 values: several value windows over a staged column
+    (76 KiB) or over columns re-read from memory (24 KiB)."""
+    say(f"count kernel windows lds={lds}")
+    _capi.set_tuning(count_lds_bytes=lds)
+    w = _random_workload(15, 8000, 6, 200, 0.5, lambda r, n: r.integers(1, 401, n))
+    for mode in ("min", "mix"):
+        m, p = _plan_of(w, mode)
+        p.run(1)
+        assert p.info()["handed_over"] == 0
+        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+        p.close(); m.close()
 
 
 def test_plan_without_stats_keeps_u1_and_p_only():

@@ -134,3 +134,20 @@ def test_sim_count_kernel_eight_byte_values():
     d, r, ip, g = _rw(27, 2000, 3, 4, 0.4, lambda rg, n: rg.integers(-30, 200, n))
     run_count(d.astype(np.float64), r, ip, g, 4, 2000, "mix", 76 * 1024, f64=True,
               expect_counted=3)
+
+
+@pytest.mark.parametrize("lds", [76 * 1024, 24 * 1024])
+def test_sim_count_kernel_value_windows(lds):
+    # 200 targets x 400 values do not fit one window: 76 KiB stages the
+    # column and takes 3 windows; 24 KiB re-reads it from memory for 9
+    d, r, ip, g = _rw(28, 8000, 3, 200, 0.5, lambda rg, n: rg.integers(1, 401, n))
+    for mode in ("min", "mix"):
+        run_count(d, r, ip, g, 200, 8000, mode, lds, expect_counted=3)
+
+
+def test_sim_count_kernel_wide_counters():
+    # a column of more than 65,535 entries: 32-bit pair counters, and a value
+    # range that needs 20 windows re-read from memory
+    d, r, ip, g = _rw(29, 70000, 1, 20, 0.96, lambda rg, n: rg.integers(1, 3001, n))
+    assert ip[1] >= 65536
+    run_count(d, r, ip, g, 20, 70000, "min", 24 * 1024, expect_counted=1)
