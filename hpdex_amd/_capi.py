@@ -38,7 +38,8 @@ class MwuOpts(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes")]
+    _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes",
+                                              "count_block")]
 
 
 class PlanInfo(ctypes.Structure):
@@ -269,8 +270,10 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None):
+def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None):
     t = get_tuning()
+    if count_block is not None:
+        t.count_block = count_block
     if hist_max_bins is not None:
         t.hist_max_bins = hist_max_bins
     if size_ordered is not None:

@@ -25,6 +25,14 @@ def _device_text(path, cut_marker=None):
     s = s.replace("extern __shared__ __align__(16) unsigned char smem[];",
                   "unsigned char* smem = sim::g_lds;")
     s = re.sub(r'asm volatile\(""\s*:\s*"\+v"\((\w+)\)\);', r"(void)\1;", s)
+    # address spaces, clang vector types and non-temporal loads have no g++
+    # spelling: plain pointers, structs with the same members, plain loads
+    s = s.replace("#define HPDEX_GLOBAL __attribute__((address_space(1)))", "#define HPDEX_GLOBAL")
+    s = re.sub(r"typedef (\w+) (\w+)x4 __attribute__\(\(ext_vector_type\(4\)\)\);",
+               r"struct \2x4 { \1 x, y, z, w; };", s)
+    s = re.sub(r"typedef (\w+) (\w+)x2 __attribute__\(\(ext_vector_type\(2\)\)\);",
+               r"struct \2x2 { \1 x, y; };", s)
+    s = s.replace("__builtin_nontemporal_load(", "sim_plain_load(")
     # workgroup-static LDS objects live in a simulator buffer that is filled
     # with a garbage pattern before every workgroup, as device LDS is
     s = re.sub(r"__shared__ (\w+) (\w+);",

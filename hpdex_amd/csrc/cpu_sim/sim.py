@@ -35,8 +35,11 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
         count_lds=76 * 1024, block=512, hist_max=8192, f64=False, timeout_s=30.0):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels)."""
-    vals = np.ascontiguousarray(data, dtype=np.float64 if f64 else np.float32)
-    rows = np.where((indices >= 0) & (indices < n_rows), indices, -1).astype(np.int32)
+    # laid out like a resident matrix: 16 padding entries (value 0, row -1)
+    vals = np.concatenate([np.asarray(data, dtype=np.float64 if f64 else np.float32),
+                           np.zeros(16, dtype=np.float64 if f64 else np.float32)])
+    rows = np.concatenate([np.where((indices >= 0) & (indices < n_rows), indices, -1),
+                           np.full(16, -1)]).astype(np.int32)
     indptr = np.ascontiguousarray(indptr, dtype=np.int64)
     gid = np.ascontiguousarray(group_id, dtype=np.int32)
     C = len(indptr) - 1

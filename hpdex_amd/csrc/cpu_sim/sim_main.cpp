@@ -76,7 +76,7 @@ bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void(
 // Returns the number of columns the count kernel finished itself.
 template <typename V>
 static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int32_t>& order,
-                        uint32_t count_lds, double timeout_s) {
+                        uint32_t count_lds, unsigned count_block, double timeout_s) {
     const JobDesc* jobs = &J;
     const int C = (int)J.C;
     std::vector<unsigned char> buf(std::max<size_t>(64 * 1024, count_lds));
@@ -87,19 +87,25 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
                                                               col_int.data(), range); }, timeout_s);
     const bool count_ok = count_lds > 0 && (uint32_t)J.G <= kCountMaxGroups &&
-                          24u * (uint32_t)J.G + 4u * kCountReserveWords <= count_lds;
-    std::vector<WorkItem> count_items, gen_items(C + 1);
+                          4u * kCountDummyWords + 24u * (uint32_t)J.G + 4u * kCountReserveWords <= count_lds;
+    std::vector<int32_t> count_cols;
+    std::vector<WorkItem> gen_items(C + 1);
     uint32_t n_static = 0;
     for (int32_t c : order) {
         const int64_t bins = (int64_t)range[2 * c + 1] - range[2 * c] + 1;
-        if (count_ok && col_int[c] && bins <= J.hist_max) count_items.push_back(WorkItem{0, c});
+        if (count_ok && col_int[c] && bins <= J.hist_max) count_cols.push_back(c);
         else gen_items[n_static++] = WorkItem{0, c};
     }
     uint32_t gen_n = n_static;
-    const unsigned nc = (unsigned)count_items.size();
+    const unsigned nc = (unsigned)count_cols.size();
     for (unsigned b = 0; b < nc; ++b)
-        run_block(b, kCountBlock, nc, [&] {
-            count_kernel<V>(jobs, count_items.data(), gen_items.data(), &gen_n, count_lds);
+        run_block(b, count_block, nc, [&] {
+            if (count_block == 256)
+                count_kernel<V, 256>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
+            else if (count_block == 1024)
+                count_kernel<V, 1024>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
+            else
+                count_kernel<V, 512>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
     const unsigned ng = n_static + nc;
     const WorkItem* it = gen_items.data();
@@ -113,14 +119,14 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
 }
 
 // count_lds: dynamic LDS of the count kernel (0 = every column takes the
-// split/reference/target kernels).  Columns are visited in reversed order.
+// split/reference/target kernels); block: its workgroup size (256, 512, 1024).  Columns are visited in reversed order.
 // Returns the number of columns the count kernel finished itself.
 extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int64_t* indptr,
                        int R, int C, const int32_t* gid, int T, int mode, int alt, int tie_corr,
                        int cc, int count_lds, int block, int hist_max, double* U1, double* P,
                        double* tie, double* n1, double* n2, unsigned long long* err_key,
                        unsigned* status, double timeout_s) {
-    (void)block;
+    const unsigned count_block = (block == 256 || block == 1024) ? (unsigned)block : 512u;
     const int G = T + 1;
     std::vector<uint32_t> grows(G, 0u);
     for (int r = 0; r < R; ++r)
@@ -134,7 +140,11 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     std::vector<ColPlan> plan(C);
     std::vector<uint32_t> reftab((size_t)C * kTabStride);
     JobDesc J{};
+    std::vector<uint16_t> gid16(R + 1, (uint16_t)0xFFFF);
+    for (int r = 0; r < R; ++r)
+        gid16[r] = (gid[r] >= 0 && gid[r] < G) ? (uint16_t)gid[r] : (uint16_t)0xFFFF;
     J.vals = vals; J.rows = rows; J.indptr = indptr; J.gid = gid; J.group_rows = grows.data();
+    J.gid16 = gid16.data();
     J.keys = keys.data(); J.cnt = cnt.data(); J.zcnt = zcnt.data(); J.goff = goff.data();
     J.plan = plan.data(); J.reftab = reftab.data();
     J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;
@@ -145,6 +155,9 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.colrange = range.data();
     std::vector<int32_t> order(C);
     for (int c = 0; c < C; ++c) order[c] = C - 1 - c;   // reversed order
-    if (f64) return run_pipeline<double>(J, range.data(), order, (uint32_t)count_lds, timeout_s);
-    return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, timeout_s);
+    if (f64)
+        return run_pipeline<double>(J, range.data(), order, (uint32_t)count_lds, count_block,
+                                    timeout_s);
+    return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, count_block,
+                               timeout_s);
 }

@@ -102,6 +102,7 @@ inline unsigned long long __ballot(int p) {
 }
 inline int __all(int p) { return __ballot(p) == ~0ull; }
 inline int __any(int p) { return __ballot(p) != 0ull; }
+template <typename T> inline T sim_plain_load(const T* p) { return *p; }
 inline int __builtin_amdgcn_readfirstlane(int v) {
     sim::t_last_op = "readfirstlane";
     const int first = sim::exchange(v, 0u);

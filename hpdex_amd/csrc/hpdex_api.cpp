@@ -33,6 +33,7 @@ struct Tuning {
     int hist_max = kHistMaxBins;
     int size_ordered = 1;
     int count_lds = kCountLdsDefault;
+    int count_block = kCountBlockDefault;
 };
 
 std::mutex g_mu;
@@ -168,8 +169,12 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     m->C = C;
     m->nnz = nnz;
     const size_t vsz = m->f64 ? 8 : 4;
-    HIPCHK(m->vals.alloc(nnz * vsz));
-    HIPCHK(m->rows.alloc(nnz * 4));
+    // kMatrixPad extra entries (rows -1, values 0) let the count kernel read
+    // whole 16-byte groups of four entries past the last column's end
+    HIPCHK(m->vals.alloc((nnz + kMatrixPad) * vsz));
+    HIPCHK(m->rows.alloc((nnz + kMatrixPad) * 4));
+    HIPCHK(hipMemsetAsync((char*)m->vals.p + nnz * vsz, 0, kMatrixPad * vsz, s));
+    HIPCHK(hipMemsetAsync((char*)m->rows.p + nnz * 4, 0xFF, kMatrixPad * 4, s));
     HIPCHK(m->indptr.alloc((C + 1) * 8));
     if (nnz) {
         if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
@@ -220,13 +225,15 @@ struct JobState {
     bool stats = false;          // tie and sample sizes kept per pair
     hpdex_mwu_opts opts{};
     std::vector<uint32_t> group_rows;
-    DevBuf gid, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    DevBuf gid, gid16, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
     DevBuf d_job;                                   // descriptor
-    DevBuf d_count_items, d_gen_items, d_gen_n;     // column queues of the two paths
+    DevBuf d_count_cols, d_gen_items, d_gen_n;      // column queues of the two paths
+    JobDesc J{};                                    // host copy of the descriptor
     int n_count = 0, n_gen_static = 0;
     int64_t count_nnz = 0;
     uint32_t count_lds = 0;
+    int count_block = kCountBlockDefault;
     // exact null tables
     std::vector<ExactTable> tabs;
     std::vector<int32_t> tidx_by_g;
@@ -256,6 +263,18 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     const size_t np = js.C * T;
     HIPCHK(js.gid.alloc(m->R * 4));
     if (m->R) HIPCHK(hipMemcpyAsync(js.gid.p, group_id, m->R * 4, hipMemcpyHostToDevice, s));
+    // 16-bit labels for the count kernel's gathers (it runs only when G <= 65535)
+    if (js.G <= kCountMaxGroups) {
+        // one label past the last row (index R, no group) for rows out of range
+        std::vector<uint16_t> g16(m->R + 1, (uint16_t)0xFFFF);
+        for (size_t r = 0; r < m->R; ++r) {
+            const int g = group_id[r];
+            g16[r] = (g >= 0 && (size_t)g < js.G) ? (uint16_t)g : (uint16_t)0xFFFF;
+        }
+        HIPCHK(js.gid16.alloc((m->R + 1) * 2));
+        HIPCHK(hipMemcpyAsync(js.gid16.p, g16.data(), (m->R + 1) * 2, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));   // g16 is a host temporary
+    }
     HIPCHK(js.grows.alloc(js.G * 4));
     HIPCHK(hipMemcpyAsync(js.grows.p, js.group_rows.data(), js.G * 4, hipMemcpyHostToDevice, s));
     HIPCHK(js.U1.alloc(np * 8));
@@ -284,6 +303,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.rows = m->rows.as<int32_t>();
     J.indptr = m->indptr.as<int64_t>();
     J.gid = js.gid.as<int32_t>();
+    J.gid16 = js.gid16.as<uint16_t>();
     J.group_rows = js.grows.as<uint32_t>();
     J.colrange = m->colrange.as<int32_t>();
     J.keys = js.keys.p;
@@ -318,31 +338,34 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
 // each queue in column order (plain form) or largest first (batched form).
 int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const Tuning& tu,
                     bool size_ordered, hipStream_t s, char* err, size_t err_len) {
+    js.J = J;
     HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
     HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
     // the count kernel takes the job if its per-group LDS state leaves room
     const bool count_ok = tu.count_lds > 0 && js.G <= kCountMaxGroups &&
-                          24u * js.G + 4u * kCountReserveWords <= (size_t)tu.count_lds;
-    std::vector<WorkItem> count_items, gen_items;
-    count_items.reserve(m.C);
+                          4u * kCountDummyWords + 24u * js.G + 4u * kCountReserveWords <= (size_t)tu.count_lds;
+    std::vector<int32_t> count_cols;
+    std::vector<WorkItem> gen_items;
+    count_cols.reserve(m.C);
     gen_items.reserve(m.C);
     for (size_t k = 0; k < m.C; ++k) {
         const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
         const int64_t len = m.h_indptr[c + 1] - m.h_indptr[c];
         const int64_t bins = (int64_t)m.h_range[2 * c + 1] - m.h_range[2 * c] + 1;
         if (count_ok && m.col_int[c] && bins <= tu.hist_max && len < ((int64_t)1 << 31)) {
-            count_items.push_back(WorkItem{0, c});
+            count_cols.push_back(c);
             js.count_nnz += len;
         } else
             gen_items.push_back(WorkItem{0, c});
     }
-    js.n_count = (int)count_items.size();
+    js.n_count = (int)count_cols.size();
     js.n_gen_static = (int)gen_items.size();
     js.count_lds = (uint32_t)tu.count_lds;
-    HIPCHK(js.d_count_items.alloc(count_items.size() * sizeof(WorkItem)));
-    if (!count_items.empty())
-        HIPCHK(hipMemcpyAsync(js.d_count_items.p, count_items.data(),
-                              count_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    js.count_block = tu.count_block;
+    HIPCHK(js.d_count_cols.alloc(count_cols.size() * sizeof(int32_t)));
+    if (!count_cols.empty())
+        HIPCHK(hipMemcpyAsync(js.d_count_cols.p, count_cols.data(),
+                              count_cols.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     // room for every column: the count kernel appends those it hands over
     HIPCHK(js.d_gen_items.alloc(m.C * sizeof(WorkItem)));
     if (!gen_items.empty())
@@ -356,7 +379,8 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
 PipelineLaunch pipeline_of(const JobState& js, bool f64) {
     PipelineLaunch L{};
     L.jobs = js.d_job.as<JobDesc>();
-    L.count_items = js.d_count_items.as<WorkItem>();
+    L.job = js.J;
+    L.count_cols = js.d_count_cols.as<int32_t>();
     L.n_count = js.n_count;
     L.gen_items = js.d_gen_items.as<WorkItem>();
     L.gen_n = js.d_gen_n.as<uint32_t>();
@@ -364,6 +388,7 @@ PipelineLaunch pipeline_of(const JobState& js, bool f64) {
     L.G = (int)js.G;
     L.f64 = f64;
     L.count_lds = js.count_lds;
+    L.count_block = js.count_block;
     return L;
 }
 
@@ -566,6 +591,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     out->hist_max_bins = t.hist_max;
     out->size_ordered = t.size_ordered;
     out->count_lds_bytes = t.count_lds;
+    out->count_block = t.count_block;
     return 0;
 }
 
@@ -580,12 +606,15 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
          t->count_lds_bytes % 16 != 0))
         return set_err(err, err_len,
                        "count_lds_bytes must be 0 or a multiple of 16 in [16384, 155648]");
+    if (t->count_block != 256 && t->count_block != 512 && t->count_block != 1024)
+        return set_err(err, err_len, "count_block must be 256, 512 or 1024");
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init && !g_count_ready && t->count_lds_bytes != 0)
         return set_err(err, err_len, "this device does not give the count kernel its LDS");
     g_tune.hist_max = t->hist_max_bins;
     g_tune.size_ordered = t->size_ordered;
     g_tune.count_lds = t->count_lds_bytes;
+    g_tune.count_block = t->count_block;
     return 0;
 }
 

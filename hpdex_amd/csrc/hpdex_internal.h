@@ -26,12 +26,14 @@ constexpr int kRefKeysF64 = 4096;           // sort path: reference keys kept on
 constexpr int kTabStride = (kHistMaxBins > kMaxBuckets ? kHistMaxBins : kMaxBuckets) + 1;
 
 // Count kernel (integer-valued columns in one pass).
-constexpr int kCountBlock = 512;            // threads per workgroup (8 waves)
+constexpr int kCountBlockDefault = 512;     // threads per workgroup: 256, 512 or 1024
 constexpr int kCountLdsDefault = 76 * 1024; // dynamic LDS per workgroup: two per CU
 constexpr int kCountLdsMax = 152 * 1024;    // largest dynamic LDS the tuning accepts
 constexpr uint32_t kCountReserveWords = 1024;  // LDS words kept for tables beside a staged column
+constexpr uint32_t kCountDummyWords = 64;      // LDS words the read pass's no-op atomics hit
 constexpr int kMaxWindows = 64;             // value windows per column before handing it over
 constexpr uint32_t kCountMaxGroups = 65535; // group ids are packed in 16 bits
+constexpr size_t kMatrixPad = 16;           // entries past nnz in a resident matrix (rows -1)
 
 // Per-column plan: written by the split kernel (and TA by the reference-table
 // kernel), read by the later kernels of the same job.
@@ -53,6 +55,7 @@ struct JobDesc {
     const int32_t* rows;         // nnz row indices, -1 = out of range
     const int64_t* indptr;       // C+1
     const int32_t* gid;          // R group labels
+    const uint16_t* gid16;       // R+1 group labels as 16 bits, 0xFFFF = none; [R] = none
     const uint32_t* group_rows;  // G rows per group
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
     // workspace
@@ -115,14 +118,15 @@ hipError_t launch_rows_i64_to_i32(const int64_t* in, int32_t* out, size_t n, int
 hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n, hipStream_t s);
 bool dtype_uses_double(int dtype);
 
-// One launch sequence of the Mann-Whitney pass.  Integer-valued columns go to
-// the count kernel (count_items); the rest, and every column the count kernel
+// One launch sequence of the Mann-Whitney pass of one job.  Integer-valued
+// columns go to the count kernel (count_cols); the rest, and every column the count kernel
 // hands over at run time, to the split -> reference -> target kernels, which
 // read their items from gen_items[0 .. *gen_n).  gen_items holds the
 // n_gen_static host-routed columns first and has room for n_count more.
 struct PipelineLaunch {
-    const JobDesc* jobs;
-    const WorkItem* count_items;
+    const JobDesc* jobs;       // device copy (split/reference/target kernels)
+    JobDesc job;               // host copy, passed by value to the count kernel
+    const int32_t* count_cols; // columns of the count kernel (one job per launch)
     int n_count;
     WorkItem* gen_items;
     uint32_t* gen_n;
@@ -130,6 +134,7 @@ struct PipelineLaunch {
     int G;                 // largest group count over the jobs (LDS sizing)
     bool f64;
     uint32_t count_lds;    // dynamic LDS bytes of the count kernel
+    int count_block;       // its workgroup size, 256, 512 or 1024
 };
 // ev (optional, 5 events): recorded before the count kernel and after the
 // count, split, reference and target kernels, for per-kernel timing.

@@ -42,11 +42,20 @@
 // The split/reference/target kernels use 256-thread workgroups and at most
 // 48 KB of LDS; the count kernel 512 threads and a tuned LDS size (76 KB).
 #include <climits>
+#include <type_traits>
 
 #include "hpdex_device.h"
 #include "hpdex_internal.h"
 
 namespace hpdex {
+
+// Pointers taken from a JobDesc are generic; reading them as global-memory
+// pointers gives global_load/global_store instead of flat accesses, whose
+// completion the LDS counter (lgkmcnt) would otherwise also wait for.
+#define HPDEX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ HPDEX_GLOBAL T* gl(T* p) { return (HPDEX_GLOBAL T*)p; }
+
 
 constexpr int kChunkK = 8;            // keys per lane in a register sort
 constexpr int kChunk = 64 * kChunkK;  // longest run sorted in registers
@@ -69,12 +78,12 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int64_t p = pb + (int64_t)u * nt + tid;
-            if (p < p1) { r[u] = (uint32_t)rows[p]; v[u] = vals[p]; }
+            if (p < p1) { r[u] = (uint32_t)gl(rows)[p]; v[u] = gl(vals)[p]; }
             else { r[u] = 0xFFFFFFFFu; v[u] = V(0); }
         }
         uint32_t g[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) g[u] = r[u] < R ? (uint32_t)gid[r[u]] : 0xFFFFFFFFu;
+        for (int u = 0; u < kUnroll; ++u) g[u] = r[u] < R ? (uint32_t)gl(gid)[r[u]] : 0xFFFFFFFFu;
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
             if (g[u] < G) f(g[u], v[u]);
@@ -668,7 +677,6 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
 // appended to the overflow queue of the split/reference/target kernels.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
-
 struct CountState {
     uint32_t bad;
     uint32_t pad[3];
@@ -724,19 +732,17 @@ struct PairCounters {
     }
 };
 
-template <typename V>
-__global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
-                                                            const WorkItem* items,
-                                                            WorkItem* overflow,
-                                                            uint32_t* n_overflow,
-                                                            uint32_t lds_bytes) {
+// J arrives by value in the kernel arguments, so its fields are read with
+// scalar loads into SGPRs; cols lists the columns of this launch.
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_t* cols,
+                                                   WorkItem* overflow, uint32_t* n_overflow,
+                                                   uint32_t lds_bytes) {
     using KT = KeyOf<V>;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CountState st;
-    int job;
-    int64_t c;
-    work_item(items, job, c);
-    const JobDesc J = jobs[job];
+    const int job = 0;
+    const int64_t c = (int64_t)uniform_u32((uint32_t)cols[blockIdx.x]);
     const int tid = tid_x(), nt = blockDim.x;
     // every value that steers control flow around a barrier is read into an
     // SGPR: the kernel writes global memory, so these loads may be vector
@@ -752,15 +758,17 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     const int32_t vmin = anyv ? vmin0 : 0;
     const uint32_t NB = anyv ? (uint32_t)(vmax0 - vmin0 + 1) : 1u;   // histogram bins
 
-    // LDS: acc[2G] u64 | cez[G] | cnz[G] | tab[NB+1] | counters | staged words
-    // (every size below is uniform and kept in SGPRs; divisions run on the
-    // vector unit, so their results are read back with readfirstlane)
-    uint64_t* acc = (uint64_t*)smem;   // per target: [2g] 2*U_target part, [2g+1] tie part
+    // LDS: dummy[64] | acc[2G] u64 | cez[G] | cnz[G] | tab[NB+1] | counters |
+    // staged words (every size below is uniform and kept in SGPRs; divisions
+    // run on the vector unit, so their results are read back with readfirstlane)
+    uint32_t* lds_words = (uint32_t*)smem;
+    uint64_t* acc = (uint64_t*)(lds_words + kCountDummyWords);   // [2g] 2*U_target part,
+                                                                  // [2g+1] tie part
     uint32_t* cez = (uint32_t*)(acc + 2u * G);   // explicit zeros per group
     uint32_t* cnz = cez + G;                     // nonzero values per group
     uint32_t* tab = cnz + G;
     const uint32_t tab_words = (NB + 1u + 3u) & ~3u;
-    const uint32_t avail = (lds_bytes - 24u * G) / 4u;
+    const uint32_t avail = (lds_bytes - 4u * kCountDummyWords - 24u * G) / 4u;
     const uint32_t free0 = avail > tab_words ? avail - tab_words : 0u;
     const bool wide = n >= 65536u;
     const uint32_t per_word = wide ? 1u : 2u;
@@ -781,9 +789,10 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     const uint32_t cap = (staged ? free0 - stage_words : free0) * per_word;
     uint32_t VS = NB;
     if (!single) {
-        VS = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap / nT));
+        // row_stride(vs) <= vs + 3, so vs = cap/nT - 3 always fits
+        const uint32_t per_row = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap / nT));
+        VS = per_row > 3u ? per_row - 3u : 0u;
         if (VS > NB) VS = NB;
-        while (VS > 0u && nT * row_stride(VS) > cap) --VS;
     }
     const uint32_t rs = (uint32_t)__builtin_amdgcn_readfirstlane((int)row_stride(VS));
     const uint32_t nwin =
@@ -804,6 +813,11 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     pc.wide = wide;
     const uint32_t cell_words = wide ? nT * rs : (nT * rs + 1u) / 2u;
     uint32_t* stage = pc.w + ((cell_words + 3u) & ~3u);
+    // word offsets from the start of LDS, for the read pass's single atomic
+    const uint32_t cez_off = (uint32_t)(cez - lds_words);
+    const uint32_t tab_off = (uint32_t)(tab - lds_words);
+    const uint32_t cell_off = (uint32_t)(pc.w - lds_words);
+    const uint32_t dummy_off = 0u;
 
     for (uint32_t x = tid; x < 2u * G; x += nt) acc[x] = 0ull;
     for (uint32_t g = tid; g < G; g += nt) { cez[g] = 0u; cnz[g] = 0u; }
@@ -813,43 +827,74 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     __syncthreads();
 
     // the one read of the column from HBM: reference histogram, counters of
-    // the first window, explicit zeros, staging for later windows
+    // the first window, explicit zeros, staging for later windows.  Lane i of
+    // a wave takes entries i, i+64, ... so a wave's 64 entries are adjacent:
+    // their row indices are sorted, and the group labels they gather (16-bit
+    // labels; rows out of range read the sentinel at index R) fall into a few
+    // cache lines.  kUnroll rows and values per lane are in flight, read with
+    // a non-temporal hint (used once) at 32-bit offsets from the column start.
+    // Whole batches run without bounds checks; the last one checks.
     {
         bool bad = false;
-        for (int64_t pb = p0; pb < p1; pb += (int64_t)nt * kUnroll) {
+        const HPDEX_GLOBAL uint16_t* gid16 = gl(J.gid16);
+        const HPDEX_GLOBAL int32_t* rbase = gl(J.rows) + p0;
+        const HPDEX_GLOBAL V* vbase = gl(vals) + p0;
+        const uint32_t step = (uint32_t)nt * kUnroll;
+        auto batch = [&](uint32_t ib, auto full_tag) {
+            constexpr bool kFull = decltype(full_tag)::value;
             uint32_t r[kUnroll];
             V v[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const int64_t p = pb + (int64_t)u * nt + tid;
-                if (p < p1) { r[u] = (uint32_t)J.rows[p]; v[u] = vals[p]; }
-                else { r[u] = 0xFFFFFFFFu; v[u] = V(0); }
+                const uint32_t i = ib + (uint32_t)(u * nt + tid);
+                if (kFull || i < n) {
+                    r[u] = (uint32_t)__builtin_nontemporal_load(rbase + i);
+                    v[u] = __builtin_nontemporal_load(vbase + i);
+                } else {
+                    r[u] = 0xFFFFFFFFu;
+                    v[u] = V(0);
+                }
             }
             uint32_t g[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) g[u] = r[u] < R ? (uint32_t)J.gid[r[u]] : 0xFFFFFFFFu;
+            for (int u = 0; u < kUnroll; ++u) g[u] = (uint32_t)gid16[min(r[u], R)];
+            // one LDS atomic per entry, its word and increment chosen without
+            // branches: a reference value -> its histogram bin, a target value
+            // of the first window -> its (target, bin) counter, an explicit
+            // zero -> the group's zero count, anything else -> this lane's
+            // dummy word
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const int64_t p = pb + (int64_t)u * nt + tid;
-                uint32_t w = kEmptySlot;
-                if (g[u] < G && KT::finite(v[u])) {
-                    if (v[u] == V(0)) {
-                        atomicAdd(&cez[g[u]], 1u);
-                    } else {
-                        int iv;
-                        if (KT::small_int(v[u], iv) && iv >= vmin && iv <= vmax0) {
-                            const uint32_t b = (uint32_t)(iv - vmin);
-                            if (g[u] == 0u) atomicAdd(&tab[b], 1u);
-                            else if (b < VS) pc.inc(g[u], b);
-                            else w = (g[u] << 16) | ((uint32_t)iv & 0xFFFFu);
-                        } else {
-                            bad = true;
-                        }
-                    }
+                const uint32_t gg = g[u];
+                const V vv = v[u];
+                int iv = 0;
+                const bool isint = KT::small_int(vv, iv);
+                const bool zero = vv == V(0);
+                const bool ingroup = gg < G;
+                const bool inrange = isint && iv >= vmin && iv <= vmax0;
+                bad |= ingroup && !zero && KT::finite(vv) && !inrange;
+                const uint32_t b = (uint32_t)(iv - vmin);
+                const bool val = ingroup && !zero && inrange;
+                const uint32_t q = (gg - 1u) * rs + b;
+                const uint32_t cell_word = cell_off + (wide ? q : (q >> 1));
+                const uint32_t cell_inc = wide ? 1u : (1u << ((q & 1u) << 4));
+                uint32_t word = dummy_off + (uint32_t)lane_id();
+                uint32_t inc = 1u;
+                if (val && gg == 0u) word = tab_off + b;
+                if (val && gg != 0u && b < VS) { word = cell_word; inc = cell_inc; }
+                if (ingroup && zero) word = cez_off + gg;
+                atomicAdd(&lds_words[word], inc);
+                if (staged) {
+                    const uint32_t i = ib + (uint32_t)(u * nt + tid);
+                    if (kFull || i < n)
+                        stage[i] = (val && gg != 0u && b >= VS)
+                                       ? ((gg << 16) | ((uint32_t)iv & 0xFFFFu)) : kEmptySlot;
                 }
-                if (staged && p < p1) stage[(uint32_t)(p - p0)] = w;
             }
-        }
+        };
+        const uint32_t nfull = n - n % step;
+        for (uint32_t ib = 0; ib < nfull; ib += step) batch(ib, std::true_type{});
+        if (nfull < n) batch(nfull, std::false_type{});
         if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
     }
     __syncthreads();
@@ -875,22 +920,36 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
     const uint32_t na = tab[NB];
     const uint32_t za = (mode == kZeroNone) ? cez[0] : (gr0 > na ? gr0 - na : 0u);
 
-    // per-target sums over the bins of one window [w0, w0 + VS)
+    // per-target sums over the bins of one window [w0, w0 + VS), one thread
+    // per target and no branches per bin: a count of 0 adds 0 to each sum.
+    // With x = eq + c, f(x) - f(eq) = c * (x^2 + x*eq + eq^2 - 1); the sums are
+    // exact modulo 2^64 like any other order of the same integer additions.
     auto window_sums = [&](uint32_t w0) {
         const uint32_t wn = min(VS, NB - w0);
         for (uint32_t g = 1u + tid; g < G; g += nt) {
             uint64_t S = 0, T = 0;
             uint32_t k = 0;
-            for (uint32_t d = 0; d < wn; ++d) {
-                const uint32_t bgv = pc.get(g, d);
-                if (bgv == 0u) continue;
-                const uint32_t b = w0 + d;
+            auto bin = [&](uint32_t b, uint32_t c) {
                 const uint32_t lt = tab[b], eq = tab[b + 1u] - lt;
                 const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
-                const uint64_t ltz = (uint64_t)lt + (zbelow ? za : 0u);
-                S += (uint64_t)bgv * (2ull * ltz + eq);
-                T += tie_f((uint64_t)eq + bgv) - tie_f(eq);
-                k += bgv;
+                const uint64_t A = 2ull * ((uint64_t)lt + (zbelow ? za : 0u)) + eq;
+                const uint32_t x = eq + c;
+                const uint64_t P = (uint64_t)x * x + (uint64_t)x * eq + (uint64_t)eq * eq - 1ull;
+                S += (uint64_t)c * A;
+                T += (uint64_t)c * (uint32_t)P + ((uint64_t)(c * (uint32_t)(P >> 32)) << 32);
+                k += c;
+            };
+            if (wide) {
+                const uint32_t* row = pc.w + (g - 1u) * rs;
+                for (uint32_t d = 0; d < wn; ++d) bin(w0 + d, row[d]);
+            } else {
+                // rows start on an even counter: two counters per word
+                const uint32_t* row = pc.w + (((g - 1u) * rs) >> 1);
+                for (uint32_t d = 0; d < wn; d += 2u) {
+                    const uint32_t cw = row[d >> 1];
+                    bin(w0 + d, cw & 0xFFFFu);
+                    if (d + 1u < wn) bin(w0 + d + 1u, cw >> 16);
+                }
             }
             acc[2u * g] += S;
             acc[2u * g + 1u] += T;
@@ -952,10 +1011,10 @@ __global__ __launch_bounds__(kCountBlock) void count_kernel(const JobDesc* jobs,
         const int64_t idx = c * J.T + (int64_t)(g - 1u);
         const double U1 = (double)u1x2 * 0.5;
         const double td = J.tie_corr ? (double)tie : 0.0;
-        J.U1[idx] = U1;
-        if (J.tie) J.tie[idx] = td;
-        if (J.n1o) { J.n1o[idx] = (double)n1; J.n2o[idx] = (double)n2; }
-        if (J.asym) J.P[idx] = p_asymptotic(U1, (double)n1, (double)n2, td, J.cc, J.alt);
+        gl(J.U1)[idx] = U1;
+        if (J.tie) gl(J.tie)[idx] = td;
+        if (J.n1o) { gl(J.n1o)[idx] = (double)n1; gl(J.n2o)[idx] = (double)n2; }
+        if (J.asym) gl(J.P)[idx] = p_asymptotic(U1, (double)n1, (double)n2, td, J.cc, J.alt);
     }
 }
 
@@ -1206,11 +1265,17 @@ static unsigned table_lds_bytes(bool f64) {
 }
 
 hipError_t prepare_count_kernel(uint32_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)count_kernel<float>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)count_kernel<double>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fns[] = {(const void*)count_kernel<float, 256>, (const void*)count_kernel<float, 512>,
+                         (const void*)count_kernel<float, 1024>,
+                         (const void*)count_kernel<double, 256>,
+                         (const void*)count_kernel<double, 512>,
+                         (const void*)count_kernel<double, 1024>};
+    for (const void* f : fns) {
+        const hipError_t e =
+            hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
@@ -1220,12 +1285,19 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
     mark(0);
     if (L.n_count > 0) {
-        if (L.f64)
-            count_kernel<double><<<L.n_count, kCountBlock, L.count_lds, s>>>(
-                L.jobs, L.count_items, L.gen_items, L.gen_n, L.count_lds);
-        else
-            count_kernel<float><<<L.n_count, kCountBlock, L.count_lds, s>>>(
-                L.jobs, L.count_items, L.gen_items, L.gen_n, L.count_lds);
+#define HPDEX_COUNT(V, NT)                                                                  \
+    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_cols, L.gen_items, \
+                                                           L.gen_n, L.count_lds)
+        if (L.f64) {
+            if (L.count_block == 256) HPDEX_COUNT(double, 256);
+            else if (L.count_block == 1024) HPDEX_COUNT(double, 1024);
+            else HPDEX_COUNT(double, 512);
+        } else {
+            if (L.count_block == 256) HPDEX_COUNT(float, 256);
+            else if (L.count_block == 1024) HPDEX_COUNT(float, 1024);
+            else HPDEX_COUNT(float, 512);
+        }
+#undef HPDEX_COUNT
     }
     mark(1);
     size_t split = 12u * (size_t)L.G;

@@ -150,6 +150,7 @@ typedef struct hpdex_tuning {
     int32_t count_lds_bytes;   /* LDS per workgroup of the one-pass kernel for integer-
                                   valued columns (0 = off: every column takes the
                                   split/reference/target kernels; else 16 KiB..152 KiB) */
+    int32_t count_block;       /* threads per workgroup of that kernel: 256, 512, 1024 */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

@@ -73,10 +73,10 @@ def test_sim_histogram_path():
 # ---- the count kernel (integer-valued columns in one pass) ----
 
 def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist_max=8192,
-              expect_counted=None):
+              expect_counted=None, block=512):
     U1, P, TIE, n1, n2, err, status = sim.mwu(data, indices, indptr, gid, T, R, mode=ZH[mode],
                                               count_lds=count_lds, f64=f64, hist_max=hist_max,
-                                              timeout_s=120)
+                                              block=block, timeout_s=120)
     assert err == 2 ** 64 - 1 and status == 0
     oU1, oP, oT, on1, on2 = oracle.mwu(data, indices, indptr, gid, T, n_rows=R,
                                        zero_handling=mode, use_continuity=True, with_stats=True)
@@ -88,11 +88,12 @@ def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist
         assert sim.mwu.last_counted == expect_counted
 
 
+@pytest.mark.parametrize("block", [256, 512])
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
-def test_sim_count_kernel_modes(mode):
-    # small signed integers with explicit zeros: staged columns, one window
+def test_sim_count_kernel_modes(mode, block):
+    # small signed integers with explicit zeros: one pass, one window
     d, r, ip, g = _rw(21, 3000, 5, 7, 0.4, lambda rg, n: rg.integers(-4, 9, n))
-    run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5)
+    run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5, block=block)
 
 
 @pytest.mark.parametrize("lds", [16 * 1024, 24 * 1024, 152 * 1024])
@@ -141,8 +142,8 @@ def test_sim_count_kernel_value_windows(lds):
     # 200 targets x 400 values do not fit one window: 76 KiB stages the
     # column and takes 3 windows; 24 KiB re-reads it from memory for 9
     d, r, ip, g = _rw(28, 8000, 3, 200, 0.5, lambda rg, n: rg.integers(1, 401, n))
-    for mode in ("min", "mix"):
-        run_count(d, r, ip, g, 200, 8000, mode, lds, expect_counted=3)
+    for mode, block in (("min", 512), ("mix", 256)):
+        run_count(d, r, ip, g, 200, 8000, mode, lds, expect_counted=3, block=block)
 
 
 def test_sim_count_kernel_wide_counters():
