@@ -28,6 +28,8 @@ EXPORTED = (
     "hpdex_plan_create", "hpdex_plan_destroy", "hpdex_plan_run", "hpdex_plan_results",
     "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
     "hpdex_bandwidth_probe", "hpdex_plan_get_info", "hpdex_plan_stage_times",
+    "hpdex_session_create", "hpdex_session_submit", "hpdex_session_wait", "hpdex_session_peak",
+    "hpdex_session_destroy",
 )
 
 
@@ -45,6 +47,24 @@ class Tuning(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "count_columns", "count_nnz", "split_columns", "handed_over")]
+
+
+class SessionOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("matrix", ctypes.c_void_p), ("dtype", ctypes.c_int32),
+                ("data", ctypes.c_void_p), ("indices", ctypes.c_void_p),
+                ("indptr", ctypes.c_void_p), ("n_rows", ctypes.c_size_t),
+                ("n_cols", ctypes.c_size_t), ("nnz", ctypes.c_size_t),
+                ("group_id", ctypes.c_void_p), ("n_targets", ctypes.c_size_t),
+                ("opts", MwuOpts), ("U1_out", ctypes.c_void_p), ("P_out", ctypes.c_void_p)]
+
+
+class JobTimes(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("footprint_bytes", ctypes.c_size_t)]
 
 
 class HpdexError(RuntimeError):
@@ -87,12 +107,21 @@ def lib() -> ctypes.CDLL:
         L.hpdex_device_info.argtypes = [ctypes.POINTER(ci), ctypes.POINTER(ci), cp, sz]
         L.hpdex_bandwidth_probe.argtypes = [sz, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
         L.hpdex_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo), cp, sz]
+        L.hpdex_session_create.argtypes = [ctypes.POINTER(SessionOpts), ctypes.POINTER(vp), cp, sz]
+        L.hpdex_session_submit.argtypes = [vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_int64),
+                                           cp, sz]
+        L.hpdex_session_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(JobTimes), cp, sz]
+        L.hpdex_session_peak.argtypes = [vp, ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_size_t)]
+        L.hpdex_session_destroy.argtypes = [vp]
+        L.hpdex_session_destroy.restype = None
         L.hpdex_plan_stage_times.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
                      "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
                      "hpdex_device_info", "hpdex_bandwidth_probe", "hpdex_plan_get_info",
-                     "hpdex_plan_stage_times"):
+                     "hpdex_plan_stage_times", "hpdex_session_create", "hpdex_session_submit",
+                     "hpdex_session_wait", "hpdex_session_peak"):
             getattr(L, name).restype = ci
         _lib = L
     return _lib
@@ -300,3 +329,78 @@ def bandwidth_probe(nbytes=1 << 30, iters=10) -> float:
     if lib().hpdex_bandwidth_probe(int(nbytes), int(iters), ctypes.byref(g), err, len(err)) != 0:
         raise HpdexError(err.value.decode())
     return g.value
+
+
+class Session:
+    """A serving session (hpdex_session_*): jobs admitted by HBM footprint and
+    run concurrently, `slots` at a time, within an arena of `budget_bytes`."""
+
+    def __init__(self, budget_bytes, slots=4):
+        o = SessionOpts(int(budget_bytes), int(slots))
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_session_create(ctypes.byref(o), ctypes.byref(h), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        self._h = h
+        self._keep = {}
+
+    def submit(self, group_id, n_targets, opts: MwuOpts, matrix: DeviceMatrix = None,
+               data=None, indices=None, indptr=None, n_rows=None) -> int:
+        """Queue a job on a resident matrix, or on a host CSC uploaded for it.
+        Returns the ticket; the arrays stay referenced until wait()."""
+        gid = np.ascontiguousarray(group_id, dtype=np.int32)
+        j = Job()
+        keep = [gid]
+        if matrix is not None:
+            j.matrix = matrix.handle
+            C = matrix.n_cols
+            keep.append(matrix)
+        else:
+            data, indices, indptr = _csc_args(data, indices, indptr)
+            keep += [data, indices, indptr]
+            j.dtype = DTYPE_CODES[data.dtype]
+            j.data, j.indices, j.indptr = _p(data), _p(indices), _p(indptr)
+            j.n_rows = len(gid) if n_rows is None else int(n_rows)
+            C = len(indptr) - 1
+            j.n_cols, j.nnz = C, len(data)
+        n = C * int(n_targets)
+        U1, P = np.empty(max(n, 1)), np.empty(max(n, 1))
+        keep += [U1, P]
+        j.group_id = _p(gid)
+        j.n_targets = int(n_targets)
+        j.opts = opts
+        j.U1_out, j.P_out = _p(U1), _p(P)
+        t = ctypes.c_int64()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_session_submit(self._h, ctypes.byref(j), ctypes.byref(t), err,
+                                      len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        self._keep[t.value] = (keep, U1[:n], P[:n])
+        return t.value
+
+    def wait(self, ticket):
+        """Block until the job ends: (U1, P, times) or HpdexError."""
+        tm = JobTimes()
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_session_wait(self._h, int(ticket), ctypes.byref(tm), err, len(err))
+        keep, U1, P = self._keep.pop(ticket, (None, None, None))
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        return U1, P, {n: getattr(tm, n) for n, _ in JobTimes._fields_}
+
+    def peak(self):
+        j, b = ctypes.c_int32(), ctypes.c_size_t()
+        lib().hpdex_session_peak(self._h, ctypes.byref(j), ctypes.byref(b))
+        return {"jobs_in_flight": j.value, "arena_bytes": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hpdex_session_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -20,21 +20,13 @@
 #include <string>
 #include <vector>
 
+#include "hpdex_host.h"
 #include "hpdex_internal.h"
 
 using namespace hpdex;
+using namespace hpdex_host;
 
-namespace {
-
-// Largest exact-null DP workspace per distinct (n1, n2), in doubles.
-constexpr size_t kExactMaxCells = (size_t)1 << 27;
-
-struct Tuning {
-    int hist_max = kHistMaxBins;
-    int size_ordered = 1;
-    int count_lds = kCountLdsDefault;
-    int count_block = kCountBlockDefault;
-};
+namespace hpdex_host {
 
 std::mutex g_mu;
 Tuning g_tune;
@@ -43,6 +35,7 @@ int g_cus = 0;
 int g_max_lds = 0;
 bool g_count_ready = false;   // the count kernel may use up to kCountLdsMax of LDS
 char g_name[256] = {0};
+thread_local BumpBlock* t_block = nullptr;
 
 int set_err(char* err, size_t len, const std::string& m) {
     if (err && len) {
@@ -51,14 +44,6 @@ int set_err(char* err, size_t len, const std::string& m) {
     }
     return 1;
 }
-
-#define HIPCHK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            return set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(e_) + \
-                                             " (" #x ")");                                 \
-    } while (0)
 
 int ensure_init(char* err, size_t err_len) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -86,34 +71,6 @@ Tuning tuning() {
     return g_tune;
 }
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { reset(); }
-    void reset() {
-        if (p) hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t bytes) {
-        reset();
-        n = bytes;
-        return hipMalloc(&p, bytes ? bytes : 16);
-    }
-    template <typename T> T* as() const { return (T*)p; }
-};
-
-struct Stream {
-    hipStream_t s = nullptr;
-    ~Stream() {
-        if (s) hipStreamDestroy(s);
-    }
-    hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
-};
-
 bool valid_dtype(int d) { return d >= HPDEX_INT8 && d <= HPDEX_FLOAT64; }
 size_t dtype_size(int d) {
     switch (d) {
@@ -136,26 +93,9 @@ int check_indptr(const int64_t* indptr, size_t C, size_t nnz, char* err, size_t 
     return 0;
 }
 
-}  // namespace
-
 // ---------------------------------------------------------------------------
-// Resident matrix
+// Resident matrix upload, per-job state
 // ---------------------------------------------------------------------------
-struct hpdex_matrix {
-    int dtype = 0;
-    bool f64 = false;
-    size_t R = 0, C = 0, nnz = 0;
-    DevBuf vals, rows, indptr;
-    DevBuf colrange;                // 2C int32: value range of integer-valued columns
-    std::vector<int64_t> h_indptr;
-    std::vector<int32_t> order;     // columns, largest first
-    std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
-                                    // non-finite) -- the column may go to the count kernel
-    std::vector<int32_t> h_range;   // host copy of colrange
-};
-
-namespace {
-
 int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,
                   const int64_t* indptr, size_t R, size_t C, size_t nnz, hipStream_t s,
                   char* err, size_t err_len) {
@@ -218,33 +158,6 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
-
-// Per-job device state shared by the one-shot call and the plan.
-struct JobState {
-    size_t T = 0, G = 0, C = 0;
-    bool stats = false;          // tie and sample sizes kept per pair
-    hpdex_mwu_opts opts{};
-    std::vector<uint32_t> group_rows;
-    DevBuf gid, gid16, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
-    DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
-    DevBuf d_job;                                   // descriptor
-    DevBuf d_count_cols, d_gen_items, d_gen_n;      // column queues of the two paths
-    JobDesc J{};                                    // host copy of the descriptor
-    int n_count = 0, n_gen_static = 0;
-    int64_t count_nnz = 0;
-    uint32_t count_lds = 0;
-    int count_block = kCountBlockDefault;
-    // exact null tables
-    std::vector<ExactTable> tabs;
-    std::vector<int32_t> tidx_by_g;
-    DevBuf d_tabs, tabbuf, tscratch, d_tidx;
-    size_t scratch_per_table = 0;
-    bool exact_by_pair = false;
-
-    unsigned long long* err_key() const { return flags.as<unsigned long long>(); }
-    unsigned long long* status_col() const { return flags.as<unsigned long long>() + 1; }
-    uint32_t* status() const { return (uint32_t*)(flags.as<unsigned long long>() + 2); }
-};
 
 int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
               const hpdex_mwu_opts* o, bool stats, hipStream_t s, char* err, size_t err_len) {
@@ -744,7 +657,7 @@ struct hpdex_plan {
     }
 };
 
-namespace {
+namespace hpdex_host {
 
 // The per-job launch sequence (everything after upload): split, reference
 // tables, targets and pairs, then the exact p lookup when asked for (tables

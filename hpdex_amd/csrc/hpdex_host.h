@@ -1,0 +1,170 @@
+// hpdex_host.h -- host-side internals shared by the C-ABI (hpdex_api.cpp) and
+// the serving session (hpdex_session.cpp).  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hpdex_internal.h"
+#include "hpdex_mi355x.h"
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return hpdex_host::set_err(err, err_len, std::string("HIP error: ") +          \
+                                                         hipGetErrorString(e_) + " (" #x ")"); \
+    } while (0)
+
+namespace hpdex_host {
+
+using namespace hpdex;
+
+// Largest exact-null DP workspace per distinct (n1, n2), in doubles.
+constexpr size_t kExactMaxCells = (size_t)1 << 27;
+
+struct Tuning {
+    int hist_max = kHistMaxBins;
+    int size_ordered = 1;
+    int count_lds = kCountLdsDefault;
+    int count_block = kCountBlockDefault;
+};
+
+extern std::mutex g_mu;
+extern Tuning g_tune;
+extern bool g_init;
+extern int g_cus;
+extern int g_max_lds;
+extern bool g_count_ready;
+extern char g_name[256];
+
+int set_err(char* err, size_t len, const std::string& m);
+int ensure_init(char* err, size_t err_len);
+Tuning tuning();
+
+// A block of device memory that a job's buffers are carved from (bump
+// allocation, freed as a whole when the job ends).  While a session worker
+// builds a job, t_block points at the job's block and DevBuf takes its memory
+// there instead of calling hipMalloc.
+struct BumpBlock {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    void* take(size_t n) {
+        n = (n + 255) & ~(size_t)255;
+        if (used + n > cap) return nullptr;
+        void* p = base + used;
+        used += n;
+        return p;
+    }
+};
+extern thread_local BumpBlock* t_block;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    bool owned = false;   // hipMalloc'd (freed here) rather than carved from a block
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p && owned) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        owned = false;
+    }
+    hipError_t alloc(size_t bytes) {
+        reset();
+        n = bytes;
+        if (t_block) {
+            p = t_block->take(bytes ? bytes : 16);
+            return p ? hipSuccess : hipErrorOutOfMemory;
+        }
+        owned = true;
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+};
+
+bool valid_dtype(int d);
+size_t dtype_size(int d);
+int check_indptr(const int64_t* indptr, size_t C, size_t nnz, char* err, size_t err_len);
+
+}  // namespace hpdex_host
+
+// Resident matrix (the C-ABI's opaque hpdex_matrix).
+struct hpdex_matrix {
+    int dtype = 0;
+    bool f64 = false;
+    size_t R = 0, C = 0, nnz = 0;
+    hpdex_host::DevBuf vals, rows, indptr;
+    hpdex_host::DevBuf colrange;    // 2C int32: value range of integer-valued columns
+    std::vector<int64_t> h_indptr;
+    std::vector<int32_t> order;     // columns, largest first
+    std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
+                                    // non-finite) -- the column may go to the count kernel
+    std::vector<int32_t> h_range;   // host copy of colrange
+};
+
+namespace hpdex_host {
+
+// Per-job device state shared by the one-shot call, the plan and the session.
+struct JobState {
+    size_t T = 0, G = 0, C = 0;
+    bool stats = false;          // tie and sample sizes kept per pair
+    hpdex_mwu_opts opts{};
+    std::vector<uint32_t> group_rows;
+    DevBuf gid, gid16, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
+    DevBuf d_job;                                   // descriptor
+    DevBuf d_count_cols, d_gen_items, d_gen_n;      // column queues of the two paths
+    JobDesc J{};                                    // host copy of the descriptor
+    int n_count = 0, n_gen_static = 0;
+    int64_t count_nnz = 0;
+    uint32_t count_lds = 0;
+    int count_block = kCountBlockDefault;
+    // exact null tables
+    std::vector<ExactTable> tabs;
+    std::vector<int32_t> tidx_by_g;
+    DevBuf d_tabs, tabbuf, tscratch, d_tidx;
+    size_t scratch_per_table = 0;
+    bool exact_by_pair = false;
+
+    unsigned long long* err_key() const { return flags.as<unsigned long long>(); }
+    unsigned long long* status_col() const { return flags.as<unsigned long long>() + 1; }
+    uint32_t* status() const { return (uint32_t*)(flags.as<unsigned long long>() + 2); }
+};
+
+int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,
+                  const int64_t* indptr, size_t R, size_t C, size_t nnz, hipStream_t s,
+                  char* err, size_t err_len);
+int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
+              const hpdex_mwu_opts* o, bool stats, hipStream_t s, char* err, size_t err_len);
+JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu);
+int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const Tuning& tu,
+                    bool size_ordered, hipStream_t s, char* err, size_t err_len);
+PipelineLaunch pipeline_of(const JobState& js, bool f64);
+int exact_plan(JobState& js, const double* n1pair, const double* n2pair, hipStream_t s,
+               char* err, size_t err_len);
+int exact_launch(const JobState& js, hipStream_t s, bool build_tables, char* err,
+                 size_t err_len);
+int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len);
+int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1, double* n2,
+             hipStream_t s, char* err, size_t err_len);
+bool opts_ok(const hpdex_mwu_opts* o);
+int run_exact_none(JobState& js, hipStream_t s, char* err, size_t err_len);
+
+}  // namespace hpdex_host

@@ -1,0 +1,370 @@
+// hpdex_session.cpp -- serving session: DE jobs of different sizes admitted by
+// HBM footprint and run concurrently on one GPU (continuous batching).
+//
+// One arena of device memory is allocated when the session starts.  Each job's
+// footprint -- an upper bound of every buffer it allocates (its uploaded matrix
+// if it brings one, the per-job workspace of hpdex_api.cpp, exact-null tables)
+// -- is computed when it is submitted.  Worker threads, one stream each, take
+// jobs from a FIFO queue whenever the head job's footprint fits in the arena;
+// the job's buffers are then carved from its block (BumpBlock) with no
+// hipMalloc, its kernels run on the worker's stream beside the other workers'
+// jobs, and the block returns to the arena when the results are in the
+// caller's host buffers.  Jobs that name a resident matrix skip the upload:
+// a new target set against cells already in HBM costs only its workspace.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hpdex_host.h"
+#include "hpdex_internal.h"
+#include "hpdex_mi355x.h"
+
+using namespace hpdex;
+using namespace hpdex_host;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over [0, cap) of the arena, with coalescing of free ranges.
+class Arena {
+  public:
+    bool take(size_t n, size_t* off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            const size_t o = it->first, rem = it->second - n;
+            free_.erase(it);
+            if (rem) free_[o + n] = rem;
+            used_ += n;
+            *off = o;
+            return true;
+        }
+        return false;
+    }
+    void give(size_t off, size_t n) {
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+        used_ -= n;
+    }
+    void reset(size_t cap) {
+        free_.clear();
+        free_[0] = cap;
+        used_ = 0;
+    }
+    size_t used() const { return used_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0;
+};
+
+struct JobRec {
+    hpdex_job job{};
+    size_t footprint = 0;
+    Clock::time_point t_submit, t_start, t_end;
+    bool done = false;
+    int rc = 0;
+    std::string err;
+};
+
+size_t up256(size_t n) { return ((n ? n : 16) + 255) & ~(size_t)255; }
+
+// Upper bound of what one job allocates through DevBuf (every allocation is
+// rounded to 256 bytes, as BumpBlock::take rounds it).
+size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows) {
+    const hpdex_matrix* m = j.matrix;
+    const size_t R = m ? m->R : j.n_rows, C = m ? m->C : j.n_cols, nnz = m ? m->nnz : j.nnz;
+    const bool f64 = m ? m->f64 : dtype_uses_double(j.dtype);
+    const size_t vsz = f64 ? 8 : 4;
+    const size_t T = j.n_targets, G = T + 1, np = C * T;
+    size_t f = 0;
+    if (!m) {   // upload_matrix
+        f += up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
+        f += up256(2 * C * 4) + up256(C);
+        if (j.dtype != HPDEX_FLOAT32 && j.dtype != HPDEX_FLOAT64)
+            f += up256(nnz * dtype_size(j.dtype));
+        f += up256(nnz * 8);
+    }
+    // job_setup
+    f += up256(R * 4) + up256((R + 1) * 2) + up256(G * 4) + 2 * up256(np * 8) + up256(32);
+    f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));
+    f += up256(C * (size_t)kTabStride * 4);
+    // job_upload_desc
+    f += up256(sizeof(JobDesc)) + up256(C * 4) + up256(C * sizeof(WorkItem)) + up256(16);
+    // exact-null tables of min/max/mix: sizes are the group row counts
+    if (j.opts.method == 1) {
+        std::set<std::pair<int32_t, int32_t>> pairs;
+        for (size_t g = 1; g <= T; ++g) pairs.emplace((int32_t)group_rows[0], (int32_t)group_rows[g]);
+        size_t tab = 0, max_cells = 0;
+        for (const auto& pr : pairs) {
+            const size_t U = (size_t)pr.first * pr.second;
+            tab += 1 + 2 * (U + 1);
+            const size_t s_ = (size_t)std::min(pr.first, pr.second);
+            max_cells = std::max(max_cells, (s_ + 1) * (U + 1));
+        }
+        f += up256(pairs.size() * sizeof(ExactTable)) + up256(tab * 8);
+        f += up256(pairs.size() * 2 * max_cells * 8) + up256(T * 4);
+    }
+    return f;
+}
+
+}  // namespace
+
+struct hpdex_session {
+    hpdex_session_opts o{};
+    void* base = nullptr;
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv_admit;   // workers: a job was queued or memory was freed
+    std::condition_variable cv_done;    // waiters: a job ended
+    std::deque<std::shared_ptr<JobRec>> pending;
+    std::map<int64_t, std::shared_ptr<JobRec>> jobs;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    int in_flight = 0, peak_in_flight = 0;
+    size_t peak_bytes = 0;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+// Everything one job does, on the worker's stream, with buffers from its block.
+void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
+    BumpBlock blk;
+    blk.base = block;
+    blk.cap = cap;
+    t_block = &blk;
+    char err[1024];
+    const size_t err_len = sizeof err;
+    err[0] = 0;
+    int rc = 0;
+    const hpdex_job& j = R.job;
+    const Tuning tu = tuning();
+    {
+        std::unique_ptr<hpdex_matrix> own;
+        const hpdex_matrix* m = j.matrix;
+        JobState js;
+        do {
+            if (!m) {
+                own.reset(new hpdex_matrix());
+                rc = upload_matrix(own.get(), j.dtype, j.data, j.indices, j.indptr, j.n_rows,
+                                   j.n_cols, j.nnz, s, err, err_len);
+                if (rc) break;
+                m = own.get();
+            }
+            if (m->C == 0) break;
+            rc = job_setup(js, m, j.group_id, j.n_targets, &j.opts, false, s, err, err_len);
+            if (rc) break;
+            const JobDesc J = make_desc(js, m, tu);
+            rc = job_upload_desc(js, J, *m, tu, tu.size_ordered != 0, s, err, err_len);
+            if (rc) break;
+            const bool exact = j.opts.method == 1;
+            if (exact) {
+                rc = exact_plan(js, nullptr, nullptr, s, err, err_len);
+                if (rc) break;
+            }
+            const hipError_t e = launch_mwu_pipeline(pipeline_of(js, m->f64), s);
+            if (e != hipSuccess) {
+                rc = set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(e));
+                break;
+            }
+            if (exact) {
+                rc = exact_launch(js, s, true, err, err_len);
+                if (rc) break;
+            }
+            rc = job_errors(js, s, err, err_len);
+            if (rc) break;
+            rc = copy_out(js, j.U1_out, j.P_out, nullptr, nullptr, nullptr, s, err, err_len);
+        } while (0);
+        // nothing of this job may still run when its block goes back to the arena
+        const hipError_t se = hipStreamSynchronize(s);
+        if (!rc && se != hipSuccess)
+            rc = set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(se));
+    }
+    t_block = nullptr;
+    R.rc = rc;
+    if (rc) R.err = err;
+}
+
+void worker(hpdex_session* S) {
+    Stream st;
+    const hipError_t se = st.create();
+    for (;;) {
+        std::shared_ptr<JobRec> J;
+        size_t off = 0;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                if (!S->pending.empty()) {
+                    if (S->arena.take(S->pending.front()->footprint, &off)) {
+                        J = S->pending.front();
+                        S->pending.pop_front();
+                        break;
+                    }
+                } else if (S->stop) {
+                    return;
+                }
+                S->cv_admit.wait(lk);
+            }
+            ++S->in_flight;
+            S->peak_in_flight = std::max(S->peak_in_flight, S->in_flight);
+            S->peak_bytes = std::max(S->peak_bytes, S->arena.used());
+            J->t_start = Clock::now();
+        }
+        if (se != hipSuccess) {
+            J->rc = 1;
+            J->err = std::string("HIP error: ") + hipGetErrorString(se) + " (stream creation)";
+        } else {
+            run_job(*J, st.s, (char*)S->base + off, J->footprint);
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->arena.give(off, J->footprint);
+            --S->in_flight;
+            J->t_end = Clock::now();
+            J->done = true;
+        }
+        S->cv_admit.notify_all();
+        S->cv_done.notify_all();
+    }
+}
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpdex_session_create(const hpdex_session_opts* o, hpdex_session** out, char* err,
+                         size_t err_len) {
+    if (!o || !out) return set_err(err, err_len, "null argument");
+    *out = nullptr;
+    if (o->hbm_budget_bytes < ((size_t)1 << 20))
+        return set_err(err, err_len, "hbm_budget_bytes must be at least 1 MiB");
+    if (o->slots < 1 || o->slots > 64) return set_err(err, err_len, "slots must be in [1, 64]");
+    if (ensure_init(err, err_len)) return 1;
+    std::unique_ptr<hpdex_session> S(new hpdex_session());
+    S->o = *o;
+    HIPCHK(hipMalloc(&S->base, o->hbm_budget_bytes));
+    S->arena.reset(o->hbm_budget_bytes);
+    for (int i = 0; i < o->slots; ++i) S->workers.emplace_back(worker, S.get());
+    *out = S.release();
+    return 0;
+}
+
+int hpdex_session_submit(hpdex_session* S, const hpdex_job* job, int64_t* ticket, char* err,
+                         size_t err_len) {
+    if (!S || !job || !ticket) return set_err(err, err_len, "null argument");
+    const hpdex_job& j = *job;
+    if (j.n_targets < 1)
+        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    if (!opts_ok(&j.opts)) return set_err(err, err_len, "invalid option block");
+    if (j.opts.method == 1 && j.opts.zero_handling == 0)
+        return set_err(err, err_len,
+                       "exact p with zero_handling none sizes its tables from the data: use "
+                       "hpdex_mwu_csc for such jobs");
+    size_t R, C;
+    if (j.matrix) {
+        R = j.matrix->R;
+        C = j.matrix->C;
+    } else {
+        if (!valid_dtype(j.dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+        if (!j.indptr || (j.nnz && (!j.data || !j.indices)))
+            return set_err(err, err_len, "null CSC array");
+        if (check_indptr(j.indptr, j.n_cols, j.nnz, err, err_len)) return 1;
+        R = j.n_rows;
+        C = j.n_cols;
+    }
+    if (R && !j.group_id) return set_err(err, err_len, "null group_id");
+    if (C && (!j.U1_out || !j.P_out)) return set_err(err, err_len, "null output buffer");
+    std::vector<uint32_t> group_rows(j.n_targets + 1, 0u);
+    for (size_t r = 0; r < R; ++r) {
+        const int g = j.group_id[r];
+        if (g >= 0 && (size_t)g <= j.n_targets) ++group_rows[g];
+    }
+    auto rec = std::make_shared<JobRec>();
+    rec->job = j;
+    rec->footprint = job_footprint(j, group_rows);
+    if (rec->footprint > S->o.hbm_budget_bytes)
+        return set_err(err, err_len,
+                       "job needs " + std::to_string(rec->footprint) +
+                           " bytes of HBM, more than the session's budget of " +
+                           std::to_string(S->o.hbm_budget_bytes));
+    rec->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return set_err(err, err_len, "session is closing");
+        *ticket = S->next_ticket++;
+        S->jobs[*ticket] = rec;
+        S->pending.push_back(rec);
+    }
+    S->cv_admit.notify_all();
+    return 0;
+}
+
+int hpdex_session_wait(hpdex_session* S, int64_t ticket, hpdex_job_times* times, char* err,
+                       size_t err_len) {
+    if (!S) return set_err(err, err_len, "null session");
+    std::shared_ptr<JobRec> J;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->jobs.find(ticket);
+        if (it == S->jobs.end()) return set_err(err, err_len, "unknown ticket");
+        J = it->second;
+        S->cv_done.wait(lk, [&] { return J->done; });
+        S->jobs.erase(ticket);
+    }
+    if (times) {
+        times->queue_ms = ms_between(J->t_submit, J->t_start);
+        times->run_ms = ms_between(J->t_start, J->t_end);
+        times->latency_ms = ms_between(J->t_submit, J->t_end);
+        times->footprint_bytes = J->footprint;
+    }
+    if (J->rc) return set_err(err, err_len, J->err);
+    return 0;
+}
+
+int hpdex_session_peak(hpdex_session* S, int32_t* jobs_in_flight, size_t* arena_bytes) {
+    if (!S) return 1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (jobs_in_flight) *jobs_in_flight = S->peak_in_flight;
+    if (arena_bytes) *arena_bytes = S->peak_bytes;
+    return 0;
+}
+
+void hpdex_session_destroy(hpdex_session* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_admit.notify_all();
+    for (auto& t : S->workers) t.join();
+    if (S->base) (void)hipFree(S->base);
+    delete S;
+}
+
+}  // extern "C"

@@ -155,6 +155,57 @@ typedef struct hpdex_tuning {
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);
 
+/* ---------------- serving session ----------------
+ * No counterpart in the reference (its API is one blocking call per job).
+ * A session serves DE jobs of different sizes concurrently on one GPU: it owns
+ * one HBM arena of hbm_budget_bytes and `slots` worker threads, each with its
+ * own stream.  Submitted jobs wait in FIFO order and are admitted as soon as a
+ * slot is free and the job's footprint (an upper bound of everything it
+ * allocates: its uploaded matrix, if any, and its workspace) fits in the
+ * arena; admitted jobs run concurrently and their kernels overlap on the
+ * device.  A job either names a resident matrix (hpdex_matrix_create; a new
+ * target set against cells already in HBM) or carries a host CSC that is
+ * uploaded into its arena block and dropped when it ends.  All pointers in a
+ * job are borrowed until hpdex_session_wait returns for it.                  */
+typedef struct hpdex_session hpdex_session;
+typedef struct hpdex_session_opts {
+    size_t hbm_budget_bytes;   /* arena size */
+    int32_t slots;             /* jobs in flight at once (1..64) */
+} hpdex_session_opts;
+typedef struct hpdex_job {
+    hpdex_matrix* matrix;      /* resident matrix, or NULL to upload the host CSC below */
+    int32_t dtype;             /* host CSC (same meaning as hpdex_mwu_csc) */
+    const void* data;
+    const int64_t* indices;
+    const int64_t* indptr;
+    size_t n_rows, n_cols, nnz;
+    const int32_t* group_id;   /* n_rows labels (of the matrix's rows) */
+    size_t n_targets;
+    hpdex_mwu_opts opts;       /* method exact needs zero_handling min, max or mix */
+    double* U1_out;            /* n_cols * n_targets each, written when the job ends */
+    double* P_out;
+} hpdex_job;
+typedef struct hpdex_job_times {
+    double queue_ms;           /* submit -> admitted */
+    double run_ms;             /* admitted -> results in the host buffers */
+    double latency_ms;         /* submit -> results in the host buffers */
+    size_t footprint_bytes;    /* arena bytes the job held */
+} hpdex_job_times;
+int hpdex_session_create(const hpdex_session_opts* o, hpdex_session** out, char* err,
+                         size_t err_len);
+/* Queue a job; *ticket identifies it.  Fails at once (nothing queued) when the
+ * job can never fit the arena or its arguments are invalid. */
+int hpdex_session_submit(hpdex_session* s, const hpdex_job* job, int64_t* ticket, char* err,
+                         size_t err_len);
+/* Block until the job has ended; returns its error (0 on success) and, if
+ * times is not NULL, its timings.  Each ticket is waited for once. */
+int hpdex_session_wait(hpdex_session* s, int64_t ticket, hpdex_job_times* times, char* err,
+                       size_t err_len);
+/* Largest number of jobs that ran at once, and most arena bytes in use, so far. */
+int hpdex_session_peak(hpdex_session* s, int32_t* jobs_in_flight, size_t* arena_bytes);
+/* Wait for every queued job, then free the arena and the workers. */
+void hpdex_session_destroy(hpdex_session* s);
+
 /* Attainable HBM bandwidth: a 16-byte streaming copy of `bytes` (read + write
  * counted), run `iters` times; gbs receives GB/s.  Also the calibration kernel
  * for HBM byte counters (its traffic is exactly 2*bytes per launch). */

@@ -1,0 +1,152 @@
+"""Serving session on the GPU: jobs of different sizes, paths and modes run
+concurrently (several slots, one HBM arena) give exactly the results of the
+same jobs run alone through the one-shot C-ABI call; admission respects the
+arena budget; re-queries of a resident matrix with new target sets match."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fixtures import rel_close
+from hpdex_amd import _capi, synthetic
+
+pytestmark = pytest.mark.gpu
+
+ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _matrix(seed, R, C, dens, values):
+    rng = np.random.default_rng(seed)
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.full(C, dens))
+    return rng, dict(data=values(rng, len(rows)).astype(np.float32), indices=rows,
+                     indptr=indptr, n_rows=R, n_cols=C)
+
+
+def _groups(rng, R, T, ref_frac=0.2, ignored=0.1):
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    nref, nign = int(R * ref_frac), int(R * ignored)
+    gid[perm[:nref]] = 0
+    gid[perm[nref + nign:]] = rng.integers(1, T + 1, R - nref - nign)
+    return gid
+
+
+def _jobs():
+    """(name, matrix dict, group_id, T, opts, resident?) of a varied job mix."""
+    jobs = []
+    rng, counts = _matrix(31, 8000, 64, 0.15, lambda r, n: r.negative_binomial(0.8, 0.15, n) + 1)
+    jobs.append(("count-min", counts, _groups(rng, 8000, 20), 20,
+                 _capi.make_opts(True, True, ZH["min"]), False))
+    rng, floats = _matrix(32, 6000, 40, 0.3, lambda r, n: np.round(r.normal(0, 2, n), 2))
+    jobs.append(("float-mix", floats, _groups(rng, 6000, 8), 8,
+                 _capi.make_opts(True, False, ZH["mix"]), False))
+    rng, zeros = _matrix(33, 5000, 30, 0.3, lambda r, n: r.integers(0, 5, n))
+    jobs.append(("count-none", zeros, _groups(rng, 5000, 6), 6,
+                 _capi.make_opts(True, True, ZH["none"]), False))
+    ex = synthetic.exact_case(n_cells=2000, n_genes=24, n_ref=40, n_targets=30)
+    jobs.append(("exact-min", dict(data=ex["data"], indices=ex["indices"], indptr=ex["indptr"],
+                                   n_rows=ex["n_rows"], n_cols=ex["n_cols"]),
+                 ex["group_id"], ex["n_targets"],
+                 _capi.make_opts(True, False, ZH["min"], method=1), False))
+    # two target sets against one resident matrix
+    rng, res = _matrix(34, 12000, 48, 0.1, lambda r, n: r.negative_binomial(1.0, 0.2, n) + 1)
+    for k, T in enumerate((12, 30)):
+        jobs.append((f"resident-{k}", res, _groups(np.random.default_rng(40 + k), 12000, T), T,
+                     _capi.make_opts(True, True, ZH["max"]), True))
+    return jobs
+
+
+def _alone(m, gid, T, opts):
+    return _capi.mwu_csc(m["data"], m["indices"], m["indptr"], gid, T, opts, n_rows=m["n_rows"])
+
+
+def _submit(sess, job, resident):
+    name, m, gid, T, opts, is_res = job
+    if is_res:
+        key = id(m)
+        if key not in resident:
+            resident[key] = _capi.DeviceMatrix(m["data"], m["indices"], m["indptr"], m["n_rows"])
+        return sess.submit(gid, T, opts, matrix=resident[key])
+    return sess.submit(gid, T, opts, data=m["data"], indices=m["indices"], indptr=m["indptr"],
+                       n_rows=m["n_rows"])
+
+
+def test_session_mix_matches_jobs_alone():
+    jobs = _jobs()
+    say("session: jobs alone")
+    alone = [_alone(m, gid, T, opts) for _, m, gid, T, opts, _ in jobs]
+    say("session: jobs in the mix (4 slots, each job twice)")
+    sess = _capi.Session(budget_bytes=4 << 30, slots=4)
+    resident = {}
+    tickets = [(i, _submit(sess, jobs[i], resident)) for i in list(range(len(jobs))) * 2]
+    for i, t in tickets:
+        U1, P, times = sess.wait(t)
+        assert np.array_equal(U1, alone[i][0]), jobs[i][0]
+        assert np.array_equal(P, alone[i][1]), jobs[i][0]
+        assert times["latency_ms"] >= times["run_ms"] >= 0.0
+    peak = sess.peak()
+    assert 1 <= peak["jobs_in_flight"] <= 4
+    assert peak["arena_bytes"] <= 4 << 30
+    sess.close()
+    for m in resident.values():
+        m.close()
+    # one job of the mix against the oracle (U1 bit-exact, p to 1e-12)
+    name, m, gid, T, opts, _ = jobs[1]
+    o = oracle.mwu(m["data"], m["indices"], m["indptr"], gid, T, n_rows=m["n_rows"],
+                   tie_correction=True, use_continuity=False, zero_handling="mix")
+    assert np.array_equal(alone[1][0], o[0]) and rel_close(alone[1][1], o[1], rtol=1e-12)
+
+
+def test_session_admission_respects_budget():
+    """A budget a little above the largest job's footprint still runs every job
+    (one or two at a time) with the same results; a job above the budget is
+    refused when submitted."""
+    jobs = _jobs()[:3]
+    say("session: footprints")
+    sess = _capi.Session(budget_bytes=2 << 30, slots=2)
+    feet = []
+    for job in jobs:
+        t = _submit(sess, job, {})
+        feet.append(sess.wait(t)[2]["footprint_bytes"])
+    sess.close()
+    budget = max(feet) + (1 << 20)
+    say(f"session: budget {budget} bytes, footprints {feet}")
+    sess = _capi.Session(budget_bytes=budget, slots=3)
+    tickets = [_submit(sess, job, {}) for job in jobs * 2]
+    for k, t in enumerate(tickets):
+        U1, P, _ = sess.wait(t)
+        _, m, gid, T, opts, _ = jobs[k % len(jobs)]
+        a = _alone(m, gid, T, opts)
+        assert np.array_equal(U1, a[0]) and np.array_equal(P, a[1])
+    assert sess.peak()["arena_bytes"] <= budget
+    sess.close()
+    small = _capi.Session(budget_bytes=2 << 20, slots=1)
+    with pytest.raises(_capi.HpdexError, match="budget"):
+        _submit(small, jobs[0], {})
+    small.close()
+
+
+def test_session_reports_job_errors():
+    """A job whose reference has fewer than two cells fails with the
+    reference's message; the session keeps serving."""
+    rng, m = _matrix(35, 3000, 10, 0.2, lambda r, n: r.integers(1, 9, n))
+    gid = _groups(rng, 3000, 4)
+    bad = gid.copy()
+    bad[bad == 0] = -1
+    bad[0] = 0
+    opts = _capi.make_opts(True, True, ZH["none"])
+    sess = _capi.Session(budget_bytes=1 << 30, slots=2)
+    t_bad = _submit(sess, ("bad", m, bad, 4, opts, False), {})
+    t_ok = _submit(sess, ("ok", m, gid, 4, opts, False), {})
+    with pytest.raises(_capi.HpdexError, match="Sample too small for reference"):
+        sess.wait(t_bad)
+    U1, P, _ = sess.wait(t_ok)
+    a = _alone(m, gid, 4, opts)
+    assert np.array_equal(U1, a[0]) and np.array_equal(P, a[1])
+    sess.close()
