@@ -115,6 +115,90 @@ def cpu_baseline(w, target_s=12.0):
                       f"visible, {threads} threads used"}
 
 
+def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
+    """Serving workload (cells U[5k,200k], genes U[2k,20k], density U[3%,12%],
+    NB(LogNormal(0,1), r=0.7)+1 counts, reference 10% of cells, 5-500 targets;
+    30% of jobs re-query a resident matrix with a new target set): Poisson
+    arrivals at `rate` jobs/s into a session of `slots` slots and a
+    `budget_gb` HBM arena.  Reports p50/p90/p99 job latency (submit -> results
+    in host memory), the same jobs submitted all at once (throughput at
+    saturation), and the same jobs as plain one-shot C-ABI calls back to back."""
+    from hpdex_amd import _capi, synthetic
+    t0 = time.perf_counter()
+    pool, jobs = synthetic.serving_workload(seed=seed, n_jobs=n_jobs)
+    log(f"serving: generated {len(pool)} matrices and {len(jobs)} jobs in "
+        f"{time.perf_counter() - t0:.1f}s")
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    resident = {k: _capi.DeviceMatrix(pool[k]["data"], pool[k]["indices"], pool[k]["indptr"],
+                                      pool[k]["n_rows"])
+                for k in sorted({j["matrix"] for j in jobs if j["resident"]})}
+    tests = [pool[j["matrix"]]["n_cols"] * j["n_targets"] for j in jobs]
+
+    def submit(sess, j):
+        m = pool[j["matrix"]]
+        if j["resident"]:
+            return sess.submit(j["group_id"], j["n_targets"], opts, matrix=resident[j["matrix"]])
+        return sess.submit(j["group_id"], j["n_targets"], opts, data=m["data"],
+                           indices=m["indices"], indptr=m["indptr"], n_rows=m["n_rows"])
+
+    sess = _capi.Session(budget_bytes=budget_gb << 30, slots=slots)
+    for j in jobs[:2]:                       # warm-up (first-call costs), not measured
+        sess.wait(submit(sess, j))
+    # Poisson arrivals
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    tickets = []
+    t0 = time.perf_counter()
+    for j, a in zip(jobs, arrivals):
+        d = t0 + a - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(submit(sess, j))
+    lat = [sess.wait(t)[2]["latency_ms"] for t in tickets]
+    t_arr = time.perf_counter() - t0
+    peak = sess.peak()
+    # saturation: every job queued at once
+    t0 = time.perf_counter()
+    tickets = [submit(sess, j) for j in jobs]
+    sat_lat = [sess.wait(t)[2]["latency_ms"] for t in tickets]
+    t_sat = time.perf_counter() - t0
+    sess.close()
+    # plain: one blocking C-ABI call per job (upload, kernels, download)
+    plain = []
+    t0 = time.perf_counter()
+    for j in jobs:
+        m = pool[j["matrix"]]
+        t1 = time.perf_counter()
+        _capi.mwu_csc(m["data"], m["indices"], m["indptr"], j["group_id"], j["n_targets"], opts,
+                      n_rows=m["n_rows"])
+        plain.append((time.perf_counter() - t1) * 1e3)
+    t_plain = time.perf_counter() - t0
+    for m in resident.values():
+        m.close()
+    pct = lambda x, q: float(np.percentile(np.asarray(x), q))
+    out = {
+        "workload": f"{len(jobs)} jobs over {len(pool)} matrices (cells x genes, density): "
+                    + ", ".join(f"{m['n_rows']}x{m['n_cols']} {m['density']:.3f}" for m in pool)
+                    + f"; {sum(j['resident'] for j in jobs)} jobs re-query a resident matrix; "
+                      "zero_handling=min, tie + continuity, asymptotic",
+        "arrival_rate_per_s": rate, "slots": slots, "hbm_budget_gb": budget_gb,
+        "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99),
+        "mean_ms": float(np.mean(lat)), "tests": int(sum(tests)),
+        "tests_per_s_at_rate": sum(tests) / t_arr,
+        "saturated_tests_per_s": sum(tests) / t_sat,
+        "saturated_p50_ms": pct(sat_lat, 50), "saturated_p99_ms": pct(sat_lat, 99),
+        "plain_tests_per_s": sum(tests) / t_plain,
+        "plain_p50_ms": pct(plain, 50), "plain_p99_ms": pct(plain, 99),
+        "saturated_over_plain": (sum(tests) / t_sat) / (sum(tests) / t_plain),
+        "peak_jobs_in_flight": peak["jobs_in_flight"],
+        "peak_arena_gb": peak["arena_bytes"] / 2 ** 30,
+    }
+    log(f"serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms at {rate}/s; "
+        f"saturated {out['saturated_tests_per_s']:.3e} tests/s vs plain "
+        f"{out['plain_tests_per_s']:.3e} tests/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +207,10 @@ def main():
     ap.add_argument("--workload", default="histogram")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving measurement (p50/p99 job latency)")
+    ap.add_argument("--serving-jobs", type=int, default=64)
+    ap.add_argument("--serving-rate", type=float, default=8.0)
     ap.add_argument("--count-lds", type=int, default=None,
                     help="tuning: LDS bytes of the count kernel (0 sends every column to the "
                          "split/reference/target kernels)")
@@ -213,9 +301,13 @@ def main():
         log(f"CPU baseline: {cpu['value']:.3e} tests/s on {cpu['cores']} threads")
     pb.close()
     m.close()
+    serving = None
+    if not args.no_serving:
+        log("serving workload")
+        serving = serving_bench(n_jobs=args.serving_jobs, rate=args.serving_rate)
     traffic = traffic_from_profiles(dom, args.workload)
     line = {
-        "metric": "gene x target MWU tests/sec on 1 MI355X (batched form)",
+        "metric": "gene\u00d7target MWU tests/sec on 1 MI355X, batched vs plain; p50/p99 job latency",
         "value": value,
         "unit": "tests/s",
         "n_gpus": 1,
@@ -239,6 +331,7 @@ def main():
         "routing": info,
         "graph_ms": kernel_ms,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

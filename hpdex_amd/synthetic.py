@@ -14,6 +14,8 @@ Rules the workload descriptions leave open, stated here:
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 
@@ -182,6 +184,70 @@ def subset_columns(w: dict, cols) -> dict:
     out = dict(w)
     out.update(data=w["data"][idx], indices=w["indices"][idx], indptr=new_ip, n_cols=len(cols))
     return out
+
+
+def _counts_matrix_parallel(seed, R, dens, mu, r, n_chunks=64, threads=None):
+    """CSC of NB(mu[c], r) + 1 counts at Bernoulli(dens[c]) positions, built
+    over n_chunks fixed column chunks, each with its own generator spawned from
+    `seed` (numpy's samplers release the GIL, so the chunks run on threads; the
+    result does not depend on the thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+    C = len(dens)
+    bounds = np.linspace(0, C, min(n_chunks, max(C, 1)) + 1).astype(np.int64)
+    seqs = np.random.SeedSequence(seed).spawn(len(bounds) - 1)
+
+    def chunk(k):
+        g = np.random.default_rng(seqs[k])
+        a, b = bounds[k], bounds[k + 1]
+        rows, ip = _bernoulli_rows(g, R, dens[a:b])
+        vals = (_nb(g, np.repeat(mu[a:b], np.diff(ip)), r, len(rows)) + 1).astype(np.float32)
+        return rows, ip, vals
+
+    n = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(n) as ex:
+        parts = list(ex.map(chunk, range(len(bounds) - 1)))
+    lens = [len(p[0]) for p in parts]
+    offs = np.concatenate([[0], np.cumsum(lens)])
+    indptr = np.concatenate([[0]] + [p[1][1:] + offs[k] for k, p in enumerate(parts)])
+    rows = np.concatenate([p[0] for p in parts])
+    vals = np.concatenate([p[2] for p in parts])
+    return rows, indptr.astype(np.int64), vals
+
+
+def serving_matrix(rng):
+    """One matrix of the serving distribution: cells U[5k, 200k], genes
+    U[2k, 20k], density U[3%, 12%] (one density per matrix), per-gene NB mean
+    ~ LogNormal(0, 1), r = 0.7, values NB + 1 as float32."""
+    R = int(rng.integers(5_000, 200_001))
+    C = int(rng.integers(2_000, 20_001))
+    dens = float(rng.uniform(0.03, 0.12))
+    mu = rng.lognormal(0.0, 1.0, C)
+    rows, indptr, vals = _counts_matrix_parallel(int(rng.integers(1 << 62)), R, np.full(C, dens),
+                                                 mu, 0.7)
+    return dict(data=vals, indices=rows, indptr=indptr, n_rows=R, n_cols=C, density=dens)
+
+
+def serving_workload(seed: int = 1005, n_matrices: int = 4, n_jobs: int = 64,
+                     resident_frac: float = 0.3):
+    """The serving workload: a pool of matrices drawn from the serving
+    distribution and a job list over it.  Each job takes a pool matrix, a
+    reference of 10% of its cells and 5-500 targets of LogNormal(median 200,
+    sigma 0.9) cells (sizes rescaled to fit the remaining cells); a fraction
+    `resident_frac` of the jobs re-query the matrix resident in HBM with a new
+    target set, the others bring it as a new host matrix."""
+    rng = np.random.default_rng(seed)
+    pool = [serving_matrix(rng) for _ in range(n_matrices)]
+    jobs = []
+    for _ in range(n_jobs):
+        k = int(rng.integers(0, n_matrices))
+        R = pool[k]["n_rows"]
+        n_ref = R // 10
+        T = int(rng.integers(5, 501))
+        sizes = _target_sizes(rng, T, 200, 0.9, 1, R, R - n_ref)
+        gid = _assign_groups(rng, R, n_ref, sizes)
+        jobs.append(dict(matrix=k, group_id=gid, n_targets=T,
+                         resident=bool(rng.random() < resident_frac)))
+    return pool, jobs
 
 
 def algorithmic_bytes(w: dict, value_bytes: int = 4) -> int:
