@@ -554,12 +554,18 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
     }
 }
 
+// Result of one target, the same in every lane of the wave.
+struct TargetResult {
+    double U1, tie;
+    uint32_t n1, n2;
+    int64_t idx;
+};
+
 template <typename V>
-__device__ __forceinline__ void process_target(const JobDesc& J, int64_t c, uint32_t g,
-                                               const ColPlan& pl,
-                                               const RefView<typename KeyOf<V>::K>& ref) {
+__device__ __forceinline__ TargetResult process_target(const JobDesc& J, int64_t c, uint32_t g,
+                                                       const ColPlan& pl,
+                                                       const RefView<typename KeyOf<V>::K>& ref) {
     using K = typename KeyOf<V>::K;
-    const int lane = lane_id();
     const int64_t b = c * J.G;
     const uint32_t nzb = J.cnt[b + g];
     K* seg = (K*)J.keys + J.indptr[c] + J.goff[b + g];
@@ -581,24 +587,35 @@ __device__ __forceinline__ void process_target(const JobDesc& J, int64_t c, uint
         else if (m <= 256) target_chunk<V, 4>(t, seg, m, q, store);
         else target_chunk<V, 8>(t, seg, m, q, store);
     }
+    // butterfly sums: every lane holds the totals
     const uint64_t S = wave_sum_u64(t.S);
     const uint64_t TT = wave_sum_u64(t.TT);
-    if (lane == 0) {
-        uint32_t ltz;
-        if (J.mode == kZeroMin) ltz = 0;
-        else if (J.mode == kZeroMax) ltz = pl.na;
-        else ltz = pl.refneg;
-        const uint64_t S2 = S + (uint64_t)zb * (2ull * ltz + za);
-        const uint64_t tie = TT + pl.TA + tie_f((uint64_t)za + zb);
-        const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
-        const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
-        const int64_t idx = c * J.T + (int64_t)(g - 1);
-        const double U1 = (double)u1x2 * 0.5;
-        const double td = J.tie_corr ? (double)tie : 0.0;
-        J.U1[idx] = U1;
-        if (J.tie) J.tie[idx] = td;
-        if (J.n1o) { J.n1o[idx] = (double)n1; J.n2o[idx] = (double)n2; }
-        if (J.asym) J.P[idx] = p_asymptotic(U1, (double)n1, (double)n2, td, J.cc, J.alt);
+    uint32_t ltz;
+    if (J.mode == kZeroMin) ltz = 0;
+    else if (J.mode == kZeroMax) ltz = pl.na;
+    else ltz = pl.refneg;
+    const uint64_t S2 = S + (uint64_t)zb * (2ull * ltz + za);
+    const uint64_t tie = TT + pl.TA + tie_f((uint64_t)za + zb);
+    const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
+    const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
+    TargetResult r;
+    r.idx = c * J.T + (int64_t)(g - 1);
+    r.U1 = (double)u1x2 * 0.5;
+    r.tie = J.tie_corr ? (double)tie : 0.0;
+    r.n1 = n1;
+    r.n2 = n2;
+    return r;
+}
+
+// Outputs of up to 64 targets held one per lane: written, and their FP64
+// asymptotic p computed, by the lanes in parallel.
+__device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResult& r, int cnt) {
+    if (lane_id() < cnt) {
+        gl(J.U1)[r.idx] = r.U1;
+        if (J.tie) gl(J.tie)[r.idx] = r.tie;
+        if (J.n1o) { gl(J.n1o)[r.idx] = (double)r.n1; gl(J.n2o)[r.idx] = (double)r.n2; }
+        if (J.asym)
+            gl(J.P)[r.idx] = p_asymptotic(r.U1, (double)r.n1, (double)r.n2, r.tie, J.cc, J.alt);
     }
 }
 
@@ -644,13 +661,22 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
     }
     if (tid == 0) st.next_t = 1u;
     __syncthreads();
+    // lane k of the wave keeps the k-th finished target until 64 are pending
+    TargetResult mine{};
+    int pending = 0;
     for (;;) {
         uint32_t g = 0;
         if (lane == 0) g = atomicAdd(&st.next_t, 1u);
         g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
         if (g >= (uint32_t)J.G) break;
-        process_target<V>(J, c, g, pl, rv);
+        const TargetResult r = process_target<V>(J, c, g, pl, rv);
+        if (lane == pending) mine = r;
+        if (++pending == kWave) {
+            flush_targets(J, mine, pending);
+            pending = 0;
+        }
     }
+    flush_targets(J, mine, pending);
 }
 
 // ---------------------------------------------------------------------------

@@ -134,8 +134,9 @@ def float_case(seed: int = 1003, n_cells: int = 400_000, n_genes: int = 18_000,
     clipped to [10, 5,000]."""
     rng = np.random.default_rng(seed)
     mu = rng.lognormal(0.0, 1.2, n_genes)
-    rows, indptr = _bernoulli_rows(rng, n_cells, np.full(n_genes, density))
-    raw = _nb(rng, np.repeat(mu, np.diff(indptr)), 0.6, len(rows)).astype(np.float64)
+    rows, indptr, raw = _counts_matrix_parallel(int(rng.integers(1 << 62)), n_cells,
+                                                np.full(n_genes, density), mu, 0.6,
+                                                plus_one=False)
     lib = np.bincount(rows, weights=raw, minlength=n_cells)
     lib[lib == 0] = 1.0
     vals = np.log1p(raw / lib[rows] * 1e4).astype(np.float32)
@@ -186,8 +187,8 @@ def subset_columns(w: dict, cols) -> dict:
     return out
 
 
-def _counts_matrix_parallel(seed, R, dens, mu, r, n_chunks=64, threads=None):
-    """CSC of NB(mu[c], r) + 1 counts at Bernoulli(dens[c]) positions, built
+def _counts_matrix_parallel(seed, R, dens, mu, r, n_chunks=64, threads=None, plus_one=True):
+    """CSC of NB(mu[c], r) (+ 1) counts at Bernoulli(dens[c]) positions, built
     over n_chunks fixed column chunks, each with its own generator spawned from
     `seed` (numpy's samplers release the GIL, so the chunks run on threads; the
     result does not depend on the thread count)."""
@@ -200,7 +201,8 @@ def _counts_matrix_parallel(seed, R, dens, mu, r, n_chunks=64, threads=None):
         g = np.random.default_rng(seqs[k])
         a, b = bounds[k], bounds[k + 1]
         rows, ip = _bernoulli_rows(g, R, dens[a:b])
-        vals = (_nb(g, np.repeat(mu[a:b], np.diff(ip)), r, len(rows)) + 1).astype(np.float32)
+        vals = _nb(g, np.repeat(mu[a:b], np.diff(ip)), r, len(rows))
+        vals = (vals + 1).astype(np.float32) if plus_one else vals.astype(np.float32)
         return rows, ip, vals
 
     n = threads or min(16, os.cpu_count() or 1)
