@@ -29,7 +29,7 @@ EXPORTED = (
     "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
     "hpdex_bandwidth_probe", "hpdex_plan_get_info", "hpdex_plan_stage_times",
     "hpdex_session_create", "hpdex_session_submit", "hpdex_session_wait", "hpdex_session_peak",
-    "hpdex_session_destroy",
+    "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc",
 )
 
 
@@ -113,6 +113,9 @@ def lib() -> ctypes.CDLL:
         L.hpdex_session_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(JobTimes), cp, sz]
         L.hpdex_session_peak.argtypes = [vp, ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_size_t)]
+        L.hpdex_fdr_bh.argtypes = [vp, sz, vp, cp, sz]
+        L.hpdex_de_csc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz, ci, ci, ctypes.c_double, ci,
+                                   vp, vp, vp, vp, vp, cp, sz]
         L.hpdex_session_destroy.argtypes = [vp]
         L.hpdex_session_destroy.restype = None
         L.hpdex_plan_stage_times.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
@@ -121,7 +124,8 @@ def lib() -> ctypes.CDLL:
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
                      "hpdex_device_info", "hpdex_bandwidth_probe", "hpdex_plan_get_info",
                      "hpdex_plan_stage_times", "hpdex_session_create", "hpdex_session_submit",
-                     "hpdex_session_wait", "hpdex_session_peak"):
+                     "hpdex_session_wait", "hpdex_session_peak", "hpdex_fdr_bh",
+                     "hpdex_de_csc"):
             getattr(L, name).restype = ci
         _lib = L
     return _lib
@@ -185,6 +189,36 @@ def group_mean_csc(data, indices, indptr, group_id, n_groups, include_zeros=True
     if rc != 0:
         raise HpdexError(err.value.decode())
     return out[:C * n_groups]
+
+
+def fdr_bh(p):
+    """Benjamini-Hochberg FDR on the device (hpdex_fdr_bh)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+    out = np.empty(max(len(p), 1))
+    err = ctypes.create_string_buffer(1024)
+    if lib().hpdex_fdr_bh(_p(p), len(p), _p(out), err, len(err)) != 0:
+        raise HpdexError(err.value.decode())
+    return out[:len(p)]
+
+
+def de_csc(data, indices, indptr, group_id, n_targets, n_rows=None, tie_correction=True,
+           use_continuity=True, clip_value=20.0, threads=-1):
+    """hpdex_de_csc: (U1, P, fold_change, log2_fold_change, fdr); fold changes
+    target-major (t*C + c), the rest gene-major (c*T + t)."""
+    data, indices, indptr = _csc_args(data, indices, indptr)
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    R = len(group_id) if n_rows is None else int(n_rows)
+    C = len(indptr) - 1
+    n = C * int(n_targets)
+    outs = [np.empty(max(n, 1)) for _ in range(5)]
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().hpdex_de_csc(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr), R, C,
+                            len(data), _p(group_id), int(n_targets), int(tie_correction),
+                            int(use_continuity), float(clip_value), int(threads),
+                            *[_p(o) for o in outs], err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return tuple(o[:n] for o in outs)
 
 
 class DeviceMatrix:

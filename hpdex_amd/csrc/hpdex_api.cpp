@@ -613,6 +613,85 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     return 0;
 }
 
+int hpdex_fdr_bh(const double* p, size_t n, double* fdr_out, char* err, size_t err_len) {
+    if (n == 0) return 0;
+    if (!p || !fdr_out) return set_err(err, err_len, "null argument");
+    if (n > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 p-values");
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    DevBuf dp, df, ws;
+    HIPCHK(dp.alloc(n * 8));
+    HIPCHK(df.alloc(n * 8));
+    const size_t wsb = fdr_workspace_bytes((int64_t)n);
+    HIPCHK(ws.alloc(wsb));
+    HIPCHK(hipMemcpyAsync(dp.p, p, n * 8, hipMemcpyHostToDevice, st.s));
+    HIPCHK(launch_fdr_bh(dp.as<double>(), (int64_t)n, df.as<double>(), ws.p, wsb, st.s));
+    HIPCHK(hipMemcpyAsync(fdr_out, df.p, n * 8, hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
+int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                 size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                 size_t n_targets, int tie_correction, int use_continuity, double clip_value,
+                 int threads, double* U1_out, double* P_out, double* fc_out,
+                 double* log2fc_out, double* fdr_out, char* err, size_t err_len) {
+    (void)threads;
+    if (n_targets < 1)
+        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (n_cols == 0) return 0;
+    const size_t G = n_targets + 1, np = n_cols * n_targets;
+    if (16 * G > (size_t)kLdsBudget)
+        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
+    if (np > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 tests");
+    if (ensure_init(err, err_len)) return 1;
+    const Tuning tu = tuning();
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix m;
+    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
+        return 1;
+    // de.py:147-158: zero_handling "min", two-sided, asymptotic
+    hpdex_mwu_opts o{};
+    o.tie_correction = tie_correction ? 1 : 0;
+    o.use_continuity = use_continuity ? 1 : 0;
+    o.zero_handling = 1;
+    o.alternative = 2;
+    o.method = 2;
+    JobState js;
+    if (job_setup(js, &m, group_id, n_targets, &o, false, st.s, err, err_len)) return 1;
+    const JobDesc J = make_desc(js, &m, tu);
+    if (job_upload_desc(js, J, m, tu, false, st.s, err, err_len)) return 1;
+    // de.py:126-142: group means with zeros, fold change, log2 fold change
+    DevBuf means, fc, lfc, fdr, ws;
+    HIPCHK(means.alloc(n_cols * G * 8));
+    HIPCHK(launch_group_mean(m.vals.p, m.f64, m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
+                             js.gid.as<int32_t>(), js.grows.as<uint32_t>(), (int64_t)n_cols,
+                             (int32_t)n_rows, (int32_t)G, 1, means.as<double>(), st.s));
+    HIPCHK(fc.alloc(np * 8));
+    HIPCHK(lfc.alloc(np * 8));
+    HIPCHK(launch_fold_change(means.as<double>(), (int64_t)n_cols, (int32_t)n_targets, clip_value,
+                              fc.as<double>(), lfc.as<double>(), st.s));
+    // de.py:147-158: the test
+    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), st.s));
+    if (job_errors(js, st.s, err, err_len)) return 1;
+    // de.py:172-184: BH FDR over every p-value
+    HIPCHK(fdr.alloc(np * 8));
+    const size_t wsb = fdr_workspace_bytes((int64_t)np);
+    HIPCHK(ws.alloc(wsb));
+    HIPCHK(launch_fdr_bh(js.P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, st.s));
+    const size_t b = np * 8;
+    if (U1_out) HIPCHK(hipMemcpyAsync(U1_out, js.U1.p, b, hipMemcpyDeviceToHost, st.s));
+    if (P_out) HIPCHK(hipMemcpyAsync(P_out, js.P.p, b, hipMemcpyDeviceToHost, st.s));
+    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, st.s));
+    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, st.s));
+    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
 int hpdex_matrix_create(int dtype, const void* data, const int64_t* indices,
                         const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
                         hpdex_matrix** out, char* err, size_t err_len) {

@@ -158,4 +158,14 @@ hipError_t launch_group_mean(const void* vals, bool f64, const int32_t* rows,
                              int include_zeros, double* out, hipStream_t s);
 hipError_t launch_stream_copy(const void* in, void* out, size_t bytes, int grid, hipStream_t s);
 
+// ---- DE post-processing (hpdex_de.hip) ----
+// fc/log2fc[t*C + c] from group means[c*(T+1) + g] (de.py:126-142)
+hipError_t launch_fold_change(const double* means, int64_t C, int32_t T, double clip, double* fc,
+                              double* log2fc, hipStream_t s);
+// Benjamini-Hochberg FDR of n p-values (n < 2^32) in a workspace of
+// fdr_workspace_bytes(n); NaN everywhere if a p-value is outside [0, 1].
+size_t fdr_workspace_bytes(int64_t n);
+hipError_t launch_fdr_bh(const double* p, int64_t n, double* fdr, void* ws, size_t ws_bytes,
+                         hipStream_t s);
+
 }  // namespace hpdex

@@ -84,6 +84,27 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
                          const int32_t* group_id, size_t n_groups, int include_zeros,
                          int use_kahan, int threads, double* out, char* err, size_t err_len);
 
+/* ---------------- differential-expression post-processing ----------------
+ * Benjamini-Hochberg FDR of n p-values on the device; replaces
+ * scipy.stats.false_discovery_control(P, method="bh") as called by
+ * src/hpdex/de.py:172-184 (including its fallback: if a p-value is NaN or
+ * outside [0, 1], every output is NaN). */
+int hpdex_fdr_bh(const double* p, size_t n, double* fdr_out, char* err, size_t err_len);
+
+/* The numeric part of parallel_differential_expression (src/hpdex/de.py:120-184)
+ * in one device pass: group means with zeros (include_zeros=True) of the
+ * reference and every target, fold change and log2 fold change with the clip
+ * rules of de.py:136-141, the Mann-Whitney U test with zero_handling "min"
+ * (two-sided, asymptotic), and the BH FDR of every p-value.  Layouts are
+ * de.py's: fc_out / log2fc_out are n_targets x n_cols target-major (t*C + c);
+ * U1_out, P_out, fdr_out are n_cols x n_targets gene-major (c*T + t), as
+ * mannwhitneyu returns them.  Any output pointer may be NULL. */
+int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                 size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                 size_t n_targets, int tie_correction, int use_continuity, double clip_value,
+                 int threads, double* U1_out, double* P_out, double* fc_out,
+                 double* log2fc_out, double* fdr_out, char* err, size_t err_len);
+
 /* ---------------- device-resident matrices and batched plans ----------------
  * No counterpart in the reference (its API is one call per job).  They let a
  * caller keep a CSC matrix in HBM and run several target sets against it.   */
