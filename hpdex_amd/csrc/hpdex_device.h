@@ -34,6 +34,8 @@ template <> struct KeyOf<float> {
         iv = (int)v;
         return (float)iv == v;
     }
+    // |v| <= lim, false for NaN
+    __device__ static inline bool abs_le(float v, float lim) { return __builtin_fabsf(v) <= lim; }
 };
 
 template <> struct KeyOf<double> {
@@ -50,6 +52,7 @@ template <> struct KeyOf<double> {
         iv = (int)v;
         return (double)iv == v;
     }
+    __device__ static inline bool abs_le(double v, double lim) { return __builtin_fabs(v) <= lim; }
 };
 
 // ---------------------------------------------------------------------------

@@ -746,14 +746,15 @@ __device__ __forceinline__ void for_each_word(bool staged, const uint32_t* stage
 struct PairCounters {
     uint32_t* w;
     uint32_t rs;
+    uint32_t row0;   // group of the first row: 0 (reference row kept here) or 1
     bool wide;
     __device__ __forceinline__ void inc(uint32_t g, uint32_t d) const {
-        const uint32_t q = (g - 1u) * rs + d;
+        const uint32_t q = (g - row0) * rs + d;
         if (wide) atomicAdd(&w[q], 1u);
         else atomicAdd(&w[q >> 1], 1u << ((q & 1u) << 4));
     }
     __device__ __forceinline__ uint32_t get(uint32_t g, uint32_t d) const {
-        const uint32_t q = (g - 1u) * rs + d;
+        const uint32_t q = (g - row0) * rs + d;
         return wide ? w[q] : ((w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu);
     }
 };
@@ -807,8 +808,10 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
     };
     // (sizes below stay far under 2^32: nT < 2^16, rs <= 8194, free0 < 2^16)
     const uint32_t cap1 = free0 * per_word;   // counters that fit without staging
+    // one pass: every group's row of NB bins plus a zero column, the reference
+    // histogram in row 0
     const bool single =
-        __builtin_amdgcn_readfirstlane(NB <= 8192u && nT * row_stride(NB) <= cap1 ? 1 : 0) != 0;
+        __builtin_amdgcn_readfirstlane(NB <= 8192u && G * row_stride(NB + 1u) <= cap1 ? 1 : 0) != 0;
     const uint32_t stage_words = (n + 3u) & ~3u;
     const bool staged = !single && __builtin_amdgcn_readfirstlane(
         (uint64_t)stage_words + kCountReserveWords <= (uint64_t)free0 ? 1 : 0) != 0;
@@ -820,7 +823,8 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
         VS = per_row > 3u ? per_row - 3u : 0u;
         if (VS > NB) VS = NB;
     }
-    const uint32_t rs = (uint32_t)__builtin_amdgcn_readfirstlane((int)row_stride(VS));
+    const uint32_t rs =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)row_stride(single ? NB + 1u : VS));
     const uint32_t nwin =
         (uint32_t)__builtin_amdgcn_readfirstlane((int)(VS ? (NB + VS - 1u) / VS : 0u));
     const int handover = __builtin_amdgcn_readfirstlane(
@@ -836,8 +840,10 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
     PairCounters pc;
     pc.w = tab + tab_words;
     pc.rs = rs;
+    pc.row0 = single ? 0u : 1u;
     pc.wide = wide;
-    const uint32_t cell_words = wide ? nT * rs : (nT * rs + 1u) / 2u;
+    const uint32_t n_rows = single ? G : nT;
+    const uint32_t cell_words = wide ? n_rows * rs : (n_rows * rs + 1u) / 2u;
     uint32_t* stage = pc.w + ((cell_words + 3u) & ~3u);
     // word offsets from the start of LDS, for the read pass's single atomic
     const uint32_t cez_off = (uint32_t)(cez - lds_words);
@@ -860,12 +866,66 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
     // cache lines.  kUnroll rows and values per lane are in flight, read with
     // a non-temporal hint (used once) at 32-bit offsets from the column start.
     // Whole batches run without bounds checks; the last one checks.
-    {
+    const HPDEX_GLOBAL uint16_t* gid16 = gl(J.gid16);
+    const HPDEX_GLOBAL int32_t* rbase = gl(J.rows) + p0;
+    const HPDEX_GLOBAL V* vbase = gl(vals) + p0;
+    const uint32_t step = (uint32_t)nt * kUnroll;
+    if (single) {
+        // every counter lives in one array: word and increment of an entry
+        // follow from (group, bin) alone -- column NB counts explicit zeros.
+        // Values are integers in [-32768, 32767] by the matrix's column
+        // metadata; a bin outside [0, NB) would mean the metadata no longer
+        // matches and hands the column over.
         bool bad = false;
-        const HPDEX_GLOBAL uint16_t* gid16 = gl(J.gid16);
-        const HPDEX_GLOBAL int32_t* rbase = gl(J.rows) + p0;
-        const HPDEX_GLOBAL V* vbase = gl(vals) + p0;
-        const uint32_t step = (uint32_t)nt * kUnroll;
+        auto batch1 = [&](uint32_t ib, auto full_tag, auto wide_tag) {
+            constexpr bool kFull = decltype(full_tag)::value;
+            constexpr bool kWide = decltype(wide_tag)::value;
+            uint32_t r[kUnroll];
+            V v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t i = ib + (uint32_t)(u * nt + tid);
+                if (kFull || i < n) {
+                    r[u] = (uint32_t)__builtin_nontemporal_load(rbase + i);
+                    v[u] = __builtin_nontemporal_load(vbase + i);
+                } else {
+                    r[u] = 0xFFFFFFFFu;
+                    v[u] = V(0);
+                }
+            }
+            uint32_t g[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) g[u] = (uint32_t)gid16[min(r[u], R)];
+            const uint32_t dummy = dummy_off + (uint32_t)lane_id();
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t gg = g[u];
+                const V vv = v[u];
+                const bool fin = KT::abs_le(vv, V(32767));
+                const bool zero = vv == V(0);
+                const uint32_t b = (uint32_t)((int)(fin ? vv : V(0)) - vmin);
+                const bool inb = b < NB;
+                const bool ingroup = gg < G;
+                bad |= ingroup && fin && !zero && !inb;
+                const uint32_t q = gg * rs + (zero ? NB : b);
+                const uint32_t word = (ingroup && (zero || (fin && inb)))
+                                          ? cell_off + (kWide ? q : (q >> 1)) : dummy;
+                atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
+            }
+        };
+        const uint32_t nfull = n - n % step;
+        if (wide) {
+            for (uint32_t ib = 0; ib < nfull; ib += step)
+                batch1(ib, std::true_type{}, std::true_type{});
+            if (nfull < n) batch1(nfull, std::false_type{}, std::true_type{});
+        } else {
+            for (uint32_t ib = 0; ib < nfull; ib += step)
+                batch1(ib, std::true_type{}, std::false_type{});
+            if (nfull < n) batch1(nfull, std::false_type{}, std::false_type{});
+        }
+        if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
+    } else {
+        bool bad = false;
         auto batch = [&](uint32_t ib, auto full_tag) {
             constexpr bool kFull = decltype(full_tag)::value;
             uint32_t r[kUnroll];
@@ -934,6 +994,14 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
         return;
     }
 
+    // one-pass layout: the reference histogram and the zero counts sit in the
+    // counter array; copy them out for the scan and the sizes
+    if (single) {
+        for (uint32_t b = tid; b < NB; b += nt) tab[b] = pc.get(0u, b);
+        for (uint32_t g = tid; g < G; g += nt) cez[g] = pc.get(g, NB);
+        __syncthreads();
+    }
+
     // reference tie sum, exclusive prefix: tab[b] = #reference values below
     // bin b, tab[NB] = #reference nonzero values
     uint64_t ta = 0;
@@ -966,11 +1034,11 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_
                 k += c;
             };
             if (wide) {
-                const uint32_t* row = pc.w + (g - 1u) * rs;
+                const uint32_t* row = pc.w + (g - pc.row0) * rs;
                 for (uint32_t d = 0; d < wn; ++d) bin(w0 + d, row[d]);
             } else {
                 // rows start on an even counter: two counters per word
-                const uint32_t* row = pc.w + (((g - 1u) * rs) >> 1);
+                const uint32_t* row = pc.w + (((g - pc.row0) * rs) >> 1);
                 for (uint32_t d = 0; d < wn; d += 2u) {
                     const uint32_t cw = row[d >> 1];
                     bin(w0 + d, cw & 0xFFFFu);
