@@ -46,7 +46,7 @@ class Tuning(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
-        "count_columns", "count_nnz", "split_columns", "handed_over")]
+        "count_columns", "count_nnz", "split_columns", "handed_over", "count_blocks")]
 
 
 class SessionOpts(ctypes.Structure):

@@ -10,6 +10,7 @@ _lib = None
 # Plain build for in-process use (AddressSanitizer needs its runtime preloaded:
 # run build_sim.py directly and LD_PRELOAD libasan for that).
 NOSAN = os.path.join(build_sim.OUT, "libhpdex_cpusim_nosan.so")
+PAD = 8192   # kMatrixPad: entries a resident matrix keeps past its end
 
 
 def lib(path=None):
@@ -27,19 +28,20 @@ def lib(path=None):
         L.sim_mwu.argtypes = [ci, vp, vp, vp, ci, ci, vp, ci, ci, ci, ci, ci, ci, ci, ci,
                               vp, vp, vp, vp, vp, vp, vp, ctypes.c_double]
         L.sim_mwu.restype = ci
+        L.sim_count_blocks.restype = ci
         _lib = L
     return _lib
 
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
-        count_lds=76 * 1024, block=512, hist_max=8192, f64=False, timeout_s=30.0):
+        count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels)."""
-    # laid out like a resident matrix: 16 padding entries (value 0, row -1)
+    # laid out like a resident matrix: PAD padding entries (value 0, row -1)
     vals = np.concatenate([np.asarray(data, dtype=np.float64 if f64 else np.float32),
-                           np.zeros(16, dtype=np.float64 if f64 else np.float32)])
+                           np.zeros(PAD, dtype=np.float64 if f64 else np.float32)])
     rows = np.concatenate([np.where((indices >= 0) & (indices < n_rows), indices, -1),
-                           np.full(16, -1)]).astype(np.int32)
+                           np.full(PAD, -1)]).astype(np.int32)
     indptr = np.ascontiguousarray(indptr, dtype=np.int64)
     gid = np.ascontiguousarray(group_id, dtype=np.int32)
     C = len(indptr) - 1
@@ -52,4 +54,5 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
                   float(timeout_s))
     mwu.last_counted = counted   # columns the count kernel finished itself
+    mwu.last_blocks = lib().sim_count_blocks()   # its workgroups (column x target slice)
     return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

@@ -26,6 +26,8 @@ using namespace hpdex;
 
 namespace {
 
+int sim_last_blocks = 0;   // count-kernel workgroups of the last sim_mwu call
+
 struct ThreadInfo { std::atomic<const char*>* last; };
 
 // Run one workgroup: nt host threads execute `body` with threadIdx set.
@@ -86,26 +88,47 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
                                                               col_int.data(), range); }, timeout_s);
-    const bool count_ok = count_lds > 0 && (uint32_t)J.G <= kCountMaxGroups &&
-                          4u * kCountDummyWords + 24u * (uint32_t)J.G + 4u * kCountReserveWords <= count_lds;
-    std::vector<int32_t> count_cols;
+    // the host's routing (job_upload_desc): count workgroups per target slice
+    const uint32_t T = (uint32_t)J.T;
+    const bool count_ok = count_lds > 0 && J.G >= 2 && (uint32_t)J.G <= kCountMaxGroups;
+    std::vector<CountItem> items;
     std::vector<WorkItem> gen_items(C + 1);
     uint32_t n_static = 0;
+    unsigned nc = 0;   // counted columns
     for (int32_t c : order) {
         const int64_t bins = (int64_t)range[2 * c + 1] - range[2 * c] + 1;
-        if (count_ok && col_int[c] && bins <= J.hist_max) count_cols.push_back(c);
-        else gen_items[n_static++] = WorkItem{0, c};
+        const int64_t len = J.indptr[c + 1] - J.indptr[c];
+        const uint32_t S = (count_ok && col_int[c] && bins <= J.hist_max && len < ((int64_t)1 << 28))
+                               ? count_slice_size(T, (uint32_t)(bins > 0 ? bins : 1), len, count_lds)
+                               : 0u;
+        if (S == 0u) {
+            gen_items[n_static++] = WorkItem{0, c};
+            continue;
+        }
+        CountItem it{};
+        it.p0 = J.indptr[c];
+        it.n = (uint32_t)len;
+        it.vmin = bins > 0 ? range[2 * c] : 0;
+        it.nb = bins > 0 ? (uint32_t)bins : 1u;
+        it.col = c;
+        it.S = (uint16_t)S;
+        for (uint32_t t0 = 0; t0 < T; t0 += S) {
+            it.t0 = (uint16_t)t0;
+            items.push_back(it);
+        }
+        ++nc;
     }
+    sim_last_blocks = (int)items.size();
     uint32_t gen_n = n_static;
-    const unsigned nc = (unsigned)count_cols.size();
-    for (unsigned b = 0; b < nc; ++b)
-        run_block(b, count_block, nc, [&] {
+    const unsigned nb = (unsigned)items.size();
+    for (unsigned b = 0; b < nb; ++b)
+        run_block(b, count_block, nb, [&] {
             if (count_block == 256)
-                count_kernel<V, 256>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 256>(J, items.data(), gen_items.data(), &gen_n, count_lds);
             else if (count_block == 1024)
-                count_kernel<V, 1024>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 1024>(J, items.data(), gen_items.data(), &gen_n, count_lds);
             else
-                count_kernel<V, 512>(J, count_cols.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
     const unsigned ng = n_static + nc;
     const WorkItem* it = gen_items.data();
@@ -149,7 +172,7 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.plan = plan.data(); J.reftab = reftab.data();
     J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
-    J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
+    J.C = C; J.n_alloc = (int64_t)nnz + (int64_t)kMatrixPad; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
     std::vector<int32_t> range(2 * (size_t)C + 2);
     J.colrange = range.data();
@@ -161,3 +184,6 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, count_block,
                                timeout_s);
 }
+
+// Count-kernel workgroups (column x target slice) of the last sim_mwu call.
+extern "C" int sim_count_blocks() { return sim_last_blocks; }

@@ -36,6 +36,8 @@ using hipEvent_t = void*;
 constexpr int hipSuccess = 0;
 constexpr int hipErrorInvalidValue = 1;
 struct uint4 { unsigned x, y, z, w; };
+struct uint2 { unsigned x, y; };
+inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
 
 struct SimDim { unsigned x = 0, y = 0, z = 0; };
 extern thread_local SimDim threadIdx;
@@ -103,6 +105,37 @@ inline unsigned long long __ballot(int p) {
 inline int __all(int p) { return __ballot(p) == ~0ull; }
 inline int __any(int p) { return __ballot(p) != 0ull; }
 template <typename T> inline T sim_plain_load(const T* p) { return *p; }
+// Buffer loads: a resource is a base address and a size in bytes; a load
+// reads base + voffset + soffset and aborts when that lies past the size
+// (stricter than the hardware, which returns 0 there).
+struct __amdgpu_buffer_rsrc_t {
+    const unsigned char* base;
+    unsigned long long bytes;
+};
+inline __amdgpu_buffer_rsrc_t __builtin_amdgcn_make_buffer_rsrc(void* p, short, unsigned bytes, int) {
+    return {(const unsigned char*)p, bytes};
+}
+template <typename T> inline T sim_buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    const unsigned long long off = (unsigned long long)(unsigned)voff + (unsigned)soff;
+    if (off + sizeof(T) > r.bytes) {
+        std::fprintf(stderr, "buffer load out of range: offset %llu + %zu > %llu bytes\n", off,
+                     sizeof(T), r.bytes);
+        std::abort();
+    }
+    T v;
+    std::memcpy(&v, r.base + off, sizeof(T));
+    return v;
+}
+inline unsigned __builtin_amdgcn_raw_buffer_load_b32(__amdgpu_buffer_rsrc_t r, int voff, int soff, int) {
+    return sim_buffer_load<unsigned>(r, voff, soff);
+}
+inline unsigned short __builtin_amdgcn_raw_buffer_load_b16(__amdgpu_buffer_rsrc_t r, int voff, int soff, int) {
+    return sim_buffer_load<unsigned short>(r, voff, soff);
+}
+inline unsigned long long __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, int voff, int soff, int) {
+    return sim_buffer_load<unsigned long long>(r, voff, soff);
+}
+inline void __builtin_amdgcn_sched_barrier(int) {}
 inline int __builtin_amdgcn_readfirstlane(int v) {
     sim::t_last_op = "readfirstlane";
     const int first = sim::exchange(v, 0u);

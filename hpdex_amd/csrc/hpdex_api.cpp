@@ -110,7 +110,7 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     m->nnz = nnz;
     const size_t vsz = m->f64 ? 8 : 4;
     // kMatrixPad extra entries (rows -1, values 0) let the count kernel read
-    // whole 16-byte groups of four entries past the last column's end
+    // a whole batch past the last column's end
     HIPCHK(m->vals.alloc((nnz + kMatrixPad) * vsz));
     HIPCHK(m->rows.alloc((nnz + kMatrixPad) * 4));
     HIPCHK(hipMemsetAsync((char*)m->vals.p + nnz * vsz, 0, kMatrixPad * vsz, s));
@@ -234,6 +234,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.status = js.status();
     J.status_col = js.status_col();
     J.C = (int64_t)m->C;
+    J.n_alloc = (int64_t)(m->nnz + kMatrixPad);
     J.R = (int32_t)m->R;
     J.G = (int32_t)js.G;
     J.T = (int32_t)js.T;
@@ -246,39 +247,59 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     return J;
 }
 
-// Put the job descriptor and its column queues on the device: integer-valued
-// columns to the count kernel, the rest to the split/reference/target kernels;
-// each queue in column order (plain form) or largest first (batched form).
+// Put the job descriptor and its work queues on the device: integer-valued
+// columns to the count kernel, one workgroup per slice of targets whose
+// counter rows fit its LDS (count_slice_size), the rest to the
+// split/reference/target kernels; each queue in column order (plain form) or
+// largest first (batched form).  Columns of 2^28 entries or more take the
+// split path: the count kernel addresses a column with 32-bit byte offsets.
 int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const Tuning& tu,
                     bool size_ordered, hipStream_t s, char* err, size_t err_len) {
     js.J = J;
     HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
     HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
-    // the count kernel takes the job if its per-group LDS state leaves room
-    const bool count_ok = tu.count_lds > 0 && js.G <= kCountMaxGroups &&
-                          4u * kCountDummyWords + 24u * js.G + 4u * kCountReserveWords <= (size_t)tu.count_lds;
-    std::vector<int32_t> count_cols;
+    const bool count_ok = tu.count_lds > 0 && js.G >= 2 && js.G <= kCountMaxGroups;
+    const uint32_t T = (uint32_t)js.T;
+    std::vector<CountItem> count_items;
     std::vector<WorkItem> gen_items;
-    count_cols.reserve(m.C);
+    count_items.reserve(m.C);
     gen_items.reserve(m.C);
+    js.n_count_cols = 0;
+    js.count_nnz = 0;
     for (size_t k = 0; k < m.C; ++k) {
         const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
         const int64_t len = m.h_indptr[c + 1] - m.h_indptr[c];
         const int64_t bins = (int64_t)m.h_range[2 * c + 1] - m.h_range[2 * c] + 1;
-        if (count_ok && m.col_int[c] && bins <= tu.hist_max && len < ((int64_t)1 << 31)) {
-            count_cols.push_back(c);
-            js.count_nnz += len;
-        } else
+        const uint32_t S =
+            (count_ok && m.col_int[c] && bins <= tu.hist_max && len < ((int64_t)1 << 28))
+                ? count_slice_size(T, (uint32_t)(bins > 0 ? bins : 1), len, (uint32_t)tu.count_lds)
+                : 0u;
+        if (S == 0u) {
             gen_items.push_back(WorkItem{0, c});
+            continue;
+        }
+        CountItem it{};
+        it.p0 = m.h_indptr[c];
+        it.n = (uint32_t)len;
+        it.vmin = bins > 0 ? m.h_range[2 * c] : 0;
+        it.nb = bins > 0 ? (uint32_t)bins : 1u;
+        it.col = c;
+        it.S = (uint16_t)S;
+        for (uint32_t t0 = 0; t0 < T; t0 += S) {
+            it.t0 = (uint16_t)t0;
+            count_items.push_back(it);
+        }
+        js.n_count_cols += 1;
+        js.count_nnz += len;
     }
-    js.n_count = (int)count_cols.size();
+    js.n_count = (int)count_items.size();
     js.n_gen_static = (int)gen_items.size();
     js.count_lds = (uint32_t)tu.count_lds;
     js.count_block = tu.count_block;
-    HIPCHK(js.d_count_cols.alloc(count_cols.size() * sizeof(int32_t)));
-    if (!count_cols.empty())
-        HIPCHK(hipMemcpyAsync(js.d_count_cols.p, count_cols.data(),
-                              count_cols.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(js.d_count_items.alloc(count_items.size() * sizeof(CountItem)));
+    if (!count_items.empty())
+        HIPCHK(hipMemcpyAsync(js.d_count_items.p, count_items.data(),
+                              count_items.size() * sizeof(CountItem), hipMemcpyHostToDevice, s));
     // room for every column: the count kernel appends those it hands over
     HIPCHK(js.d_gen_items.alloc(m.C * sizeof(WorkItem)));
     if (!gen_items.empty())
@@ -293,8 +314,9 @@ PipelineLaunch pipeline_of(const JobState& js, bool f64) {
     PipelineLaunch L{};
     L.jobs = js.d_job.as<JobDesc>();
     L.job = js.J;
-    L.count_cols = js.d_count_cols.as<int32_t>();
+    L.count_items = js.d_count_items.as<CountItem>();
     L.n_count = js.n_count;
+    L.n_count_cols = js.n_count_cols;
     L.gen_items = js.d_gen_items.as<WorkItem>();
     L.gen_n = js.d_gen_n.as<uint32_t>();
     L.n_gen_static = js.n_gen_static;
@@ -805,7 +827,8 @@ void hpdex_plan_destroy(hpdex_plan* p) { delete p; }
 
 int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len) {
     if (!p || !out) return set_err(err, err_len, "null handle");
-    out->count_columns = p->js.n_count;
+    out->count_columns = p->js.n_count_cols;
+    out->count_blocks = p->js.n_count;
     out->count_nnz = p->js.count_nnz;
     out->split_columns = p->js.n_gen_static;
     out->handed_over = 0;

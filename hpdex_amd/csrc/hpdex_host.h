@@ -130,9 +130,9 @@ struct JobState {
     DevBuf gid, gid16, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
     DevBuf d_job;                                   // descriptor
-    DevBuf d_count_cols, d_gen_items, d_gen_n;      // column queues of the two paths
+    DevBuf d_count_items, d_gen_items, d_gen_n;     // work queues of the two paths
     JobDesc J{};                                    // host copy of the descriptor
-    int n_count = 0, n_gen_static = 0;
+    int n_count = 0, n_count_cols = 0, n_gen_static = 0;   // count workgroups, their columns
     int64_t count_nnz = 0;
     uint32_t count_lds = 0;
     int count_block = kCountBlockDefault;

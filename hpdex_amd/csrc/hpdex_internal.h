@@ -26,14 +26,14 @@ constexpr int kRefKeysF64 = 4096;           // sort path: reference keys kept on
 constexpr int kTabStride = (kHistMaxBins > kMaxBuckets ? kHistMaxBins : kMaxBuckets) + 1;
 
 // Count kernel (integer-valued columns in one pass).
-constexpr int kCountBlockDefault = 512;     // threads per workgroup: 256, 512 or 1024
-constexpr int kCountLdsDefault = 76 * 1024; // dynamic LDS per workgroup: two per CU
+constexpr int kCountBlockDefault = 256;     // threads per workgroup: 256, 512 or 1024
+constexpr int kCountLdsDefault = 40448;     // dynamic LDS per workgroup: four per CU
 constexpr int kCountLdsMax = 152 * 1024;    // largest dynamic LDS the tuning accepts
-constexpr uint32_t kCountReserveWords = 1024;  // LDS words kept for tables beside a staged column
 constexpr uint32_t kCountDummyWords = 64;      // LDS words the read pass's no-op atomics hit
-constexpr int kMaxWindows = 64;             // value windows per column before handing it over
+constexpr uint32_t kMaxSlices = 64;         // target slices per column, beyond: sort path
 constexpr uint32_t kCountMaxGroups = 65535; // group ids are packed in 16 bits
-constexpr size_t kMatrixPad = 16;           // entries past nnz in a resident matrix (rows -1)
+constexpr size_t kMatrixPad = 8192;         // entries past nnz in a resident matrix (rows -1,
+                                            // values 0): one count-kernel batch
 
 // Per-column plan: written by the split kernel (and TA by the reference-table
 // kernel), read by the later kernels of the same job.
@@ -75,6 +75,7 @@ struct JobDesc {
     uint32_t* status;            // kStatus* bits
     unsigned long long* status_col;  // lowest column that raised a status bit
     int64_t C;
+    int64_t n_alloc;             // entries allocated in vals and rows (nnz + kMatrixPad)
     int32_t R;
     int32_t G;                   // n_targets + 1
     int32_t T;
@@ -90,6 +91,67 @@ struct WorkItem {
     int32_t job;
     int32_t col;
 };
+
+// One workgroup of the count kernel: column `col` (its entries [p0, p0+n),
+// its histogram bins: values vmin .. vmin+nb-1) and targets t0+1 ..
+// t0+min(S, T-t0).  S, the column's slice size, is the same in every slice of
+// the column (T <= 65534, so both fit 16 bits).  The host fills the column
+// fields from the matrix's own indptr and value-range metadata.
+struct CountItem {
+    int64_t p0;
+    uint32_t n;
+    int32_t vmin;   // smallest stored nonzero value (0 when there is none)
+    uint32_t nb;    // vmax - vmin + 1 (1 when there is no nonzero value)
+    int32_t col;
+    uint16_t t0;
+    uint16_t S;
+    uint32_t pad;
+};
+
+// LDS of a count-kernel workgroup with `rows` rows (the reference and a
+// slice's targets) over NB value bins plus a zero column:
+//   dummy[kCountDummyWords] | acc[2*rows] u64 | cez, cnz, grs [rows] u32
+//   (rounded to whole 8 B) | bins[NB] of (lt', eq) u32 pairs | counters[rows x rs]
+// Counters are 16-bit (two per word) unless `wide` (a column of 2^16 entries
+// or more: a count could reach 2^16).  The host's slicing and the kernel's
+// own check use these same functions.
+__host__ __device__ inline uint32_t count_row_stride(uint32_t nb1, bool wide) {
+    // counters per row: an odd number of words, so threads that each walk
+    // one row hit different banks
+    if (wide) return nb1 | 1u;
+    uint32_t r = (nb1 + 1u) & ~1u;
+    if (((r >> 1) & 1u) == 0u) r += 2u;
+    return r;
+}
+__host__ __device__ inline uint32_t count_row_words(uint32_t rows) { return (3u * rows + 1u) & ~1u; }
+__host__ __device__ inline uint32_t count_cell_words(uint32_t rows, uint32_t rs, bool wide) {
+    return wide ? rows * rs : (rows * rs + 1u) / 2u;
+}
+__host__ __device__ inline uint64_t count_lds_need(uint32_t rows, uint32_t NB, bool wide) {
+    const uint32_t rs = count_row_stride(NB + 1u, wide);
+    return 4ull * kCountDummyWords + 16ull * rows + 4ull * count_row_words(rows) + 8ull * NB +
+           4ull * (uint64_t)count_cell_words(rows, rs, wide);
+}
+
+// Slice size of a count-kernel column with T targets, NB bins and n stored
+// entries in workgroups of lds_bytes: the largest S whose S + 1 rows fit,
+// then evened out over the slices that takes.  0 when one target does not
+// fit or the column would need more than kMaxSlices slices.
+inline uint32_t count_slice_size(uint32_t T, uint32_t NB, int64_t n, uint32_t lds_bytes) {
+    const bool wide = n >= 65536;
+    if (T == 0u || T > 65534u || NB == 0u || NB > (uint32_t)kHistMaxBins ||
+        count_lds_need(2u, NB, wide) > lds_bytes)
+        return 0u;
+    uint32_t lo = 1u, hi = T;   // need() grows with rows
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1u) / 2u;
+        if (count_lds_need(mid + 1u, NB, wide) <= lds_bytes) lo = mid;
+        else hi = mid - 1u;
+    }
+    const uint32_t k = (T + lo - 1u) / lo;
+    if (k > kMaxSlices) return 0u;
+    return (T + k - 1u) / k;
+}
 
 // Exact-null table for one distinct (n1, n2): layout in the table buffer is
 // [total][counts 0..U][cdf 0..U] doubles starting at `offset`.
@@ -119,15 +181,17 @@ hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n,
 bool dtype_uses_double(int dtype);
 
 // One launch sequence of the Mann-Whitney pass of one job.  Integer-valued
-// columns go to the count kernel (count_cols); the rest, and every column the count kernel
-// hands over at run time, to the split -> reference -> target kernels, which
-// read their items from gen_items[0 .. *gen_n).  gen_items holds the
-// n_gen_static host-routed columns first and has room for n_count more.
+// columns go to the count kernel (count_items, one workgroup per column and
+// target slice); the rest, and every column the count kernel hands over at
+// run time, to the split -> reference -> target kernels, which read their
+// items from gen_items[0 .. *gen_n).  gen_items holds the n_gen_static
+// host-routed columns first and has room for every counted column more.
 struct PipelineLaunch {
     const JobDesc* jobs;       // device copy (split/reference/target kernels)
     JobDesc job;               // host copy, passed by value to the count kernel
-    const int32_t* count_cols; // columns of the count kernel (one job per launch)
-    int n_count;
+    const CountItem* count_items; // workgroups of the count kernel (one job per launch)
+    int n_count;                  // their number
+    int n_count_cols;             // columns among them (hand-overs: at most one each)
     WorkItem* gen_items;
     uint32_t* gen_n;
     int n_gen_static;

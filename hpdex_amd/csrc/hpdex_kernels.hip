@@ -56,6 +56,34 @@ namespace hpdex {
 template <typename T>
 __device__ __forceinline__ HPDEX_GLOBAL T* gl(T* p) { return (HPDEX_GLOBAL T*)p; }
 
+// Buffer loads (gfx9 family): word 3 of a raw buffer resource, and the
+// cache-policy bits of the loads (default policy: the non-temporal hint made
+// the count kernel's column reads slower, 0.328 -> 0.307 ms without it)
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kBufferNT = 0;
+__device__ __forceinline__ uint32_t buffer_load_u32(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
+                                                    uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, kBufferNT);
+}
+__device__ __forceinline__ uint32_t buffer_load_u16(__amdgpu_buffer_rsrc_t rs, uint32_t voff) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, 0, 0);
+}
+template <typename V>
+__device__ __forceinline__ V buffer_load_value(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
+                                               uint32_t soff);
+template <>
+__device__ __forceinline__ float buffer_load_value<float>(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
+                                                          uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff,
+                                                                          kBufferNT));
+}
+template <>
+__device__ __forceinline__ double buffer_load_value<double>(__amdgpu_buffer_rsrc_t rs,
+                                                            uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff,
+                                                                           kBufferNT));
+}
+
 
 constexpr int kChunkK = 8;            // keys per lane in a register sort
 constexpr int kChunk = 64 * kChunkK;  // longest run sorted in registers
@@ -66,6 +94,8 @@ __device__ inline uint64_t tie_f(uint64_t t) { return t * t * t - t; }
 // kUnroll row/value loads and kUnroll group_id gathers in flight so the
 // stream is not bound by one load latency per entry.
 constexpr int kUnroll = 8;
+// the count kernel's loads run up to one batch past a column's end
+static_assert(kMatrixPad >= 1024u * kUnroll, "matrix padding below one count-kernel batch");
 template <typename V, typename F>
 __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
                                                const V* __restrict__ vals,
@@ -686,23 +716,25 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
 // The resident matrix carries each column's value range [vmin, vmax] (stored
 // nonzero finite values), so histogram bins b = value - vmin are known before
 // the column is read.  One read of the column from HBM then does everything
-// with one LDS atomic per stored entry: a reference value increments its bin
-// of the reference histogram, a target value the counter of its (target,
-// bin) pair.  Afterwards, with lt(b)/eq(b) from the histogram's exclusive
-// prefix and b_gv from the counters, each target's sums follow per bin:
+// with one LDS atomic per stored entry into a table of counter rows: row 0
+// counts the reference's values per bin, row r the values of target t0 + r,
+// and an extra column NB of every row counts the group's explicit zeros.
+// Afterwards, with lt(b)/eq(b) from the reference row's exclusive prefix and
+// b_gv from the counters, each target's sums follow per bin:
 //
 //   2*U_target(g) += b_gv * (2*lt(v) + eq(v))      (zero block added below)
 //   tie(g)        += f(eq(v) + b_gv) - f(eq(v))      f(t) = t^3 - t
 //
 // so no entry needs to know its position inside a run of equal values.  When
-// the counters of every (target, bin) do not fit beside the histogram, the
-// bins are taken in windows of VS: the first window during the read, the rest
-// in further passes over a staged copy of the column in LDS (or over HBM
-// again when the column is too long to stage).  Columns that do not fit at
-// all (more windows than kMaxWindows, values outside the recorded range) are
-// appended to the overflow queue of the split/reference/target kernels.
+// the rows of every target do not fit the workgroup's LDS, the host splits
+// the targets into slices of S (CountItem), one workgroup each: every slice
+// reads the whole column and counts the reference row and its own targets,
+// so the slices of a column run side by side rather than one after another.
+// Columns whose slices would be too many, and columns whose values leave the
+// recorded range (the matrix's metadata no longer matches), take the
+// split/reference/target kernels instead (the latter are appended to their
+// queue by the column's first slice).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 struct CountState {
     uint32_t bad;
     uint32_t pad[3];
@@ -710,396 +742,339 @@ struct CountState {
     uint64_t red[16];
 };
 
-template <typename V>
-__device__ __forceinline__ uint32_t slot_word(uint32_t g, V v, uint32_t G) {
-    using KT = KeyOf<V>;
-    if (g >= G || !KT::finite(v) || v == V(0)) return kEmptySlot;
-    int iv;
-    if (!KT::small_int(v, iv) || iv < -32768 || iv > 32767) return kEmptySlot;
-    return (g << 16) | ((uint32_t)iv & 0xFFFFu);
-}
-
-__device__ __forceinline__ int32_t slot_value(uint32_t w) {
-    return (int32_t)(int16_t)(uint16_t)(w & 0xFFFFu);
-}
-
-// Visit the LDS word of every stored entry of [p0, p1): from the staged copy,
-// or recomputed from HBM when the column was too long to stage.
-template <typename V, typename F>
-__device__ __forceinline__ void for_each_word(bool staged, const uint32_t* stage,
-                                              const JobDesc& J, uint32_t G, int64_t p0,
-                                              int64_t p1, F&& f) {
-    const int tid = tid_x(), nt = blockDim.x;
-    if (staged) {
-        const uint32_t n = (uint32_t)(p1 - p0);
-        for (uint32_t x = tid; x < n; x += nt) f(stage[x]);
-    } else {
-        for_each_entry<V>(J.rows, (const V*)J.vals, J.gid, (uint32_t)J.R, G, p0, p1,
-                          [&](uint32_t g, V v) { f(slot_word<V>(g, v, G)); });
-    }
-}
-
-// Counters of (target, bin) pairs: 16-bit halves of LDS words when no count
-// can reach 2^16 (a column of fewer than 2^16 entries), else whole words.
-// Rows are `rs` counters apart, with an odd word stride so that the per-target
-// epilogue (one thread per row) reads without bank conflicts.
+// Counters of (row, bin) pairs: 16-bit halves of LDS words when no count can
+// reach 2^16 (a column of fewer than 2^16 entries), else whole words.  Rows
+// are `rs` counters apart (count_row_stride: an odd number of words), so the
+// per-target sums (one thread per row) read without bank conflicts.
 struct PairCounters {
     uint32_t* w;
     uint32_t rs;
-    uint32_t row0;   // group of the first row: 0 (reference row kept here) or 1
     bool wide;
-    __device__ __forceinline__ void inc(uint32_t g, uint32_t d) const {
-        const uint32_t q = (g - row0) * rs + d;
-        if (wide) atomicAdd(&w[q], 1u);
-        else atomicAdd(&w[q >> 1], 1u << ((q & 1u) << 4));
-    }
-    __device__ __forceinline__ uint32_t get(uint32_t g, uint32_t d) const {
-        const uint32_t q = (g - row0) * rs + d;
+    __device__ __forceinline__ uint32_t get(uint32_t row, uint32_t d) const {
+        const uint32_t q = row * rs + d;
         return wide ? w[q] : ((w[q >> 1] >> ((q & 1u) << 4)) & 0xFFFFu);
     }
 };
 
 // J arrives by value in the kernel arguments, so its fields are read with
-// scalar loads into SGPRs; cols lists the columns of this launch.
+// scalar loads into SGPRs; items lists the (column, target slice) workgroups
+// of this launch with their columns' extents and value ranges.
 template <typename V, int NT>
-__global__ __launch_bounds__(NT) void count_kernel(const JobDesc J, const int32_t* cols,
+__global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
+                                                   const CountItem* __restrict__ items,
                                                    WorkItem* overflow, uint32_t* n_overflow,
                                                    uint32_t lds_bytes) {
     using KT = KeyOf<V>;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CountState st;
     const int job = 0;
-    const int64_t c = (int64_t)uniform_u32((uint32_t)cols[blockIdx.x]);
-    const int tid = tid_x(), nt = blockDim.x;
     // every value that steers control flow around a barrier is read into an
-    // SGPR: the kernel writes global memory, so these loads may be vector
-    // loads, and a branch on a VGPR would put the barriers under exec masks
+    // SGPR (a branch on a VGPR would put the barriers under exec masks)
+    const CountItem item = items[blockIdx.x];
+    const int64_t c = (int64_t)uniform_u32((uint32_t)item.col);
+    const uint32_t t0 = uniform_u32((uint32_t)item.t0);
+    const uint32_t S = uniform_u32((uint32_t)item.S);
+    const int64_t p0 = uniform_i64(item.p0);
+    const uint32_t n = uniform_u32(item.n);
+    const int32_t vmin = (int32_t)uniform_u32((uint32_t)item.vmin);
+    const uint32_t NB = uniform_u32(item.nb);   // histogram bins
+    const int tid = tid_x(), nt = NT;
     const uint32_t G = uniform_u32((uint32_t)J.G), nT = G - 1u;
     const uint32_t R = uniform_u32((uint32_t)J.R);
+    const uint32_t Ts = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(S, nT - t0));
+    const uint32_t rows = Ts + 1u;   // reference + this slice's targets
+    const bool sliced = Ts != nT;
     const V* vals = (const V*)J.vals;
-    const int64_t p0 = uniform_i64(J.indptr[c]), p1 = uniform_i64(J.indptr[c + 1]);
-    const uint32_t n = (uint32_t)(p1 - p0);
-    const int32_t vmin0 = (int32_t)uniform_u32((uint32_t)J.colrange[2 * c]);
-    const int32_t vmax0 = (int32_t)uniform_u32((uint32_t)J.colrange[2 * c + 1]);
-    const bool anyv = vmax0 >= vmin0;
-    const int32_t vmin = anyv ? vmin0 : 0;
-    const uint32_t NB = anyv ? (uint32_t)(vmax0 - vmin0 + 1) : 1u;   // histogram bins
-
-    // LDS: dummy[64] | acc[2G] u64 | cez[G] | cnz[G] | tab[NB+1] | counters |
-    // staged words (every size below is uniform and kept in SGPRs; divisions
-    // run on the vector unit, so their results are read back with readfirstlane)
-    uint32_t* lds_words = (uint32_t*)smem;
-    uint64_t* acc = (uint64_t*)(lds_words + kCountDummyWords);   // [2g] 2*U_target part,
-                                                                  // [2g+1] tie part
-    uint32_t* cez = (uint32_t*)(acc + 2u * G);   // explicit zeros per group
-    uint32_t* cnz = cez + G;                     // nonzero values per group
-    uint32_t* tab = cnz + G;
-    const uint32_t tab_words = (NB + 1u + 3u) & ~3u;
-    const uint32_t avail = (lds_bytes - 4u * kCountDummyWords - 24u * G) / 4u;
-    const uint32_t free0 = avail > tab_words ? avail - tab_words : 0u;
     const bool wide = n >= 65536u;
-    const uint32_t per_word = wide ? 1u : 2u;
-    // row stride in counters: an odd number of words per row
-    auto row_stride = [&](uint32_t vs) -> uint32_t {
-        if (wide) return vs | 1u;
-        uint32_t r = (vs + 1u) & ~1u;
-        if (((r >> 1) & 1u) == 0u) r += 2u;
-        return r;
-    };
-    // (sizes below stay far under 2^32: nT < 2^16, rs <= 8194, free0 < 2^16)
-    const uint32_t cap1 = free0 * per_word;   // counters that fit without staging
-    // one pass: every group's row of NB bins plus a zero column, the reference
-    // histogram in row 0
-    const bool single =
-        __builtin_amdgcn_readfirstlane(NB <= 8192u && G * row_stride(NB + 1u) <= cap1 ? 1 : 0) != 0;
-    const uint32_t stage_words = (n + 3u) & ~3u;
-    const bool staged = !single && __builtin_amdgcn_readfirstlane(
-        (uint64_t)stage_words + kCountReserveWords <= (uint64_t)free0 ? 1 : 0) != 0;
-    const uint32_t cap = (staged ? free0 - stage_words : free0) * per_word;
-    uint32_t VS = NB;
-    if (!single) {
-        // row_stride(vs) <= vs + 3, so vs = cap/nT - 3 always fits
-        const uint32_t per_row = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cap / nT));
-        VS = per_row > 3u ? per_row - 3u : 0u;
-        if (VS > NB) VS = NB;
-    }
-    const uint32_t rs =
-        (uint32_t)__builtin_amdgcn_readfirstlane((int)row_stride(single ? NB + 1u : VS));
-    const uint32_t nwin =
-        (uint32_t)__builtin_amdgcn_readfirstlane((int)(VS ? (NB + VS - 1u) / VS : 0u));
+
+    // the same test on every slice of the column (rows of S + 1), so that they
+    // all agree whether the column is theirs
     const int handover = __builtin_amdgcn_readfirstlane(
-        ((int64_t)NB > (int64_t)uniform_u32((uint32_t)J.hist_max) || free0 == 0u || VS == 0u ||
-         nwin > (uint32_t)kMaxWindows) ? 1 : 0);
+        ((int64_t)NB > (int64_t)uniform_u32((uint32_t)J.hist_max) ||
+         count_lds_need(S + 1u, NB, wide) > (uint64_t)lds_bytes) ? 1 : 0);
     if (handover) {
-        if (tid == 0) {
+        if (t0 == 0u && tid == 0) {
             const uint32_t k = atomicAdd(n_overflow, 1u);
             overflow[k] = WorkItem{job, (int32_t)c};
         }
         return;
     }
-    PairCounters pc;
-    pc.w = tab + tab_words;
-    pc.rs = rs;
-    pc.row0 = single ? 0u : 1u;
-    pc.wide = wide;
-    const uint32_t n_rows = single ? G : nT;
-    const uint32_t cell_words = wide ? n_rows * rs : (n_rows * rs + 1u) / 2u;
-    uint32_t* stage = pc.w + ((cell_words + 3u) & ~3u);
-    // word offsets from the start of LDS, for the read pass's single atomic
-    const uint32_t cez_off = (uint32_t)(cez - lds_words);
-    const uint32_t tab_off = (uint32_t)(tab - lds_words);
-    const uint32_t cell_off = (uint32_t)(pc.w - lds_words);
-    const uint32_t dummy_off = 0u;
 
-    for (uint32_t x = tid; x < 2u * G; x += nt) acc[x] = 0ull;
-    for (uint32_t g = tid; g < G; g += nt) { cez[g] = 0u; cnz[g] = 0u; }
-    for (uint32_t b = tid; b <= NB; b += nt) tab[b] = 0u;
+    // group sizes of the reference and this slice's targets, fetched now and
+    // kept in registers through the read pass (row x: group 0 or t0 + x)
+    const uint32_t* grows = J.group_rows;
+    const uint32_t gpre0 = (uint32_t)tid < rows ? grows[tid == 0 ? 0u : t0 + (uint32_t)tid] : 0u;
+    const uint32_t gpre1 = (uint32_t)tid + (uint32_t)nt < rows ? grows[t0 + (uint32_t)(tid + nt)] : 0u;
+
+    // LDS: dummy[64] | acc[2*rows] u64 | cez, cnz, grs [rows] | bins[NB] |
+    // counters[rows x rs]
+    uint32_t* lds_words = (uint32_t*)smem;
+    uint64_t* acc = (uint64_t*)(lds_words + kCountDummyWords);   // [2x] 2*U_target part,
+                                                                  // [2x+1] tie part
+    uint32_t* cez = (uint32_t*)(acc + 2u * rows);   // explicit zeros per row
+    uint32_t* cnz = cez + rows;                     // nonzero values per row
+    uint32_t* grs = cnz + rows;                     // group size per row
+    uint2* bins = (uint2*)(cez + count_row_words(rows));   // per bin: (lt', eq), below
+    const uint32_t rs = (uint32_t)__builtin_amdgcn_readfirstlane((int)count_row_stride(NB + 1u, wide));
+    PairCounters pc;
+    pc.w = (uint32_t*)(bins + NB);
+    pc.rs = rs;
+    pc.wide = wide;
+    const uint32_t cell_words = count_cell_words(rows, rs, wide);
+    const uint32_t cell_off = (uint32_t)(pc.w - lds_words);
+
     for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
     if (tid == 0) st.bad = 0u;
     __syncthreads();
 
-    // the one read of the column from HBM: reference histogram, counters of
-    // the first window, explicit zeros, staging for later windows.  Lane i of
-    // a wave takes entries i, i+64, ... so a wave's 64 entries are adjacent:
-    // their row indices are sorted, and the group labels they gather (16-bit
-    // labels; rows out of range read the sentinel at index R) fall into a few
-    // cache lines.  kUnroll rows and values per lane are in flight, read with
-    // a non-temporal hint (used once) at 32-bit offsets from the column start.
-    // Whole batches run without bounds checks; the last one checks.
-    const HPDEX_GLOBAL uint16_t* gid16 = gl(J.gid16);
-    const HPDEX_GLOBAL int32_t* rbase = gl(J.rows) + p0;
-    const HPDEX_GLOBAL V* vbase = gl(vals) + p0;
-    const uint32_t step = (uint32_t)nt * kUnroll;
-    if (single) {
-        // every counter lives in one array: word and increment of an entry
-        // follow from (group, bin) alone -- column NB counts explicit zeros.
-        // Values are integers in [-32768, 32767] by the matrix's column
-        // metadata; a bin outside [0, NB) would mean the metadata no longer
-        // matches and hands the column over.
-        bool bad = false;
-        auto batch1 = [&](uint32_t ib, auto full_tag, auto wide_tag) {
-            constexpr bool kFull = decltype(full_tag)::value;
-            constexpr bool kWide = decltype(wide_tag)::value;
-            uint32_t r[kUnroll];
-            V v[kUnroll];
+    // the one read of the column from HBM.  Lane i of a wave takes entries i,
+    // i+64, ... so a wave's 64 entries are adjacent: their row indices are
+    // sorted, and the group labels they gather (16-bit labels; rows out of
+    // range read the sentinel at index R) fall into a few cache lines.
+    // Batches of kUnroll rows and values per lane are software-pipelined: the
+    // next batch's loads are issued after this batch's label gathers and
+    // before its atomics, so they are in flight while this batch is counted
+    // (loads complete in order: waiting for the gathers leaves them
+    // outstanding).  Loads run without bounds checks -- the matrix keeps
+    // kMatrixPad >= one batch of entries past its end -- and entries past the
+    // column's end are masked in the last batch.  Each entry costs one LDS
+    // atomic, its word and increment computed without branches; entries of no
+    // row of this workgroup hit the lane's dummy word.
+    //
+    // The loads are buffer loads: the column start and its size in a scalar
+    // resource, the lane's offset in VGPRs fixed for the whole loop, the
+    // batch start in the scalar offset.  The resources reach to the end of
+    // their arrays (at most 2^32 - 1 bytes on), so every load the loop makes
+    // lies inside its array whatever the hardware range check adds.
+    const uint64_t left = (uint64_t)(uniform_i64(J.n_alloc) - p0);   // entries from p0 on
+    const auto bytes32 = [](uint64_t b) { return (uint32_t)(b < 0xFFFFFFFFull ? b : 0xFFFFFFFFull); };
+    const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(J.rows + p0), (short)0, bytes32(left * 4u), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(vals + p0), (short)0, bytes32(left * sizeof(V)), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)J.gid16, (short)0, (uint32_t)(R + 1u) * 2u, kBufferWord3);
+    const uint32_t step = (uint32_t)NT * kUnroll;
+    bool bad = false;
+    auto load = [&](uint32_t ib, uint32_t (&r)[kUnroll], V (&v)[kUnroll]) {
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t i = ib + (uint32_t)(u * nt + tid);
-                if (kFull || i < n) {
-                    r[u] = (uint32_t)__builtin_nontemporal_load(rbase + i);
-                    v[u] = __builtin_nontemporal_load(vbase + i);
-                } else {
-                    r[u] = 0xFFFFFFFFu;
-                    v[u] = V(0);
-                }
-            }
-            uint32_t g[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) g[u] = (uint32_t)gid16[min(r[u], R)];
-            const uint32_t dummy = dummy_off + (uint32_t)lane_id();
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t gg = g[u];
-                const V vv = v[u];
-                const bool fin = KT::abs_le(vv, V(32767));
-                const bool zero = vv == V(0);
-                const uint32_t b = (uint32_t)((int)(fin ? vv : V(0)) - vmin);
-                const bool inb = b < NB;
-                const bool ingroup = gg < G;
-                bad |= ingroup && fin && !zero && !inb;
-                const uint32_t q = gg * rs + (zero ? NB : b);
-                const uint32_t word = (ingroup && (zero || (fin && inb)))
-                                          ? cell_off + (kWide ? q : (q >> 1)) : dummy;
-                atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
-            }
-        };
-        const uint32_t nfull = n - n % step;
-        if (wide) {
-            for (uint32_t ib = 0; ib < nfull; ib += step)
-                batch1(ib, std::true_type{}, std::true_type{});
-            if (nfull < n) batch1(nfull, std::false_type{}, std::true_type{});
-        } else {
-            for (uint32_t ib = 0; ib < nfull; ib += step)
-                batch1(ib, std::true_type{}, std::false_type{});
-            if (nfull < n) batch1(nfull, std::false_type{}, std::false_type{});
+        for (int u = 0; u < kUnroll; ++u) {
+            // per lane: entry tid + u*NT of the batch (loop-invariant); per
+            // batch: its first entry ib (one scalar)
+            const uint32_t e = (uint32_t)tid + (uint32_t)(u * NT);
+            r[u] = buffer_load_u32(rows_rsrc, e * 4u, ib * 4u);
+            v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), ib * (uint32_t)sizeof(V));
         }
-        if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
-    } else {
-        bool bad = false;
-        auto batch = [&](uint32_t ib, auto full_tag) {
-            constexpr bool kFull = decltype(full_tag)::value;
-            uint32_t r[kUnroll];
-            V v[kUnroll];
+    };
+    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) g[u] = buffer_load_u16(gid_rsrc, min(r[u], R) * 2u);
+    };
+    auto count = [&](const uint32_t (&g)[kUnroll], const V (&v)[kUnroll], auto wide_tag,
+                     auto sliced_tag) {
+        constexpr bool kWide = decltype(wide_tag)::value;
+        constexpr bool kSliced = decltype(sliced_tag)::value;
+        const uint32_t dummy = (uint32_t)lane_id();
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t gg = g[u];
+            const V vv = v[u];
+            // |v| <= 32768 covers every integer the column metadata admits
+            // ([-32768, 32767]); a finite nonzero value outside the recorded
+            // range means the metadata no longer matches (checked for every
+            // entry, whatever its group, so all slices agree)
+            const bool near = KT::abs_le(vv, V(32768));
+            const bool zero = vv == V(0);
+            const uint32_t b = (uint32_t)((int)(near ? vv : V(0)) - vmin);
+            const bool inb = near && b < NB;
+            bad |= !zero && !inb && KT::finite(vv);
+            // the entry's row: reference 0, target t0 + k -> k
+            uint32_t row;
+            bool mine;
+            if (kSliced) {
+                const uint32_t rr = gg - t0;
+                row = gg == 0u ? 0u : rr;
+                mine = gg == 0u || rr - 1u < Ts;
+            } else {
+                row = gg;
+                mine = gg < G;
+            }
+            const uint32_t q = row * rs + (zero ? NB : b);
+            const uint32_t word = (mine && (zero || inb)) ? cell_off + (kWide ? q : (q >> 1)) : dummy;
+            atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
+        }
+    };
+    auto read_all = [&](auto wide_tag, auto sliced_tag) {
+        uint32_t r[kUnroll];
+        V v[kUnroll];
+        load(0u, r, v);
+        uint32_t ib = 0;
+        for (; ib + step < n; ib += step) {   // a whole batch with another after it
+            uint32_t g[kUnroll];
+            gather(r, g);
+            // keep the gathers ahead of the next batch's loads (the counter
+            // that waits for the gathers would otherwise wait for those too)
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t r2[kUnroll];
+            V v2[kUnroll];
+            load(ib + step, r2, v2);
+            __builtin_amdgcn_sched_barrier(0);
+            count(g, v, wide_tag, sliced_tag);
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
+        }
+        if (ib < n) {   // the last batch: entries past the column's end masked
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t i = ib + (uint32_t)(u * nt + tid);
-                if (kFull || i < n) {
-                    r[u] = (uint32_t)__builtin_nontemporal_load(rbase + i);
-                    v[u] = __builtin_nontemporal_load(vbase + i);
-                } else {
-                    r[u] = 0xFFFFFFFFu;
-                    v[u] = V(0);
-                }
+                const bool in = ib + (uint32_t)(u * NT + tid) < n;
+                r[u] = in ? r[u] : 0xFFFFFFFFu;
+                v[u] = in ? v[u] : V(0);
             }
             uint32_t g[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) g[u] = (uint32_t)gid16[min(r[u], R)];
-            // one LDS atomic per entry, its word and increment chosen without
-            // branches: a reference value -> its histogram bin, a target value
-            // of the first window -> its (target, bin) counter, an explicit
-            // zero -> the group's zero count, anything else -> this lane's
-            // dummy word
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const uint32_t gg = g[u];
-                const V vv = v[u];
-                int iv = 0;
-                const bool isint = KT::small_int(vv, iv);
-                const bool zero = vv == V(0);
-                const bool ingroup = gg < G;
-                const bool inrange = isint && iv >= vmin && iv <= vmax0;
-                bad |= ingroup && !zero && KT::finite(vv) && !inrange;
-                const uint32_t b = (uint32_t)(iv - vmin);
-                const bool val = ingroup && !zero && inrange;
-                const uint32_t q = (gg - 1u) * rs + b;
-                const uint32_t cell_word = cell_off + (wide ? q : (q >> 1));
-                const uint32_t cell_inc = wide ? 1u : (1u << ((q & 1u) << 4));
-                uint32_t word = dummy_off + (uint32_t)lane_id();
-                uint32_t inc = 1u;
-                if (val && gg == 0u) word = tab_off + b;
-                if (val && gg != 0u && b < VS) { word = cell_word; inc = cell_inc; }
-                if (ingroup && zero) word = cez_off + gg;
-                atomicAdd(&lds_words[word], inc);
-                if (staged) {
-                    const uint32_t i = ib + (uint32_t)(u * nt + tid);
-                    if (kFull || i < n)
-                        stage[i] = (val && gg != 0u && b >= VS)
-                                       ? ((gg << 16) | ((uint32_t)iv & 0xFFFFu)) : kEmptySlot;
-                }
-            }
-        };
-        const uint32_t nfull = n - n % step;
-        for (uint32_t ib = 0; ib < nfull; ib += step) batch(ib, std::true_type{});
-        if (nfull < n) batch(nfull, std::false_type{});
-        if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
+            gather(r, g);
+            count(g, v, wide_tag, sliced_tag);
+        }
+    };
+    if (sliced) {
+        if (wide) read_all(std::true_type{}, std::true_type{});
+        else read_all(std::false_type{}, std::true_type{});
+    } else {
+        if (wide) read_all(std::true_type{}, std::false_type{});
+        else read_all(std::false_type{}, std::false_type{});
     }
+    if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
+    if ((uint32_t)tid < rows) grs[tid] = gpre0;
+    if ((uint32_t)tid + (uint32_t)nt < rows) grs[tid + nt] = gpre1;
+    for (uint32_t x = (uint32_t)tid + 2u * (uint32_t)nt; x < rows; x += nt) grs[x] = grows[t0 + x];
     __syncthreads();
-    // a value outside the recorded range (the range no longer matches the
-    // data): leave the column to the split path
+    // a value outside the recorded range: the column goes to the split path
+    // (queued once, by its first slice)
     if (__builtin_amdgcn_readfirstlane((int)st.bad)) {
-        if (tid == 0) {
+        if (t0 == 0u && tid == 0) {
             const uint32_t k = atomicAdd(n_overflow, 1u);
             overflow[k] = WorkItem{job, (int32_t)c};
         }
         return;
     }
 
-    // one-pass layout: the reference histogram and the zero counts sit in the
-    // counter array; copy them out for the scan and the sizes
-    if (single) {
-        for (uint32_t b = tid; b < NB; b += nt) tab[b] = pc.get(0u, b);
-        for (uint32_t g = tid; g < G; g += nt) cez[g] = pc.get(g, NB);
-        __syncthreads();
-    }
-
-    // reference tie sum, exclusive prefix: tab[b] = #reference values below
-    // bin b, tab[NB] = #reference nonzero values
-    uint64_t ta = 0;
-    for (uint32_t b = tid; b < NB; b += nt) ta += tie_f(tab[b]);
-    ta = block_sum_u64(ta, st.red);
-    block_excl_scan(tab, (int)NB + 1, st.scan);
-
+    // one pass over the reference row: each thread takes a run of `chunk`
+    // bins, the waves' inclusive scans and a scan of the wave totals give
+    // every bin the number of reference values below it; with them the tie
+    // sum of the reference values and the explicit zeros of every row
     const int32_t mode = (int32_t)uniform_u32((uint32_t)J.mode);
-    const uint32_t gr0 = J.group_rows[0];
-    const uint32_t na = tab[NB];
+    const uint32_t chunk = (NB + (uint32_t)NT - 1u) / (uint32_t)NT;
+    const uint32_t b0 = min((uint32_t)tid * chunk, NB), b1 = min(b0 + chunk, NB);
+    uint32_t sh = 0;
+    uint64_t stie = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t h = pc.get(0u, b);
+        sh += h;
+        stie += tie_f(h);
+    }
+    for (uint32_t x = tid; x < rows; x += nt) cez[x] = pc.get(x, NB);
+    const uint32_t incl = wave_incl_scan_u32(sh);
+    const uint64_t wtie = wave_sum_u64(stie);
+    const int w = tid / kWave;
+    if (lane_id() == kWave - 1) st.scan[w] = incl;
+    if (lane_id() == 0) st.red[w] = wtie;
+    __syncthreads();
+    uint32_t below = 0, na = 0;   // reference values in earlier waves' bins; in all bins
+    uint64_t ta = 0;              // reference tie sum over nonzero values
+#pragma unroll
+    for (int i = 0; i < NT / kWave; ++i) {
+        const uint32_t t = st.scan[i];
+        below += i < w ? t : 0u;
+        na += t;
+        ta += st.red[i];
+    }
+    const uint32_t gr0 = grs[0];
     const uint32_t za = (mode == kZeroNone) ? cez[0] : (gr0 > na ? gr0 - na : 0u);
-
-    // per-target sums over the bins of one window [w0, w0 + VS), one thread
-    // per target and no branches per bin: a count of 0 adds 0 to each sum.
-    // With x = eq + c, f(x) - f(eq) = c * (x^2 + x*eq + eq^2 - 1); the sums are
-    // exact modulo 2^64 like any other order of the same integer additions.
-    auto window_sums = [&](uint32_t w0) {
-        const uint32_t wn = min(VS, NB - w0);
-        for (uint32_t g = 1u + tid; g < G; g += nt) {
-            uint64_t S = 0, T = 0;
-            uint32_t k = 0;
-            auto bin = [&](uint32_t b, uint32_t c) {
-                const uint32_t lt = tab[b], eq = tab[b + 1u] - lt;
-                const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
-                const uint64_t A = 2ull * ((uint64_t)lt + (zbelow ? za : 0u)) + eq;
-                const uint32_t x = eq + c;
-                const uint64_t P = (uint64_t)x * x + (uint64_t)x * eq + (uint64_t)eq * eq - 1ull;
-                S += (uint64_t)c * A;
-                T += (uint64_t)c * (uint32_t)P + ((uint64_t)(c * (uint32_t)(P >> 32)) << 32);
-                k += c;
-            };
-            if (wide) {
-                const uint32_t* row = pc.w + (g - pc.row0) * rs;
-                for (uint32_t d = 0; d < wn; ++d) bin(w0 + d, row[d]);
-            } else {
-                // rows start on an even counter: two counters per word
-                const uint32_t* row = pc.w + (((g - pc.row0) * rs) >> 1);
-                for (uint32_t d = 0; d < wn; d += 2u) {
-                    const uint32_t cw = row[d >> 1];
-                    bin(w0 + d, cw & 0xFFFFu);
-                    if (d + 1u < wn) bin(w0 + d + 1u, cw >> 16);
-                }
-            }
-            acc[2u * g] += S;
-            acc[2u * g + 1u] += T;
-            cnz[g] += k;
+    // bins[b] = (reference values below b, plus the zero block when it lies
+    // below b; reference values in b)
+    {
+        uint32_t lt = below + incl - sh;
+        for (uint32_t b = b0; b < b1; ++b) {
+            const uint32_t h = pc.get(0u, b);
+            const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
+            bins[b] = make_uint2(lt + (zbelow ? za : 0u), h);
+            if ((int32_t)b + vmin == 0) st.scan[16] = lt;   // value 0's bin: values below 0
+            lt += h;
         }
-    };
-    window_sums(0u);
-    for (uint32_t w0 = VS; w0 < NB; w0 += VS) {
-        __syncthreads();
-        for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
-        __syncthreads();
-        for_each_word<V>(staged, stage, J, G, p0, p1, [&](uint32_t w) {
-            const uint32_t g = w >> 16;
-            if (g == 0u || g >= G) return;
-            const uint32_t d = (uint32_t)(slot_value(w) - vmin) - w0;
-            if (d < VS) pc.inc(g, d);
-        });
-        __syncthreads();
-        window_sums(w0);
     }
     __syncthreads();
 
-    // errors of this column (mannwhitneyu.cpp:936-1038)
+    // per-target sums over the bins, one thread per target row, no branches
+    // per bin (a count of 0 adds 0 to each sum).  With x = eq + c,
+    //   2*U part: c * (2*lt' + eq)
+    //   tie part: f(x) - f(eq) = c * (x*(x + eq) + eq^2 - 1)
+    // exact modulo 2^64 like any other order of the same integer additions.
+    for (uint32_t x = 1u + tid; x < rows; x += nt) {
+        uint64_t Slt = 0, Seq = 0, Tx = 0;
+        uint32_t k = 0;
+        auto bin = [&](uint32_t d, uint32_t cnt) {
+            const uint2 be = bins[d];
+            Slt += (uint64_t)cnt * be.x;
+            Seq += (uint64_t)cnt * be.y;
+            const uint32_t xx = be.y + cnt;
+            const uint64_t P = (uint64_t)xx * (xx + be.y) + ((uint64_t)be.y * be.y - 1ull);
+            Tx += (uint64_t)cnt * P;
+            k += cnt;
+        };
+        if (wide) {
+            const uint32_t* row = pc.w + x * rs;
+            for (uint32_t d = 0; d < NB; ++d) bin(d, row[d]);
+        } else {
+            // rows start on an even counter: two counters per word
+            const uint32_t* row = pc.w + ((x * rs) >> 1);
+            uint32_t d = 0;
+            for (; d + 4u <= NB; d += 4u) {
+                const uint32_t w0 = row[d >> 1], w1 = row[(d >> 1) + 1u];
+                bin(d, w0 & 0xFFFFu);
+                bin(d + 1u, w0 >> 16);
+                bin(d + 2u, w1 & 0xFFFFu);
+                bin(d + 3u, w1 >> 16);
+            }
+            for (; d < NB; d += 2u) {
+                const uint32_t cw = row[d >> 1];
+                bin(d, cw & 0xFFFFu);
+                if (d + 1u < NB) bin(d + 1u, cw >> 16);
+            }
+        }
+        acc[2u * x] = 2ull * Slt + Seq;
+        acc[2u * x + 1u] = Tx;
+        cnz[x] = k;
+    }
     if (tid == 0) cnz[0] = na;
     __syncthreads();
-    {
-        const uint64_t ebase = (uint64_t)c * (uint64_t)(2u * G + 1u);
-        for (uint32_t g = tid; g < G; g += nt) {
-            const uint32_t nz = cnz[g], ez = cez[g], gr = J.group_rows[g];
-            if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
-            const uint32_t ns = (mode == kZeroNone) ? nz + ez : gr;
-            if (ns < 2u)
-                atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
-        }
-    }
 
-    // per target: zero block, U1, tie, sizes, asymptotic p
-    uint32_t ltz0;   // reference values below the zero block
+    // reference values below the zero block
+    uint32_t ltz0;
     if (mode == kZeroMin) ltz0 = 0u;
     else if (mode == kZeroMax) ltz0 = na;
-    else ltz0 = vmin < 0 ? tab[min((uint32_t)(-vmin), NB)] : 0u;
-    for (uint32_t g = 1u + tid; g < G; g += nt) {
-        const uint32_t nzb = cnz[g];
+    else ltz0 = vmin >= 0 ? 0u : ((uint32_t)(-vmin) < NB ? st.scan[16] : na);
+
+    // per row: errors of this column (mannwhitneyu.cpp:936-1038); per target
+    // row: zero block, U1, tie, sizes, asymptotic p
+    const uint64_t ebase = (uint64_t)c * (uint64_t)(2u * G + 1u);
+    for (uint32_t x = tid; x < rows; x += nt) {
+        const uint32_t g = x == 0u ? 0u : t0 + x;
+        const uint32_t nz = cnz[x], ez = cez[x], gr = grs[x];
+        if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
+        const uint32_t ns = (mode == kZeroNone) ? nz + ez : gr;
+        if (ns < 2u) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
+        if (x == 0u) continue;
         uint32_t zb, n1, n2;
         if (mode == kZeroNone) {
-            zb = cez[g];
+            zb = ez;
             n1 = na + za;
-            n2 = nzb + zb;
+            n2 = nz + zb;
         } else {
-            const uint32_t grg = J.group_rows[g];
-            zb = grg > nzb ? grg - nzb : 0u;
+            zb = gr > nz ? gr - nz : 0u;
             n1 = gr0;
-            n2 = grg;
+            n2 = gr;
         }
-        const uint64_t S2 = acc[2u * g] + (uint64_t)zb * (2ull * ltz0 + za);
-        const uint64_t tie = acc[2u * g + 1u] + ta + tie_f((uint64_t)za + zb);
+        const uint64_t S2 = acc[2u * x] + (uint64_t)zb * (2ull * ltz0 + za);
+        const uint64_t tie = acc[2u * x + 1u] + ta + tie_f((uint64_t)za + zb);
         const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
         const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
         const int64_t idx = c * J.T + (int64_t)(g - 1u);
@@ -1373,14 +1348,14 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
 }
 
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
-    const int n_gen = L.n_gen_static + L.n_count;
+    const int n_gen = L.n_gen_static + L.n_count_cols;
     if (n_gen <= 0) return hipSuccess;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
     mark(0);
     if (L.n_count > 0) {
 #define HPDEX_COUNT(V, NT)                                                                  \
-    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_cols, L.gen_items, \
+    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_items, L.gen_items, \
                                                            L.gen_n, L.count_lds)
         if (L.f64) {
             if (L.count_block == 256) HPDEX_COUNT(double, 256);

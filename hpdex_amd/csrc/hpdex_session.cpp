@@ -111,7 +111,8 @@ size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows
     f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));
     f += up256(C * (size_t)kTabStride * 4);
     // job_upload_desc
-    f += up256(sizeof(JobDesc)) + up256(C * 4) + up256(C * sizeof(WorkItem)) + up256(16);
+    f += up256(sizeof(JobDesc)) + up256(C * kMaxSlices * sizeof(CountItem)) +
+         up256(C * sizeof(WorkItem)) + up256(16);
     // exact-null tables of min/max/mix: sizes are the group row counts
     if (j.opts.method == 1) {
         std::set<std::pair<int32_t, int32_t>> pairs;

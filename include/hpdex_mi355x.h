@@ -148,12 +148,15 @@ int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie
 /* Where the plan's columns go: the one-pass count kernel (integer-valued
  * columns) or the split/reference/target kernels; handed_over counts the
  * count-kernel columns that the last run passed on to the split kernels
- * (value range too wide for the on-chip histogram).                         */
+ * (values outside the column's recorded range).  count_blocks is the count
+ * kernel's workgroups: one per column, or one per slice of targets when a
+ * column's counters for every target do not fit one workgroup's LDS.        */
 typedef struct hpdex_plan_info {
     int64_t count_columns;
     int64_t count_nnz;
     int64_t split_columns;
     int64_t handed_over;
+    int64_t count_blocks;
 } hpdex_plan_info;
 int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len);
 /* Per-kernel timing: runs the plain launch sequence `iters` times with HIP

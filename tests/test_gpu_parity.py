@@ -200,11 +200,11 @@ def _plan_of(w, mode, form=1, stats=True):
 @pytest.mark.parametrize("lds", [0, 16 * 1024, 24 * 1024, 152 * 1024])
 def test_count_kernel_lds_sizes(lds, restore_tuning):
     """The one-pass count kernel at several LDS sizes: 0 sends every column to
-    the split path; 16 KiB re-reads columns too long to stage and needs
-    several value windows; 152 KiB stages every column in one window."""
+    the split path; 16 and 24 KiB split the 40 targets of the wider columns
+    into slices; 152 KiB takes each column in one workgroup."""
     _capi.set_tuning(count_lds_bytes=lds)
     w = _random_workload(11, 9000, 16, 40, 0.6,
-                         lambda r, n: r.negative_binomial(0.8, 0.1, n) - 2)
+                         lambda r, n: r.negative_binomial(0.8, 0.02, n) - 2)
     for mode in ("min", "max", "mix", "none"):
         say(f"count kernel lds={lds} mode={mode}")
         m, p = _plan_of(w, mode)
@@ -212,6 +212,10 @@ def test_count_kernel_lds_sizes(lds, restore_tuning):
         info = p.info()
         assert info["count_columns"] == (16 if lds else 0)
         assert info["handed_over"] == 0
+        if lds == 152 * 1024:
+            assert info["count_blocks"] == 16
+        elif lds:
+            assert info["count_blocks"] > 16
         assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
         p.close(); m.close()
 
@@ -229,31 +233,56 @@ def test_count_kernel_routes_wide_ranges_to_split():
     p.close(); m.close()
 
 
-def test_count_kernel_hands_over_at_run_time(restore_tuning):
-    """300 targets x 1,000 values in 16 KiB would need 200 value windows: the
-    count kernel hands the columns to the split path while running."""
-    say("count kernel hand-over")
+def test_count_kernel_routes_too_many_slices_to_split(restore_tuning):
+    """300 targets x 1,000 values in 16 KiB: a workgroup holds 3 targets' rows,
+    100 slices per column, more than 64: the columns take the split path."""
+    say("count kernel too many slices")
     _capi.set_tuning(count_lds_bytes=16 * 1024)
     w = _random_workload(14, 6000, 4, 300, 0.5, lambda r, n: r.integers(1, 1001, n))
     m, p = _plan_of(w, "min")
     p.run(2)
     info = p.info()
-    assert info["count_columns"] == 4 and info["handed_over"] == 4
+    assert info["count_columns"] == 0 and info["split_columns"] == 4
     assert_parity(p.results(with_stats=True), oracle_stats(w, "min"))
     p.close(); m.close()
 
 
 @pytest.mark.parametrize("lds", [76 * 1024, 24 * 1024])
-def test_count_kernel_value_windows(lds, restore_tuning):
-    """200 targets x 400 values: several value windows over a staged column
-    (76 KiB) or over columns re-read from memory (24 KiB)."""
-    say(f"count kernel windows lds={lds}")
+def test_count_kernel_target_slices(lds, restore_tuning):
+    """200 targets x 400 values do not fit one workgroup: 3 slices of 67
+    targets in 76 KiB, 9 slices of 23 in 24 KiB, each reading the column."""
+    say(f"count kernel slices lds={lds}")
     _capi.set_tuning(count_lds_bytes=lds)
     w = _random_workload(15, 8000, 6, 200, 0.5, lambda r, n: r.integers(1, 401, n))
-    for mode in ("min", "mix"):
+    for mode in ("min", "mix", "none"):
         m, p = _plan_of(w, mode)
         p.run(1)
-        assert p.info()["handed_over"] == 0
+        info = p.info()
+        assert info["handed_over"] == 0 and info["count_columns"] == 6
+        assert info["count_blocks"] == 6 * (3 if lds == 76 * 1024 else 9)
+        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+        p.close(); m.close()
+
+
+def test_count_kernel_extreme_values():
+    """Values at the ends of the 16-bit range of the column metadata:
+    -32768 .. -32700 and 32700 .. 32767 in the count kernel; both ends in one
+    column (65,536 values) in the split path."""
+    say("count kernel extreme values")
+    w = _random_workload(16, 5000, 3, 6, 0.4, lambda r, n: r.integers(0, 69, n))
+    d, ip = w["data"].copy(), w["indptr"]
+    d[ip[0]:ip[1]] = np.where(d[ip[0]:ip[1]] == 0, 0, d[ip[0]:ip[1]] - 32768)
+    d[ip[0]] = -32768
+    d[ip[1]:ip[2]] = np.where(d[ip[1]:ip[2]] == 0, 0, d[ip[1]:ip[2]] + 32699)
+    d[ip[1]] = 32767
+    d[ip[2]] = -32768
+    d[ip[2] + 1] = 32767
+    w = dict(w, data=d)
+    for mode in ("min", "mix", "none"):
+        m, p = _plan_of(w, mode)
+        p.run(1)
+        info = p.info()
+        assert info["count_columns"] == 2 and info["split_columns"] == 1
         assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
         p.close(); m.close()
 

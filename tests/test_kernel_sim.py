@@ -73,7 +73,7 @@ def test_sim_histogram_path():
 # ---- the count kernel (integer-valued columns in one pass) ----
 
 def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist_max=8192,
-              expect_counted=None, block=512):
+              expect_counted=None, block=512, expect_blocks=None):
     U1, P, TIE, n1, n2, err, status = sim.mwu(data, indices, indptr, gid, T, R, mode=ZH[mode],
                                               count_lds=count_lds, f64=f64, hist_max=hist_max,
                                               block=block, timeout_s=120)
@@ -86,23 +86,26 @@ def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist
     assert np.array_equal(P, oP)
     if expect_counted is not None:
         assert sim.mwu.last_counted == expect_counted
+    if expect_blocks is not None:
+        assert sim.mwu.last_blocks == expect_blocks
 
 
 @pytest.mark.parametrize("block", [256, 512])
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
 def test_sim_count_kernel_modes(mode, block):
-    # small signed integers with explicit zeros: one pass, one window
+    # small signed integers with explicit zeros: one workgroup per column
     d, r, ip, g = _rw(21, 3000, 5, 7, 0.4, lambda rg, n: rg.integers(-4, 9, n))
-    run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5, block=block)
+    run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5, expect_blocks=5,
+              block=block)
 
 
 @pytest.mark.parametrize("lds", [16 * 1024, 24 * 1024, 152 * 1024])
 def test_sim_count_kernel_lds_sizes(lds):
-    # 16 KiB: columns too long to stage (~3,600 entries; re-read from memory,
-    # 32-bit counters) and several value windows; 24 KiB: staged, 16-bit
-    # counters, several windows; 152 KiB: staged, one window
-    d, r, ip, g = _rw(22, 6000, 4, 40, 0.6, lambda rg, n: rg.negative_binomial(0.8, 0.1, n) + 1)
-    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4)
+    # 40 targets over ranges of 330-355 values: 3 target slices per column in
+    # 16 KiB, 2 in 24 KiB, one workgroup per column in 152 KiB
+    d, r, ip, g = _rw(22, 6000, 4, 40, 0.6, lambda rg, n: rg.negative_binomial(0.8, 0.02, n) + 1)
+    blocks = {16 * 1024: 12, 24 * 1024: 8, 152 * 1024: 4}[lds]
+    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4, expect_blocks=blocks)
 
 
 def test_sim_count_kernel_hands_over_wide_ranges():
@@ -114,10 +117,11 @@ def test_sim_count_kernel_hands_over_wide_ranges():
     run_count(d, r, ip, g, 5, 3000, "min", 76 * 1024, hist_max=128, expect_counted=0)
 
 
-def test_sim_count_kernel_hands_over_too_many_windows():
-    # 300 targets in 16 KiB leave 5-value windows: a 1,000-value range needs 200
+def test_sim_count_kernel_too_many_slices_take_split_path():
+    # 300 targets x 1,000 values in 16 KiB: a workgroup holds 3 targets' rows,
+    # 100 slices per column (more than 64): the columns take the split path
     d, r, ip, g = _rw(25, 4000, 2, 300, 0.5, lambda rg, n: rg.integers(1, 1001, n))
-    run_count(d, r, ip, g, 300, 4000, "min", 16 * 1024, expect_counted=0)
+    run_count(d, r, ip, g, 300, 4000, "min", 16 * 1024, expect_counted=0, expect_blocks=0)
 
 
 def test_sim_count_kernel_nonfinite_and_mixed_columns():
@@ -131,6 +135,22 @@ def test_sim_count_kernel_nonfinite_and_mixed_columns():
         run_count(d, r, ip, g, 4, 2500, mode, 76 * 1024, expect_counted=3)
 
 
+def test_sim_count_kernel_extreme_values():
+    # values at the ends of the 16-bit range the column metadata admits:
+    # -32768 .. -32700 in column 0, 32700 .. 32767 in column 1, both ends
+    # (a 65,536-value range: split path) in column 2
+    d, r, ip, g = _rw(30, 3000, 3, 6, 0.4, lambda rg, n: rg.integers(0, 69, n))
+    d = d.copy()
+    d[ip[0]:ip[1]] = np.where(d[ip[0]:ip[1]] == 0, 0, d[ip[0]:ip[1]] - 32768)
+    d[ip[0]] = -32768
+    d[ip[1]:ip[2]] = np.where(d[ip[1]:ip[2]] == 0, 0, d[ip[1]:ip[2]] + 32699)
+    d[ip[1]] = 32767
+    d[ip[2]] = -32768
+    d[ip[2] + 1] = 32767
+    for mode in ("min", "mix", "none"):
+        run_count(d, r, ip, g, 6, 3000, mode, 76 * 1024, expect_counted=2, expect_blocks=2)
+
+
 def test_sim_count_kernel_eight_byte_values():
     d, r, ip, g = _rw(27, 2000, 3, 4, 0.4, lambda rg, n: rg.integers(-30, 200, n))
     run_count(d.astype(np.float64), r, ip, g, 4, 2000, "mix", 76 * 1024, f64=True,
@@ -138,17 +158,19 @@ def test_sim_count_kernel_eight_byte_values():
 
 
 @pytest.mark.parametrize("lds", [76 * 1024, 24 * 1024])
-def test_sim_count_kernel_value_windows(lds):
-    # 200 targets x 400 values do not fit one window: 76 KiB stages the
-    # column and takes 3 windows; 24 KiB re-reads it from memory for 9
+def test_sim_count_kernel_target_slices(lds):
+    # 200 targets x 400 values do not fit one workgroup: 76 KiB holds 88
+    # targets' rows (3 slices of 67), 24 KiB 24 (9 slices of 23)
     d, r, ip, g = _rw(28, 8000, 3, 200, 0.5, lambda rg, n: rg.integers(1, 401, n))
-    for mode, block in (("min", 512), ("mix", 256)):
-        run_count(d, r, ip, g, 200, 8000, mode, lds, expect_counted=3, block=block)
+    blocks = 3 * (3 if lds == 76 * 1024 else 9)
+    for mode, block in (("min", 512), ("mix", 256), ("none", 512)):
+        run_count(d, r, ip, g, 200, 8000, mode, lds, expect_counted=3, block=block,
+                  expect_blocks=blocks)
 
 
 def test_sim_count_kernel_wide_counters():
-    # a column of more than 65,535 entries: 32-bit pair counters, and a value
-    # range that needs 20 windows re-read from memory
+    # a column of more than 65,535 entries: 32-bit counters, and a 3,000-value
+    # range of which a 76 KiB workgroup holds 3 targets' rows (7 slices)
     d, r, ip, g = _rw(29, 70000, 1, 20, 0.96, lambda rg, n: rg.integers(1, 3001, n))
     assert ip[1] >= 65536
-    run_count(d, r, ip, g, 20, 70000, "min", 24 * 1024, expect_counted=1)
+    run_count(d, r, ip, g, 20, 70000, "min", 76 * 1024, expect_counted=1, expect_blocks=7)
