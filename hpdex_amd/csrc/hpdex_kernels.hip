@@ -823,29 +823,8 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     const uint32_t cell_words = count_cell_words(rows, rs, wide);
     const uint32_t cell_off = (uint32_t)(pc.w - lds_words);
 
-    for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
-    if (tid == 0) st.bad = 0u;
-    __syncthreads();
-
-    // the one read of the column from HBM.  Lane i of a wave takes entries i,
-    // i+64, ... so a wave's 64 entries are adjacent: their row indices are
-    // sorted, and the group labels they gather (16-bit labels; rows out of
-    // range read the sentinel at index R) fall into a few cache lines.
-    // Batches of kUnroll rows and values per lane are software-pipelined: the
-    // next batch's loads are issued after this batch's label gathers and
-    // before its atomics, so they are in flight while this batch is counted
-    // (loads complete in order: waiting for the gathers leaves them
-    // outstanding).  Loads run without bounds checks -- the matrix keeps
-    // kMatrixPad >= one batch of entries past its end -- and entries past the
-    // column's end are masked in the last batch.  Each entry costs one LDS
-    // atomic, its word and increment computed without branches; entries of no
-    // row of this workgroup hit the lane's dummy word.
-    //
-    // The loads are buffer loads: the column start and its size in a scalar
-    // resource, the lane's offset in VGPRs fixed for the whole loop, the
-    // batch start in the scalar offset.  The resources reach to the end of
-    // their arrays (at most 2^32 - 1 bytes on), so every load the loop makes
-    // lies inside its array whatever the hardware range check adds.
+    // column loads (buffer loads, described with the read pass below); the
+    // first batch is issued now so that it overlaps the set-up of LDS
     const uint64_t left = (uint64_t)(uniform_i64(J.n_alloc) - p0);   // entries from p0 on
     const auto bytes32 = [](uint64_t b) { return (uint32_t)(b < 0xFFFFFFFFull ? b : 0xFFFFFFFFull); };
     const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -870,6 +849,33 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) g[u] = buffer_load_u16(gid_rsrc, min(r[u], R) * 2u);
     };
+    uint32_t r0[kUnroll];
+    V v0[kUnroll];
+    load(0u, r0, v0);
+
+    for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
+    if (tid == 0) st.bad = 0u;
+    __syncthreads();
+
+    // the one read of the column from HBM.  Lane i of a wave takes entries i,
+    // i+64, ... so a wave's 64 entries are adjacent: their row indices are
+    // sorted, and the group labels they gather (16-bit labels; rows out of
+    // range read the sentinel at index R) fall into a few cache lines.
+    // Batches of kUnroll rows and values per lane are software-pipelined: the
+    // next batch's loads are issued after this batch's label gathers and
+    // before its atomics, so they are in flight while this batch is counted
+    // (loads complete in order: waiting for the gathers leaves them
+    // outstanding).  Loads run without bounds checks -- the matrix keeps
+    // kMatrixPad >= one batch of entries past its end -- and entries past the
+    // column's end are masked in the last batch.  Each entry costs one LDS
+    // atomic, its word and increment computed without branches; entries of no
+    // row of this workgroup hit the lane's dummy word.
+    //
+    // The loads are buffer loads: the column start and its size in a scalar
+    // resource, the lane's offset in VGPRs fixed for the whole loop, the
+    // batch start in the scalar offset.  The resources reach to the end of
+    // their arrays (at most 2^32 - 1 bytes on), so every load the loop makes
+    // lies inside its array whatever the hardware range check adds.
     auto count = [&](const uint32_t (&g)[kUnroll], const V (&v)[kUnroll], auto wide_tag,
                      auto sliced_tag) {
         constexpr bool kWide = decltype(wide_tag)::value;
@@ -907,7 +913,8 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     auto read_all = [&](auto wide_tag, auto sliced_tag) {
         uint32_t r[kUnroll];
         V v[kUnroll];
-        load(0u, r, v);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) { r[u] = r0[u]; v[u] = v0[u]; }
         uint32_t ib = 0;
         for (; ib + step < n; ib += step) {   // a whole batch with another after it
             uint32_t g[kUnroll];
