@@ -10,7 +10,7 @@ _lib = None
 # Plain build for in-process use (AddressSanitizer needs its runtime preloaded:
 # run build_sim.py directly and LD_PRELOAD libasan for that).
 NOSAN = os.path.join(build_sim.OUT, "libhpdex_cpusim_nosan.so")
-PAD = 8192   # kMatrixPad: entries a resident matrix keeps past its end
+PAD = 16   # kMatrixPad: entries a resident matrix keeps past its end
 
 
 def lib(path=None):

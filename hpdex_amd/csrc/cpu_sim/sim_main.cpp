@@ -172,7 +172,7 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.plan = plan.data(); J.reftab = reftab.data();
     J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
-    J.C = C; J.n_alloc = (int64_t)nnz + (int64_t)kMatrixPad; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
+    J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
     std::vector<int32_t> range(2 * (size_t)C + 2);
     J.colrange = range.data();

@@ -109,8 +109,7 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     m->C = C;
     m->nnz = nnz;
     const size_t vsz = m->f64 ? 8 : 4;
-    // kMatrixPad extra entries (rows -1, values 0) let the count kernel read
-    // a whole batch past the last column's end
+    // kMatrixPad extra entries (rows -1, values 0) past the last column
     HIPCHK(m->vals.alloc((nnz + kMatrixPad) * vsz));
     HIPCHK(m->rows.alloc((nnz + kMatrixPad) * 4));
     HIPCHK(hipMemsetAsync((char*)m->vals.p + nnz * vsz, 0, kMatrixPad * vsz, s));
@@ -234,7 +233,6 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.status = js.status();
     J.status_col = js.status_col();
     J.C = (int64_t)m->C;
-    J.n_alloc = (int64_t)(m->nnz + kMatrixPad);
     J.R = (int32_t)m->R;
     J.G = (int32_t)js.G;
     J.T = (int32_t)js.T;

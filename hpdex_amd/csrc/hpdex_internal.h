@@ -32,8 +32,8 @@ constexpr int kCountLdsMax = 152 * 1024;    // largest dynamic LDS the tuning ac
 constexpr uint32_t kCountDummyWords = 64;      // LDS words the read pass's no-op atomics hit
 constexpr uint32_t kMaxSlices = 64;         // target slices per column, beyond: sort path
 constexpr uint32_t kCountMaxGroups = 65535; // group ids are packed in 16 bits
-constexpr size_t kMatrixPad = 8192;         // entries past nnz in a resident matrix (rows -1,
-                                            // values 0): one count-kernel batch
+constexpr size_t kMatrixPad = 16;           // entries past nnz in a resident matrix (rows -1,
+                                            // values 0)
 
 // Per-column plan: written by the split kernel (and TA by the reference-table
 // kernel), read by the later kernels of the same job.
@@ -75,7 +75,6 @@ struct JobDesc {
     uint32_t* status;            // kStatus* bits
     unsigned long long* status_col;  // lowest column that raised a status bit
     int64_t C;
-    int64_t n_alloc;             // entries allocated in vals and rows (nnz + kMatrixPad)
     int32_t R;
     int32_t G;                   // n_targets + 1
     int32_t T;

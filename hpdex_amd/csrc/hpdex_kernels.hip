@@ -94,8 +94,6 @@ __device__ inline uint64_t tie_f(uint64_t t) { return t * t * t - t; }
 // kUnroll row/value loads and kUnroll group_id gathers in flight so the
 // stream is not bound by one load latency per entry.
 constexpr int kUnroll = 8;
-// the count kernel's loads run up to one batch past a column's end
-static_assert(kMatrixPad >= 1024u * kUnroll, "matrix padding below one count-kernel batch");
 template <typename V, typename F>
 __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
                                                const V* __restrict__ vals,
@@ -825,24 +823,25 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
 
     // column loads (buffer loads, described with the read pass below); the
     // first batch is issued now so that it overlaps the set-up of LDS
-    const uint64_t left = (uint64_t)(uniform_i64(J.n_alloc) - p0);   // entries from p0 on
-    const auto bytes32 = [](uint64_t b) { return (uint32_t)(b < 0xFFFFFFFFull ? b : 0xFFFFFFFFull); };
-    const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(J.rows + p0), (short)0, bytes32(left * 4u), kBufferWord3);
-    const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(vals + p0), (short)0, bytes32(left * sizeof(V)), kBufferWord3);
+    const int32_t* col_rows = J.rows + p0;
+    const V* col_vals = vals + p0;
     const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)J.gid16, (short)0, (uint32_t)(R + 1u) * 2u, kBufferWord3);
     const uint32_t step = (uint32_t)NT * kUnroll;
     bool bad = false;
     auto load = [&](uint32_t ib, uint32_t (&r)[kUnroll], V (&v)[kUnroll]) {
+        // resources from the batch's first entry to the column's end: lanes
+        // past the end read 0 without a memory access
+        const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_rows + ib), (short)0, (n - ib) * 4u, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_vals + ib), (short)0, (n - ib) * (uint32_t)sizeof(V), kBufferWord3);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            // per lane: entry tid + u*NT of the batch (loop-invariant); per
-            // batch: its first entry ib (one scalar)
+            // per lane: entry tid + u*NT of the batch (loop-invariant)
             const uint32_t e = (uint32_t)tid + (uint32_t)(u * NT);
-            r[u] = buffer_load_u32(rows_rsrc, e * 4u, ib * 4u);
-            v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), ib * (uint32_t)sizeof(V));
+            r[u] = buffer_load_u32(rows_rsrc, e * 4u, 0u);
+            v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
         }
     };
     auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
@@ -865,17 +864,16 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     // next batch's loads are issued after this batch's label gathers and
     // before its atomics, so they are in flight while this batch is counted
     // (loads complete in order: waiting for the gathers leaves them
-    // outstanding).  Loads run without bounds checks -- the matrix keeps
-    // kMatrixPad >= one batch of entries past its end -- and entries past the
-    // column's end are masked in the last batch.  Each entry costs one LDS
-    // atomic, its word and increment computed without branches; entries of no
-    // row of this workgroup hit the lane's dummy word.
+    // outstanding).  Each entry costs one LDS atomic, its word and increment
+    // computed without branches; entries of no row of this workgroup hit the
+    // lane's dummy word.
     //
-    // The loads are buffer loads: the column start and its size in a scalar
-    // resource, the lane's offset in VGPRs fixed for the whole loop, the
-    // batch start in the scalar offset.  The resources reach to the end of
-    // their arrays (at most 2^32 - 1 bytes on), so every load the loop makes
-    // lies inside its array whatever the hardware range check adds.
+    // The loads are buffer loads: each batch's start and the bytes left to
+    // the column's end in a scalar resource, the lane's offset in VGPRs fixed
+    // for the whole loop.  The hardware range check turns the lanes past the
+    // column's end into zeros that cost no memory traffic (they are masked in
+    // the last batch), so no load leaves the column.  Columns are shorter
+    // than 2^28 entries, so the byte sizes fit 32 bits.
     auto count = [&](const uint32_t (&g)[kUnroll], const V (&v)[kUnroll], auto wide_tag,
                      auto sliced_tag) {
         constexpr bool kWide = decltype(wide_tag)::value;
