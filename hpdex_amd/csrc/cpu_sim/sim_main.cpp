@@ -163,11 +163,13 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     std::vector<ColPlan> plan(C);
     std::vector<uint32_t> reftab((size_t)C * kTabStride);
     JobDesc J{};
-    std::vector<uint16_t> gid16(R + 1, (uint16_t)0xFFFF);
-    for (int r = 0; r < R; ++r)
-        gid16[r] = (gid[r] >= 0 && gid[r] < G) ? (uint16_t)gid[r] : (uint16_t)0xFFFF;
+    const uint32_t bits = label_bits((uint32_t)G);
+    std::vector<uint32_t> gidp(label_words((size_t)R, bits));
+    pack_labels(gid, (size_t)R, (uint32_t)G, bits, gidp.data());
     J.vals = vals; J.rows = rows; J.indptr = indptr; J.gid = gid; J.group_rows = grows.data();
-    J.gid16 = gid16.data();
+    J.gidp = gidp.data();
+    J.gid_bits = bits;
+    J.gid_words = (uint32_t)gidp.size();
     J.keys = keys.data(); J.cnt = cnt.data(); J.zcnt = zcnt.data(); J.goff = goff.data();
     J.plan = plan.data(); J.reftab = reftab.data();
     J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;

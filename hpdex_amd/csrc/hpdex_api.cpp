@@ -175,17 +175,21 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     const size_t np = js.C * T;
     HIPCHK(js.gid.alloc(m->R * 4));
     if (m->R) HIPCHK(hipMemcpyAsync(js.gid.p, group_id, m->R * 4, hipMemcpyHostToDevice, s));
-    // 16-bit labels for the count kernel's gathers (it runs only when G <= 65535)
+    // packed labels for the count kernel's gathers (it runs only when
+    // G <= 65535 and every label's bit position fits 32 bits)
+    js.gid_bits = 0;
+    js.gid_words = 0;
     if (js.G <= kCountMaxGroups) {
-        // one label past the last row (index R, no group) for rows out of range
-        std::vector<uint16_t> g16(m->R + 1, (uint16_t)0xFFFF);
-        for (size_t r = 0; r < m->R; ++r) {
-            const int g = group_id[r];
-            g16[r] = (g >= 0 && (size_t)g < js.G) ? (uint16_t)g : (uint16_t)0xFFFF;
+        const uint32_t bits = label_bits((uint32_t)js.G);
+        if ((uint64_t)(m->R + 1) * bits < ((uint64_t)1 << 32)) {
+            js.gid_bits = bits;
+            js.gid_words = label_words(m->R, bits);
+            std::vector<uint32_t> packed(js.gid_words);
+            pack_labels(group_id, m->R, (uint32_t)js.G, bits, packed.data());
+            HIPCHK(js.gidp.alloc(js.gid_words * 4));
+            HIPCHK(hipMemcpyAsync(js.gidp.p, packed.data(), js.gid_words * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));   // packed is a host temporary
         }
-        HIPCHK(js.gid16.alloc((m->R + 1) * 2));
-        HIPCHK(hipMemcpyAsync(js.gid16.p, g16.data(), (m->R + 1) * 2, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));   // g16 is a host temporary
     }
     HIPCHK(js.grows.alloc(js.G * 4));
     HIPCHK(hipMemcpyAsync(js.grows.p, js.group_rows.data(), js.G * 4, hipMemcpyHostToDevice, s));
@@ -215,7 +219,9 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.rows = m->rows.as<int32_t>();
     J.indptr = m->indptr.as<int64_t>();
     J.gid = js.gid.as<int32_t>();
-    J.gid16 = js.gid16.as<uint16_t>();
+    J.gidp = js.gidp.as<uint32_t>();
+    J.gid_bits = js.gid_bits;
+    J.gid_words = (uint32_t)js.gid_words;
     J.group_rows = js.grows.as<uint32_t>();
     J.colrange = m->colrange.as<int32_t>();
     J.keys = js.keys.p;
@@ -256,7 +262,7 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     js.J = J;
     HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
     HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
-    const bool count_ok = tu.count_lds > 0 && js.G >= 2 && js.G <= kCountMaxGroups;
+    const bool count_ok = tu.count_lds > 0 && js.G >= 2 && js.G <= kCountMaxGroups && js.gid_bits > 0;
     const uint32_t T = (uint32_t)js.T;
     std::vector<CountItem> count_items;
     std::vector<WorkItem> gen_items;

@@ -127,7 +127,9 @@ struct JobState {
     bool stats = false;          // tie and sample sizes kept per pair
     hpdex_mwu_opts opts{};
     std::vector<uint32_t> group_rows;
-    DevBuf gid, gid16, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    DevBuf gid, gidp, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    uint32_t gid_bits = 0;
+    size_t gid_words = 0;
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
     DevBuf d_job;                                   // descriptor
     DevBuf d_count_items, d_gen_items, d_gen_n;     // work queues of the two paths

@@ -55,7 +55,9 @@ struct JobDesc {
     const int32_t* rows;         // nnz row indices, -1 = out of range
     const int64_t* indptr;       // C+1
     const int32_t* gid;          // R group labels
-    const uint16_t* gid16;       // R+1 group labels as 16 bits, 0xFFFF = none; [R] = none
+    const uint32_t* gidp;        // R+1 group labels packed at gid_bits bits (pack_labels)
+    uint32_t gid_bits;
+    uint32_t gid_words;          // 32-bit words of gidp
     const uint32_t* group_rows;  // G rows per group
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
     // workspace
@@ -130,6 +132,32 @@ __host__ __device__ inline uint64_t count_lds_need(uint32_t rows, uint32_t NB, b
     const uint32_t rs = count_row_stride(NB + 1u, wide);
     return 4ull * kCountDummyWords + 16ull * rows + 4ull * count_row_words(rows) + 8ull * NB +
            4ull * (uint64_t)count_cell_words(rows, rs, wide);
+}
+
+// Group labels for the count kernel's gathers: rows 0..R-1 and one more,
+// row R (where rows out of range are sent), packed at `bits` bits each, least
+// significant first within 32-bit words (a label may straddle two words; the
+// kernel reads the 8 bytes from the label's first word).  A row with no group
+// (label outside [0, G)) and row R hold 2^bits - 1, which is >= G.
+inline uint32_t label_bits(uint32_t G) {
+    uint32_t b = 1;
+    while (((1u << b) - 1u) < G) ++b;
+    return b;
+}
+inline size_t label_words(size_t R, uint32_t bits) { return ((R + 1) * bits + 31) / 32 + 2; }
+inline void pack_labels(const int32_t* gid, size_t R, uint32_t G, uint32_t bits, uint32_t* out) {
+    const size_t nw = label_words(R, bits);
+    for (size_t w = 0; w < nw; ++w) out[w] = 0u;
+    const uint32_t none = (1u << bits) - 1u;
+    for (size_t r = 0; r <= R; ++r) {
+        const uint32_t lab =
+            (r < R && gid[r] >= 0 && (uint32_t)gid[r] < G) ? (uint32_t)gid[r] : none;
+        const uint64_t pos = (uint64_t)r * bits;
+        const size_t w = (size_t)(pos >> 5);
+        const uint32_t sh = (uint32_t)(pos & 31u);
+        out[w] |= lab << sh;
+        if (sh + bits > 32u) out[w + 1] |= lab >> (32u - sh);
+    }
 }
 
 // Slice size of a count-kernel column with T targets, NB bins and n stored

@@ -65,8 +65,8 @@ __device__ __forceinline__ uint32_t buffer_load_u32(__amdgpu_buffer_rsrc_t rs, u
                                                     uint32_t soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, kBufferNT);
 }
-__device__ __forceinline__ uint32_t buffer_load_u16(__amdgpu_buffer_rsrc_t rs, uint32_t voff) {
-    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, 0, 0);
+__device__ __forceinline__ uint64_t buffer_load_u64(__amdgpu_buffer_rsrc_t rs, uint32_t voff) {
+    return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, 0, 0));
 }
 template <typename V>
 __device__ __forceinline__ V buffer_load_value(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
@@ -826,7 +826,9 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     const int32_t* col_rows = J.rows + p0;
     const V* col_vals = vals + p0;
     const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)J.gid16, (short)0, (uint32_t)(R + 1u) * 2u, kBufferWord3);
+        (void*)J.gidp, (short)0, uniform_u32(J.gid_words) * 4u, kBufferWord3);
+    const uint32_t lbits = uniform_u32(J.gid_bits);
+    const uint32_t lmask = (1u << lbits) - 1u;
     const uint32_t step = (uint32_t)NT * kUnroll;
     bool bad = false;
     auto load = [&](uint32_t ib, uint32_t (&r)[kUnroll], V (&v)[kUnroll]) {
@@ -846,7 +848,12 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     };
     auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) g[u] = buffer_load_u16(gid_rsrc, min(r[u], R) * 2u);
+        for (int u = 0; u < kUnroll; ++u) {
+            // the label's bit position; 8 bytes from its word hold all of it
+            const uint32_t pos = min(r[u], R) * lbits;
+            const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
+            g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
+        }
     };
     uint32_t r0[kUnroll];
     V v0[kUnroll];
@@ -858,8 +865,9 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
 
     // the one read of the column from HBM.  Lane i of a wave takes entries i,
     // i+64, ... so a wave's 64 entries are adjacent: their row indices are
-    // sorted, and the group labels they gather (16-bit labels; rows out of
-    // range read the sentinel at index R) fall into a few cache lines.
+    // sorted, and the group labels they gather (bit-packed at ceil(log2(G+1))
+    // bits, so a column's gathers touch as few cache lines as possible; rows
+    // out of range read the sentinel at index R) fall into a few lines.
     // Batches of kUnroll rows and values per lane are software-pipelined: the
     // next batch's loads are issued after this batch's label gathers and
     // before its atomics, so they are in flight while this batch is counted

@@ -107,7 +107,7 @@ size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows
         f += up256(nnz * 8);
     }
     // job_setup
-    f += up256(R * 4) + up256((R + 1) * 2) + up256(G * 4) + 2 * up256(np * 8) + up256(32);
+    f += up256(R * 4) + up256(label_words(R, 16) * 4) + up256(G * 4) + 2 * up256(np * 8) + up256(32);
     f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));
     f += up256(C * (size_t)kTabStride * 4);
     // job_upload_desc
