@@ -130,14 +130,16 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
             else
                 count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
-    const unsigned ng = n_static + nc;
+    // the looping form on a grid of at most 3 workgroups, so that each takes
+    // several queue items (the other form runs the same item code once)
+    const unsigned ng = std::min(n_static + nc, 3u);
     const WorkItem* it = gen_items.data();
     for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+        run_block(b, kColBlock, ng, [&] { split_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
     for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+        run_block(b, kColBlock, ng, [&] { ref_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
     for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+        run_block(b, kColBlock, ng, [&] { target_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
     return (int)(nc - (gen_n - n_static));
 }
 

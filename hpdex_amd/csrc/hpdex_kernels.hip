@@ -168,10 +168,16 @@ __device__ inline uint32_t upper_bound_k(const K* a, uint32_t lo, uint32_t hi, K
 }
 
 // The work item of this workgroup, uniform across it (kept in SGPRs).
-__device__ inline void work_item(const WorkItem* items, int& job, int64_t& col) {
-    job = __builtin_amdgcn_readfirstlane(items[blockIdx.x].job);
-    col = (int64_t)__builtin_amdgcn_readfirstlane(items[blockIdx.x].col);
+__device__ inline void work_item(const WorkItem* items, uint32_t k, int& job, int64_t& col) {
+    job = __builtin_amdgcn_readfirstlane(items[k].job);
+    col = (int64_t)__builtin_amdgcn_readfirstlane(items[k].col);
 }
+
+// Workgroups of the split / reference / target kernels when the host routed
+// no column to them (their queue then holds only the count kernel's rare
+// hand-overs): at most this many, each taking several queue items
+constexpr unsigned kHandoverGrid = 256;
+template <typename T> struct TypeTag { using type = T; };
 
 // A workgroup-uniform value held in an SGPR.
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t x) {
@@ -215,18 +221,14 @@ struct SplitState {
 };
 
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs,
-                                                          const WorkItem* items,
-                                                          const uint32_t* n_items) {
+__device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ SplitState st;
-    // items past the queue's length (the grid is sized for its capacity)
-    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
-    work_item(items, job, c);
+    work_item(items, k, job, c);
     const JobDesc J = jobs[job];
     const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
     const int G = J.G;
@@ -345,6 +347,22 @@ __global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs,
     });
 }
 
+// kLoop false: workgroup b takes queue item b (a grid as large as the
+// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
+template <typename V, bool kLoop>
+__global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs, const WorkItem* items,
+                                                   const uint32_t* n_items) {
+    const uint32_t n = uniform_u32(*n_items);
+    if (!kLoop) {
+        if (blockIdx.x < n) split_item<V>(jobs, items, blockIdx.x);
+        return;
+    }
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        split_item<V>(jobs, items, k);
+        __syncthreads();   // the next item reuses LDS
+    }
+}
+
 // Rank table of the reference slice seen by the target phase.
 template <typename K>
 struct RefView {
@@ -436,18 +454,14 @@ struct RefState {
 };
 
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs,
-                                                        const WorkItem* items,
-                                                        const uint32_t* n_items) {
+__device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ RefState st;
-    // items past the queue's length (the grid is sized for its capacity)
-    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
-    work_item(items, job, c);
+    work_item(items, k, job, c);
     const JobDesc J = jobs[job];
     const int tid = tid_x(), nt = blockDim.x;
     const ColPlan pl = J.plan[c];
@@ -497,6 +511,22 @@ __global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs,
         ta = block_sum_u64(ta, st.red);
     }
     if (tid == 0 && (hist || na <= cap)) J.plan[c].TA = ta;
+}
+
+// kLoop false: workgroup b takes queue item b (a grid as large as the
+// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
+template <typename V, bool kLoop>
+__global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
+                                                   const uint32_t* n_items) {
+    const uint32_t n = uniform_u32(*n_items);
+    if (!kLoop) {
+        if (blockIdx.x < n) ref_item<V>(jobs, items, blockIdx.x);
+        return;
+    }
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        ref_item<V>(jobs, items, k);
+        __syncthreads();   // the next item reuses LDS
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -652,17 +682,13 @@ struct TargetState {
 };
 
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
-                                                           const WorkItem* items,
-                                                           const uint32_t* n_items) {
+__device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using K = typename KeyOf<V>::K;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TargetState st;
-    // items past the queue's length (the grid is sized for its capacity)
-    if (blockIdx.x >= (unsigned)__builtin_amdgcn_readfirstlane((int)*n_items)) return;
     int job;
     int64_t c;
-    work_item(items, job, c);
+    work_item(items, k, job, c);
     const JobDesc J = jobs[job];
     const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
     const ColPlan pl = J.plan[c];
@@ -705,6 +731,22 @@ __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs,
         }
     }
     flush_targets(J, mine, pending);
+}
+
+// kLoop false: workgroup b takes queue item b (a grid as large as the
+// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
+template <typename V, bool kLoop>
+__global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs, const WorkItem* items,
+                                                   const uint32_t* n_items) {
+    const uint32_t n = uniform_u32(*n_items);
+    if (!kLoop) {
+        if (blockIdx.x < n) target_item<V>(jobs, items, blockIdx.x);
+        return;
+    }
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        target_item<V>(jobs, items, k);
+        __syncthreads();   // the next item reuses LDS
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1363,6 +1405,11 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
     const int n_gen = L.n_gen_static + L.n_count_cols;
     if (n_gen <= 0) return hipSuccess;
+    // host-routed columns: one workgroup per queue slot; hand-overs only: a
+    // small grid of looping workgroups
+    const bool loop = L.n_gen_static == 0;
+    const unsigned sort_grid =
+        loop ? ((unsigned)n_gen < kHandoverGrid ? (unsigned)n_gen : kHandoverGrid) : (unsigned)n_gen;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
     reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
     mark(0);
@@ -1386,18 +1433,21 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
     const unsigned split_lds = align16(split);
     const unsigned tab_lds = table_lds_bytes(L.f64);
+    auto sort_path = [&](auto v_tag, auto loop_tag) {
+        using V = typename decltype(v_tag)::type;
+        constexpr bool kLoop = decltype(loop_tag)::value;
+        split_kernel<V, kLoop><<<sort_grid, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(2);
+        ref_kernel<V, kLoop><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(3);
+        target_kernel<V, kLoop><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+    };
     if (L.f64) {
-        split_kernel<double><<<n_gen, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        mark(2);
-        ref_kernel<double><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        mark(3);
-        target_kernel<double><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        if (loop) sort_path(TypeTag<double>{}, std::true_type{});
+        else sort_path(TypeTag<double>{}, std::false_type{});
     } else {
-        split_kernel<float><<<n_gen, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        mark(2);
-        ref_kernel<float><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        mark(3);
-        target_kernel<float><<<n_gen, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        if (loop) sort_path(TypeTag<float>{}, std::true_type{});
+        else sort_path(TypeTag<float>{}, std::false_type{});
     }
     mark(4);
     return hipGetLastError();
