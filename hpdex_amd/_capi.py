@@ -29,8 +29,18 @@ EXPORTED = (
     "hpdex_get_tuning", "hpdex_set_tuning", "hpdex_version", "hpdex_device_info",
     "hpdex_bandwidth_probe", "hpdex_plan_get_info", "hpdex_plan_stage_times",
     "hpdex_session_create", "hpdex_session_submit", "hpdex_session_wait", "hpdex_session_peak",
-    "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc",
+    "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc", "hpdex_matrix_create_from",
+    "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
 )
+
+LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
+
+
+class MatrixSrc(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_int32), ("dtype", ctypes.c_int32), ("data", ctypes.c_void_p),
+                ("indices", ctypes.c_void_p), ("indptr", ctypes.c_void_p),
+                ("index_bits", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n_rows", ctypes.c_size_t), ("n_cols", ctypes.c_size_t), ("nnz", ctypes.c_size_t)]
 
 
 class MwuOpts(ctypes.Structure):
@@ -119,6 +129,11 @@ def lib() -> ctypes.CDLL:
         L.hpdex_session_destroy.argtypes = [vp]
         L.hpdex_session_destroy.restype = None
         L.hpdex_plan_stage_times.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_matrix_create_from.argtypes = [ctypes.POINTER(MatrixSrc), ctypes.POINTER(vp), cp, sz]
+        L.hpdex_matrix_shape.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                         ctypes.POINTER(sz)]
+        L.hpdex_matrix_download.argtypes = [vp, vp, vp, vp, cp, sz]
+        L.hpdex_de_matrix.argtypes = [vp, vp, sz, ci, ci, ctypes.c_double, vp, vp, vp, vp, vp, cp, sz]
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
                      "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
@@ -221,8 +236,39 @@ def de_csc(data, indices, indptr, group_id, n_targets, n_rows=None, tie_correcti
     return tuple(o[:n] for o in outs)
 
 
+def _index_array(a):
+    """Index array as given when it is int32 or int64 (no copy), else int64."""
+    a = np.asarray(a)
+    if a.dtype not in (np.dtype(np.int32), np.dtype(np.int64)):
+        a = a.astype(np.int64)
+    return np.ascontiguousarray(a)
+
+
+def _data_array(a):
+    a = np.asarray(a)
+    if a.dtype not in DTYPE_CODES:
+        raise HpdexError("Unsupported dtype for data")
+    return np.ascontiguousarray(a)
+
+
+def de_matrix(m, group_id, n_targets, tie_correction=True, use_continuity=True, clip_value=20.0):
+    """hpdex_de_matrix on a DeviceMatrix: as de_csc."""
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    n = m.n_cols * int(n_targets)
+    outs = [np.empty(max(n, 1)) for _ in range(5)]
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().hpdex_de_matrix(m.handle, _p(group_id), int(n_targets), int(tie_correction),
+                               int(use_continuity), float(clip_value), *[_p(o) for o in outs],
+                               err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return tuple(o[:n] for o in outs)
+
+
 class DeviceMatrix:
-    """A CSC matrix resident in HBM (hpdex_matrix_create)."""
+    """A CSC matrix resident in HBM (hpdex_matrix_create), or built on the
+    device from CSR / dense input or 32-bit indices (from_scipy, from_dense:
+    hpdex_matrix_create_from)."""
 
     def __init__(self, data, indices, indptr, n_rows):
         data, indices, indptr = _csc_args(data, indices, indptr)
@@ -237,6 +283,67 @@ class DeviceMatrix:
         if rc != 0:
             raise HpdexError(err.value.decode())
         self._h = h
+
+    @classmethod
+    def _create(cls, src, keep):
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().hpdex_matrix_create_from(ctypes.byref(src), ctypes.byref(h), err, len(err))
+        del keep
+        if rc != 0:
+            raise HpdexError(err.value.decode())
+        self._h = h
+        R, C, nnz = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        lib().hpdex_matrix_shape(h, ctypes.byref(R), ctypes.byref(C), ctypes.byref(nnz))
+        self.n_rows, self.n_cols, self.nnz = int(R.value), int(C.value), int(nnz.value)
+        return self
+
+    @classmethod
+    def from_arrays(cls, layout, data, indices, indptr, n_rows, n_cols):
+        """CSC (indices = rows) or CSR (indices = columns) arrays as they are:
+        int32 or int64 indices, any of the ten dtypes; converted on the device."""
+        data, indices, indptr = _data_array(data), _index_array(indices), _index_array(indptr)
+        if indices.dtype != indptr.dtype:
+            indices = indices.astype(np.int64)
+            indptr = indptr.astype(np.int64)
+        src = MatrixSrc(layout=layout, dtype=DTYPE_CODES[data.dtype], data=_p(data),
+                        indices=_p(indices), indptr=_p(indptr),
+                        index_bits=64 if indices.dtype == np.int64 else 32,
+                        n_rows=int(n_rows), n_cols=int(n_cols), nnz=len(data))
+        return cls._create(src, (data, indices, indptr))
+
+    @classmethod
+    def from_scipy(cls, m):
+        """A scipy CSC or CSR matrix (cells x genes), converted on the device."""
+        import scipy.sparse
+        if isinstance(m, scipy.sparse.csc_matrix) or getattr(m, "format", None) == "csc":
+            layout = LAYOUT_CSC
+        elif isinstance(m, scipy.sparse.csr_matrix) or getattr(m, "format", None) == "csr":
+            layout = LAYOUT_CSR
+        else:
+            raise HpdexError("sparse_matrix must be CSR or CSC")
+        return cls.from_arrays(layout, m.data, m.indices, m.indptr, m.shape[0], m.shape[1])
+
+    @classmethod
+    def from_dense(cls, X):
+        """A dense cells x genes array (row-major), its nonzeros kept."""
+        X = _data_array(X)
+        if X.ndim != 2:
+            raise HpdexError("dense input must be 2-D")
+        src = MatrixSrc(layout=LAYOUT_DENSE, dtype=DTYPE_CODES[X.dtype], data=_p(X),
+                        index_bits=64, n_rows=X.shape[0], n_cols=X.shape[1], nnz=0)
+        return cls._create(src, (X,))
+
+    def download(self):
+        """(data as float64, rows int32, indptr int64) of the resident CSC."""
+        data = np.empty(max(self.nnz, 1))
+        rows = np.empty(max(self.nnz, 1), dtype=np.int32)
+        indptr = np.empty(self.n_cols + 1, dtype=np.int64)
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_matrix_download(self._h, _p(data), _p(rows), _p(indptr), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        return data[:self.nnz], rows[:self.nnz], indptr
 
     @property
     def handle(self):

@@ -58,15 +58,17 @@ def build(verbose: bool = False) -> None:
     a_obj = os.path.join(BUILD, "hpdex_api.o")
     s_obj = os.path.join(BUILD, "hpdex_session.o")
     d_obj = os.path.join(BUILD, "hpdex_de.o")
+    i_obj = os.path.join(BUILD, "hpdex_ingest.o")
     _run([
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_kernels.hip"), "-o", k_obj],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_de.hip"), "-o", d_obj],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_ingest.hip"), "-o", i_obj],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "hpdex_api.cpp"), "-o", a_obj],
         [HIPCC, *HIP_FLAGS, "-pthread", "-x", "hip", "-c", os.path.join(CSRC, "hpdex_session.cpp"),
          "-o", s_obj],
     ])
     tmp = LIB + ".tmp"
-    _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", k_obj, d_obj, a_obj,
+    _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", k_obj, d_obj, i_obj, a_obj,
            s_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so"]])
     os.replace(tmp, LIB)
 

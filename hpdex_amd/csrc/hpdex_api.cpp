@@ -134,8 +134,16 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     }
     HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (C + 1) * 8, hipMemcpyHostToDevice, s));
     m->h_indptr.assign(indptr, indptr + C + 1);
-    // column metadata: integer-valued columns and their value ranges, which
-    // let the count kernel bin values during its one read of a column
+    return finish_matrix(m, s, err, err_len);
+}
+
+// Column metadata of a matrix whose values, rows and column pointers are on
+// the device (h_indptr set): integer-valued columns and their value ranges,
+// which let the count kernel bin values during its one read of a column, and
+// the columns by size.
+int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
+    const size_t C = m->C;
+    const int64_t* indptr = m->h_indptr.data();
     m->col_int.assign(C, 0);
     m->h_range.assign(2 * C, 0);
     HIPCHK(m->colrange.alloc(2 * C * 4));
@@ -156,6 +164,119 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     });
     HIPCHK(hipStreamSynchronize(s));
     return 0;
+}
+
+// Row or column pointers of a source layout, widened to int64 on the host and
+// checked as a CSC index is.
+static int widen_indptr(const void* p, int bits, size_t n, size_t nnz, std::vector<int64_t>& out,
+                        char* err, size_t err_len) {
+    out.resize(n + 1);
+    for (size_t i = 0; i <= n; ++i)
+        out[i] = bits == 32 ? (int64_t)((const int32_t*)p)[i] : ((const int64_t*)p)[i];
+    return check_indptr(out.data(), n, nnz, err, err_len);
+}
+
+// Values of the source's dtype on the device, converted to the matrix's value
+// type later by the kernels of each layout.
+static hipError_t upload_raw(DevBuf& b, const void* host, size_t bytes, hipStream_t s) {
+    hipError_t e = b.alloc(bytes);
+    if (e != hipSuccess || bytes == 0) return e;
+    return hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s);
+}
+
+int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t s, char* err,
+                       size_t err_len) {
+    const int dtype = src->dtype;
+    const size_t R = src->n_rows, C = src->n_cols;
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (R > (size_t)INT32_MAX) return set_err(err, err_len, "more than 2^31-1 rows is unsupported");
+    if (C > (size_t)INT32_MAX) return set_err(err, err_len, "more than 2^31-1 columns is unsupported");
+    const int bits = src->index_bits;
+    if (src->layout != HPDEX_LAYOUT_DENSE && bits != 32 && bits != 64)
+        return set_err(err, err_len, "index_bits must be 32 or 64");
+    if (src->layout == HPDEX_LAYOUT_CSC && bits == 64)
+        return upload_matrix(m, dtype, src->data, (const int64_t*)src->indices,
+                             (const int64_t*)src->indptr, R, C, src->nnz, s, err, err_len);
+    m->dtype = dtype;
+    m->f64 = dtype_uses_double(dtype);
+    m->R = R;
+    m->C = C;
+    const size_t vsz = m->f64 ? 8 : 4;
+    auto alloc_out = [&](size_t nnz) -> hipError_t {
+        m->nnz = nnz;
+        hipError_t e;
+        if ((e = m->vals.alloc((nnz + kMatrixPad) * vsz)) != hipSuccess) return e;
+        if ((e = m->rows.alloc((nnz + kMatrixPad) * 4)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync((char*)m->vals.p + nnz * vsz, 0, kMatrixPad * vsz, s)) != hipSuccess)
+            return e;
+        if ((e = hipMemsetAsync((char*)m->rows.p + nnz * 4, 0xFF, kMatrixPad * 4, s)) != hipSuccess)
+            return e;
+        return m->indptr.alloc((C + 1) * 8);
+    };
+    if (src->layout == HPDEX_LAYOUT_CSC) {   // 32-bit indices
+        const size_t nnz = src->nnz;
+        if (widen_indptr(src->indptr, 32, C, nnz, m->h_indptr, err, err_len)) return 1;
+        HIPCHK(alloc_out(nnz));
+        if (nnz) {
+            if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
+                HIPCHK(hipMemcpyAsync(m->vals.p, src->data, nnz * vsz, hipMemcpyHostToDevice, s));
+            } else {
+                DevBuf raw;
+                HIPCHK(upload_raw(raw, src->data, nnz * dtype_size(dtype), s));
+                HIPCHK(launch_convert_values(dtype, raw.p, m->vals.p, nnz, s));
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            HIPCHK(hipMemcpyAsync(m->rows.p, src->indices, nnz * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(launch_rows_i32_check(m->rows.as<int32_t>(), nnz, (int64_t)R, s));
+        }
+        HIPCHK(hipMemcpyAsync(m->indptr.p, m->h_indptr.data(), (C + 1) * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return finish_matrix(m, s, err, err_len);
+    }
+    if (src->layout == HPDEX_LAYOUT_CSR) {
+        const size_t nnz = src->nnz;
+        if (nnz > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 stored values in CSR input");
+        std::vector<int64_t> rptr;
+        if (widen_indptr(src->indptr, bits, R, nnz, rptr, err, err_len)) return 1;
+        if ((size_t)rptr[R] != nnz || rptr[0] != 0)
+            return set_err(err, err_len, "CSR indptr must start at 0 and end at nnz");
+        HIPCHK(alloc_out(nnz));
+        DevBuf raw, cols, ptr, ws;
+        HIPCHK(upload_raw(raw, src->data, nnz * dtype_size(dtype), s));
+        HIPCHK(upload_raw(cols, src->indices, nnz * (size_t)(bits / 8), s));
+        HIPCHK(upload_raw(ptr, src->indptr, (R + 1) * (size_t)(bits / 8), s));
+        const size_t wsb = csr_workspace_bytes(nnz);
+        HIPCHK(ws.alloc(wsb));
+        uint32_t bad = 0;
+        HIPCHK(launch_csr_to_csc(dtype, m->f64, raw.p, cols.p, ptr.p, bits, R, C, nnz, m->vals.p,
+                                 m->rows.as<int32_t>(), m->indptr.as<int64_t>(), &bad, ws.p, wsb, s));
+        if (bad) return set_err(err, err_len, "column index out of range in CSR input");
+        m->h_indptr.resize(C + 1);
+        HIPCHK(hipMemcpyAsync(m->h_indptr.data(), m->indptr.p, (C + 1) * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return finish_matrix(m, s, err, err_len);
+    }
+    if (src->layout == HPDEX_LAYOUT_DENSE) {
+        if (R && C > (size_t)-1 / R / 8) return set_err(err, err_len, "dense input too large");
+        DevBuf raw, ws, ptr;
+        HIPCHK(upload_raw(raw, src->data, R * C * dtype_size(dtype), s));
+        const size_t wsb = dense_workspace_bytes(R, C, m->f64);
+        HIPCHK(ws.alloc(wsb));
+        HIPCHK(ptr.alloc((C + 1) * 8));
+        HIPCHK(launch_dense_count(dtype, m->f64, raw.p, R, C, ptr.as<int64_t>(), ws.p, wsb, s));
+        m->h_indptr.resize(C + 1);
+        HIPCHK(hipMemcpyAsync(m->h_indptr.data(), ptr.p, (C + 1) * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        raw.reset();
+        const size_t nnz = (size_t)m->h_indptr[C];
+        HIPCHK(alloc_out(nnz));
+        HIPCHK(hipMemcpyAsync(m->indptr.p, ptr.p, (C + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(launch_dense_compact(m->f64, R, C, ptr.as<int64_t>(), m->vals.p, m->rows.as<int32_t>(),
+                                    ws.p, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return finish_matrix(m, s, err, err_len);
+    }
+    return set_err(err, err_len, "unknown layout");
 }
 
 int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
@@ -658,27 +779,14 @@ int hpdex_fdr_bh(const double* p, size_t n, double* fdr_out, char* err, size_t e
     return 0;
 }
 
-int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
-                 size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
-                 size_t n_targets, int tie_correction, int use_continuity, double clip_value,
-                 int threads, double* U1_out, double* P_out, double* fc_out,
-                 double* log2fc_out, double* fdr_out, char* err, size_t err_len) {
-    (void)threads;
-    if (n_targets < 1)
-        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
-    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
-    if (n_cols == 0) return 0;
+// de.py:120-184 on a matrix already on the device (steps 4-7 of the driver)
+static int de_run(hpdex_matrix& m, const int32_t* group_id, size_t n_targets, int tie_correction,
+                  int use_continuity, double clip_value, double* U1_out, double* P_out,
+                  double* fc_out, double* log2fc_out, double* fdr_out, hipStream_t s, char* err,
+                  size_t err_len) {
+    const size_t n_cols = m.C, n_rows = m.R;
     const size_t G = n_targets + 1, np = n_cols * n_targets;
-    if (16 * G > (size_t)kLdsBudget)
-        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
-    if (np > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 tests");
-    if (ensure_init(err, err_len)) return 1;
     const Tuning tu = tuning();
-    Stream st;
-    HIPCHK(st.create());
-    hpdex_matrix m;
-    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
-        return 1;
     // de.py:147-158: zero_handling "min", two-sided, asymptotic
     hpdex_mwu_opts o{};
     o.tie_correction = tie_correction ? 1 : 0;
@@ -687,33 +795,125 @@ int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int6
     o.alternative = 2;
     o.method = 2;
     JobState js;
-    if (job_setup(js, &m, group_id, n_targets, &o, false, st.s, err, err_len)) return 1;
+    if (job_setup(js, &m, group_id, n_targets, &o, false, s, err, err_len)) return 1;
     const JobDesc J = make_desc(js, &m, tu);
-    if (job_upload_desc(js, J, m, tu, false, st.s, err, err_len)) return 1;
+    if (job_upload_desc(js, J, m, tu, false, s, err, err_len)) return 1;
     // de.py:126-142: group means with zeros, fold change, log2 fold change
     DevBuf means, fc, lfc, fdr, ws;
     HIPCHK(means.alloc(n_cols * G * 8));
     HIPCHK(launch_group_mean(m.vals.p, m.f64, m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
                              js.gid.as<int32_t>(), js.grows.as<uint32_t>(), (int64_t)n_cols,
-                             (int32_t)n_rows, (int32_t)G, 1, means.as<double>(), st.s));
+                             (int32_t)n_rows, (int32_t)G, 1, means.as<double>(), s));
     HIPCHK(fc.alloc(np * 8));
     HIPCHK(lfc.alloc(np * 8));
     HIPCHK(launch_fold_change(means.as<double>(), (int64_t)n_cols, (int32_t)n_targets, clip_value,
-                              fc.as<double>(), lfc.as<double>(), st.s));
+                              fc.as<double>(), lfc.as<double>(), s));
     // de.py:147-158: the test
-    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), st.s));
-    if (job_errors(js, st.s, err, err_len)) return 1;
+    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), s));
+    if (job_errors(js, s, err, err_len)) return 1;
     // de.py:172-184: BH FDR over every p-value
     HIPCHK(fdr.alloc(np * 8));
     const size_t wsb = fdr_workspace_bytes((int64_t)np);
     HIPCHK(ws.alloc(wsb));
-    HIPCHK(launch_fdr_bh(js.P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, st.s));
+    HIPCHK(launch_fdr_bh(js.P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, s));
     const size_t b = np * 8;
-    if (U1_out) HIPCHK(hipMemcpyAsync(U1_out, js.U1.p, b, hipMemcpyDeviceToHost, st.s));
-    if (P_out) HIPCHK(hipMemcpyAsync(P_out, js.P.p, b, hipMemcpyDeviceToHost, st.s));
-    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, st.s));
-    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, st.s));
-    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, st.s));
+    if (U1_out) HIPCHK(hipMemcpyAsync(U1_out, js.U1.p, b, hipMemcpyDeviceToHost, s));
+    if (P_out) HIPCHK(hipMemcpyAsync(P_out, js.P.p, b, hipMemcpyDeviceToHost, s));
+    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, s));
+    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, s));
+    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+static int de_checks(size_t n_targets, size_t n_cols, char* err, size_t err_len) {
+    if (n_targets < 1)
+        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    const size_t G = n_targets + 1;
+    if (16 * G > (size_t)kLdsBudget)
+        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
+    if (n_cols * n_targets > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 tests");
+    return 0;
+}
+
+int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                 size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                 size_t n_targets, int tie_correction, int use_continuity, double clip_value,
+                 int threads, double* U1_out, double* P_out, double* fc_out,
+                 double* log2fc_out, double* fdr_out, char* err, size_t err_len) {
+    (void)threads;
+    if (de_checks(n_targets, n_cols, err, err_len)) return 1;
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (n_cols == 0) return 0;
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix m;
+    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
+        return 1;
+    return de_run(m, group_id, n_targets, tie_correction, use_continuity, clip_value, U1_out, P_out,
+                  fc_out, log2fc_out, fdr_out, st.s, err, err_len);
+}
+
+int hpdex_de_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                    int tie_correction, int use_continuity, double clip_value, double* U1_out,
+                    double* P_out, double* fc_out, double* log2fc_out, double* fdr_out, char* err,
+                    size_t err_len) {
+    if (!m) return set_err(err, err_len, "null handle");
+    if (de_checks(n_targets, m->C, err, err_len)) return 1;
+    if (m->C == 0) return 0;
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    return de_run(*m, group_id, n_targets, tie_correction, use_continuity, clip_value, U1_out, P_out,
+                  fc_out, log2fc_out, fdr_out, st.s, err, err_len);
+}
+
+int hpdex_matrix_create_from(const hpdex_matrix_src* src, hpdex_matrix** out, char* err,
+                             size_t err_len) {
+    if (!out || !src) return set_err(err, err_len, "null argument");
+    *out = nullptr;
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    hpdex_matrix* m = new hpdex_matrix();
+    if (upload_matrix_from(m, src, st.s, err, err_len)) {
+        delete m;
+        return 1;
+    }
+    *out = m;
+    return 0;
+}
+
+int hpdex_matrix_shape(const hpdex_matrix* m, size_t* n_rows, size_t* n_cols, size_t* nnz) {
+    if (!m) return 1;
+    if (n_rows) *n_rows = m->R;
+    if (n_cols) *n_cols = m->C;
+    if (nnz) *nnz = m->nnz;
+    return 0;
+}
+
+int hpdex_matrix_download(const hpdex_matrix* m, double* data_out, int32_t* rows_out,
+                          int64_t* indptr_out, char* err, size_t err_len) {
+    if (!m) return set_err(err, err_len, "null handle");
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    const size_t nnz = m->nnz;
+    if (rows_out && nnz)
+        HIPCHK(hipMemcpyAsync(rows_out, m->rows.p, nnz * 4, hipMemcpyDeviceToHost, st.s));
+    if (indptr_out)
+        HIPCHK(hipMemcpyAsync(indptr_out, m->indptr.p, (m->C + 1) * 8, hipMemcpyDeviceToHost, st.s));
+    if (data_out && nnz) {
+        if (m->f64) {
+            HIPCHK(hipMemcpyAsync(data_out, m->vals.p, nnz * 8, hipMemcpyDeviceToHost, st.s));
+        } else {
+            std::vector<float> f(nnz);
+            HIPCHK(hipMemcpyAsync(f.data(), m->vals.p, nnz * 4, hipMemcpyDeviceToHost, st.s));
+            HIPCHK(hipStreamSynchronize(st.s));
+            for (size_t i = 0; i < nnz; ++i) data_out[i] = (double)f[i];
+        }
+    }
     HIPCHK(hipStreamSynchronize(st.s));
     return 0;
 }

@@ -150,6 +150,9 @@ struct JobState {
     uint32_t* status() const { return (uint32_t*)(flags.as<unsigned long long>() + 2); }
 };
 
+int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len);
+int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t s, char* err,
+                       size_t err_len);
 int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,
                   const int64_t* indptr, size_t R, size_t C, size_t nnz, hipStream_t s,
                   char* err, size_t err_len);

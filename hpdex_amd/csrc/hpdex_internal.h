@@ -207,6 +207,26 @@ hipError_t launch_rows_i64_to_i32(const int64_t* in, int32_t* out, size_t n, int
 hipError_t launch_convert_values(int dtype, const void* in, void* out, size_t n, hipStream_t s);
 bool dtype_uses_double(int dtype);
 
+// ---- ingest (hpdex_ingest.hip) ----
+hipError_t launch_rows_i32_check(int32_t* rows, size_t n, int64_t R, hipStream_t s);
+size_t csr_workspace_bytes(size_t nnz);
+// CSR (values of `dtype`, column indices and row pointers of index_bits 32 or
+// 64, all on the device) -> CSC rows, values (float, or double when f64) and
+// column pointers; *bad_host = 1 when a column index lies outside [0, C).
+// Synchronises the stream.
+hipError_t launch_csr_to_csc(int dtype, bool f64, const void* vals, const void* cols,
+                             const void* indptr, int index_bits, size_t R, size_t C, size_t nnz,
+                             void* vals_out, int32_t* rows_out, int64_t* indptr_out,
+                             uint32_t* bad_host, void* ws, size_t ws_bytes, hipStream_t s);
+// dense row-major R x C (values of `dtype` on the device) -> column pointers of
+// its nonzeros (count phase, leaving a converted column-major copy at the
+// start of ws), then rows and values (compact phase, reading that copy)
+size_t dense_workspace_bytes(size_t R, size_t C, bool f64);
+hipError_t launch_dense_count(int dtype, bool f64, const void* dense, size_t R, size_t C,
+                              int64_t* indptr_out, void* ws, size_t ws_bytes, hipStream_t s);
+hipError_t launch_dense_compact(bool f64, size_t R, size_t C, const int64_t* indptr, void* vals_out,
+                                int32_t* rows_out, void* ws, hipStream_t s);
+
 // One launch sequence of the Mann-Whitney pass of one job.  Integer-valued
 // columns go to the count kernel (count_items, one workgroup per column and
 // target slice); the rest, and every column the count kernel hands over at

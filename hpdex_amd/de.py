@@ -8,7 +8,8 @@ fold change and log2 fold change, the Mann-Whitney U test with zero_handling
 scipy.stats.false_discovery_control on the host.
 
 `adata` is anything with AnnData's `obs` (pandas DataFrame), `X` (numpy array
-or scipy sparse matrix, cells x genes), `var_names` and `n_vars`.
+or scipy CSR/CSC matrix, cells x genes), `var_names` and `n_vars`.  X goes
+to the device as it is and becomes a CSC there (hpdex_matrix_create_from).
 
 The row labels keep the reference's layout: target/feature labels and fold
 changes are target-major, while p_value, u_statistic and fdr are gene-major as
@@ -92,20 +93,24 @@ def parallel_differential_expression(
         group_id[np.asarray(obs[groupby_key] == g)] = idx
     n_targets = len(groups)
 
-    # Step 3: CSC (de.py:106-112)
+    # Step 3: CSC (de.py:106-112), built on the device from X as it is: a
+    # dense array, CSR or CSC, with its own index width (no host tocsc, no
+    # int64 copies)
     if isinstance(adata.X, np.ndarray):
-        matrix = scipy.sparse.csc_matrix(adata.X)
-    elif isinstance(adata.X, scipy.sparse.csr_matrix):
-        matrix = adata.X.tocsc()
+        matrix = _capi.DeviceMatrix.from_dense(adata.X)
+    elif scipy.sparse.issparse(adata.X) and adata.X.format in ("csr", "csc"):
+        matrix = _capi.DeviceMatrix.from_scipy(adata.X)
     else:
-        matrix = adata.X
+        raise ValueError("sparse_matrix must be CSR or CSC")
 
     # Steps 4, 5, 7 on the device (de.py:126-184)
-    U1, P, fc, lfc, fdr = _capi.de_csc(matrix.data, matrix.indices, matrix.indptr, group_id,
-                                       n_targets, n_rows=matrix.shape[0],
-                                       tie_correction=tie_correction,
-                                       use_continuity=use_continuity, clip_value=clip_value,
-                                       threads=threads)
+    try:
+        U1, P, fc, lfc, fdr = _capi.de_matrix(matrix, group_id, n_targets,
+                                              tie_correction=tie_correction,
+                                              use_continuity=use_continuity,
+                                              clip_value=clip_value)
+    finally:
+        matrix.close()
 
     # Step 6/8: labels and output (de.py:160-196)
     n_genes = adata.n_vars

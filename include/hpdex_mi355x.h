@@ -115,6 +115,48 @@ typedef struct hpdex_plan hpdex_plan;
 int hpdex_matrix_create(int dtype, const void* data, const int64_t* indices,
                         const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
                         hpdex_matrix** out, char* err, size_t err_len);
+/* A resident matrix built on the device from the layouts callers hold, with
+ * 32- or 64-bit indices.  No counterpart in the reference: it replaces the
+ * host conversions before the call -- scipy's tocsc of AnnData's CSR X and
+ * csc_matrix of a dense X (src/hpdex/de.py:103-109), and the int64 copies of
+ * indices and indptr (src/hpdex/backend/mannwhitneyu_.py:59-62).  The result
+ * is the matrix hpdex_matrix_create would build from that CSC: rows in order
+ * inside each column, explicit zeros of sparse input kept, zeros of dense
+ * input (and -0.0) dropped, NaN kept.
+ *   HPDEX_LAYOUT_CSC:   indices = nnz rows, indptr = n_cols+1 offsets
+ *   HPDEX_LAYOUT_CSR:   indices = nnz columns, indptr = n_rows+1 offsets
+ *                       (a column index outside [0, n_cols) is an error)
+ *   HPDEX_LAYOUT_DENSE: data = n_rows*n_cols values, row-major; indices,
+ *                       indptr and nnz unused
+ * index_bits: 32 or 64, the width of indices and indptr.                    */
+#define HPDEX_LAYOUT_CSC 0
+#define HPDEX_LAYOUT_CSR 1
+#define HPDEX_LAYOUT_DENSE 2
+typedef struct hpdex_matrix_src {
+    int32_t layout;
+    int32_t dtype;
+    const void* data;
+    const void* indices;
+    const void* indptr;
+    int32_t index_bits;
+    int32_t reserved;
+    size_t n_rows, n_cols, nnz;
+} hpdex_matrix_src;
+int hpdex_matrix_create_from(const hpdex_matrix_src* src, hpdex_matrix** out, char* err,
+                             size_t err_len);
+/* Shape and contents of a resident matrix as CSC: values as double (nnz),
+ * row indices (nnz; -1 marks rows outside [0, n_rows) of the input), column
+ * pointers (n_cols+1).  Any output pointer may be NULL.                      */
+int hpdex_matrix_shape(const hpdex_matrix* m, size_t* n_rows, size_t* n_cols, size_t* nnz);
+int hpdex_matrix_download(const hpdex_matrix* m, double* data_out, int32_t* rows_out,
+                          int64_t* indptr_out, char* err, size_t err_len);
+/* hpdex_de_csc on a resident matrix, whatever layout it was built from (the
+ * driver's steps 4-7 after its step 3 ran on the device).  No counterpart in
+ * the reference; outputs and layouts as hpdex_de_csc.                        */
+int hpdex_de_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                    int tie_correction, int use_continuity, double clip_value, double* U1_out,
+                    double* P_out, double* fc_out, double* log2fc_out, double* fdr_out, char* err,
+                    size_t err_len);
 void hpdex_matrix_destroy(hpdex_matrix* m);
 /* Bytes of HBM the resident matrix occupies. */
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
