@@ -115,6 +115,47 @@ def cpu_baseline(w, target_s=12.0):
                       f"visible, {threads} threads used"}
 
 
+def ingest_bench(w, reps=3):
+    """The matrix as AnnData holds it -- CSR, int32 indices -- made resident
+    two ways: the host route the reference's driver and wrapper take
+    (scipy tocsc, int64 copies of indices and indptr, then the CSC upload)
+    and the device route (hpdex_matrix_create_from: CSR upload, transpose on
+    the device).  Median wall time of `reps` runs each, after one warm-up."""
+    import scipy.sparse as sp
+    from hpdex_amd import _capi
+    R, C = w["n_rows"], w["n_cols"]
+    csr = sp.csc_matrix((w["data"], w["indices"], w["indptr"]), shape=(R, C)).tocsr()
+    csr.indices = csr.indices.astype(np.int32)
+    csr.indptr = csr.indptr.astype(np.int32)
+
+    def host_route():
+        t0 = time.perf_counter()
+        h = csr.tocsc()
+        ind, ptr = h.indices.astype(np.int64), h.indptr.astype(np.int64)
+        t1 = time.perf_counter()
+        dm = _capi.DeviceMatrix(h.data, ind, ptr, R)
+        t2 = time.perf_counter()
+        dm.close()
+        return t1 - t0, t2 - t1
+
+    def device_route():
+        t0 = time.perf_counter()
+        dm = _capi.DeviceMatrix.from_scipy(csr)
+        t1 = time.perf_counter()
+        dm.close()
+        return t1 - t0
+
+    device_route()
+    dev = sorted(device_route() for _ in range(reps))[reps // 2]
+    host = sorted(host_route() for _ in range(reps))
+    conv = sorted(h[0] for h in host)[reps // 2]
+    up = sorted(h[1] for h in host)[reps // 2]
+    return {"input": f"CSR {R} x {C}, nnz {csr.nnz}, float32 values, int32 indices",
+            "host_tocsc_and_int64_ms": conv * 1e3, "host_csc_upload_ms": up * 1e3,
+            "host_route_ms": (conv + up) * 1e3, "device_route_ms": dev * 1e3,
+            "host_over_device": (conv + up) / dev}
+
+
 def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
     """Serving workload (cells U[5k,200k], genes U[2k,20k], density U[3%,12%],
     NB(LogNormal(0,1), r=0.7)+1 counts, reference 10% of cells, 5-500 targets;
@@ -207,6 +248,8 @@ def main():
     ap.add_argument("--workload", default="histogram")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip the host-vs-device ingest measurement")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving measurement (p50/p99 job latency)")
     ap.add_argument("--serving-jobs", type=int, default=64)
@@ -294,6 +337,12 @@ def main():
                           "batched_over_plain_per_call": value / (tests / e2e),
                           "batched_over_plain_resident": value / (tests / (pker / 1e3)),
                           "bit_identical_to_batched": same}
+    if not args.no_ingest and args.workload in ("histogram", "small"):
+        log("ingest: CSR with int32 indices, host route vs device route")
+        extra["ingest"] = ingest_bench(w)
+        ig = extra["ingest"]
+        log(f"ingest: host {ig['host_route_ms']:.0f} ms (tocsc + int64 {ig['host_tocsc_and_int64_ms']:.0f}, "
+            f"upload {ig['host_csc_upload_ms']:.0f}), device {ig['device_route_ms']:.0f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
