@@ -156,6 +156,33 @@ def ingest_bench(w, reps=3):
             "host_over_device": (conv + up) / dev}
 
 
+def stream_bench(w, limit_gb, workers=3):
+    """The workload's DE (hpdex_amd.stream) streamed from memory-mapped .npy
+    files on local disk under a memory limit, against one hpdex_de_csc call
+    on the arrays in host memory; identical outputs are checked."""
+    import tempfile
+    from hpdex_amd import _capi, stream
+    R, T = w["n_rows"], w["n_targets"]
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        arrs = []
+        for name in ("data", "indices", "indptr"):
+            f = os.path.join(d, name + ".npy")
+            np.save(f, w[name])
+            arrs.append(np.load(f, mmap_mode="r"))
+        runs = stream.plan_gene_runs(arrs[2], T, int(limit_gb * 2**30) // workers)
+        t0 = time.perf_counter()
+        got = stream.stream_de(*arrs, R, w["group_id"], T, memory_limit_gb=limit_gb,
+                               num_workers=workers)
+        ts = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    want = _capi.de_csc(w["data"], w["indices"], w["indptr"], w["group_id"], T, n_rows=R)
+    to = time.perf_counter() - t0
+    same = all(np.array_equal(g, x, equal_nan=True) for g, x in zip(got, want))
+    return {"memory_limit_gb": limit_gb, "workers": workers, "runs": len(runs),
+            "stream_ms": ts * 1e3, "one_shot_ms": to * 1e3, "identical": same,
+            "source": "memory-mapped .npy files (data, int64 indices, indptr) on local disk"}
+
+
 def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
     """Serving workload (cells U[5k,200k], genes U[2k,20k], density U[3%,12%],
     NB(LogNormal(0,1), r=0.7)+1 counts, reference 10% of cells, 5-500 targets;
@@ -248,6 +275,9 @@ def main():
     ap.add_argument("--workload", default="histogram")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--stream-limit-gb", type=float, default=0.0,
+                    help="also stream the workload's DE from memory-mapped files under this "
+                         "memory limit (GB) and compare with the one-shot call")
     ap.add_argument("--no-ingest", action="store_true",
                     help="skip the host-vs-device ingest measurement")
     ap.add_argument("--no-serving", action="store_true",
@@ -343,6 +373,12 @@ def main():
         ig = extra["ingest"]
         log(f"ingest: host {ig['host_route_ms']:.0f} ms (tocsc + int64 {ig['host_tocsc_and_int64_ms']:.0f}, "
             f"upload {ig['host_csc_upload_ms']:.0f}), device {ig['device_route_ms']:.0f} ms")
+    if args.stream_limit_gb > 0:
+        log(f"streaming DE under {args.stream_limit_gb} GB")
+        extra["stream"] = stream_bench(w, args.stream_limit_gb)
+        sb = extra["stream"]
+        log(f"stream: {sb['runs']} runs, {sb['stream_ms']:.0f} ms vs one-shot {sb['one_shot_ms']:.0f} ms, "
+            f"identical {sb['identical']}")
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
