@@ -130,6 +130,16 @@ __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
     return v;
 }
 
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint64_t o = shfl_up_u64(v, d);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Register bitonic sort of 64*K keys held K per lane, element e = i*64 + lane
 // (slot i).  Exchanges at distance >= 64 stay inside a lane; shorter ones use

@@ -677,6 +677,120 @@ __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResu
     }
 }
 
+// Targets g0 .. g0+ng-1 of column c, target g0+j held by lane j (4-byte keys).
+// The keys of consecutive groups lie next to each other in the column's key
+// range, so a window of consecutive targets whose keys total at most 64 is
+// ranked in one pass of the wave: lane e loads key e of the window and finds
+// its target (the last one starting at or before e); a bitonic sort of
+// (target, key) orders every target's keys; runs of equal (target, key) give
+// each key its position in its run of equal values (the tie term); inclusive
+// scans of the per-key terms, read at each target's first and last position,
+// give every target's sums.  A target with more than 64 keys alone goes
+// through process_target.  Each lane returns its own target's result.
+template <typename V>
+__device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t c, uint32_t g0,
+                                                       uint32_t ng, const ColPlan& pl,
+                                                       const RefView<typename KeyOf<V>::K>& ref,
+                                                       const typename KeyOf<V>::K* colkeys) {
+    using K = typename KeyOf<V>::K;
+    static_assert(sizeof(K) == 4, "target windows pack (target, key) into 64 bits");
+    const uint32_t lane = (uint32_t)lane_id();
+    const int64_t b = c * J.G;
+    const uint32_t gj = g0 + lane;
+    const bool own = lane < ng;
+    const uint32_t sz = own ? J.cnt[b + gj] : 0u;
+    const uint32_t off = own ? J.goff[b + gj] : 0u;
+    uint32_t zaj = 0, zb = 0, n1 = 0, n2 = 0;
+    if (own) pair_sizes(J, c, gj, zaj, zb, n1, n2);
+    // the reference's zero block, the same for every target
+    const uint32_t nza = J.cnt[b];
+    const uint32_t za = (J.mode == kZeroNone) ? J.zcnt[b]
+                                             : (J.group_rows[0] > nza ? J.group_rows[0] - nza : 0u);
+    const uint32_t inc = wave_incl_scan_u32(sz);
+    TargetCtx<K> t;
+    t.ref = ref;
+    t.mode = J.mode;
+    t.vmin = pl.vmin;
+    t.za = za;
+    uint64_t S = 0, TT = 0;
+    bool big = false;
+    TargetResult mine{};
+    uint32_t j0 = 0;
+    while (j0 < ng) {
+        const uint32_t base = j0 ? shfl_u32(inc, (int)j0 - 1) : 0u;
+        const uint64_t fit = __ballot(lane >= j0 && own && inc - base <= (uint32_t)kWave);
+        const uint32_t j1 = j0 + (uint32_t)__builtin_popcountll(fit);
+        if (j1 == j0) {   // one target of more than 64 keys
+            const TargetResult r = process_target<V>(J, c, g0 + j0, pl, ref);
+            if (lane == j0) { mine = r; big = true; }
+            ++j0;
+            continue;
+        }
+        const uint32_t m = shfl_u32(inc, (int)j1 - 1) - base;   // keys in the window
+        const uint32_t off0 = shfl_u32(off, (int)j0);
+        const bool valid = lane < m;
+        const K key = valid ? colkeys[off0 + lane] : K(0);
+        // owner: the last target j in [j0, j1) whose first key is at or before
+        // this lane (empty targets share the next one's start and lose)
+        const uint32_t excl = inc - sz - base;
+        uint32_t lo = j0, hi = j1;
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const uint32_t e = shfl_u32(excl, (int)mid);
+            if (hi - lo > 1u) {
+                if (e <= lane) lo = mid;
+                else hi = mid;
+            }
+        }
+        uint64_t comp[1] = {valid ? (((uint64_t)lo << 32) | (uint64_t)key) : ~0ull};
+        wave_sort<1, uint64_t>(comp);
+        const uint64_t prev = shfl_up_u64(comp[0], 1);
+        const bool head = valid && (lane == 0 || prev != comp[0]);
+        const uint64_t hm = __ballot(head);
+        const uint64_t upto = (lane == (uint32_t)kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
+        const uint64_t below = hm & upto;
+        uint64_t s_e = 0, t_e = 0;
+        if (valid) {
+            const K v = (K)comp[0];
+            const uint32_t h = (uint32_t)(63 - __clzll(below));
+            uint32_t lt, eq;
+            t.ref.lookup(v, lt, eq);
+            const uint64_t jj = (uint64_t)(lane - h + 1u);
+            const uint64_t ltz = (uint64_t)lt + (zero_below<V>(t, v) ? za : 0u);
+            s_e = 2ull * ltz + eq;
+            const uint64_t x = (uint64_t)eq + jj;
+            t_e = 3ull * x * (x - 1ull);
+        }
+        const uint64_t ps = wave_incl_scan_u64(s_e), pt = wave_incl_scan_u64(t_e);
+        // target j's keys sit at [excl_j, excl_j + sz_j) of the sorted window
+        const uint32_t last = excl + sz - 1u, before = excl - 1u;
+        const uint64_t ps_last = shfl_u64(ps, (int)(last & 63u)), pt_last = shfl_u64(pt, (int)(last & 63u));
+        const uint64_t ps_bef = shfl_u64(ps, (int)(before & 63u)), pt_bef = shfl_u64(pt, (int)(before & 63u));
+        if (lane >= j0 && lane < j1 && sz > 0u) {
+            S = ps_last - (excl ? ps_bef : 0ull);
+            TT = pt_last - (excl ? pt_bef : 0ull);
+        }
+        j0 = j1;
+    }
+    if (own && !big) {
+        uint32_t ltz;
+        if (J.mode == kZeroMin) ltz = 0;
+        else if (J.mode == kZeroMax) ltz = pl.na;
+        else ltz = pl.refneg;
+        const uint64_t S2 = S + (uint64_t)zb * (2ull * ltz + za);
+        const uint64_t tie = TT + pl.TA + tie_f((uint64_t)za + zb);
+        const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
+        const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
+        mine.idx = c * J.T + (int64_t)(gj - 1);
+        mine.U1 = (double)u1x2 * 0.5;
+        mine.tie = J.tie_corr ? (double)tie : 0.0;
+        mine.n1 = n1;
+        mine.n2 = n2;
+    }
+    return mine;
+}
+
 struct TargetState {
     uint32_t next_t;
 };
@@ -715,22 +829,37 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     }
     if (tid == 0) st.next_t = 1u;
     __syncthreads();
-    // lane k of the wave keeps the k-th finished target until 64 are pending
-    TargetResult mine{};
-    int pending = 0;
-    for (;;) {
-        uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(&st.next_t, 1u);
-        g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
-        if (g >= (uint32_t)J.G) break;
-        const TargetResult r = process_target<V>(J, c, g, pl, rv);
-        if (lane == pending) mine = r;
-        if (++pending == kWave) {
-            flush_targets(J, mine, pending);
-            pending = 0;
+    if constexpr (sizeof(K) == 4) {
+        // a wave claims 64 consecutive targets, one per lane, and ranks the
+        // small ones several at a time (target_windows)
+        const K* colkeys = (const K*)J.keys + J.indptr[c];
+        for (;;) {
+            uint32_t g0 = 0;
+            if (lane == 0) g0 = atomicAdd(&st.next_t, (uint32_t)kWave);
+            g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
+            if (g0 >= (uint32_t)J.G) break;
+            const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
+            const TargetResult r = target_windows<V>(J, c, g0, ng, pl, rv, colkeys);
+            flush_targets(J, r, (int)ng);
         }
+    } else {
+        // lane k of the wave keeps the k-th finished target until 64 are pending
+        TargetResult mine{};
+        int pending = 0;
+        for (;;) {
+            uint32_t g = 0;
+            if (lane == 0) g = atomicAdd(&st.next_t, 1u);
+            g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
+            if (g >= (uint32_t)J.G) break;
+            const TargetResult r = process_target<V>(J, c, g, pl, rv);
+            if (lane == pending) mine = r;
+            if (++pending == kWave) {
+                flush_targets(J, mine, pending);
+                pending = 0;
+            }
+        }
+        flush_targets(J, mine, pending);
     }
-    flush_targets(J, mine, pending);
 }
 
 // kLoop false: workgroup b takes queue item b (a grid as large as the

@@ -60,6 +60,29 @@ def test_sim_sort_path_long_slices(mode):
     run_both(d, r, ip, g, 2, 6000, mode)
 
 
+@pytest.mark.parametrize("mode", ["min", "none", "mix"])
+def test_sim_sort_path_many_small_targets(mode):
+    # 150 targets of 2-40 cells (tiny or, in min/mix, empty slices, packed
+    # several to a wave window) and four targets of ~400 cells (more than 64
+    # keys: ranked alone), float values with ties and negatives; mode none
+    # counts stored values only, so it gets a dense matrix (two per target)
+    rng = np.random.default_rng(31)
+    R, C, T = 8000, 6, 154
+    dens = 0.97 if mode == "none" else 0.4
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.full(C, dens))
+    data = np.round(rng.normal(0, 1.5, len(rows)), 1).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:600]] = 0
+    sizes = np.concatenate([rng.integers(2 if mode != "none" else 6, 41, 150),
+                            [400, 380, 420, 390]])
+    at = 600
+    for g, n in enumerate(sizes, start=1):
+        gid[perm[at:at + n]] = g
+        at += n
+    run_both(data, rows, indptr, gid, T, R, mode)
+
+
 def test_sim_eight_byte_keys():
     d, r, ip, g = _rw(9, 1500, 3, 3, 0.5, lambda rg, n: np.round(rg.normal(0, 2, n), 1))
     run_both(d.astype(np.float64), r, ip, g, 3, 1500, "mix", f64=True)
