@@ -118,6 +118,48 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
     }
 }
 
+// Visit (group, value) of every entry of the column [p0, p0 + n) whose row is
+// in a group, for workgroups of NT threads: batches of NT*kUnroll entries read
+// as buffer loads whose resources end at the column's end (lanes past it read
+// zeros and are masked), group labels gathered from the bit-packed array.
+// Needs the packed labels and n < 2^28 (32-bit byte offsets).
+template <typename V, int NT, typename F>
+__device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, uint32_t n, F&& f) {
+    const uint32_t tid = (uint32_t)tid_x();
+    const uint32_t R = (uint32_t)J.R, G = (uint32_t)J.G;
+    const uint32_t lbits = J.gid_bits, lmask = (1u << lbits) - 1u;
+    const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)J.gidp, (short)0, J.gid_words * 4u, kBufferWord3);
+    const int32_t* col_rows = J.rows + p0;
+    const V* col_vals = (const V*)J.vals + p0;
+    constexpr uint32_t step = (uint32_t)NT * kUnroll;
+    for (uint32_t ib = 0; ib < n; ib += step) {
+        const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_rows + ib), (short)0, (n - ib) * 4u, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_vals + ib), (short)0, (n - ib) * (uint32_t)sizeof(V), kBufferWord3);
+        uint32_t r[kUnroll];
+        V v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t e = tid + (uint32_t)(u * NT);
+            r[u] = buffer_load_u32(rows_rsrc, e * 4u, 0u);
+            v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
+            if (ib + e >= n) r[u] = 0xFFFFFFFFu;
+        }
+        uint32_t g[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t pos = min(r[u], R) * lbits;
+            const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
+            g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            if (g[u] < G) f(g[u], v[u]);
+    }
+}
+
 __device__ inline void raise_status(const JobDesc& J, uint32_t bit, int64_t c) {
     atomicOr(J.status, bit);
     atomicMin(J.status_col, (unsigned long long)c);
@@ -242,6 +284,18 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     uint32_t* cnt_nz = (uint32_t*)smem;
     uint32_t* cnt_ez = cnt_nz + G;
     uint32_t* cur = cnt_ez + G;
+    // the column's entries in a group: batched buffer loads and bit-packed
+    // labels for a sparse column (its rows are far apart, so 32-bit label
+    // gathers would touch a cache line per entry); the plain loop for a column
+    // holding a quarter of the rows or more (consecutive rows: 32-bit label
+    // gathers are coalesced, unpacking would only add work) or one the packed
+    // labels cannot address
+    const bool packed = __builtin_amdgcn_readfirstlane(
+        (J.gid_bits > 0u && p1 - p0 < ((int64_t)1 << 28) && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
+    auto grouped = [&](auto&& f) {
+        if (packed) for_each_grouped<V, kColBlock>(J, p0, (uint32_t)(p1 - p0), f);
+        else for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, f);
+    };
 
     for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; }
     if (tid == 0) {
@@ -256,7 +310,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         int lmin = INT_MAX, lmax = INT_MIN;
         K kmn = KT::kSentinel, kmx = 0;
         uint32_t neg = 0;
-        for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, [&](uint32_t g, V v) {
+        grouped([&](uint32_t g, V v) {
             if (!KT::finite(v)) return;
             if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
             atomicAdd(&cnt_nz[g], 1u);
@@ -340,7 +394,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t hist = st.hist;
     const int32_t vmin = st.vmin;
     K* keys = (K*)J.keys + p0;
-    for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, [&](uint32_t g, V v) {
+    grouped([&](uint32_t g, V v) {
         if (!KT::finite(v) || v == V(0)) return;
         const uint32_t pos = atomicAdd(&cur[g], 1u);
         keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
