@@ -29,12 +29,15 @@ def lib(path=None):
                               vp, vp, vp, vp, vp, vp, vp, ctypes.c_double]
         L.sim_mwu.restype = ci
         L.sim_count_blocks.restype = ci
+        L.sim_set_shrink_range.argtypes = [ci]
+        L.sim_set_shrink_range.restype = None
         _lib = L
     return _lib
 
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
-        count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0):
+        count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0,
+        shrink_range=False):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels)."""
     # laid out like a resident matrix: PAD padding entries (value 0, row -1)
@@ -49,6 +52,7 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     outs = [np.zeros(max(n, 1)) for _ in range(5)]
     err = np.zeros(1, dtype=np.uint64)
     status = np.zeros(1, dtype=np.uint32)
+    lib().sim_set_shrink_range(int(shrink_range))   # test hook: forces hand-overs
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,

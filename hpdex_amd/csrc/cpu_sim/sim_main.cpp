@@ -27,6 +27,7 @@ using namespace hpdex;
 namespace {
 
 int sim_last_blocks = 0;   // count-kernel workgroups of the last sim_mwu call
+int sim_shrink_range = 0;  // test hook: record each column's range one value short
 
 struct ThreadInfo { std::atomic<const char*>* last; };
 
@@ -88,6 +89,11 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
                                                               col_int.data(), range); }, timeout_s);
+    // test hook: ranges one value short of the data, so that the count kernel
+    // meets values outside them and hands the columns over
+    if (sim_shrink_range)
+        for (int c = 0; c < C; ++c)
+            if (range[2 * c + 1] > range[2 * c]) range[2 * c + 1] -= 1;
     // the host's routing (job_upload_desc): count workgroups per target slice
     const uint32_t T = (uint32_t)J.T;
     const bool count_ok = count_lds > 0 && J.G >= 2 && (uint32_t)J.G <= kCountMaxGroups;
@@ -130,16 +136,22 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
             else
                 count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
-    // the looping form on a grid of at most 3 workgroups, so that each takes
-    // several queue items (the other form runs the same item code once)
-    const unsigned ng = std::min(n_static + nc, 3u);
     const WorkItem* it = gen_items.data();
-    for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { split_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
-    for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { ref_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
-    for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { target_kernel<V, true>(jobs, it, &gen_n); }, timeout_s);
+    if (n_static == 0) {
+        // hand-overs only: the combined kernel, on a grid of at most 3
+        // workgroups so that each takes several queue items
+        const unsigned hg = std::min(nc, 3u);
+        for (unsigned b = 0; b < hg; ++b)
+            run_block(b, kColBlock, hg, [&] { handover_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+    } else {
+        const unsigned ng = n_static + nc;
+        for (unsigned b = 0; b < ng; ++b)
+            run_block(b, kColBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+        for (unsigned b = 0; b < ng; ++b)
+            run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+        for (unsigned b = 0; b < ng; ++b)
+            run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+    }
     return (int)(nc - (gen_n - n_static));
 }
 
@@ -191,3 +203,4 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
 
 // Count-kernel workgroups (column x target slice) of the last sim_mwu call.
 extern "C" int sim_count_blocks() { return sim_last_blocks; }
+extern "C" void sim_set_shrink_range(int on) { sim_shrink_range = on; }

@@ -215,9 +215,9 @@ __device__ inline void work_item(const WorkItem* items, uint32_t k, int& job, in
     col = (int64_t)__builtin_amdgcn_readfirstlane(items[k].col);
 }
 
-// Workgroups of the split / reference / target kernels when the host routed
-// no column to them (their queue then holds only the count kernel's rare
-// hand-overs): at most this many, each taking several queue items
+// Workgroups of the hand-over kernel (used when the host routed no column to
+// the sort path, so the queue holds only the count kernel's rare hand-overs):
+// at most this many, each taking several queue items
 constexpr unsigned kHandoverGrid = 256;
 template <typename T> struct TypeTag { using type = T; };
 
@@ -401,20 +401,12 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     });
 }
 
-// kLoop false: workgroup b takes queue item b (a grid as large as the
-// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
-template <typename V, bool kLoop>
+// workgroup b takes queue item b (the grid is sized for the queue's capacity)
+template <typename V>
 __global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs, const WorkItem* items,
                                                    const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
-    if (!kLoop) {
-        if (blockIdx.x < n) split_item<V>(jobs, items, blockIdx.x);
-        return;
-    }
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        split_item<V>(jobs, items, k);
-        __syncthreads();   // the next item reuses LDS
-    }
+    if (blockIdx.x < n) split_item<V>(jobs, items, blockIdx.x);
 }
 
 // Rank table of the reference slice seen by the target phase.
@@ -567,20 +559,12 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     if (tid == 0 && (hist || na <= cap)) J.plan[c].TA = ta;
 }
 
-// kLoop false: workgroup b takes queue item b (a grid as large as the
-// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
-template <typename V, bool kLoop>
+// workgroup b takes queue item b (the grid is sized for the queue's capacity)
+template <typename V>
 __global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
                                                    const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
-    if (!kLoop) {
-        if (blockIdx.x < n) ref_item<V>(jobs, items, blockIdx.x);
-        return;
-    }
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        ref_item<V>(jobs, items, k);
-        __syncthreads();   // the next item reuses LDS
-    }
+    if (blockIdx.x < n) ref_item<V>(jobs, items, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -916,17 +900,31 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     }
 }
 
-// kLoop false: workgroup b takes queue item b (a grid as large as the
-// queue's capacity); true: items b, b + gridDim.x, ... (a small grid)
-template <typename V, bool kLoop>
+// workgroup b takes queue item b (the grid is sized for the queue's capacity)
+template <typename V>
 __global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs, const WorkItem* items,
                                                    const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
-    if (!kLoop) {
-        if (blockIdx.x < n) target_item<V>(jobs, items, blockIdx.x);
-        return;
-    }
+    if (blockIdx.x < n) target_item<V>(jobs, items, blockIdx.x);
+}
+
+// The count kernel's hand-overs when the host routed no column to the sort
+// path: split, reference table and targets of each queued column in one
+// workgroup, one launch instead of three (each phase reads only what the
+// previous one wrote for the same column; the workgroup's barrier and fence
+// order them).
+template <typename V>
+__global__ __launch_bounds__(kColBlock) void handover_kernel(const JobDesc* jobs,
+                                                             const WorkItem* items,
+                                                             const uint32_t* n_items) {
+    const uint32_t n = uniform_u32(*n_items);
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        split_item<V>(jobs, items, k);
+        __threadfence();
+        __syncthreads();
+        ref_item<V>(jobs, items, k);
+        __threadfence();
+        __syncthreads();
         target_item<V>(jobs, items, k);
         __syncthreads();   // the next item reuses LDS
     }
@@ -1616,21 +1614,24 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
     const unsigned split_lds = align16(split);
     const unsigned tab_lds = table_lds_bytes(L.f64);
-    auto sort_path = [&](auto v_tag, auto loop_tag) {
+    auto sort_path = [&](auto v_tag) {
         using V = typename decltype(v_tag)::type;
-        constexpr bool kLoop = decltype(loop_tag)::value;
-        split_kernel<V, kLoop><<<sort_grid, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        split_kernel<V><<<sort_grid, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
         mark(2);
-        ref_kernel<V, kLoop><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        ref_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
         mark(3);
-        target_kernel<V, kLoop><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        target_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
     };
-    if (L.f64) {
-        if (loop) sort_path(TypeTag<double>{}, std::true_type{});
-        else sort_path(TypeTag<double>{}, std::false_type{});
+    const unsigned both_lds = split_lds > tab_lds ? split_lds : tab_lds;
+    if (loop) {   // hand-overs only: the three phases in one launch
+        if (L.f64) handover_kernel<double><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        else handover_kernel<float><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        mark(2);
+        mark(3);
+    } else if (L.f64) {
+        sort_path(TypeTag<double>{});
     } else {
-        if (loop) sort_path(TypeTag<float>{}, std::true_type{});
-        else sort_path(TypeTag<float>{}, std::false_type{});
+        sort_path(TypeTag<float>{});
     }
     mark(4);
     return hipGetLastError();

@@ -174,6 +174,27 @@ def test_sim_count_kernel_extreme_values():
         run_count(d, r, ip, g, 6, 3000, mode, 76 * 1024, expect_counted=2, expect_blocks=2)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("mode", ["min", "none"])
+def test_sim_count_kernel_hand_overs(mode, mixed):
+    # recorded ranges one value short of the data: every counted column meets
+    # a value outside its range and is handed to the sort path at run time --
+    # with no host-routed column (mixed False) the combined hand-over kernel
+    # takes them, otherwise the per-column kernels after the routed float one
+    d, r, ip, g = _rw(32, 3000, 5, 6, 0.5, lambda rg, n: rg.integers(-3, 12, n))
+    if mixed:
+        d = d.copy()
+        d[ip[4]] = 0.5          # column 4 is not integer-valued: routed by the host
+    U1, P, TIE, n1, n2, err, status = sim.mwu(d, r, ip, g, 6, 3000, mode=ZH[mode],
+                                              shrink_range=True, timeout_s=120)
+    assert err == 2 ** 64 - 1 and status == 0
+    assert sim.mwu.last_counted == 0
+    oU1, oP, oT, on1, on2 = oracle.mwu(d, r, ip, g, 6, n_rows=3000, zero_handling=mode,
+                                       use_continuity=True, with_stats=True)
+    assert np.array_equal(U1, oU1) and np.array_equal(TIE, oT)
+    assert np.array_equal(n1, on1) and np.array_equal(n2, on2) and np.array_equal(P, oP)
+
+
 def test_sim_count_kernel_eight_byte_values():
     d, r, ip, g = _rw(27, 2000, 3, 4, 0.4, lambda rg, n: rg.integers(-30, 200, n))
     run_count(d.astype(np.float64), r, ip, g, 4, 2000, "mix", 76 * 1024, f64=True,
