@@ -69,7 +69,8 @@ class Job(ctypes.Structure):
                 ("indptr", ctypes.c_void_p), ("n_rows", ctypes.c_size_t),
                 ("n_cols", ctypes.c_size_t), ("nnz", ctypes.c_size_t),
                 ("group_id", ctypes.c_void_p), ("n_targets", ctypes.c_size_t),
-                ("opts", MwuOpts), ("U1_out", ctypes.c_void_p), ("P_out", ctypes.c_void_p)]
+                ("opts", MwuOpts), ("U1_out", ctypes.c_void_p), ("P_out", ctypes.c_void_p),
+                ("index_bits", ctypes.c_int32), ("layout", ctypes.c_int32)]
 
 
 class JobTimes(ctypes.Structure):
@@ -486,9 +487,11 @@ class Session:
         self._keep = {}
 
     def submit(self, group_id, n_targets, opts: MwuOpts, matrix: DeviceMatrix = None,
-               data=None, indices=None, indptr=None, n_rows=None) -> int:
-        """Queue a job on a resident matrix, or on a host CSC uploaded for it.
-        Returns the ticket; the arrays stay referenced until wait()."""
+               data=None, indices=None, indptr=None, n_rows=None, n_cols=None,
+               layout=LAYOUT_CSC) -> int:
+        """Queue a job on a resident matrix, or on a host CSC (or CSR, with
+        n_cols) uploaded for it; int32 indices are sent as they are.  Returns
+        the ticket; the arrays stay referenced until wait()."""
         gid = np.ascontiguousarray(group_id, dtype=np.int32)
         j = Job()
         keep = [gid]
@@ -497,12 +500,16 @@ class Session:
             C = matrix.n_cols
             keep.append(matrix)
         else:
-            data, indices, indptr = _csc_args(data, indices, indptr)
+            data, indices, indptr = _data_array(data), _index_array(indices), _index_array(indptr)
+            if indices.dtype != indptr.dtype:
+                indices, indptr = indices.astype(np.int64), indptr.astype(np.int64)
             keep += [data, indices, indptr]
             j.dtype = DTYPE_CODES[data.dtype]
             j.data, j.indices, j.indptr = _p(data), _p(indices), _p(indptr)
+            j.index_bits = 32 if indices.dtype == np.int32 else 64
+            j.layout = layout
             j.n_rows = len(gid) if n_rows is None else int(n_rows)
-            C = len(indptr) - 1
+            C = len(indptr) - 1 if layout == LAYOUT_CSC else int(n_cols)
             j.n_cols, j.nnz = C, len(data)
         n = C * int(n_targets)
         U1, P = np.empty(max(n, 1)), np.empty(max(n, 1))

@@ -168,7 +168,7 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
 
 // Row or column pointers of a source layout, widened to int64 on the host and
 // checked as a CSC index is.
-static int widen_indptr(const void* p, int bits, size_t n, size_t nnz, std::vector<int64_t>& out,
+int widen_indptr(const void* p, int bits, size_t n, size_t nnz, std::vector<int64_t>& out,
                         char* err, size_t err_len) {
     out.resize(n + 1);
     for (size_t i = 0; i <= n; ++i)

@@ -151,6 +151,8 @@ struct JobState {
 };
 
 int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len);
+int widen_indptr(const void* p, int bits, size_t n, size_t nnz, std::vector<int64_t>& out,
+                 char* err, size_t err_len);
 int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t s, char* err,
                        size_t err_len);
 int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,

@@ -90,6 +90,21 @@ struct JobRec {
 
 size_t up256(size_t n) { return ((n ? n : 16) + 255) & ~(size_t)255; }
 
+// The host matrix a job brings, as hpdex_matrix_create_from takes it.
+hpdex_matrix_src source_of(const hpdex_job& j) {
+    hpdex_matrix_src src{};
+    src.layout = j.layout;
+    src.dtype = j.dtype;
+    src.data = j.data;
+    src.indices = j.indices;
+    src.indptr = j.indptr;
+    src.index_bits = j.index_bits == 32 ? 32 : 64;
+    src.n_rows = j.n_rows;
+    src.n_cols = j.n_cols;
+    src.nnz = j.nnz;
+    return src;
+}
+
 // Upper bound of what one job allocates through DevBuf (every allocation is
 // rounded to 256 bytes, as BumpBlock::take rounds it).
 size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows) {
@@ -99,12 +114,19 @@ size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows
     const size_t vsz = f64 ? 8 : 4;
     const size_t T = j.n_targets, G = T + 1, np = C * T;
     size_t f = 0;
-    if (!m) {   // upload_matrix
+    if (!m) {   // upload_matrix_from
         f += up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
         f += up256(2 * C * 4) + up256(C);
-        if (j.dtype != HPDEX_FLOAT32 && j.dtype != HPDEX_FLOAT64)
-            f += up256(nnz * dtype_size(j.dtype));
-        f += up256(nnz * 8);
+        const size_t ib = j.index_bits == 32 ? 4 : 8;
+        if (j.layout == HPDEX_LAYOUT_CSR) {
+            // raw values, column indices, row pointers, transpose workspace
+            f += up256(nnz * dtype_size(j.dtype)) + up256(nnz * ib) + up256((R + 1) * ib);
+            f += up256(csr_workspace_bytes(nnz));
+        } else {
+            if (j.dtype != HPDEX_FLOAT32 && j.dtype != HPDEX_FLOAT64)
+                f += up256(nnz * dtype_size(j.dtype));
+            if (ib == 8) f += up256(nnz * 8);   // int64 rows staged for the conversion
+        }
     }
     // job_setup
     f += up256(R * 4) + up256(label_words(R, 16) * 4) + up256(G * 4) + 2 * up256(np * 8) + up256(32);
@@ -169,8 +191,8 @@ void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
         do {
             if (!m) {
                 own.reset(new hpdex_matrix());
-                rc = upload_matrix(own.get(), j.dtype, j.data, j.indices, j.indptr, j.n_rows,
-                                   j.n_cols, j.nnz, s, err, err_len);
+                const hpdex_matrix_src src = source_of(j);
+                rc = upload_matrix_from(own.get(), &src, s, err, err_len);
                 if (rc) break;
                 m = own.get();
             }
@@ -293,9 +315,19 @@ int hpdex_session_submit(hpdex_session* S, const hpdex_job* job, int64_t* ticket
         C = j.matrix->C;
     } else {
         if (!valid_dtype(j.dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+        if (j.layout != HPDEX_LAYOUT_CSC && j.layout != HPDEX_LAYOUT_CSR)
+            return set_err(err, err_len, "a session job brings a CSC or CSR matrix");
+        if (j.index_bits != 0 && j.index_bits != 32 && j.index_bits != 64)
+            return set_err(err, err_len, "index_bits must be 32 or 64");
         if (!j.indptr || (j.nnz && (!j.data || !j.indices)))
-            return set_err(err, err_len, "null CSC array");
-        if (check_indptr(j.indptr, j.n_cols, j.nnz, err, err_len)) return 1;
+            return set_err(err, err_len, "null matrix array");
+        std::vector<int64_t> ptr;
+        const bool csr = j.layout == HPDEX_LAYOUT_CSR;
+        if (widen_indptr(j.indptr, j.index_bits == 32 ? 32 : 64, csr ? j.n_rows : j.n_cols, j.nnz,
+                         ptr, err, err_len))
+            return 1;
+        if (csr && (ptr[0] != 0 || (size_t)ptr[j.n_rows] != j.nnz))
+            return set_err(err, err_len, "CSR indptr must start at 0 and end at nnz");
         R = j.n_rows;
         C = j.n_cols;
     }

@@ -239,17 +239,20 @@ typedef struct hpdex_session_opts {
     int32_t slots;             /* jobs in flight at once (1..64) */
 } hpdex_session_opts;
 typedef struct hpdex_job {
-    hpdex_matrix* matrix;      /* resident matrix, or NULL to upload the host CSC below */
-    int32_t dtype;             /* host CSC (same meaning as hpdex_mwu_csc) */
+    hpdex_matrix* matrix;      /* resident matrix, or NULL to upload the host matrix below */
+    int32_t dtype;             /* host matrix (same meaning as hpdex_mwu_csc) */
     const void* data;
-    const int64_t* indices;
-    const int64_t* indptr;
+    const void* indices;       /* int64, or int32 with index_bits 32 */
+    const void* indptr;
     size_t n_rows, n_cols, nnz;
     const int32_t* group_id;   /* n_rows labels (of the matrix's rows) */
     size_t n_targets;
     hpdex_mwu_opts opts;       /* method exact needs zero_handling min, max or mix */
     double* U1_out;            /* n_cols * n_targets each, written when the job ends */
     double* P_out;
+    int32_t index_bits;        /* 0 or 64: int64 indices and indptr; 32: int32 */
+    int32_t layout;            /* HPDEX_LAYOUT_CSC (0) or HPDEX_LAYOUT_CSR, converted on the
+                                  device as hpdex_matrix_create_from does */
 } hpdex_job;
 typedef struct hpdex_job_times {
     double queue_ms;           /* submit -> admitted */

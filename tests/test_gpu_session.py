@@ -132,6 +132,30 @@ def test_session_admission_respects_budget():
     small.close()
 
 
+def test_session_jobs_with_int32_indices_and_csr():
+    """Jobs that bring their matrix with int32 indices, or as CSR, return the
+    results of the same matrix as an int64 CSC."""
+    import scipy.sparse as sp
+    rng, m = _matrix(36, 5000, 40, 0.15, lambda r, n: r.negative_binomial(0.8, 0.2, n) + 1)
+    m["data"][::7] += 0.5                       # some float columns: both paths
+    gid = _groups(rng, 5000, 6)
+    opts = _capi.make_opts(True, True, ZH["mix"])
+    a = _alone(m, gid, 6, opts)
+    csc = sp.csc_matrix((m["data"], m["indices"], m["indptr"]), shape=(5000, 40))
+    csr = csc.tocsr()
+    say("session: int32 CSC and CSR jobs")
+    sess = _capi.Session(budget_bytes=1 << 30, slots=2)
+    t1 = sess.submit(gid, 6, opts, data=m["data"], indices=m["indices"].astype(np.int32),
+                     indptr=m["indptr"].astype(np.int32), n_rows=5000)
+    t2 = sess.submit(gid, 6, opts, data=csr.data, indices=csr.indices.astype(np.int32),
+                     indptr=csr.indptr.astype(np.int32), n_rows=5000, n_cols=40,
+                     layout=_capi.LAYOUT_CSR)
+    for t in (t1, t2):
+        U1, P, _ = sess.wait(t)
+        assert np.array_equal(U1, a[0]) and np.array_equal(P, a[1])
+    sess.close()
+
+
 def test_session_reports_job_errors():
     """A job whose reference has fewer than two cells fails with the
     reference's message; the session keeps serving."""
