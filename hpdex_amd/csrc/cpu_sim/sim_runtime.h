@@ -153,6 +153,8 @@ inline int __clzll(unsigned long long x) { return x ? __builtin_clzll(x) : 64; }
 inline int __clz(unsigned x) { return x ? __builtin_clz(x) : 32; }
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline long long __double_as_longlong(double d) { long long u; std::memcpy(&u, &d, 8); return u; }
+inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline double __longlong_as_double(long long u) { double d; std::memcpy(&d, &u, 8); return d; }
 using std::min;
 using std::max;
 inline int min(int a, int b) { return a < b ? a : b; }

@@ -27,6 +27,10 @@ template <> struct KeyOf<float> {
         uint32_t b = __float_as_uint(v);
         return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
     }
+    // the value of a key (key's inverse)
+    __device__ static inline float value(K k) {
+        return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+    }
     __device__ static inline bool finite(float v) { return __builtin_isfinite(v); }
     // integral and small enough to bin exactly
     __device__ static inline bool small_int(float v, int& iv) {
@@ -45,6 +49,10 @@ template <> struct KeyOf<double> {
     __device__ static inline K key(double v) {
         uint64_t b = (uint64_t)__double_as_longlong(v);
         return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+    }
+    __device__ static inline double value(K k) {
+        return __longlong_as_double(
+            (long long)((k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
     }
     __device__ static inline bool finite(double v) { return __builtin_isfinite(v); }
     __device__ static inline bool small_int(double v, int& iv) {

@@ -40,7 +40,8 @@
 // only explicit zeros count).
 //
 // The split/reference/target kernels use 256-thread workgroups and at most
-// 48 KB of LDS; the count kernel 512 threads and a tuned LDS size (76 KB).
+// 48 KB of LDS; the count kernel 256 threads and a tuned LDS size (40 KB,
+// four workgroups per CU).
 #include <climits>
 #include <type_traits>
 
@@ -119,45 +120,81 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
 }
 
 // Visit (group, value) of every entry of the column [p0, p0 + n) whose row is
-// in a group, for workgroups of NT threads: batches of NT*kUnroll entries read
-// as buffer loads whose resources end at the column's end (lanes past it read
-// zeros and are masked), group labels gathered from the bit-packed array.
-// Needs the packed labels and n < 2^28 (32-bit byte offsets).
-template <typename V, int NT, typename F>
-__device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, uint32_t n, F&& f) {
+// in a group, for workgroups of NT threads.  Batches of NT*kUnroll entries are
+// read as buffer loads whose resources end at the batch's last entry (lanes
+// past it read zeros and are masked) and software-pipelined: the next batch's
+// rows and values are issued after this batch's label gathers and before its
+// visits, so they are in flight while this batch is visited (loads complete in
+// order: waiting for the gathers leaves them outstanding).  kPacked gathers the
+// labels from the bit-packed array (rows out of range read its sentinel at
+// index R): a sparse column's gathers then touch as few cache lines as
+// possible.  Otherwise the 32-bit labels are gathered: for a column holding a
+// large share of the rows they are coalesced and unpacking would only add work.
+template <typename V, int NT, bool kPacked, typename F>
+__device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, int64_t n, F&& f) {
     const uint32_t tid = (uint32_t)tid_x();
     const uint32_t R = (uint32_t)J.R, G = (uint32_t)J.G;
     const uint32_t lbits = J.gid_bits, lmask = (1u << lbits) - 1u;
     const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)J.gidp, (short)0, J.gid_words * 4u, kBufferWord3);
+        (void*)J.gidp, (short)0, kPacked ? J.gid_words * 4u : 0u, kBufferWord3);
     const int32_t* col_rows = J.rows + p0;
     const V* col_vals = (const V*)J.vals + p0;
     constexpr uint32_t step = (uint32_t)NT * kUnroll;
-    for (uint32_t ib = 0; ib < n; ib += step) {
+    auto load = [&](int64_t ib, uint32_t (&r)[kUnroll], V (&v)[kUnroll]) {
+        const int64_t left = n - ib;
+        const uint32_t m = left < (int64_t)step ? (uint32_t)left : step;
         const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(col_rows + ib), (short)0, (n - ib) * 4u, kBufferWord3);
+            (void*)(col_rows + ib), (short)0, m * 4u, kBufferWord3);
         const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(col_vals + ib), (short)0, (n - ib) * (uint32_t)sizeof(V), kBufferWord3);
-        uint32_t r[kUnroll];
-        V v[kUnroll];
+            (void*)(col_vals + ib), (short)0, m * (uint32_t)sizeof(V), kBufferWord3);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const uint32_t e = tid + (uint32_t)(u * NT);
             r[u] = buffer_load_u32(rows_rsrc, e * 4u, 0u);
             v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
-            if (ib + e >= n) r[u] = 0xFFFFFFFFu;
         }
-        uint32_t g[kUnroll];
+    };
+    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            const uint32_t pos = min(r[u], R) * lbits;
-            const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
-            g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
+            if (kPacked) {
+                const uint32_t pos = min(r[u], R) * lbits;
+                const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
+                g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
+            } else {
+                g[u] = r[u] < R ? (uint32_t)gl(J.gid)[r[u]] : 0xFFFFFFFFu;
+            }
         }
+    };
+    if (n <= 0) return;
+    uint32_t r[kUnroll];
+    V v[kUnroll];
+    load(0, r, v);
+    int64_t ib = 0;
+    for (; ib + (int64_t)step < n; ib += step) {   // a whole batch with another after it
+        uint32_t g[kUnroll];
+        gather(r, g);
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t r2[kUnroll];
+        V v2[kUnroll];
+        load(ib + step, r2, v2);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
             if (g[u] < G) f(g[u], v[u]);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
     }
+    // the last batch: lanes past the column's end masked
+    const uint32_t m = (uint32_t)(n - ib);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+        if (tid + (uint32_t)(u * NT) >= m) r[u] = 0xFFFFFFFFu;
+    uint32_t g[kUnroll];
+    gather(r, g);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+        if (g[u] < G) f(g[u], v[u]);
 }
 
 __device__ inline void raise_status(const JobDesc& J, uint32_t bit, int64_t c) {
@@ -284,28 +321,35 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     uint32_t* cnt_nz = (uint32_t*)smem;
     uint32_t* cnt_ez = cnt_nz + G;
     uint32_t* cur = cnt_ez + G;
-    // the column's entries in a group: batched buffer loads and bit-packed
-    // labels for a sparse column (its rows are far apart, so 32-bit label
-    // gathers would touch a cache line per entry); the plain loop for a column
-    // holding a quarter of the rows or more (consecutive rows: 32-bit label
-    // gathers are coalesced, unpacking would only add work) or one the packed
-    // labels cannot address
+    // the column's entries in a group: bit-packed labels for a sparse column
+    // (its rows are far apart, so 32-bit label gathers would touch a cache
+    // line per entry), 32-bit labels for a column holding a quarter of the
+    // rows or more (consecutive rows: the gathers are coalesced)
     const bool packed = __builtin_amdgcn_readfirstlane(
-        (J.gid_bits > 0u && p1 - p0 < ((int64_t)1 << 28) && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
+        (J.gid_bits > 0u && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
     auto grouped = [&](auto&& f) {
-        if (packed) for_each_grouped<V, kColBlock>(J, p0, (uint32_t)(p1 - p0), f);
-        else for_each_entry<V>(J.rows, vals, J.gid, R, (uint32_t)G, p0, p1, f);
+        if (packed) for_each_grouped<V, kColBlock, true>(J, p0, p1 - p0, f);
+        else for_each_grouped<V, kColBlock, false>(J, p0, p1 - p0, f);
     };
 
-    for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; }
+    // Slots: group g's keys at the exclusive prefix of the group sizes.  When
+    // the column's key range holds them all (a column storing at least as many
+    // entries as there are grouped rows, e.g. a dense one), every stored value
+    // goes to its slot in the counting pass itself and the column is read once;
+    // otherwise the counts are scanned into offsets and a second pass scatters.
+    for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; cur[g] = J.group_rows[g]; }
     if (tid == 0) {
         st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
         st.kmin = ~0ull; st.kmax = 0;
     }
     __syncthreads();
+    const uint32_t slots = block_excl_scan(cur, G, st.scan);
+    const bool fixed = __builtin_amdgcn_readfirstlane((int64_t)slots <= p1 - p0 ? 1 : 0) != 0;
+    K* keys = (K*)J.keys + p0;
 
-    // counts and value statistics
-    {
+    // counts and value statistics (and, in slots, the keys)
+    auto count_pass = [&](auto fixed_tag) {
+        constexpr bool kFixed = decltype(fixed_tag)::value;
         bool li = true;
         int lmin = INT_MAX, lmax = INT_MIN;
         K kmn = KT::kSentinel, kmx = 0;
@@ -313,12 +357,19 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         grouped([&](uint32_t g, V v) {
             if (!KT::finite(v)) return;
             if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
-            atomicAdd(&cnt_nz[g], 1u);
+            const uint32_t i = atomicAdd(&cnt_nz[g], 1u);
+            const K k = KT::key(v);
+            if (kFixed) {
+                // a group holding more entries than rows (repeated row
+                // indices) is an error of the column; its keys stay inside
+                // the column's range
+                const uint32_t pos = cur[g] + i;
+                if ((int64_t)pos < p1 - p0) keys[pos] = k;
+            }
             int iv;
             if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
             else li = false;
             if (g == 0) {
-                const K k = KT::key(v);
                 kmn = k < kmn ? k : kmn;
                 kmx = k > kmx ? k : kmx;
                 neg += (v < V(0)) ? 1u : 0u;
@@ -338,7 +389,9 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
             atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
-    }
+    };
+    if (fixed) count_pass(std::true_type{});
+    else count_pass(std::false_type{});
     __syncthreads();
 
     // errors of this column (mannwhitneyu.cpp:936-1038), counts out
@@ -352,11 +405,11 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             if (n < 2) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
             J.cnt[b + g] = nz;
             J.zcnt[b + g] = ez;
-            cur[g] = nz;
+            if (!fixed) cur[g] = nz;
         }
     }
     __syncthreads();
-    block_excl_scan(cur, G, st.scan);
+    if (!fixed) block_excl_scan(cur, G, st.scan);
     for (int g = tid; g < G; g += nt) J.goff[c * G + g] = cur[g];
 
     // plan of the column
@@ -390,15 +443,25 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     }
     __syncthreads();
 
-    // scatter nonzero values into the groups' key ranges
     const uint32_t hist = st.hist;
     const int32_t vmin = st.vmin;
-    K* keys = (K*)J.keys + p0;
-    grouped([&](uint32_t g, V v) {
-        if (!KT::finite(v) || v == V(0)) return;
-        const uint32_t pos = atomicAdd(&cur[g], 1u);
-        keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
-    });
+    if (!fixed) {
+        // scatter nonzero values into the groups' key ranges
+        grouped([&](uint32_t g, V v) {
+            if (!KT::finite(v) || v == V(0)) return;
+            const uint32_t pos = atomicAdd(&cur[g], 1u);
+            keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
+        });
+    } else if (hist) {
+        // the slots hold keys; a histogram plan reads bins: each group's
+        // keys (integers, as the statistics found them) become v - vmin
+        const int w = tid / kWave, nw = nt / kWave;
+        for (int g = w; g < G; g += nw) {
+            const uint32_t o = cur[g], m = min(cnt_nz[g], J.group_rows[g]);
+            for (uint32_t i = (uint32_t)lane; i < m; i += kWave)
+                keys[o + i] = (K)(uint32_t)((int)KT::value(keys[o + i]) - vmin);
+        }
+    }
 }
 
 // workgroup b takes queue item b (the grid is sized for the queue's capacity)
@@ -716,10 +779,10 @@ __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResu
 }
 
 // Targets g0 .. g0+ng-1 of column c, target g0+j held by lane j (4-byte keys).
-// The keys of consecutive groups lie next to each other in the column's key
-// range, so a window of consecutive targets whose keys total at most 64 is
-// ranked in one pass of the wave: lane e loads key e of the window and finds
-// its target (the last one starting at or before e); a bitonic sort of
+// A window of consecutive targets whose keys total at most 64 is ranked in one
+// pass of the wave: lane e finds the target of key e of the window (the last
+// one starting at or before e in the window's packed order) and loads it from
+// that target's keys; a bitonic sort of
 // (target, key) orders every target's keys; runs of equal (target, key) give
 // each key its position in its run of equal values (the tie term); inclusive
 // scans of the per-key terms, read at each target's first and last position,
@@ -765,9 +828,7 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
             continue;
         }
         const uint32_t m = shfl_u32(inc, (int)j1 - 1) - base;   // keys in the window
-        const uint32_t off0 = shfl_u32(off, (int)j0);
         const bool valid = lane < m;
-        const K key = valid ? colkeys[off0 + lane] : K(0);
         // owner: the last target j in [j0, j1) whose first key is at or before
         // this lane (empty targets share the next one's start and lose)
         const uint32_t excl = inc - sz - base;
@@ -781,6 +842,10 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
                 else hi = mid;
             }
         }
+        // the lane's key: position lane - excl(owner) of the owner's keys (a
+        // group's keys need not end where the next group's start)
+        const uint32_t at = shfl_u32(off, (int)lo) + lane - shfl_u32(excl, (int)lo);
+        const K key = valid ? colkeys[at] : K(0);
         uint64_t comp[1] = {valid ? (((uint64_t)lo << 32) | (uint64_t)key) : ~0ull};
         wave_sort<1, uint64_t>(comp);
         const uint64_t prev = shfl_up_u64(comp[0], 1);
@@ -900,10 +965,15 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     }
 }
 
-// workgroup b takes queue item b (the grid is sized for the queue's capacity)
+// workgroup b takes queue item b (the grid is sized for the queue's capacity).
+// At most 168 VGPRs, so that three waves share a SIMD: the table LDS
+// (49 KB) fits three workgroups per CU, and the third wave hides the
+// latency of the per-target gathers (6.0 -> 4.4 ms on 2,000 small targets
+// x 6,000 genes, against 2 waves at the 178 VGPRs the kernel takes freely;
+// the cap spills 4 VGPRs).
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void target_kernel(const JobDesc* jobs, const WorkItem* items,
-                                                   const uint32_t* n_items) {
+__global__ __launch_bounds__(kColBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void target_kernel(const JobDesc* jobs, const WorkItem* items, const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
     if (blockIdx.x < n) target_item<V>(jobs, items, blockIdx.x);
 }

@@ -83,6 +83,92 @@ def test_sim_sort_path_many_small_targets(mode):
     run_both(data, rows, indptr, gid, T, R, mode)
 
 
+@pytest.mark.parametrize("mode", ["min", "none"])
+def test_sim_sort_path_sparse_columns(mode):
+    # columns holding under a quarter of the rows are split through the
+    # bit-packed labels: ~5,200 entries (two whole batches of 2,048 and a
+    # partial one), ~2,000 (one partial batch) and, in mode min (mode none
+    # needs two stored values per group), ~30 entries and an empty column;
+    # rows in no group (label -1) interleaved
+    rng = np.random.default_rng(33)
+    R, T = 40000, 5
+    dens = [0.13, 0.13, 0.05] + ([0.00075, 0.0] if mode == "min" else [])
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array(dens))
+    data = np.round(rng.normal(0, 2, len(rows)), 2).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:9000]] = 0
+    gid[perm[9000:30000]] = rng.integers(1, T + 1, 21000)
+    run_both(data, rows, indptr, gid, T, R, mode)
+
+
+def _dense(R, C):
+    return np.tile(np.arange(R, dtype=np.int64), C), np.arange(C + 1, dtype=np.int64) * R
+
+
+@pytest.mark.parametrize("mode", ["min", "none", "mix"])
+def test_sim_sort_path_dense_slots(mode):
+    # dense columns are split in one pass into slots sized by the groups' row
+    # counts; explicit zeros and NaN among the stored values leave unused slot
+    # entries after a group's keys, which the small-target windows step over
+    rng = np.random.default_rng(34)
+    R, C, T = 6000, 4, 200
+    rows, indptr = _dense(R, C)
+    data = np.round(rng.normal(0, 1.5, R * C), 1).astype(np.float32)
+    z = rng.random(R * C)
+    data[z < 0.1] = 0.0
+    data[(z >= 0.1) & (z < 0.12)] = np.nan
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:500]] = 0
+    at = 500
+    for g, n in enumerate(rng.integers(4, 31, T), start=1):
+        gid[perm[at:at + n]] = g
+        at += n
+    run_both(data, rows, indptr, gid, T, R, mode)
+
+
+def test_sim_sort_path_dense_slots_histogram_plan():
+    # grouped values all small integers, one non-integer in a row of no group:
+    # the column takes the sort path, whose split picks a histogram plan after
+    # its keys are already in their slots
+    rng = np.random.default_rng(35)
+    R, C, T = 5000, 3, 40
+    rows, indptr = _dense(R, C)
+    data = rng.integers(-3, 12, R * C).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:800]] = 0
+    gid[perm[800:3500]] = rng.integers(1, T + 1, 2700)
+    for c in range(C):
+        data[c * R + perm[4000 + c]] = 2.5
+    U1, P, TIE, n1, n2, err, status = sim.mwu(data, rows, indptr, gid, T, R, mode=ZH["min"],
+                                              timeout_s=120)
+    assert sim.mwu.last_counted == 0
+    assert err == 2 ** 64 - 1 and status == 0
+    oU1, oP, oT, _, _ = oracle.mwu(data, rows, indptr, gid, T, n_rows=R, zero_handling="min",
+                                   use_continuity=True, with_stats=True)
+    assert np.array_equal(U1, oU1) and np.array_equal(TIE, oT) and np.array_equal(P, oP)
+
+
+def test_sim_dense_slots_repeated_row_is_an_error():
+    # a column storing one row twice (every row grouped, so the slots fill the
+    # column's key range): the last group overflows its slot; the error is
+    # reported and no key lands outside the column (checked by the sanitizer)
+    rng = np.random.default_rng(36)
+    R, T = 3000, 6
+    gid = rng.integers(0, T + 1, R).astype(np.int32)
+    last = np.nonzero(gid == T)[0]
+    rows = np.arange(R, dtype=np.int64)
+    other = np.nonzero(gid == 1)[0][0]
+    rows[other] = last[0]
+    rows.sort()
+    data = rng.normal(0, 1, R).astype(np.float32)
+    U1, P, TIE, n1, n2, err, status = sim.mwu(data, rows, np.array([0, R]), gid, T, R,
+                                              mode=ZH["min"], timeout_s=60)
+    assert err != 2 ** 64 - 1
+
+
 def test_sim_eight_byte_keys():
     d, r, ip, g = _rw(9, 1500, 3, 3, 0.5, lambda rg, n: np.round(rg.normal(0, 2, n), 1))
     run_both(d.astype(np.float64), r, ip, g, 3, 1500, "mix", f64=True)
