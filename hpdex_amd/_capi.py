@@ -51,12 +51,13 @@ class MwuOpts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes",
-                                              "count_block")]
+                                              "count_block", "count_packed")]
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
-        "count_columns", "count_nnz", "split_columns", "handed_over", "count_blocks")]
+        "count_columns", "count_nnz", "split_columns", "handed_over", "count_blocks",
+        "count_packed_columns")]
 
 
 class SessionOpts(ctypes.Structure):
@@ -441,8 +442,11 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None):
+def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None,
+               count_packed=None):
     t = get_tuning()
+    if count_packed is not None:
+        t.count_packed = int(count_packed)
     if count_block is not None:
         t.count_block = count_block
     if hist_max_bins is not None:

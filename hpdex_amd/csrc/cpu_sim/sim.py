@@ -31,13 +31,16 @@ def lib(path=None):
         L.sim_count_blocks.restype = ci
         L.sim_set_shrink_range.argtypes = [ci]
         L.sim_set_shrink_range.restype = None
+        L.sim_set_pack.argtypes = [ci]
+        L.sim_set_pack.restype = None
+        L.sim_packed_blocks.restype = ci
         _lib = L
     return _lib
 
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
         count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0,
-        shrink_range=False):
+        shrink_range=False, pack=True):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels)."""
     # laid out like a resident matrix: PAD padding entries (value 0, row -1)
@@ -53,10 +56,12 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     err = np.zeros(1, dtype=np.uint64)
     status = np.zeros(1, dtype=np.uint32)
     lib().sim_set_shrink_range(int(shrink_range))   # test hook: forces hand-overs
+    lib().sim_set_pack(int(pack))   # packed entries for the count kernel where they fit
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
                   float(timeout_s))
     mwu.last_counted = counted   # columns the count kernel finished itself
     mwu.last_blocks = lib().sim_count_blocks()   # its workgroups (column x target slice)
+    mwu.last_packed = lib().sim_packed_blocks()   # of them, those that read packed entries
     return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

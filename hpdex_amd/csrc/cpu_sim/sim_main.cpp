@@ -28,6 +28,8 @@ namespace {
 
 int sim_last_blocks = 0;   // count-kernel workgroups of the last sim_mwu call
 int sim_shrink_range = 0;  // test hook: record each column's range one value short
+int sim_pack = 1;          // the count kernel reads packed entries where a column has them
+int sim_last_packed = 0;   // count-kernel workgroups of the last call that read packed entries
 
 struct ThreadInfo { std::atomic<const char*>* last; };
 
@@ -78,7 +80,7 @@ bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void(
 // target kernels over the queue (host-routed columns plus hand-overs).
 // Returns the number of columns the count kernel finished itself.
 template <typename V>
-static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int32_t>& order,
+static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& order,
                         uint32_t count_lds, unsigned count_block, double timeout_s) {
     const JobDesc* jobs = &J;
     const int C = (int)J.C;
@@ -89,6 +91,24 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
                                                               col_int.data(), range); }, timeout_s);
+    // the matrix's packed entries (finish_matrix), made from the recorded ranges
+    const uint32_t vb = 32u - pack_row_bits((uint32_t)J.R);
+    std::vector<int32_t> lo(C, INT_MIN);
+    std::vector<uint8_t> col_packed(C, 0);
+    for (int c = 0; c < C; ++c) {
+        const int64_t bins = (int64_t)range[2 * c + 1] - range[2 * c] + 1;
+        if (col_int[c] && pack_fits(vb, bins)) {
+            lo[c] = bins > 0 ? range[2 * c] : 0;
+            col_packed[c] = 1;
+        }
+    }
+    std::vector<uint32_t> packed((size_t)J.indptr[C] + 1, 0xDEADBEEFu);
+    for (int c = 0; c < C; ++c)
+        run_block(c, 256, C, [&] { pack_columns_kernel<V>((const V*)J.vals, J.rows, J.indptr,
+                                                          lo.data(), (uint32_t)J.R, vb,
+                                                          packed.data()); }, timeout_s);
+    J.packed = packed.data();
+    J.pack_vb = vb;
     // test hook: ranges one value short of the data, so that the count kernel
     // meets values outside them and hands the columns over
     if (sim_shrink_range)
@@ -118,6 +138,7 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
         it.nb = bins > 0 ? (uint32_t)bins : 1u;
         it.col = c;
         it.S = (uint16_t)S;
+        it.flags = (sim_pack && col_packed[c]) ? kCountPacked : 0u;
         for (uint32_t t0 = 0; t0 < T; t0 += S) {
             it.t0 = (uint16_t)t0;
             items.push_back(it);
@@ -125,6 +146,8 @@ static int run_pipeline(const JobDesc& J, int32_t* range, const std::vector<int3
         ++nc;
     }
     sim_last_blocks = (int)items.size();
+    sim_last_packed = 0;
+    for (const CountItem& x : items) sim_last_packed += (x.flags & kCountPacked) ? 1 : 0;
     uint32_t gen_n = n_static;
     const unsigned nb = (unsigned)items.size();
     for (unsigned b = 0; b < nb; ++b)
@@ -204,3 +227,5 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
 // Count-kernel workgroups (column x target slice) of the last sim_mwu call.
 extern "C" int sim_count_blocks() { return sim_last_blocks; }
 extern "C" void sim_set_shrink_range(int on) { sim_shrink_range = on; }
+extern "C" void sim_set_pack(int on) { sim_pack = on; }
+extern "C" int sim_packed_blocks() { return sim_last_packed; }

@@ -157,6 +157,32 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
                               s));
         HIPCHK(hipStreamSynchronize(s));
     }
+    // packed entries of the integer-valued columns whose bins fit the code
+    // bits: one 32-bit word per entry for the count kernel's read
+    m->col_packed.assign(C, 0);
+    m->pack_vb = 32u - pack_row_bits((uint32_t)m->R);
+    std::vector<int32_t> lo(C, INT_MIN);
+    bool any = false;
+    for (size_t c = 0; c < C; ++c) {
+        const int64_t bins = (int64_t)m->h_range[2 * c + 1] - m->h_range[2 * c] + 1;
+        if (m->col_int[c] && pack_fits(m->pack_vb, bins)) {
+            lo[c] = bins > 0 ? m->h_range[2 * c] : 0;
+            m->col_packed[c] = 1;
+            any = true;
+        }
+    }
+    if (any && m->nnz) {
+        DevBuf d_lo;
+        HIPCHK(d_lo.alloc(C * 4));
+        HIPCHK(hipMemcpyAsync(d_lo.p, lo.data(), C * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(m->packed.alloc(m->nnz * 4));
+        HIPCHK(launch_pack_columns(m->vals.p, m->f64, m->rows.as<int32_t>(),
+                                   m->indptr.as<int64_t>(), (int64_t)C, d_lo.as<int32_t>(),
+                                   (uint32_t)m->R, m->pack_vb, m->packed.as<uint32_t>(), s));
+        HIPCHK(hipStreamSynchronize(s));   // d_lo is freed on return
+    } else {
+        m->col_packed.assign(C, 0);
+    }
     m->order.resize(C);
     std::iota(m->order.begin(), m->order.end(), 0);
     std::stable_sort(m->order.begin(), m->order.end(), [&](int32_t a, int32_t b) {
@@ -345,6 +371,8 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.gid_words = (uint32_t)js.gid_words;
     J.group_rows = js.grows.as<uint32_t>();
     J.colrange = m->colrange.as<int32_t>();
+    J.packed = m->packed.p ? m->packed.as<uint32_t>() : nullptr;
+    J.pack_vb = m->pack_vb;
     J.keys = js.keys.p;
     J.cnt = js.cnt.as<uint32_t>();
     J.zcnt = js.zcnt.as<uint32_t>();
@@ -390,6 +418,7 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     count_items.reserve(m.C);
     gen_items.reserve(m.C);
     js.n_count_cols = 0;
+    js.n_count_packed = 0;
     js.count_nnz = 0;
     for (size_t k = 0; k < m.C; ++k) {
         const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
@@ -410,11 +439,13 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
         it.nb = bins > 0 ? (uint32_t)bins : 1u;
         it.col = c;
         it.S = (uint16_t)S;
+        it.flags = (tu.count_packed && m.col_packed[c]) ? kCountPacked : 0u;
         for (uint32_t t0 = 0; t0 < T; t0 += S) {
             it.t0 = (uint16_t)t0;
             count_items.push_back(it);
         }
         js.n_count_cols += 1;
+        js.n_count_packed += it.flags ? 1 : 0;
         js.count_nnz += len;
     }
     js.n_count = (int)count_items.size();
@@ -652,6 +683,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     out->size_ordered = t.size_ordered;
     out->count_lds_bytes = t.count_lds;
     out->count_block = t.count_block;
+    out->count_packed = t.count_packed;
     return 0;
 }
 
@@ -668,6 +700,8 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
                        "count_lds_bytes must be 0 or a multiple of 16 in [16384, 155648]");
     if (t->count_block != 256 && t->count_block != 512 && t->count_block != 1024)
         return set_err(err, err_len, "count_block must be 256, 512 or 1024");
+    if (t->count_packed != 0 && t->count_packed != 1)
+        return set_err(err, err_len, "count_packed must be 0 or 1");
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init && !g_count_ready && t->count_lds_bytes != 0)
         return set_err(err, err_len, "this device does not give the count kernel its LDS");
@@ -675,6 +709,7 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     g_tune.size_ordered = t->size_ordered;
     g_tune.count_lds = t->count_lds_bytes;
     g_tune.count_block = t->count_block;
+    g_tune.count_packed = t->count_packed;
     return 0;
 }
 
@@ -939,7 +974,7 @@ void hpdex_matrix_destroy(hpdex_matrix* m) { delete m; }
 
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
     if (!m) return 0;
-    return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n;
+    return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n + m->packed.n;
 }
 
 }  // extern "C"
@@ -1033,6 +1068,7 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
     if (!p || !out) return set_err(err, err_len, "null handle");
     out->count_columns = p->js.n_count_cols;
     out->count_blocks = p->js.n_count;
+    out->count_packed_columns = p->js.n_count_packed;
     out->count_nnz = p->js.count_nnz;
     out->split_columns = p->js.n_gen_static;
     out->handed_over = 0;

@@ -32,6 +32,7 @@ struct Tuning {
     int size_ordered = 1;
     int count_lds = kCountLdsDefault;
     int count_block = kCountBlockDefault;
+    int count_packed = 1;
 };
 
 extern std::mutex g_mu;
@@ -117,6 +118,9 @@ struct hpdex_matrix {
     std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
                                     // non-finite) -- the column may go to the count kernel
     std::vector<int32_t> h_range;   // host copy of colrange
+    hpdex_host::DevBuf packed;      // packed entries (pack_row_bits) of the packable columns
+    uint32_t pack_vb = 0;           // their value bits
+    std::vector<uint8_t> col_packed;   // 1: column c has packed entries
 };
 
 namespace hpdex_host {
@@ -135,6 +139,7 @@ struct JobState {
     DevBuf d_count_items, d_gen_items, d_gen_n;     // work queues of the two paths
     JobDesc J{};                                    // host copy of the descriptor
     int n_count = 0, n_count_cols = 0, n_gen_static = 0;   // count workgroups, their columns
+    int n_count_packed = 0;                                  // of them read packed
     int64_t count_nnz = 0;
     uint32_t count_lds = 0;
     int count_block = kCountBlockDefault;

@@ -60,6 +60,8 @@ struct JobDesc {
     uint32_t gid_words;          // 32-bit words of gidp
     const uint32_t* group_rows;  // G rows per group
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
+    const uint32_t* packed;      // nnz packed entries (pack_entry) or null; CountItem flags
+    uint32_t pack_vb;            // value bits of a packed entry
     // workspace
     void* keys;                  // nnz keys (uint32 for float values, uint64 for double)
     uint32_t* cnt;               // C*G stored nonzero finite values per (column, group)
@@ -106,8 +108,28 @@ struct CountItem {
     int32_t col;
     uint16_t t0;
     uint16_t S;
-    uint32_t pad;
+    uint32_t flags;   // kCountPacked: read the column's packed entries
 };
+constexpr uint32_t kCountPacked = 1u;
+
+// Packed entries of a resident matrix.  A column whose finite nonzero values
+// are all integers of one range [lo, lo + bins) (the columns the count kernel
+// takes) keeps each entry also as one 32-bit word (row << vb) | code, with
+// code = value - lo, pack_zero_code for an explicit zero and pack_skip_code
+// for a non-finite value, and a row outside [0, R) stored as R.  The count
+// kernel then reads one word per entry instead of a row and a value.  The row
+// takes rb = pack_row_bits(R) bits (R itself included), the code vb = 32 - rb.
+__host__ __device__ inline uint32_t pack_row_bits(uint32_t R) {
+    uint32_t b = 1;
+    while (b < 32u && (R >> b) != 0u) ++b;
+    return b;
+}
+__host__ __device__ inline uint32_t pack_zero_code(uint32_t vb) { return (1u << vb) - 2u; }
+__host__ __device__ inline uint32_t pack_skip_code(uint32_t vb) { return (1u << vb) - 1u; }
+// bins (0 or less: no nonzero value) leave both special codes free
+__host__ __device__ inline bool pack_fits(uint32_t vb, int64_t bins) {
+    return vb >= 2u && vb <= 30u && bins <= (int64_t)(1u << vb) - 2;
+}
 
 // LDS of a count-kernel workgroup with `rows` rows (the reference and a
 // slice's targets) over NB value bins plus a zero column:
@@ -258,6 +280,9 @@ hipError_t prepare_count_kernel(uint32_t bytes);
 // smallest and largest stored nonzero finite value of such a column.
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
                                    uint8_t* ok, int32_t* range, hipStream_t s);
+hipError_t launch_pack_columns(const void* vals, bool f64, const int32_t* rows,
+                               const int64_t* indptr, int64_t C, const int32_t* lo, uint32_t R,
+                               uint32_t vb, uint32_t* out, hipStream_t s);
 hipError_t launch_init_job_state(unsigned long long* err_key, uint32_t* status,
                                  unsigned long long* status_col, hipStream_t s);
 hipError_t launch_exact_tables(const ExactTable* tabs, int n_tabs, double* tabbuf,

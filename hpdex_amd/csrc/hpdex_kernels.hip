@@ -1115,9 +1115,16 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     const uint32_t cell_off = (uint32_t)(pc.w - lds_words);
 
     // column loads (buffer loads, described with the read pass below); the
-    // first batch is issued now so that it overlaps the set-up of LDS
+    // first batch is issued now so that it overlaps the set-up of LDS.  A
+    // packed column (kCountPacked) is read as one word per entry, (row << vb)
+    // | code, from the matrix's packed entries instead of a row and a value.
+    const bool packed = uniform_u32(item.flags & kCountPacked) != 0u;
+    const uint32_t pvb = uniform_u32(J.pack_vb);
+    const uint32_t pmask = (1u << pvb) - 1u;
+    const uint32_t pzero = pack_zero_code(pvb);
     const int32_t* col_rows = J.rows + p0;
     const V* col_vals = vals + p0;
+    const uint32_t* col_packed = J.packed + p0;
     const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)J.gidp, (short)0, uniform_u32(J.gid_words) * 4u, kBufferWord3);
     const uint32_t lbits = uniform_u32(J.gid_bits);
@@ -1139,18 +1146,27 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
             v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
         }
     };
-    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
+    auto load_packed = [&](uint32_t ib, uint32_t (&r)[kUnroll]) {
+        const __amdgpu_buffer_rsrc_t pk_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_packed + ib), (short)0, (n - ib) * 4u, kBufferWord3);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            r[u] = buffer_load_u32(pk_rsrc, ((uint32_t)tid + (uint32_t)(u * NT)) * 4u, 0u);
+    };
+    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll], auto packed_tag) {
+        constexpr bool kPk = decltype(packed_tag)::value;
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             // the label's bit position; 8 bytes from its word hold all of it
-            const uint32_t pos = min(r[u], R) * lbits;
+            const uint32_t pos = min(kPk ? r[u] >> pvb : r[u], R) * lbits;
             const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
             g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
         }
     };
     uint32_t r0[kUnroll];
     V v0[kUnroll];
-    load(0u, r0, v0);
+    if (packed) load_packed(0u, r0);
+    else load(0u, r0, v0);
 
     for (uint32_t x = tid; x < cell_words; x += nt) pc.w[x] = 0u;
     if (tid == 0) st.bad = 0u;
@@ -1175,24 +1191,38 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     // column's end into zeros that cost no memory traffic (they are masked in
     // the last batch), so no load leaves the column.  Columns are shorter
     // than 2^28 entries, so the byte sizes fit 32 bits.
-    auto count = [&](const uint32_t (&g)[kUnroll], const V (&v)[kUnroll], auto wide_tag,
-                     auto sliced_tag) {
+    auto count = [&](const uint32_t (&g)[kUnroll], const V (&v)[kUnroll],
+                     const uint32_t (&r)[kUnroll], auto wide_tag, auto sliced_tag,
+                     auto packed_tag) {
         constexpr bool kWide = decltype(wide_tag)::value;
         constexpr bool kSliced = decltype(sliced_tag)::value;
+        constexpr bool kPk = decltype(packed_tag)::value;
         const uint32_t dummy = (uint32_t)lane_id();
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const uint32_t gg = g[u];
-            const V vv = v[u];
-            // |v| <= 32768 covers every integer the column metadata admits
-            // ([-32768, 32767]); a finite nonzero value outside the recorded
-            // range means the metadata no longer matches (checked for every
-            // entry, whatever its group, so all slices agree)
-            const bool near = KT::abs_le(vv, V(32768));
-            const bool zero = vv == V(0);
-            const uint32_t b = (uint32_t)((int)(near ? vv : V(0)) - vmin);
-            const bool inb = near && b < NB;
-            bad |= !zero && !inb && KT::finite(vv);
+            uint32_t b;
+            bool zero, inb;
+            if (kPk) {
+                // the code: value - vmin, the zero code, or the skip code of
+                // a non-finite value (neither counted nor out of range)
+                b = r[u] & pmask;
+                zero = b == pzero;
+                inb = b < NB;
+                bad |= b < pzero && !inb;
+            } else {
+                const V vv = v[u];
+                // |v| <= 32768 covers every integer the column metadata
+                // admits ([-32768, 32767]); a finite nonzero value outside
+                // the recorded range means the metadata no longer matches
+                // (checked for every entry, whatever its group, so all
+                // slices agree)
+                const bool near = KT::abs_le(vv, V(32768));
+                zero = vv == V(0);
+                b = (uint32_t)((int)(near ? vv : V(0)) - vmin);
+                inb = near && b < NB;
+                bad |= !zero && !inb && KT::finite(vv);
+            }
             // the entry's row: reference 0, target t0 + k -> k
             uint32_t row;
             bool mine;
@@ -1209,27 +1239,30 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
             atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
         }
     };
-    auto read_all = [&](auto wide_tag, auto sliced_tag) {
+    auto read_all = [&](auto wide_tag, auto sliced_tag, auto packed_tag) {
+        constexpr bool kPk = decltype(packed_tag)::value;
         uint32_t r[kUnroll];
         V v[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) { r[u] = r0[u]; v[u] = v0[u]; }
+        for (int u = 0; u < kUnroll; ++u) { r[u] = r0[u]; v[u] = kPk ? V(0) : v0[u]; }
         uint32_t ib = 0;
         for (; ib + step < n; ib += step) {   // a whole batch with another after it
             uint32_t g[kUnroll];
-            gather(r, g);
+            gather(r, g, packed_tag);
             // keep the gathers ahead of the next batch's loads (the counter
             // that waits for the gathers would otherwise wait for those too)
             __builtin_amdgcn_sched_barrier(0);
             uint32_t r2[kUnroll];
             V v2[kUnroll];
-            load(ib + step, r2, v2);
+            if (kPk) load_packed(ib + step, r2);
+            else load(ib + step, r2, v2);
             __builtin_amdgcn_sched_barrier(0);
-            count(g, v, wide_tag, sliced_tag);
+            count(g, v, r, wide_tag, sliced_tag, packed_tag);
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
+            for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = kPk ? V(0) : v2[u]; }
         }
         if (ib < n) {   // the last batch: entries past the column's end masked
+            // (a masked packed word reads as row R with the skip code)
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const bool in = ib + (uint32_t)(u * NT + tid) < n;
@@ -1237,17 +1270,21 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
                 v[u] = in ? v[u] : V(0);
             }
             uint32_t g[kUnroll];
-            gather(r, g);
-            count(g, v, wide_tag, sliced_tag);
+            gather(r, g, packed_tag);
+            count(g, v, r, wide_tag, sliced_tag, packed_tag);
         }
     };
-    if (sliced) {
-        if (wide) read_all(std::true_type{}, std::true_type{});
-        else read_all(std::false_type{}, std::true_type{});
-    } else {
-        if (wide) read_all(std::true_type{}, std::false_type{});
-        else read_all(std::false_type{}, std::false_type{});
-    }
+    auto read_as = [&](auto packed_tag) {
+        if (sliced) {
+            if (wide) read_all(std::true_type{}, std::true_type{}, packed_tag);
+            else read_all(std::false_type{}, std::true_type{}, packed_tag);
+        } else {
+            if (wide) read_all(std::true_type{}, std::false_type{}, packed_tag);
+            else read_all(std::false_type{}, std::false_type{}, packed_tag);
+        }
+    };
+    if (packed) read_as(std::true_type{});
+    else read_as(std::false_type{});
     if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
     if ((uint32_t)tid < rows) grs[tid] = gpre0;
     if ((uint32_t)tid + (uint32_t)nt < rows) grs[tid + nt] = gpre1;
@@ -1427,6 +1464,31 @@ __global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
         ok_out[c] = (uint8_t)ok;
         range[2 * c] = ok ? lo : 1;
         range[2 * c + 1] = ok ? hi : 0;
+    }
+}
+
+// Packed entries (pack_row_bits) of the columns with lo[c] != INT_MIN, one
+// workgroup per column.  Classification found every finite nonzero value of
+// such a column an integer in [lo, lo + bins), and the host checked that the
+// bins fit the code bits.
+template <typename V>
+__global__ __launch_bounds__(256) void pack_columns_kernel(const V* vals, const int32_t* rows,
+                                                           const int64_t* indptr,
+                                                           const int32_t* lo, uint32_t R,
+                                                           uint32_t vb, uint32_t* out) {
+    const int64_t c = blockIdx.x;
+    const int32_t base = lo[c];
+    if (base == INT_MIN) return;
+    const uint32_t zc = pack_zero_code(vb), sc = pack_skip_code(vb);
+    for (int64_t p = indptr[c] + threadIdx.x; p < indptr[c + 1]; p += blockDim.x) {
+        const V v = vals[p];
+        const int32_t r = rows[p];
+        const uint32_t row = (r >= 0 && (uint32_t)r < R) ? (uint32_t)r : R;
+        uint32_t code;
+        if (!KeyOf<V>::finite(v)) code = sc;
+        else if (v == V(0)) code = zc;
+        else code = (uint32_t)((int32_t)v - base);
+        out[p] = (row << vb) | code;
     }
 }
 
@@ -1716,6 +1778,19 @@ hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* in
     else
         classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, indptr,
                                                                     ok, range);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_columns(const void* vals, bool f64, const int32_t* rows,
+                               const int64_t* indptr, int64_t C, const int32_t* lo, uint32_t R,
+                               uint32_t vb, uint32_t* out, hipStream_t s) {
+    if (C <= 0) return hipSuccess;
+    if (f64)
+        pack_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, rows, indptr,
+                                                                 lo, R, vb, out);
+    else
+        pack_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, rows, indptr,
+                                                                lo, R, vb, out);
     return hipGetLastError();
 }
 

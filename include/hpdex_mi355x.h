@@ -192,13 +192,15 @@ int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie
  * count-kernel columns that the last run passed on to the split kernels
  * (values outside the column's recorded range).  count_blocks is the count
  * kernel's workgroups: one per column, or one per slice of targets when a
- * column's counters for every target do not fit one workgroup's LDS.        */
+ * column's counters for every target do not fit one workgroup's LDS;
+ * count_packed_columns the count-kernel columns read as packed entries.     */
 typedef struct hpdex_plan_info {
     int64_t count_columns;
     int64_t count_nnz;
     int64_t split_columns;
     int64_t handed_over;
     int64_t count_blocks;
+    int64_t count_packed_columns;
 } hpdex_plan_info;
 int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len);
 /* Per-kernel timing: runs the plain launch sequence `iters` times with HIP
@@ -217,6 +219,9 @@ typedef struct hpdex_tuning {
                                   valued columns (0 = off: every column takes the
                                   split/reference/target kernels; else 16 KiB..152 KiB) */
     int32_t count_block;       /* threads per workgroup of that kernel: 256, 512, 1024 */
+    int32_t count_packed;      /* 1 = that kernel reads a matrix's packed entries (one
+                                  32-bit word per entry, made when the matrix is
+                                  created) where it has them; 0 = rows and values  */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

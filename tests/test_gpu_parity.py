@@ -188,7 +188,8 @@ def test_exact_path(mode, alt):
 def restore_tuning():
     t = _capi.get_tuning()
     yield
-    _capi.set_tuning(t.hist_max_bins, t.size_ordered, t.count_lds_bytes)
+    _capi.set_tuning(t.hist_max_bins, t.size_ordered, t.count_lds_bytes, t.count_block,
+                     t.count_packed)
 
 
 def _plan_of(w, mode, form=1, stats=True):
@@ -218,6 +219,45 @@ def test_count_kernel_lds_sizes(lds, restore_tuning):
             assert info["count_blocks"] > 16
         assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
         p.close(); m.close()
+
+
+def test_count_kernel_packed_entries(restore_tuning):
+    """The count kernel reading a column's packed entries (one word per entry:
+    row and value code) and reading its rows and values give the same results,
+    equal to the oracle's: explicit zeros, NaN and +-Inf (the skip code),
+    rows outside [0, R), at R = 300,000 (13 code bits), where a column of
+    8,190 bins is packed and one of 8,191 is not."""
+    rng = np.random.default_rng(17)
+    R, T = 300_000, 5
+    rows, ip = synthetic._bernoulli_rows(rng, R, np.array([0.03, 0.03, 0.05, 0.05, 0.2]))
+    rows = rows.copy()
+    d = rng.integers(1, 8190, len(rows)).astype(np.float32)
+    d[ip[0]], d[ip[0] + 1] = 1.0, 8190.0
+    d[ip[1]], d[ip[1] + 1] = 1.0, 8191.0
+    d[ip[2]:] = rng.negative_binomial(0.8, 0.3, len(d) - ip[2]) - 1
+    d[ip[2]:ip[3]:13] = np.nan
+    d[ip[3]:ip[4]:17] = np.inf
+    d[ip[3] + 5] = -np.inf
+    rows[ip[4] + 3] = -4
+    rows[ip[4] + 11] = R + 9
+    gid = rng.integers(-1, T + 1, R).astype(np.int32)
+    w = dict(data=d, indices=rows, indptr=ip, group_id=gid, n_targets=T, n_rows=R, n_cols=5)
+    for mode in ("min", "none", "mix"):
+        say(f"count kernel packed entries mode={mode}")
+        o = oracle_stats(w, mode)
+        res = {}
+        for packed in (1, 0):
+            # 152 KiB: the 8,190- and 8,191-bin columns fit (two target slices)
+            _capi.set_tuning(count_packed=packed, count_lds_bytes=152 * 1024)
+            m, p = _plan_of(w, mode)
+            p.run(2)
+            info = p.info()
+            assert info["count_columns"] == 5 and info["handed_over"] == 0
+            assert info["count_packed_columns"] == (4 if packed else 0)
+            res[packed] = p.results(with_stats=True)
+            p.close(); m.close()
+            assert_parity(res[packed], o)
+        assert all(np.array_equal(a, b) for a, b in zip(res[0], res[1]))
 
 
 def test_count_kernel_routes_wide_ranges_to_split():

@@ -182,10 +182,13 @@ def test_sim_histogram_path():
 # ---- the count kernel (integer-valued columns in one pass) ----
 
 def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist_max=8192,
-              expect_counted=None, block=512, expect_blocks=None):
+              expect_counted=None, block=512, expect_blocks=None, pack=True,
+              expect_packed=None):
     U1, P, TIE, n1, n2, err, status = sim.mwu(data, indices, indptr, gid, T, R, mode=ZH[mode],
                                               count_lds=count_lds, f64=f64, hist_max=hist_max,
-                                              block=block, timeout_s=120)
+                                              block=block, timeout_s=120, pack=pack)
+    if expect_packed is not None:
+        assert sim.mwu.last_packed == expect_packed
     assert err == 2 ** 64 - 1 and status == 0
     oU1, oP, oT, on1, on2 = oracle.mwu(data, indices, indptr, gid, T, n_rows=R,
                                        zero_handling=mode, use_continuity=True, with_stats=True)
@@ -199,13 +202,15 @@ def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist
         assert sim.mwu.last_blocks == expect_blocks
 
 
+@pytest.mark.parametrize("pack", [True, False])
 @pytest.mark.parametrize("block", [256, 512])
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
-def test_sim_count_kernel_modes(mode, block):
-    # small signed integers with explicit zeros: one workgroup per column
+def test_sim_count_kernel_modes(mode, block, pack):
+    # small signed integers with explicit zeros: one workgroup per column, read
+    # as packed entries (one word per entry) or as rows and values
     d, r, ip, g = _rw(21, 3000, 5, 7, 0.4, lambda rg, n: rg.integers(-4, 9, n))
     run_count(d, r, ip, g, 7, 3000, mode, 76 * 1024, expect_counted=5, expect_blocks=5,
-              block=block)
+              block=block, pack=pack, expect_packed=5 if pack else 0)
 
 
 @pytest.mark.parametrize("lds", [16 * 1024, 24 * 1024, 152 * 1024])
@@ -240,8 +245,31 @@ def test_sim_count_kernel_nonfinite_and_mixed_columns():
     d[c0][::7] = np.nan          # column 0: NaNs are skipped (zero block outside none)
     d[ip[1]:ip[2]][::11] = np.inf
     d[ip[2]] = 0.5               # column 2: one non-integer -> split path
+    d[ip[3] + 2] = -np.inf
     for mode in ("min", "none"):
-        run_count(d, r, ip, g, 4, 2500, mode, 76 * 1024, expect_counted=3)
+        for pack in (True, False):   # non-finite values take the skip code when packed
+            run_count(d, r, ip, g, 4, 2500, mode, 76 * 1024, expect_counted=3, pack=pack,
+                      expect_packed=3 if pack else 0)
+
+
+def test_sim_count_kernel_packed_rows_out_of_range_and_sizes():
+    # R = 300,000 leaves 13 code bits beside a row: a column of 8,190 bins is
+    # packed, one of 8,191 (count path, read as rows and values) is not; rows
+    # outside [0, R) are packed as R (no group)
+    rng = np.random.default_rng(37)
+    R, T = 300_000, 3
+    rows, ip = synthetic._bernoulli_rows(rng, R, np.array([0.03, 0.03, 0.01]))
+    rows = rows.copy()
+    d = rng.integers(1, 8190, len(rows)).astype(np.float32)
+    d[ip[0]], d[ip[0] + 1] = 1.0, 8190.0        # column 0: 8,190 bins
+    d[ip[1]], d[ip[1] + 1] = 1.0, 8191.0        # column 1: 8,191 bins
+    d[ip[2]:ip[3]] = rng.integers(0, 9, ip[3] - ip[2])
+    rows[ip[2] + 5] = -4
+    rows[ip[2] + 9] = R + 7
+    gid = rng.integers(-1, T + 1, R).astype(np.int32)
+    for mode in ("min", "none"):
+        run_count(d, rows, ip, gid, T, R, mode, 152 * 1024, expect_counted=3, expect_blocks=3,
+                  expect_packed=2)
 
 
 def test_sim_count_kernel_extreme_values():
