@@ -154,6 +154,8 @@ inline int __clz(unsigned x) { return x ? __builtin_clz(x) : 32; }
 inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
 inline long long __double_as_longlong(double d) { long long u; std::memcpy(&u, &d, 8); return u; }
 inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
+// low 32 bits of the product of the low 24 bits of each operand (v_mul_u32_u24)
+inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 inline double __longlong_as_double(long long u) { double d; std::memcpy(&d, &u, 8); return d; }
 using std::min;
 using std::max;

@@ -1153,12 +1153,20 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
         for (int u = 0; u < kUnroll; ++u)
             r[u] = buffer_load_u32(pk_rsrc, ((uint32_t)tid + (uint32_t)(u * NT)) * 4u, 0u);
     };
-    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll], auto packed_tag) {
+    // kClamp: rows may lie past R (unpacked rows of -1, masked lanes); a
+    // packed word of a whole batch holds a row of at most R.  kR24: R < 2^24,
+    // so the bit position takes the full-rate 24-bit multiply.
+    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll], auto packed_tag,
+                      auto clamp_tag, auto r24_tag) {
         constexpr bool kPk = decltype(packed_tag)::value;
+        constexpr bool kClamp = decltype(clamp_tag)::value;
+        constexpr bool kR24 = decltype(r24_tag)::value;
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             // the label's bit position; 8 bytes from its word hold all of it
-            const uint32_t pos = min(kPk ? r[u] >> pvb : r[u], R) * lbits;
+            uint32_t row = kPk ? r[u] >> pvb : r[u];
+            if (kClamp) row = min(row, R);
+            const uint32_t pos = kR24 ? __umul24(row, lbits) : row * lbits;
             const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
             g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
         }
@@ -1234,57 +1242,79 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
                 row = gg;
                 mine = gg < G;
             }
-            const uint32_t q = row * rs + (zero ? NB : b);
+            // row < 2^16 whenever the entry is counted (a garbage row only
+            // feeds a word that is not used)
+            const uint32_t q = __umul24(row, rs) + (zero ? NB : b);
             const uint32_t word = (mine && (zero || inb)) ? cell_off + (kWide ? q : (q >> 1)) : dummy;
             atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
         }
     };
-    auto read_all = [&](auto wide_tag, auto sliced_tag, auto packed_tag) {
+    auto read_all = [&](auto wide_tag, auto sliced_tag, auto packed_tag, auto r24_tag) {
         constexpr bool kPk = decltype(packed_tag)::value;
-        uint32_t r[kUnroll];
-        V v[kUnroll];
+        // two register sets: each batch is counted from the set it was loaded
+        // into while the next one loads into the other (no copies)
+        uint32_t rA[kUnroll], rB[kUnroll];
+        V vA[kUnroll], vB[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) { r[u] = r0[u]; v[u] = kPk ? V(0) : v0[u]; }
-        uint32_t ib = 0;
-        for (; ib + step < n; ib += step) {   // a whole batch with another after it
+        for (int u = 0; u < kUnroll; ++u) { rA[u] = r0[u]; vA[u] = kPk ? V(0) : v0[u]; }
+        // a whole batch with another after it
+        auto whole = [&](const uint32_t (&rc)[kUnroll], const V (&vc)[kUnroll],
+                         uint32_t (&rn)[kUnroll], V (&vn)[kUnroll], uint32_t ibn) {
             uint32_t g[kUnroll];
-            gather(r, g, packed_tag);
+            gather(rc, g, packed_tag, std::integral_constant<bool, !kPk>{}, r24_tag);
             // keep the gathers ahead of the next batch's loads (the counter
             // that waits for the gathers would otherwise wait for those too)
             __builtin_amdgcn_sched_barrier(0);
-            uint32_t r2[kUnroll];
-            V v2[kUnroll];
-            if (kPk) load_packed(ib + step, r2);
-            else load(ib + step, r2, v2);
+            if (kPk) load_packed(ibn, rn);
+            else load(ibn, rn, vn);
             __builtin_amdgcn_sched_barrier(0);
-            count(g, v, r, wide_tag, sliced_tag, packed_tag);
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = kPk ? V(0) : v2[u]; }
-        }
-        if (ib < n) {   // the last batch: entries past the column's end masked
-            // (a masked packed word reads as row R with the skip code)
+            count(g, vc, rc, wide_tag, sliced_tag, packed_tag);
+        };
+        // the last batch: entries past the column's end masked (a masked
+        // packed word reads as row R with the skip code)
+        auto last = [&](uint32_t (&rc)[kUnroll], V (&vc)[kUnroll], uint32_t ib) {
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const bool in = ib + (uint32_t)(u * NT + tid) < n;
-                r[u] = in ? r[u] : 0xFFFFFFFFu;
-                v[u] = in ? v[u] : V(0);
+                rc[u] = in ? rc[u] : 0xFFFFFFFFu;
+                vc[u] = in ? vc[u] : V(0);
             }
             uint32_t g[kUnroll];
-            gather(r, g, packed_tag);
-            count(g, v, r, wide_tag, sliced_tag, packed_tag);
+            gather(rc, g, packed_tag, std::true_type{}, r24_tag);
+            count(g, vc, rc, wide_tag, sliced_tag, packed_tag);
+        };
+        uint32_t ib = 0;
+        for (;;) {
+            if (ib + step >= n) {
+                if (ib < n) last(rA, vA, ib);
+                break;
+            }
+            whole(rA, vA, rB, vB, ib + step);
+            ib += step;
+            if (ib + step >= n) {
+                last(rB, vB, ib);
+                break;
+            }
+            whole(rB, vB, rA, vA, ib + step);
+            ib += step;
         }
     };
-    auto read_as = [&](auto packed_tag) {
+    auto read_as = [&](auto packed_tag, auto r24_tag) {
         if (sliced) {
-            if (wide) read_all(std::true_type{}, std::true_type{}, packed_tag);
-            else read_all(std::false_type{}, std::true_type{}, packed_tag);
+            if (wide) read_all(std::true_type{}, std::true_type{}, packed_tag, r24_tag);
+            else read_all(std::false_type{}, std::true_type{}, packed_tag, r24_tag);
         } else {
-            if (wide) read_all(std::true_type{}, std::false_type{}, packed_tag);
-            else read_all(std::false_type{}, std::false_type{}, packed_tag);
+            if (wide) read_all(std::true_type{}, std::false_type{}, packed_tag, r24_tag);
+            else read_all(std::false_type{}, std::false_type{}, packed_tag, r24_tag);
         }
     };
-    if (packed) read_as(std::true_type{});
-    else read_as(std::false_type{});
+    if (R < (1u << 24)) {
+        if (packed) read_as(std::true_type{}, std::true_type{});
+        else read_as(std::false_type{}, std::true_type{});
+    } else {
+        if (packed) read_as(std::true_type{}, std::false_type{});
+        else read_as(std::false_type{}, std::false_type{});
+    }
     if (__any(bad ? 1 : 0) && lane_id() == 0) atomicOr(&st.bad, 1u);
     if ((uint32_t)tid < rows) grs[tid] = gpre0;
     if ((uint32_t)tid + (uint32_t)nt < rows) grs[tid + nt] = gpre1;

@@ -260,6 +260,36 @@ def test_count_kernel_packed_entries(restore_tuning):
         assert all(np.array_equal(a, b) for a, b in zip(res[0], res[1]))
 
 
+def test_count_kernel_rows_beyond_24_bits():
+    """R = 2^24 + 100 rows: the label positions take the 32-bit multiply; 25
+    row bits leave 7 code bits, so a 100-bin column is packed and a 300-bin
+    column is read as rows and values; rows up to R - 1."""
+    say("count kernel rows beyond 2^24")
+    rng = np.random.default_rng(18)
+    R, T, n = (1 << 24) + 100, 4, 6000
+    rows0 = np.sort(rng.choice(R, n, replace=False))
+    rows0[-3:] = [R - 3, R - 2, R - 1]
+    rows1 = np.sort(rng.choice(R, n, replace=False))
+    rows = np.concatenate([rows0, rows1]).astype(np.int64)
+    ip = np.array([0, n, 2 * n], dtype=np.int64)
+    d = np.concatenate([rng.integers(1, 101, n), rng.integers(1, 301, n)]).astype(np.float32)
+    d[0], d[1] = 1.0, 100.0
+    d[n], d[n + 1] = 1.0, 300.0
+    gid = np.full(R, -1, np.int32)
+    pick = rng.choice(R, 400_000, replace=False)
+    gid[pick] = rng.integers(0, T + 1, len(pick))
+    gid[rows] = rng.integers(0, T + 1, len(rows))
+    w = dict(data=d, indices=rows, indptr=ip, group_id=gid, n_targets=T, n_rows=R, n_cols=2)
+    for mode in ("min", "none"):
+        m, p = _plan_of(w, mode)
+        p.run(1)
+        info = p.info()
+        assert info["count_columns"] == 2 and info["count_packed_columns"] == 1
+        assert info["handed_over"] == 0
+        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+        p.close(); m.close()
+
+
 def test_count_kernel_routes_wide_ranges_to_split():
     """Integer columns whose value range exceeds the histogram limit go to the
     split path when the plan is made; results are unchanged."""

@@ -272,6 +272,30 @@ def test_sim_count_kernel_packed_rows_out_of_range_and_sizes():
                   expect_packed=2)
 
 
+def test_sim_count_kernel_rows_beyond_24_bits():
+    # R = 2^24 + 100: label positions take the 32-bit multiply; 25 row bits
+    # leave 7 code bits, so the 100-bin column is packed and the 300-bin one
+    # is read as rows and values; rows near R
+    rng = np.random.default_rng(38)
+    R, T = (1 << 24) + 100, 4
+    n = 6000
+    rows0 = np.sort(rng.choice(R, n, replace=False))
+    rows0[-3:] = [R - 3, R - 2, R - 1]
+    rows1 = np.sort(rng.choice(R, n, replace=False))
+    rows = np.concatenate([rows0, rows1]).astype(np.int64)
+    ip = np.array([0, n, 2 * n], dtype=np.int64)
+    d = np.concatenate([rng.integers(1, 101, n), rng.integers(1, 301, n)]).astype(np.float32)
+    d[0], d[1] = 1.0, 100.0
+    d[n], d[n + 1] = 1.0, 300.0
+    gid = np.full(R, -1, np.int32)
+    pick = rng.choice(R, 400_000, replace=False)
+    gid[pick] = rng.integers(0, T + 1, len(pick))
+    gid[rows] = rng.integers(0, T + 1, len(rows))
+    for mode in ("min", "none"):
+        run_count(d, rows, ip, gid, T, R, mode, 76 * 1024, expect_counted=2, expect_blocks=2,
+                  expect_packed=1)
+
+
 def test_sim_count_kernel_extreme_values():
     # values at the ends of the 16-bit range the column metadata admits:
     # -32768 .. -32700 in column 0, 32700 .. 32767 in column 1, both ends
