@@ -1362,14 +1362,16 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     }
     const uint32_t gr0 = grs[0];
     const uint32_t za = (mode == kZeroNone) ? cez[0] : (gr0 > na ? gr0 - na : 0u);
-    // bins[b] = (reference values below b, plus the zero block when it lies
-    // below b; reference values in b)
+    // bins[b] = (2*lt' + eq, eq): lt' = reference values below b, plus the
+    // zero block when it lies below b; eq = reference values in b.  lt' + eq
+    // counts distinct reference rows, at most R < 2^31, so 2*lt' + eq fits
+    // 32 bits.
     {
         uint32_t lt = below + incl - sh;
         for (uint32_t b = b0; b < b1; ++b) {
             const uint32_t h = pc.get(0u, b);
             const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
-            bins[b] = make_uint2(lt + (zbelow ? za : 0u), h);
+            bins[b] = make_uint2(2u * (lt + (zbelow ? za : 0u)) + h, h);
             if ((int32_t)b + vmin == 0) st.scan[16] = lt;   // value 0's bin: values below 0
             lt += h;
         }
@@ -1377,26 +1379,37 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     __syncthreads();
 
     // per-target sums over the bins, one thread per target row, no branches
-    // per bin (a count of 0 adds 0 to each sum).  With x = eq + c,
-    //   2*U part: c * (2*lt' + eq)
-    //   tie part: f(x) - f(eq) = c * (x*(x + eq) + eq^2 - 1)
-    // exact modulo 2^64 like any other order of the same integer additions.
+    // per bin (a count of 0 adds 0 to each sum).  With c the row's count in
+    // bin b, a = 2*lt' + eq and t = eq + c:
+    //   2*U part: sum_b c * a
+    //   tie part: sum_b f(t) - f(eq) = sum_b t^3 - (na + k) - ta
+    // for f(t) = t^3 - t, since sum_b t = na + k (k = the row's count) and
+    // sum_b f(eq) = ta, the reference's own tie sum over every bin; exact
+    // modulo 2^64 like any other order of the same integer additions.  A
+    // column of fewer than 2^16 entries has t < 2^16, so t^2 takes the
+    // full-rate 24-bit multiply.
     for (uint32_t x = 1u + tid; x < rows; x += nt) {
-        uint64_t Slt = 0, Seq = 0, Tx = 0;
+        uint64_t S2 = 0, T3 = 0;
         uint32_t k = 0;
-        auto bin = [&](uint32_t d, uint32_t cnt) {
-            const uint2 be = bins[d];
-            Slt += (uint64_t)cnt * be.x;
-            Seq += (uint64_t)cnt * be.y;
-            const uint32_t xx = be.y + cnt;
-            const uint64_t P = (uint64_t)xx * (xx + be.y) + ((uint64_t)be.y * be.y - 1ull);
-            Tx += (uint64_t)cnt * P;
-            k += cnt;
-        };
         if (wide) {
             const uint32_t* row = pc.w + x * rs;
-            for (uint32_t d = 0; d < NB; ++d) bin(d, row[d]);
+            for (uint32_t d = 0; d < NB; ++d) {
+                const uint32_t cnt = row[d];
+                const uint2 be = bins[d];
+                S2 += (uint64_t)cnt * be.x;
+                const uint32_t t = be.y + cnt;
+                const uint64_t t2 = (uint64_t)t * t;
+                T3 += t2 * t;
+                k += cnt;
+            }
         } else {
+            auto bin = [&](uint32_t d, uint32_t cnt) {
+                const uint2 be = bins[d];
+                S2 += (uint64_t)cnt * be.x;
+                const uint32_t t = be.y + cnt;
+                T3 += (uint64_t)(uint32_t)__umul24(t, t) * t;
+                k += cnt;
+            };
             // rows start on an even counter: two counters per word
             const uint32_t* row = pc.w + ((x * rs) >> 1);
             uint32_t d = 0;
@@ -1413,8 +1426,8 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
                 if (d + 1u < NB) bin(d + 1u, cw >> 16);
             }
         }
-        acc[2u * x] = 2ull * Slt + Seq;
-        acc[2u * x + 1u] = Tx;
+        acc[2u * x] = S2;
+        acc[2u * x + 1u] = T3 - (uint64_t)(na + k) - ta;
         cnz[x] = k;
     }
     if (tid == 0) cnz[0] = na;
