@@ -103,6 +103,45 @@ inline unsigned long long __ballot(int p) {
     return m;
 }
 inline int __all(int p) { return __ballot(p) == ~0ull; }
+// v_readlane: lane l's value in every lane (l uniform)
+inline int __builtin_amdgcn_readlane(int v, int l) {
+    sim::t_last_op = "readlane";
+    return sim::exchange(v, (unsigned)l);
+}
+// DPP moves (gfx9 controls): row_shr:n (0x111..0x11F) within rows of 16,
+// wave_shr:1 (0x138), row_bcast:15 (0x142: lane 15 of the row before) and
+// row_bcast:31 (0x143: lane 31, to rows 2 and 3).  A lane of a row outside
+// row_mask keeps `old`; a lane whose source is invalid reads 0 with
+// bound_ctrl, else keeps `old`.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask,
+                                       bool bound_ctrl) {
+    sim::t_last_op = "dpp";
+    const unsigned l = sim::lane(), row = l / 16u, c = l % 16u;
+    bool valid;
+    unsigned from;
+    if (ctrl >= 0x111 && ctrl <= 0x11F) {
+        const unsigned n = (unsigned)ctrl - 0x110u;
+        valid = c >= n;
+        from = l - n;
+    } else if (ctrl == 0x138) {
+        valid = l >= 1u;
+        from = l - 1u;
+    } else if (ctrl == 0x142) {
+        valid = row >= 1u;
+        from = 16u * row - 1u;
+    } else if (ctrl == 0x143) {
+        valid = row >= 2u;
+        from = 31u;
+    } else {
+        std::fprintf(stderr, "sim: unsupported dpp control %#x\n", ctrl);
+        std::abort();
+    }
+    const int r = sim::exchange(src, valid ? from : l);
+    const bool on = ((row_mask >> row) & 1) && ((bank_mask >> (c / 4u)) & 1);
+    if (!on) return old;
+    if (!valid) return bound_ctrl ? 0 : old;
+    return r;
+}
 inline int __any(int p) { return __ballot(p) != 0ull; }
 template <typename T> inline T sim_plain_load(const T* p) { return *p; }
 // Buffer loads: a resource is a base address and a size in bytes; a load

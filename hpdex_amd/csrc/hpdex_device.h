@@ -128,6 +128,56 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 // inclusive prefix sum across the wave
+// Wave-wide inclusive scans by DPP moves: row shifts by 1, 2, 4 and 8 inside
+// rows of 16 lanes, then lane 15 of a row into the next row (rows 1 and 3)
+// and lane 31 into rows 2 and 3 -- register-to-register moves with no LDS
+// round trip.  `op` combines a lane's value with one from a lower lane; a
+// lane with no source combines with 0 (the identity of add, or and unsigned
+// max).  Every lane of the wave must be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xF, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    return ((uint64_t)dpp_u32<kCtrl, kRowMask>((uint32_t)(v >> 32)) << 32) |
+           dpp_u32<kCtrl, kRowMask>((uint32_t)v);
+}
+template <int kCtrl, int kRowMask, typename T>
+__device__ __forceinline__ T dpp_t(T v) {
+    if constexpr (sizeof(T) == 8) return dpp_u64<kCtrl, kRowMask>(v);
+    else return dpp_u32<kCtrl, kRowMask>(v);
+}
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_incl_scan_dpp(T v, Op op) {
+    v = op(v, dpp_t<0x111, 0xF>(v));   // row_shr:1
+    v = op(v, dpp_t<0x112, 0xF>(v));   // row_shr:2
+    v = op(v, dpp_t<0x114, 0xF>(v));   // row_shr:4
+    v = op(v, dpp_t<0x118, 0xF>(v));   // row_shr:8
+    v = op(v, dpp_t<0x142, 0xA>(v));   // row_bcast:15 into rows 1 and 3
+    v = op(v, dpp_t<0x143, 0xC>(v));   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+// Lane l - 1's value in lane l (lane 0 reads 0): wave_shr:1.
+__device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
+}
+// A lane's value read by every lane (l uniform).
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t l) {
+    return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
+}
+// Index of the k-th (1-based) set bit of m; 63 when m has fewer than k.
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t s = 32; s >= 1; s >>= 1)   // pos + s stays below 64
+        if ((uint32_t)__builtin_popcountll(m & ((1ull << (pos + s)) - 1ull)) < k) pos += s;
+    return pos;
+}
+
 __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
     const int l = lane_id();
 #pragma unroll

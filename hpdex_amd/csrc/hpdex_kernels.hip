@@ -782,19 +782,20 @@ __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResu
 // A window of consecutive targets whose keys total at most 64 is ranked in one
 // pass of the wave: lane e finds the target of key e of the window (the last
 // one starting at or before e in the window's packed order) and loads it from
-// that target's keys; a bitonic sort of
-// (target, key) orders every target's keys; runs of equal (target, key) give
-// each key its position in its run of equal values (the tie term); inclusive
-// scans of the per-key terms, read at each target's first and last position,
-// give every target's sums.  A target with more than 64 keys alone goes
-// through process_target.  Each lane returns its own target's result.
+// that target's keys; a key's position among the equal keys of its target
+// comes from comparing it with the keys of the lanes before it in its
+// segment, so nothing is sorted; inclusive scans of the per-key terms, read at
+// each target's first and last position, give every target's sums.  Every
+// cross-lane step but the key offset and those reads is a DPP move.  A target
+// with more than 64 keys alone goes through process_target.  Each lane
+// returns its own target's result.
 template <typename V>
 __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t c, uint32_t g0,
                                                        uint32_t ng, const ColPlan& pl,
                                                        const RefView<typename KeyOf<V>::K>& ref,
                                                        const typename KeyOf<V>::K* colkeys) {
     using K = typename KeyOf<V>::K;
-    static_assert(sizeof(K) == 4, "target windows pack (target, key) into 64 bits");
+    static_assert(sizeof(K) == 4, "target windows compare 32-bit keys across lanes");
     const uint32_t lane = (uint32_t)lane_id();
     const int64_t b = c * J.G;
     const uint32_t gj = g0 + lane;
@@ -818,7 +819,7 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
     TargetResult mine{};
     uint32_t j0 = 0;
     while (j0 < ng) {
-        const uint32_t base = j0 ? shfl_u32(inc, (int)j0 - 1) : 0u;
+        const uint32_t base = j0 ? readlane_u32(inc, j0 - 1u) : 0u;
         const uint64_t fit = __ballot(lane >= j0 && own && inc - base <= (uint32_t)kWave);
         const uint32_t j1 = j0 + (uint32_t)__builtin_popcountll(fit);
         if (j1 == j0) {   // one target of more than 64 keys
@@ -827,50 +828,56 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
             ++j0;
             continue;
         }
-        const uint32_t m = shfl_u32(inc, (int)j1 - 1) - base;   // keys in the window
+        const uint32_t m = readlane_u32(inc, j1 - 1u) - base;   // keys in the window
         const bool valid = lane < m;
-        // owner: the last target j in [j0, j1) whose first key is at or before
-        // this lane (empty targets share the next one's start and lose)
+        // target j's keys take positions [excl_j, excl_j + sz_j) of the window
         const uint32_t excl = inc - sz - base;
-        uint32_t lo = j0, hi = j1;
-#pragma unroll
-        for (int it = 0; it < 6; ++it) {
-            const uint32_t mid = (lo + hi) >> 1;
-            const uint32_t e = shfl_u32(excl, (int)mid);
-            if (hi - lo > 1u) {
-                if (e <= lane) lo = mid;
-                else hi = mid;
-            }
-        }
-        // the lane's key: position lane - excl(owner) of the owner's keys (a
-        // group's keys need not end where the next group's start)
-        const uint32_t at = shfl_u32(off, (int)lo) + lane - shfl_u32(excl, (int)lo);
-        const K key = valid ? colkeys[at] : K(0);
-        uint64_t comp[1] = {valid ? (((uint64_t)lo << 32) | (uint64_t)key) : ~0ull};
-        wave_sort<1, uint64_t>(comp);
-        const uint64_t prev = shfl_up_u64(comp[0], 1);
-        const bool head = valid && (lane == 0 || prev != comp[0]);
-        const uint64_t hm = __ballot(head);
+        const bool starts = lane >= j0 && lane < j1 && sz > 0u;
+        // the window's segment starts as bits (an OR over the wave) and the
+        // targets that start them; a key's target is the k-th of those, k =
+        // the starts at or before the key
+        const uint64_t smask = readlane_u64(
+            wave_incl_scan_dpp(starts ? (1ull << excl) : 0ull,
+                               [](uint64_t a, uint64_t x) { return a | x; }),
+            (uint32_t)kWave - 1u);
+        const uint64_t tmask = __ballot(starts);
         const uint64_t upto = (lane == (uint32_t)kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
-        const uint64_t below = hm & upto;
+        const uint64_t sm = smask & upto;
+        const uint32_t first = sm ? (uint32_t)(63 - __clzll(sm)) : 0u;   // its first key
+        const uint32_t owner = nth_set_bit(tmask, (uint32_t)__builtin_popcountll(sm));
+        // the lane's key: position lane - first of the owner's keys (a
+        // group's keys need not end where the next group's start)
+        const uint32_t at = shfl_u32(off, (int)owner) + lane - first;
+        const K key = valid ? colkeys[at] : K(0);
+        // the key's position among the equal keys of its target at lower
+        // lanes, plus one (any order of equal keys gives the same sums): the
+        // keys of lanes 1 .. r back, r = its rank in its segment, compared
+        // through a chain of one-lane shifts as long as the longest segment
+        const uint32_t r = valid ? lane - first : 0u;
+        const uint32_t rmax = readlane_u32(
+            wave_incl_scan_dpp(r, [](uint32_t a, uint32_t x) { return a > x ? a : x; }),
+            (uint32_t)kWave - 1u);
+        uint32_t jj = 1u;
+        uint32_t kd = (uint32_t)key;
+        for (uint32_t d = 1; d <= rmax; ++d) {
+            kd = wave_shr1_u32(kd);
+            jj += (d <= r && kd == (uint32_t)key) ? 1u : 0u;
+        }
         uint64_t s_e = 0, t_e = 0;
         if (valid) {
-            const K v = (K)comp[0];
-            const uint32_t h = (uint32_t)(63 - __clzll(below));
             uint32_t lt, eq;
-            t.ref.lookup(v, lt, eq);
-            const uint64_t jj = (uint64_t)(lane - h + 1u);
-            const uint64_t ltz = (uint64_t)lt + (zero_below<V>(t, v) ? za : 0u);
+            t.ref.lookup(key, lt, eq);
+            const uint64_t ltz = (uint64_t)lt + (zero_below<V>(t, key) ? za : 0u);
             s_e = 2ull * ltz + eq;
             const uint64_t x = (uint64_t)eq + jj;
             t_e = 3ull * x * (x - 1ull);
         }
-        const uint64_t ps = wave_incl_scan_u64(s_e), pt = wave_incl_scan_u64(t_e);
-        // target j's keys sit at [excl_j, excl_j + sz_j) of the sorted window
+        const auto add = [](uint64_t a, uint64_t x) { return a + x; };
+        const uint64_t ps = wave_incl_scan_dpp(s_e, add), pt = wave_incl_scan_dpp(t_e, add);
         const uint32_t last = excl + sz - 1u, before = excl - 1u;
         const uint64_t ps_last = shfl_u64(ps, (int)(last & 63u)), pt_last = shfl_u64(pt, (int)(last & 63u));
         const uint64_t ps_bef = shfl_u64(ps, (int)(before & 63u)), pt_bef = shfl_u64(pt, (int)(before & 63u));
-        if (lane >= j0 && lane < j1 && sz > 0u) {
+        if (starts) {
             S = ps_last - (excl ? ps_bef : 0ull);
             TT = pt_last - (excl ? pt_bef : 0ull);
         }
