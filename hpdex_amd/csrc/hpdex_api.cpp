@@ -171,7 +171,7 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
             any = true;
         }
     }
-    if (any && m->nnz) {
+    if (any && m->nnz && m->pack) {
         DevBuf d_lo;
         HIPCHK(d_lo.alloc(C * 4));
         HIPCHK(hipMemcpyAsync(d_lo.p, lo.data(), C * 4, hipMemcpyHostToDevice, s));
@@ -729,6 +729,8 @@ int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
+    m.pack = false;   // used once: no packed entries
+
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
     JobState js;
@@ -773,6 +775,8 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
+    m.pack = false;   // used once: no packed entries
+
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
     std::vector<uint32_t> grows(n_groups, 0u);
@@ -884,6 +888,8 @@ int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int6
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
+    m.pack = false;   // used once: no packed entries
+
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
     return de_run(m, group_id, n_targets, tie_correction, use_continuity, clip_value, U1_out, P_out,

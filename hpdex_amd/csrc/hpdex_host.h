@@ -118,6 +118,8 @@ struct hpdex_matrix {
     std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
                                     // non-finite) -- the column may go to the count kernel
     std::vector<int32_t> h_range;   // host copy of colrange
+    bool pack = true;               // make packed entries (a matrix that is queried more
+                                    // than once: they cost one pass over the entries)
     hpdex_host::DevBuf packed;      // packed entries (pack_row_bits) of the packable columns
     uint32_t pack_vb = 0;           // their value bits
     std::vector<uint8_t> col_packed;   // 1: column c has packed entries

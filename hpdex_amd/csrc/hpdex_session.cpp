@@ -117,7 +117,6 @@ size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows
     if (!m) {   // upload_matrix_from
         f += up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
         f += up256(2 * C * 4) + up256(C);
-        f += up256(nnz * 4) + up256(C * 4);   // packed entries and their columns' bases
         const size_t ib = j.index_bits == 32 ? 4 : 8;
         if (j.layout == HPDEX_LAYOUT_CSR) {
             // raw values, column indices, row pointers, transpose workspace
@@ -192,6 +191,7 @@ void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
         do {
             if (!m) {
                 own.reset(new hpdex_matrix());
+                own->pack = false;   // the job's own matrix serves this job only
                 const hpdex_matrix_src src = source_of(j);
                 rc = upload_matrix_from(own.get(), &src, s, err, err_len);
                 if (rc) break;

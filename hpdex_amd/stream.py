@@ -33,9 +33,10 @@ logger = logging.getLogger(__name__)
 COLUMNS = ["target", "feature", "p_value", "u_statistic", "fold_change", "log2_fold_change", "fdr"]
 
 # Bytes one stored value costs while its run is in flight: the host copy of
-# its value and int64 row index, and on the device its value, int32 row,
-# packed entry, the int64 staging of the upload and a sort-path key.
-BYTES_PER_VALUE = 44
+# its value and int64 row index, and on the device its value, int32 row, the
+# int64 staging of the upload and a sort-path key (a run's matrix is used
+# once, so it gets no packed entries).
+BYTES_PER_VALUE = 40
 # Bytes per (gene, target) pair and per (gene, group) in flight: five float64
 # outputs on the host, their device buffers and sort-path counters, group means.
 BYTES_PER_PAIR = 96
