@@ -162,6 +162,9 @@ __device__ __forceinline__ T wave_incl_scan_dpp(T v, Op op) {
 __device__ __forceinline__ uint32_t wave_shr1_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
 }
+__device__ __forceinline__ uint64_t dpp_shr1_u64(uint64_t v) {
+    return ((uint64_t)wave_shr1_u32((uint32_t)(v >> 32)) << 32) | wave_shr1_u32((uint32_t)v);
+}
 // A lane's value read by every lane (l uniform).
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);

@@ -808,7 +808,7 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
     const uint32_t nza = J.cnt[b];
     const uint32_t za = (J.mode == kZeroNone) ? J.zcnt[b]
                                              : (J.group_rows[0] > nza ? J.group_rows[0] - nza : 0u);
-    const uint32_t inc = wave_incl_scan_u32(sz);
+    const uint32_t inc = wave_incl_scan_dpp(sz, [](uint32_t a, uint32_t x) { return a + x; });
     TargetCtx<K> t;
     t.ref = ref;
     t.mode = J.mode;
@@ -874,12 +874,18 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         }
         const auto add = [](uint64_t a, uint64_t x) { return a + x; };
         const uint64_t ps = wave_incl_scan_dpp(s_e, add), pt = wave_incl_scan_dpp(t_e, add);
-        const uint32_t last = excl + sz - 1u, before = excl - 1u;
-        const uint64_t ps_last = shfl_u64(ps, (int)(last & 63u)), pt_last = shfl_u64(pt, (int)(last & 63u));
-        const uint64_t ps_bef = shfl_u64(ps, (int)(before & 63u)), pt_bef = shfl_u64(pt, (int)(before & 63u));
+        // each target's scans at its last key (an empty target ends where
+        // the one before it ends; none before the window's first key), less
+        // those of the target before it, one lane down
+        // (every lane takes part in the shuffles; the select comes after)
+        const uint32_t end = excl + sz;
+        const uint64_t ps_at = shfl_u64(ps, (int)((end - 1u) & 63u));
+        const uint64_t pt_at = shfl_u64(pt, (int)((end - 1u) & 63u));
+        const uint64_t ps_end = end ? ps_at : 0ull, pt_end = end ? pt_at : 0ull;
+        const uint64_t ps_prev = dpp_shr1_u64(ps_end), pt_prev = dpp_shr1_u64(pt_end);
         if (starts) {
-            S = ps_last - (excl ? ps_bef : 0ull);
-            TT = pt_last - (excl ? pt_bef : 0ull);
+            S = ps_end - (lane == j0 ? 0ull : ps_prev);
+            TT = pt_end - (lane == j0 ? 0ull : pt_prev);
         }
         j0 = j1;
     }
