@@ -103,10 +103,12 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
         }
     }
     std::vector<uint32_t> packed((size_t)J.indptr[C] + 1, 0xDEADBEEFu);
+    std::vector<uint32_t> dense(C, 0), zneg(C, 0);
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { pack_columns_kernel<V>((const V*)J.vals, J.rows, J.indptr,
-                                                          lo.data(), (uint32_t)J.R, vb,
-                                                          packed.data()); }, timeout_s);
+                                                          lo.data(), range, (uint32_t)J.R, vb,
+                                                          packed.data(), dense.data(),
+                                                          zneg.data()); }, timeout_s);
     J.packed = packed.data();
     J.pack_vb = vb;
     // test hook: ranges one value short of the data, so that the count kernel
@@ -122,7 +124,9 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     uint32_t n_static = 0;
     unsigned nc = 0;   // counted columns
     for (int32_t c : order) {
-        const int64_t bins = (int64_t)range[2 * c + 1] - range[2 * c] + 1;
+        // a packed column's bins are its distinct values; any other's its range
+        const bool pk = sim_pack && col_packed[c];
+        const int64_t bins = pk ? (int64_t)dense[c] : (int64_t)range[2 * c + 1] - range[2 * c] + 1;
         const int64_t len = J.indptr[c + 1] - J.indptr[c];
         const uint32_t S = (count_ok && col_int[c] && bins <= J.hist_max && len < ((int64_t)1 << 28))
                                ? count_slice_size(T, (uint32_t)(bins > 0 ? bins : 1), len, count_lds)
@@ -134,11 +138,11 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
         CountItem it{};
         it.p0 = J.indptr[c];
         it.n = (uint32_t)len;
-        it.vmin = bins > 0 ? range[2 * c] : 0;
+        it.vmin = pk ? (int32_t)zneg[c] : (bins > 0 ? range[2 * c] : 0);
         it.nb = bins > 0 ? (uint32_t)bins : 1u;
         it.col = c;
         it.S = (uint16_t)S;
-        it.flags = (sim_pack && col_packed[c]) ? kCountPacked : 0u;
+        it.flags = pk ? kCountPacked : 0u;
         for (uint32_t t0 = 0; t0 < T; t0 += S) {
             it.t0 = (uint16_t)t0;
             items.push_back(it);

@@ -172,14 +172,22 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
         }
     }
     if (any && m->nnz && m->pack) {
-        DevBuf d_lo;
+        DevBuf d_lo, d_dense, d_zneg;
         HIPCHK(d_lo.alloc(C * 4));
+        HIPCHK(d_dense.alloc(C * 4));
+        HIPCHK(d_zneg.alloc(C * 4));
         HIPCHK(hipMemcpyAsync(d_lo.p, lo.data(), C * 4, hipMemcpyHostToDevice, s));
         HIPCHK(m->packed.alloc(m->nnz * 4));
         HIPCHK(launch_pack_columns(m->vals.p, m->f64, m->rows.as<int32_t>(),
                                    m->indptr.as<int64_t>(), (int64_t)C, d_lo.as<int32_t>(),
-                                   (uint32_t)m->R, m->pack_vb, m->packed.as<uint32_t>(), s));
-        HIPCHK(hipStreamSynchronize(s));   // d_lo is freed on return
+                                   m->colrange.as<int32_t>(), (uint32_t)m->R, m->pack_vb,
+                                   m->packed.as<uint32_t>(), d_dense.as<uint32_t>(),
+                                   d_zneg.as<uint32_t>(), s));
+        m->h_dense.assign(C, 0);
+        m->h_zneg.assign(C, 0);
+        HIPCHK(hipMemcpyAsync(m->h_dense.data(), d_dense.p, C * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(m->h_zneg.data(), d_zneg.p, C * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));   // the staging buffers are freed on return
     } else {
         m->col_packed.assign(C, 0);
     }
@@ -423,7 +431,10 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     for (size_t k = 0; k < m.C; ++k) {
         const int32_t c = size_ordered ? m.order[k] : (int32_t)k;
         const int64_t len = m.h_indptr[c + 1] - m.h_indptr[c];
-        const int64_t bins = (int64_t)m.h_range[2 * c + 1] - m.h_range[2 * c] + 1;
+        // a packed column's bins are its distinct values; any other's its range
+        const bool packed = tu.count_packed && m.col_packed[c];
+        const int64_t bins = packed ? (int64_t)m.h_dense[c]
+                                    : (int64_t)m.h_range[2 * c + 1] - m.h_range[2 * c] + 1;
         const uint32_t S =
             (count_ok && m.col_int[c] && bins <= tu.hist_max && len < ((int64_t)1 << 28))
                 ? count_slice_size(T, (uint32_t)(bins > 0 ? bins : 1), len, (uint32_t)tu.count_lds)
@@ -435,11 +446,11 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
         CountItem it{};
         it.p0 = m.h_indptr[c];
         it.n = (uint32_t)len;
-        it.vmin = bins > 0 ? m.h_range[2 * c] : 0;
+        it.vmin = packed ? (int32_t)m.h_zneg[c] : (bins > 0 ? m.h_range[2 * c] : 0);
         it.nb = bins > 0 ? (uint32_t)bins : 1u;
         it.col = c;
         it.S = (uint16_t)S;
-        it.flags = (tu.count_packed && m.col_packed[c]) ? kCountPacked : 0u;
+        it.flags = packed ? kCountPacked : 0u;
         for (uint32_t t0 = 0; t0 < T; t0 += S) {
             it.t0 = (uint16_t)t0;
             count_items.push_back(it);

@@ -123,6 +123,8 @@ struct hpdex_matrix {
     hpdex_host::DevBuf packed;      // packed entries (pack_row_bits) of the packable columns
     uint32_t pack_vb = 0;           // their value bits
     std::vector<uint8_t> col_packed;   // 1: column c has packed entries
+    std::vector<uint32_t> h_dense;     // packed columns: distinct nonzero values (their bins)
+    std::vector<uint32_t> h_zneg;      // packed columns: distinct values below zero
 };
 
 namespace hpdex_host {

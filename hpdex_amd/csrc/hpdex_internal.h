@@ -103,19 +103,24 @@ struct WorkItem {
 struct CountItem {
     int64_t p0;
     uint32_t n;
-    int32_t vmin;   // smallest stored nonzero value (0 when there is none)
-    uint32_t nb;    // vmax - vmin + 1 (1 when there is no nonzero value)
+    int32_t vmin;   // smallest stored nonzero value (0 when there is none); for a packed
+                    // item, its distinct values below zero (bins below the zero block)
+    uint32_t nb;    // vmax - vmin + 1 (1 when there is no nonzero value); for a packed item,
+                    // its distinct nonzero values (at least 1)
     int32_t col;
     uint16_t t0;
     uint16_t S;
     uint32_t flags;   // kCountPacked: read the column's packed entries
 };
 constexpr uint32_t kCountPacked = 1u;
+// words of the pack kernel's presence bitmap: every 16-bit integer value
+constexpr uint32_t kPackBitmapWords = 2048u;
 
 // Packed entries of a resident matrix.  A column whose finite nonzero values
 // are all integers of one range [lo, lo + bins) (the columns the count kernel
 // takes) keeps each entry also as one 32-bit word (row << vb) | code, with
-// code = value - lo, pack_zero_code for an explicit zero and pack_skip_code
+// code = the value's rank among the column's distinct nonzero values (the
+// count kernel's bin), pack_zero_code for an explicit zero and pack_skip_code
 // for a non-finite value, and a row outside [0, R) stored as R.  The count
 // kernel then reads one word per entry instead of a row and a value.  The row
 // takes rb = pack_row_bits(R) bits (R itself included), the code vb = 32 - rb.
@@ -281,8 +286,9 @@ hipError_t prepare_count_kernel(uint32_t bytes);
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
                                    uint8_t* ok, int32_t* range, hipStream_t s);
 hipError_t launch_pack_columns(const void* vals, bool f64, const int32_t* rows,
-                               const int64_t* indptr, int64_t C, const int32_t* lo, uint32_t R,
-                               uint32_t vb, uint32_t* out, hipStream_t s);
+                               const int64_t* indptr, int64_t C, const int32_t* lo,
+                               const int32_t* range, uint32_t R, uint32_t vb, uint32_t* out,
+                               uint32_t* dense, uint32_t* zneg, hipStream_t s);
 hipError_t launch_init_job_state(unsigned long long* err_key, uint32_t* status,
                                  unsigned long long* status_col, hipStream_t s);
 hipError_t launch_exact_tables(const ExactTable* tabs, int n_tabs, double* tabbuf,

@@ -1383,9 +1383,13 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
         uint32_t lt = below + incl - sh;
         for (uint32_t b = b0; b < b1; ++b) {
             const uint32_t h = pc.get(0u, b);
-            const bool zbelow = mode == kZeroMin || (mode != kZeroMax && (int32_t)b + vmin > 0);
+            // the first bin of a positive value (a packed column's bins are
+            // ranks: vmin holds its distinct negative values)
+            const bool pos = packed ? b >= (uint32_t)vmin : (int32_t)b + vmin > 0;
+            const bool zbelow = mode == kZeroMin || (mode != kZeroMax && pos);
             bins[b] = make_uint2(2u * (lt + (zbelow ? za : 0u)) + h, h);
-            if ((int32_t)b + vmin == 0) st.scan[16] = lt;   // value 0's bin: values below 0
+            // where 0 would sit: the reference values below it
+            if (packed ? b == (uint32_t)vmin : (int32_t)b + vmin == 0) st.scan[16] = lt;
             lt += h;
         }
     }
@@ -1450,6 +1454,7 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     uint32_t ltz0;
     if (mode == kZeroMin) ltz0 = 0u;
     else if (mode == kZeroMax) ltz0 = na;
+    else if (packed) ltz0 = vmin == 0 ? 0u : ((uint32_t)vmin < NB ? st.scan[16] : na);
     else ltz0 = vmin >= 0 ? 0u : ((uint32_t)(-vmin) < NB ? st.scan[16] : na);
 
     // per row: errors of this column (mannwhitneyu.cpp:936-1038); per target
@@ -1525,25 +1530,60 @@ __global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
 
 // Packed entries (pack_row_bits) of the columns with lo[c] != INT_MIN, one
 // workgroup per column.  Classification found every finite nonzero value of
-// such a column an integer in [lo, lo + bins), and the host checked that the
-// bins fit the code bits.
+// such a column an integer in [lo, hi] = range[2c .. 2c+1], and the host
+// checked that hi - lo + 1 leaves the two special codes free.  The code of a
+// value is its rank among the column's distinct nonzero values, so the count
+// kernel's bins are only the values that occur: a first pass marks them in an
+// LDS bitmap (at most 2^16 values: 2,048 words), a prefix of the word
+// popcounts gives the ranks, a second pass writes the words.  dense[c] = the
+// distinct values, zneg[c] = those below zero.
 template <typename V>
 __global__ __launch_bounds__(256) void pack_columns_kernel(const V* vals, const int32_t* rows,
                                                            const int64_t* indptr,
-                                                           const int32_t* lo, uint32_t R,
-                                                           uint32_t vb, uint32_t* out) {
+                                                           const int32_t* lo,
+                                                           const int32_t* range, uint32_t R,
+                                                           uint32_t vb, uint32_t* out,
+                                                           uint32_t* dense, uint32_t* zneg) {
+    __shared__ uint32_t bits[kPackBitmapWords];
+    __shared__ uint32_t pre[kPackBitmapWords];
+    __shared__ uint32_t scan[20];
     const int64_t c = blockIdx.x;
     const int32_t base = lo[c];
     if (base == INT_MIN) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int64_t span = (int64_t)range[2 * c + 1] - base + 1;   // 0 or less: no nonzero value
+    const uint32_t words = span > 0 ? (uint32_t)((span + 31) / 32) : 0u;
+    for (uint32_t w = tid; w < words; w += nt) bits[w] = 0u;
+    __syncthreads();
+    const int64_t p0 = indptr[c], p1 = indptr[c + 1];
+    for (int64_t p = p0 + tid; p < p1; p += nt) {
+        const V v = vals[p];
+        if (KeyOf<V>::finite(v) && v != V(0)) {
+            const uint32_t k = (uint32_t)((int32_t)v - base);
+            atomicOr(&bits[k >> 5], 1u << (k & 31u));
+        }
+    }
+    __syncthreads();
+    for (uint32_t w = tid; w < words; w += nt) pre[w] = (uint32_t)__builtin_popcount(bits[w]);
+    __syncthreads();
+    const uint32_t total = block_excl_scan(pre, (int)words, scan);
+    // rank of offset k: present values at smaller offsets
+    auto rank = [&](uint32_t k) {
+        return pre[k >> 5] + (uint32_t)__builtin_popcount(bits[k >> 5] & ((1u << (k & 31u)) - 1u));
+    };
+    if (tid == 0) {
+        dense[c] = total;
+        zneg[c] = base >= 0 ? 0u : ((int64_t)(-(int64_t)base) < span ? rank((uint32_t)(-base)) : total);
+    }
     const uint32_t zc = pack_zero_code(vb), sc = pack_skip_code(vb);
-    for (int64_t p = indptr[c] + threadIdx.x; p < indptr[c + 1]; p += blockDim.x) {
+    for (int64_t p = p0 + tid; p < p1; p += nt) {
         const V v = vals[p];
         const int32_t r = rows[p];
         const uint32_t row = (r >= 0 && (uint32_t)r < R) ? (uint32_t)r : R;
         uint32_t code;
         if (!KeyOf<V>::finite(v)) code = sc;
         else if (v == V(0)) code = zc;
-        else code = (uint32_t)((int32_t)v - base);
+        else code = rank((uint32_t)((int32_t)v - base));
         out[p] = (row << vb) | code;
     }
 }
@@ -1838,15 +1878,16 @@ hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* in
 }
 
 hipError_t launch_pack_columns(const void* vals, bool f64, const int32_t* rows,
-                               const int64_t* indptr, int64_t C, const int32_t* lo, uint32_t R,
-                               uint32_t vb, uint32_t* out, hipStream_t s) {
+                               const int64_t* indptr, int64_t C, const int32_t* lo,
+                               const int32_t* range, uint32_t R, uint32_t vb, uint32_t* out,
+                               uint32_t* dense, uint32_t* zneg, hipStream_t s) {
     if (C <= 0) return hipSuccess;
     if (f64)
-        pack_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, rows, indptr,
-                                                                 lo, R, vb, out);
+        pack_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>(
+            (const double*)vals, rows, indptr, lo, range, R, vb, out, dense, zneg);
     else
-        pack_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, rows, indptr,
-                                                                lo, R, vb, out);
+        pack_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>(
+            (const float*)vals, rows, indptr, lo, range, R, vb, out, dense, zneg);
     return hipGetLastError();
 }
 

@@ -203,22 +203,26 @@ def test_count_kernel_lds_sizes(lds, restore_tuning):
     """The one-pass count kernel at several LDS sizes: 0 sends every column to
     the split path; 16 and 24 KiB split the 40 targets of the wider columns
     into slices; 152 KiB takes each column in one workgroup."""
-    _capi.set_tuning(count_lds_bytes=lds)
     w = _random_workload(11, 9000, 16, 40, 0.6,
                          lambda r, n: r.negative_binomial(0.8, 0.02, n) - 2)
-    for mode in ("min", "max", "mix", "none"):
-        say(f"count kernel lds={lds} mode={mode}")
-        m, p = _plan_of(w, mode)
-        p.run(1)
-        info = p.info()
-        assert info["count_columns"] == (16 if lds else 0)
-        assert info["handed_over"] == 0
-        if lds == 152 * 1024:
-            assert info["count_blocks"] == 16
-        elif lds:
-            assert info["count_blocks"] > 16
-        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
-        p.close(); m.close()
+    for packed in (0, 1):
+        # read as rows and values the bins are the value range; packed, the
+        # distinct values (fewer, so at most as many slices)
+        _capi.set_tuning(count_lds_bytes=lds, count_packed=packed)
+        for mode in ("min", "max", "mix", "none"):
+            say(f"count kernel lds={lds} packed={packed} mode={mode}")
+            m, p = _plan_of(w, mode)
+            p.run(1)
+            info = p.info()
+            assert info["count_columns"] == (16 if lds else 0)
+            assert info["count_packed_columns"] == (16 if lds and packed else 0)
+            assert info["handed_over"] == 0
+            if lds == 152 * 1024:
+                assert info["count_blocks"] == 16
+            elif lds and not packed:
+                assert info["count_blocks"] > 16
+            assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+            p.close(); m.close()
 
 
 def test_count_kernel_packed_entries(restore_tuning):
@@ -290,24 +294,28 @@ def test_count_kernel_rows_beyond_24_bits():
         p.close(); m.close()
 
 
-def test_count_kernel_routes_wide_ranges_to_split():
+def test_count_kernel_routes_wide_ranges_to_split(restore_tuning):
     """Integer columns whose value range exceeds the histogram limit go to the
-    split path when the plan is made; results are unchanged."""
+    split path when they are read as rows and values; packed, their bins are
+    the ~2,400 distinct values each holds, and the count kernel takes them.
+    Results are the same either way."""
     say("count kernel routing")
     w = _random_workload(12, 6000, 10, 6, 0.4, lambda r, n: r.integers(-20000, 20000, n))
-    m, p = _plan_of(w, "mix")
-    p.run(2)
-    info = p.info()
-    assert info["count_columns"] == 0 and info["split_columns"] == 10
-    assert_parity(p.results(with_stats=True), oracle_stats(w, "mix"))
-    p.close(); m.close()
+    for packed, counted in ((0, 0), (1, 10)):
+        _capi.set_tuning(count_packed=packed)
+        m, p = _plan_of(w, "mix")
+        p.run(2)
+        info = p.info()
+        assert info["count_columns"] == counted and info["split_columns"] == 10 - counted
+        assert_parity(p.results(with_stats=True), oracle_stats(w, "mix"))
+        p.close(); m.close()
 
 
 def test_count_kernel_routes_too_many_slices_to_split(restore_tuning):
     """300 targets x 1,000 values in 16 KiB: a workgroup holds 3 targets' rows,
     100 slices per column, more than 64: the columns take the split path."""
     say("count kernel too many slices")
-    _capi.set_tuning(count_lds_bytes=16 * 1024)
+    _capi.set_tuning(count_lds_bytes=16 * 1024, count_packed=0)
     w = _random_workload(14, 6000, 4, 300, 0.5, lambda r, n: r.integers(1, 1001, n))
     m, p = _plan_of(w, "min")
     p.run(2)
@@ -334,10 +342,11 @@ def test_count_kernel_target_slices(lds, restore_tuning):
         p.close(); m.close()
 
 
-def test_count_kernel_extreme_values():
+def test_count_kernel_extreme_values(restore_tuning):
     """Values at the ends of the 16-bit range of the column metadata:
     -32768 .. -32700 and 32700 .. 32767 in the count kernel; both ends in one
-    column (65,536 values) in the split path."""
+    column (65,536 values) in the split path when read as rows and values, in
+    the count kernel when packed (its ~70 distinct values are the bins)."""
     say("count kernel extreme values")
     w = _random_workload(16, 5000, 3, 6, 0.4, lambda r, n: r.integers(0, 69, n))
     d, ip = w["data"].copy(), w["indptr"]
@@ -348,13 +357,15 @@ def test_count_kernel_extreme_values():
     d[ip[2]] = -32768
     d[ip[2] + 1] = 32767
     w = dict(w, data=d)
-    for mode in ("min", "mix", "none"):
-        m, p = _plan_of(w, mode)
-        p.run(1)
-        info = p.info()
-        assert info["count_columns"] == 2 and info["split_columns"] == 1
-        assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
-        p.close(); m.close()
+    for packed, counted in ((0, 2), (1, 3)):
+        _capi.set_tuning(count_packed=packed)
+        for mode in ("min", "mix", "none"):
+            m, p = _plan_of(w, mode)
+            p.run(1)
+            info = p.info()
+            assert info["count_columns"] == counted and info["split_columns"] == 3 - counted
+            assert_parity(p.results(with_stats=True), oracle_stats(w, mode))
+            p.close(); m.close()
 
 
 def test_plan_without_stats_keeps_u1_and_p_only():

@@ -215,17 +215,25 @@ def test_sim_count_kernel_modes(mode, block, pack):
 
 @pytest.mark.parametrize("lds", [16 * 1024, 24 * 1024, 152 * 1024])
 def test_sim_count_kernel_lds_sizes(lds):
-    # 40 targets over ranges of 330-355 values: 3 target slices per column in
-    # 16 KiB, 2 in 24 KiB, one workgroup per column in 152 KiB
+    # 40 targets over ranges of 330-355 values: read as rows and values, 3
+    # target slices per column in 16 KiB, 2 in 24 KiB, one workgroup per column
+    # in 152 KiB; as packed entries the bins are only the values that occur
+    # (fewer), so at most as many workgroups
     d, r, ip, g = _rw(22, 6000, 4, 40, 0.6, lambda rg, n: rg.negative_binomial(0.8, 0.02, n) + 1)
     blocks = {16 * 1024: 12, 24 * 1024: 8, 152 * 1024: 4}[lds]
-    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4, expect_blocks=blocks)
+    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4, expect_blocks=blocks,
+              pack=False)
+    run_count(d, r, ip, g, 40, 6000, "min", lds, expect_counted=4, expect_packed=None)
+    assert 4 <= sim.mwu.last_blocks <= blocks and sim.mwu.last_packed == sim.mwu.last_blocks
 
 
 def test_sim_count_kernel_hands_over_wide_ranges():
-    # values span more than hist_max bins: every column goes to the split path
+    # values span more than hist_max bins: read as rows and values every column
+    # goes to the split path; as packed entries the bins are the ~1,200
+    # distinct values a column holds, and the count kernel takes them
     d, r, ip, g = _rw(23, 3000, 3, 5, 0.4, lambda rg, n: rg.integers(-20000, 20000, n))
-    run_count(d, r, ip, g, 5, 3000, "mix", 76 * 1024, expect_counted=0)
+    run_count(d, r, ip, g, 5, 3000, "mix", 76 * 1024, expect_counted=0, pack=False)
+    run_count(d, r, ip, g, 5, 3000, "mix", 76 * 1024, expect_counted=3, expect_packed=3)
     # a low histogram limit hands over a narrow range too
     d, r, ip, g = _rw(24, 3000, 3, 5, 0.4, lambda rg, n: rg.integers(1, 300, n))
     run_count(d, r, ip, g, 5, 3000, "min", 76 * 1024, hist_max=128, expect_counted=0)
@@ -309,7 +317,12 @@ def test_sim_count_kernel_extreme_values():
     d[ip[2]] = -32768
     d[ip[2] + 1] = 32767
     for mode in ("min", "mix", "none"):
-        run_count(d, r, ip, g, 6, 3000, mode, 76 * 1024, expect_counted=2, expect_blocks=2)
+        # rows and values: column 2's 65,536-value range takes the split path;
+        # packed, its ~70 distinct values are the bins and it is counted too
+        run_count(d, r, ip, g, 6, 3000, mode, 76 * 1024, expect_counted=2, expect_blocks=2,
+                  pack=False)
+        run_count(d, r, ip, g, 6, 3000, mode, 76 * 1024, expect_counted=3, expect_blocks=3,
+                  expect_packed=3)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
@@ -318,13 +331,15 @@ def test_sim_count_kernel_hand_overs(mode, mixed):
     # recorded ranges one value short of the data: every counted column meets
     # a value outside its range and is handed to the sort path at run time --
     # with no host-routed column (mixed False) the combined hand-over kernel
-    # takes them, otherwise the per-column kernels after the routed float one
+    # takes them, otherwise the per-column kernels after the routed float one.
+    # The columns are read as rows and values: packed entries rank the values
+    # a column holds, so they cannot meet one outside their bins.
     d, r, ip, g = _rw(32, 3000, 5, 6, 0.5, lambda rg, n: rg.integers(-3, 12, n))
     if mixed:
         d = d.copy()
         d[ip[4]] = 0.5          # column 4 is not integer-valued: routed by the host
     U1, P, TIE, n1, n2, err, status = sim.mwu(d, r, ip, g, 6, 3000, mode=ZH[mode],
-                                              shrink_range=True, timeout_s=120)
+                                              shrink_range=True, timeout_s=120, pack=False)
     assert err == 2 ** 64 - 1 and status == 0
     assert sim.mwu.last_counted == 0
     oU1, oP, oT, on1, on2 = oracle.mwu(d, r, ip, g, 6, n_rows=3000, zero_handling=mode,
