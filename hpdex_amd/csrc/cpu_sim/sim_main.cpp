@@ -87,10 +87,13 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     std::vector<unsigned char> buf(std::max<size_t>(64 * 1024, count_lds));
     sim::g_lds = buf.data();
     sim::g_lds_bytes = buf.size();
-    std::vector<uint8_t> col_int(C, 0);
+    std::vector<uint8_t> col_int(C, 0), col_ident(C, 0);
     for (int c = 0; c < C; ++c)
         run_block(c, 256, C, [&] { classify_columns_kernel<V>((const V*)J.vals, J.indptr,
-                                                              col_int.data(), range); }, timeout_s);
+                                                              col_int.data(), range, J.rows,
+                                                              (uint32_t)J.R, col_ident.data()); },
+                  timeout_s);
+    J.colident = col_ident.data();
     // the matrix's packed entries (finish_matrix), made from the recorded ranges
     const uint32_t vb = 32u - pack_row_bits((uint32_t)J.R);
     std::vector<int32_t> lo(C, INT_MIN);

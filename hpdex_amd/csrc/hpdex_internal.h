@@ -60,6 +60,7 @@ struct JobDesc {
     uint32_t gid_words;          // 32-bit words of gidp
     const uint32_t* group_rows;  // G rows per group
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
+    const uint8_t* colident;     // C: 1 = the column stores rows 0 .. R-1 in order
     const uint32_t* packed;      // nnz packed entries (pack_entry) or null; CountItem flags
     uint32_t pack_vb;            // value bits of a packed entry
     // workspace
@@ -284,7 +285,8 @@ hipError_t prepare_count_kernel(uint32_t bytes);
 // column c is non-finite or an integer in [-32768, 32767]; range[2c..2c+1] =
 // smallest and largest stored nonzero finite value of such a column.
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
-                                   uint8_t* ok, int32_t* range, hipStream_t s);
+                                   uint8_t* ok, int32_t* range, const int32_t* rows, uint32_t R,
+                                   uint8_t* ident, hipStream_t s);
 hipError_t launch_pack_columns(const void* vals, bool f64, const int32_t* rows,
                                const int64_t* indptr, int64_t C, const int32_t* lo,
                                const int32_t* range, uint32_t R, uint32_t vb, uint32_t* out,

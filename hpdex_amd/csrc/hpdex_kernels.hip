@@ -130,7 +130,7 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
 // index R): a sparse column's gathers then touch as few cache lines as
 // possible.  Otherwise the 32-bit labels are gathered: for a column holding a
 // large share of the rows they are coalesced and unpacking would only add work.
-template <typename V, int NT, bool kPacked, typename F>
+template <typename V, int NT, bool kPacked, bool kInOrder = false, typename F>
 __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, int64_t n, F&& f) {
     const uint32_t tid = (uint32_t)tid_x();
     const uint32_t R = (uint32_t)J.R, G = (uint32_t)J.G;
@@ -150,7 +150,9 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const uint32_t e = tid + (uint32_t)(u * NT);
-            r[u] = buffer_load_u32(rows_rsrc, e * 4u, 0u);
+            // kInOrder: the column stores rows 0 .. R-1 in order, so an
+            // entry's row is its position (R < 2^31 keeps it in 32 bits)
+            r[u] = kInOrder ? (uint32_t)ib + e : buffer_load_u32(rows_rsrc, e * 4u, 0u);
             v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
         }
     };
@@ -327,8 +329,12 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // rows or more (consecutive rows: the gathers are coalesced)
     const bool packed = __builtin_amdgcn_readfirstlane(
         (J.gid_bits > 0u && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
+    // a column storing rows 0 .. R-1 in order (a dense matrix held as CSC):
+    // rows from positions, so only its values are read
+    const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
     auto grouped = [&](auto&& f) {
         if (packed) for_each_grouped<V, kColBlock, true>(J, p0, p1 - p0, f);
+        else if (in_order) for_each_grouped<V, kColBlock, false, true>(J, p0, p1 - p0, f);
         else for_each_grouped<V, kColBlock, false>(J, p0, p1 - p0, f);
     };
 
@@ -1498,12 +1504,25 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
 template <typename V>
 __global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
                                                                const int64_t* indptr,
-                                                               uint8_t* ok_out, int32_t* range) {
-    __shared__ uint32_t ok;
+                                                               uint8_t* ok_out, int32_t* range,
+                                                               const int32_t* rows, uint32_t R,
+                                                               uint8_t* ident_out) {
+    __shared__ uint32_t ok, ident;
     __shared__ int32_t lo, hi;
     const int64_t c = blockIdx.x;
-    if (threadIdx.x == 0) { ok = 1u; lo = INT_MAX; hi = INT_MIN; }
+    if (threadIdx.x == 0) { ok = 1u; lo = INT_MAX; hi = INT_MIN; ident = 1u; }
     __syncthreads();
+    // a column storing exactly rows 0 .. R-1 in order (a dense matrix held as
+    // CSC): its readers can take the row from the position
+    const int64_t q0 = indptr[c], q1 = indptr[c + 1];
+    if (q1 - q0 == (int64_t)R) {
+        bool in_order = true;
+        for (int64_t p = q0 + threadIdx.x; p < q1; p += blockDim.x)
+            in_order &= rows[p] == (int32_t)(p - q0);
+        if (!in_order) atomicAnd(&ident, 0u);
+    } else if (threadIdx.x == 0) {
+        ident = 0u;
+    }
     bool li = true;
     int mn = INT_MAX, mx = INT_MIN;
     for (int64_t p = indptr[c] + threadIdx.x; p < indptr[c + 1]; p += blockDim.x) {
@@ -1525,6 +1544,7 @@ __global__ __launch_bounds__(256) void classify_columns_kernel(const V* vals,
         ok_out[c] = (uint8_t)ok;
         range[2 * c] = ok ? lo : 1;
         range[2 * c + 1] = ok ? hi : 0;
+        ident_out[c] = (uint8_t)ident;
     }
 }
 
@@ -1866,14 +1886,15 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
 }
 
 hipError_t launch_classify_columns(const void* vals, bool f64, const int64_t* indptr, int64_t C,
-                                   uint8_t* ok, int32_t* range, hipStream_t s) {
+                                   uint8_t* ok, int32_t* range, const int32_t* rows, uint32_t R,
+                                   uint8_t* ident, hipStream_t s) {
     if (C <= 0) return hipSuccess;
     if (f64)
-        classify_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>((const double*)vals, indptr,
-                                                                     ok, range);
+        classify_columns_kernel<double><<<(unsigned)C, 256, 0, s>>>(
+            (const double*)vals, indptr, ok, range, rows, R, ident);
     else
-        classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>((const float*)vals, indptr,
-                                                                    ok, range);
+        classify_columns_kernel<float><<<(unsigned)C, 256, 0, s>>>(
+            (const float*)vals, indptr, ok, range, rows, R, ident);
     return hipGetLastError();
 }
 

@@ -116,7 +116,7 @@ size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows
     size_t f = 0;
     if (!m) {   // upload_matrix_from
         f += up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
-        f += up256(2 * C * 4) + up256(C);
+        f += up256(2 * C * 4) + 2 * up256(C);
         const size_t ib = j.index_bits == 32 ? 4 : 8;
         if (j.layout == HPDEX_LAYOUT_CSR) {
             // raw values, column indices, row pointers, transpose workspace

@@ -128,6 +128,24 @@ def test_sim_sort_path_dense_slots(mode):
     run_both(data, rows, indptr, gid, T, R, mode)
 
 
+@pytest.mark.parametrize("mode", ["min", "none"])
+def test_sim_sort_path_dense_rows_out_of_order(mode):
+    # three columns storing every row: in order (rows taken from positions),
+    # reversed, and in order but for two rows swapped (both read their rows)
+    rng = np.random.default_rng(39)
+    R, T = 4000, 30
+    rows = [np.arange(R), np.arange(R)[::-1].copy(), np.arange(R)]
+    rows[2][[100, 2500]] = rows[2][[2500, 100]]
+    rows = np.concatenate(rows).astype(np.int64)
+    ip = np.arange(4, dtype=np.int64) * R
+    data = np.round(rng.normal(0, 1.5, 3 * R), 1).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:600]] = 0
+    gid[perm[600:2600]] = rng.integers(1, T + 1, 2000)
+    run_both(data, rows, ip, gid, T, R, mode)
+
+
 def test_sim_sort_path_dense_slots_histogram_plan():
     # grouped values all small integers, one non-integer in a row of no group:
     # the column takes the sort path, whose split picks a histogram plan after
