@@ -176,7 +176,7 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     } else {
         const unsigned ng = n_static + nc;
         for (unsigned b = 0; b < ng; ++b)
-            run_block(b, kColBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+            run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
         for (unsigned b = 0; b < ng; ++b)
             run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
         for (unsigned b = 0; b < ng; ++b)

@@ -18,6 +18,7 @@ enum : uint32_t {
 
 // Launch geometry shared by the per-column kernels.
 constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
+constexpr int kSplitBlock = 1024;           // threads per split workgroup (16 waves)
 constexpr int kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup, at most
 constexpr int kHistMaxBins = 8192;          // histogram path: widest integer range
 constexpr int kMaxBuckets = 4096;           // sort path: bucket index entries (pow2)

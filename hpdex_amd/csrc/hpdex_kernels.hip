@@ -301,7 +301,7 @@ struct SplitState {
     uint32_t scan[20];
 };
 
-template <typename V>
+template <typename V, int NT>
 __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
@@ -333,9 +333,9 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // rows from positions, so only its values are read
     const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
     auto grouped = [&](auto&& f) {
-        if (packed) for_each_grouped<V, kColBlock, true>(J, p0, p1 - p0, f);
-        else if (in_order) for_each_grouped<V, kColBlock, false, true>(J, p0, p1 - p0, f);
-        else for_each_grouped<V, kColBlock, false>(J, p0, p1 - p0, f);
+        if (packed) for_each_grouped<V, NT, true>(J, p0, p1 - p0, f);
+        else if (in_order) for_each_grouped<V, NT, false, true>(J, p0, p1 - p0, f);
+        else for_each_grouped<V, NT, false>(J, p0, p1 - p0, f);
     };
 
     // Slots: group g's keys at the exclusive prefix of the group sizes.  When
@@ -470,12 +470,18 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     }
 }
 
-// workgroup b takes queue item b (the grid is sized for the queue's capacity)
+// workgroup b takes queue item b (the grid is sized for the queue's capacity).
+// kSplitBlock threads, one workgroup per CU: a column's keys are scattered
+// over its key range group by group, and with few columns in flight those
+// ranges stay in L2 while their partial lines fill (256-thread workgroups, five
+// per CU, wrote 8x the key bytes to HBM: 4.0 -> 3.6 ms on a dense 150,000 x
+// 6,000 split, 3.2 -> 2.3 ms on a sparse 400,000 x 18,000 one).
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void split_kernel(const JobDesc* jobs, const WorkItem* items,
-                                                   const uint32_t* n_items) {
+__global__ __launch_bounds__(kSplitBlock) void split_kernel(const JobDesc* jobs,
+                                                            const WorkItem* items,
+                                                            const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
-    if (blockIdx.x < n) split_item<V>(jobs, items, blockIdx.x);
+    if (blockIdx.x < n) split_item<V, kSplitBlock>(jobs, items, blockIdx.x);
 }
 
 // Rank table of the reference slice seen by the target phase.
@@ -1008,7 +1014,7 @@ __global__ __launch_bounds__(kColBlock) void handover_kernel(const JobDesc* jobs
                                                              const uint32_t* n_items) {
     const uint32_t n = uniform_u32(*n_items);
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        split_item<V>(jobs, items, k);
+        split_item<V, kColBlock>(jobs, items, k);
         __threadfence();
         __syncthreads();
         ref_item<V>(jobs, items, k);
@@ -1864,7 +1870,7 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     const unsigned tab_lds = table_lds_bytes(L.f64);
     auto sort_path = [&](auto v_tag) {
         using V = typename decltype(v_tag)::type;
-        split_kernel<V><<<sort_grid, kColBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        split_kernel<V><<<sort_grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
         mark(2);
         ref_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
         mark(3);
