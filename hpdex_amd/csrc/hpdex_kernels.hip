@@ -817,10 +817,17 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
     uint32_t zaj = 0, zb = 0, n1 = 0, n2 = 0;
     if (own) pair_sizes(J, c, gj, zaj, zb, n1, n2);
     // the reference's zero block, the same for every target
-    const uint32_t nza = J.cnt[b];
-    const uint32_t za = (J.mode == kZeroNone) ? J.zcnt[b]
-                                             : (J.group_rows[0] > nza ? J.group_rows[0] - nza : 0u);
+    const uint32_t nza = uniform_u32(J.cnt[b]);
+    const uint32_t za = uniform_u32((J.mode == kZeroNone)
+                                        ? J.zcnt[b]
+                                        : (J.group_rows[0] > nza ? J.group_rows[0] - nza : 0u));
     const uint32_t inc = wave_incl_scan_dpp(sz, [](uint32_t a, uint32_t x) { return a + x; });
+    // a window's per-key terms sum below 2^32 when 64 keys of 2*lt' + eq
+    // (at most 2*(nza + za) + nza) and of 3x(x - 1) (x = eq + position, at
+    // most nza + 64) do: 32-bit scans then
+    const uint64_t xm = (uint64_t)nza + 64u;
+    const bool narrow = 64ull * (2ull * ((uint64_t)nza + za) + nza) < (1ull << 32) &&
+                        192ull * xm * xm < (1ull << 32);
     TargetCtx<K> t;
     t.ref = ref;
     t.mode = J.mode;
@@ -856,7 +863,10 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         const uint64_t upto = (lane == (uint32_t)kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
         const uint64_t sm = smask & upto;
         const uint32_t first = sm ? (uint32_t)(63 - __clzll(sm)) : 0u;   // its first key
-        const uint32_t owner = nth_set_bit(tmask, (uint32_t)__builtin_popcountll(sm));
+        const uint32_t k = (uint32_t)__builtin_popcountll(sm);
+        // with no empty target in the window the k-th starting target is j0 + k - 1
+        const uint64_t wmask = (j1 == 64u ? ~0ull : ((1ull << j1) - 1ull)) & ~((1ull << j0) - 1ull);
+        const uint32_t owner = tmask == wmask ? j0 + k - 1u : nth_set_bit(tmask, k);
         // the lane's key: position lane - first of the owner's keys (a
         // group's keys need not end where the next group's start)
         const uint32_t at = shfl_u32(off, (int)owner) + lane - first;
@@ -884,21 +894,39 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
             const uint64_t x = (uint64_t)eq + jj;
             t_e = 3ull * x * (x - 1ull);
         }
-        const auto add = [](uint64_t a, uint64_t x) { return a + x; };
-        const uint64_t ps = wave_incl_scan_dpp(s_e, add), pt = wave_incl_scan_dpp(t_e, add);
-        // each target's scans at its last key (an empty target ends where
-        // the one before it ends; none before the window's first key), less
-        // those of the target before it, one lane down
-        // (every lane takes part in the shuffles; the select comes after)
+        // inclusive scans of the terms; each target's scans at its last key
+        // (an empty target ends where the one before it ends; none before
+        // the window's first key), less those of the target before it, one
+        // lane down (every lane takes part in the shuffles; the select comes
+        // after)
         const uint32_t end = excl + sz;
-        const uint64_t ps_at = shfl_u64(ps, (int)((end - 1u) & 63u));
-        const uint64_t pt_at = shfl_u64(pt, (int)((end - 1u) & 63u));
-        const uint64_t ps_end = end ? ps_at : 0ull, pt_end = end ? pt_at : 0ull;
-        const uint64_t ps_prev = dpp_shr1_u64(ps_end), pt_prev = dpp_shr1_u64(pt_end);
-        if (starts) {
-            S = ps_end - (lane == j0 ? 0ull : ps_prev);
-            TT = pt_end - (lane == j0 ? 0ull : pt_prev);
-        }
+        auto sums = [&](auto word_tag) {
+            using U = typename decltype(word_tag)::type;
+            const auto add = [](U a, U x) { return (U)(a + x); };
+            const U ps = wave_incl_scan_dpp((U)s_e, add), pt = wave_incl_scan_dpp((U)t_e, add);
+            U ps_at, pt_at, ps_prev, pt_prev;
+            if constexpr (sizeof(U) == 8) {
+                ps_at = shfl_u64(ps, (int)((end - 1u) & 63u));
+                pt_at = shfl_u64(pt, (int)((end - 1u) & 63u));
+            } else {
+                ps_at = shfl_u32(ps, (int)((end - 1u) & 63u));
+                pt_at = shfl_u32(pt, (int)((end - 1u) & 63u));
+            }
+            const U ps_end = end ? ps_at : U(0), pt_end = end ? pt_at : U(0);
+            if constexpr (sizeof(U) == 8) {
+                ps_prev = dpp_shr1_u64(ps_end);
+                pt_prev = dpp_shr1_u64(pt_end);
+            } else {
+                ps_prev = wave_shr1_u32(ps_end);
+                pt_prev = wave_shr1_u32(pt_end);
+            }
+            if (starts) {
+                S = (uint64_t)(U)(ps_end - (lane == j0 ? U(0) : ps_prev));
+                TT = (uint64_t)(U)(pt_end - (lane == j0 ? U(0) : pt_prev));
+            }
+        };
+        if (narrow) sums(TypeTag<uint32_t>{});
+        else sums(TypeTag<uint64_t>{});
         j0 = j1;
     }
     if (own && !big) {

@@ -160,6 +160,27 @@ def test_float_sort_path(mode):
     assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
 
 
+@pytest.mark.parametrize("mode", ["min", "none"])
+def test_float_windows_large_reference(mode):
+    """Small targets against a reference of ~5,800 stored values per column:
+    a window's sums may pass 2^32, so the target windows scan in 64 bits
+    (smaller references take 32-bit scans)."""
+    say(f"float windows large reference {mode}")
+    rng = np.random.default_rng(19)
+    R, C, T = 9000, 8, 120
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.full(C, 0.97))
+    data = np.round(rng.normal(0, 1.0, len(rows)), 1).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:6000]] = 0
+    at = 6000
+    for g, n in enumerate(rng.integers(3, 31, T), start=1):
+        gid[perm[at:at + n]] = g
+        at += n
+    w = dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R)
+    assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+
+
 @pytest.mark.parametrize("mode", ["min", "mix", "none"])
 def test_integer_hist_path(mode):
     say(f"integer hist path {mode}")

@@ -129,6 +129,24 @@ def test_sim_sort_path_dense_slots(mode):
 
 
 @pytest.mark.parametrize("mode", ["min", "none"])
+def test_sim_sort_path_windows_with_a_large_reference(mode):
+    # a reference of ~5,800 stored values: a window's sums may pass 2^32, so
+    # the windows scan in 64 bits; 120 targets of 3-30 cells, ties
+    rng = np.random.default_rng(40)
+    R, C, T = 9000, 2, 120
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.full(C, 0.97))
+    data = np.round(rng.normal(0, 1.0, len(rows)), 1).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:6000]] = 0
+    at = 6000
+    for g, n in enumerate(rng.integers(3, 31, T), start=1):
+        gid[perm[at:at + n]] = g
+        at += n
+    run_both(data, rows, indptr, gid, T, R, mode)
+
+
+@pytest.mark.parametrize("mode", ["min", "none"])
 def test_sim_sort_path_dense_rows_out_of_order(mode):
     # three columns storing every row: in order (rows taken from positions),
     # reversed, and in order but for two rows swapped (both read their rows)
