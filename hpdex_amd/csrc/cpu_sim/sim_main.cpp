@@ -84,7 +84,7 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                         uint32_t count_lds, unsigned count_block, double timeout_s) {
     const JobDesc* jobs = &J;
     const int C = (int)J.C;
-    std::vector<unsigned char> buf(std::max<size_t>(64 * 1024, count_lds));
+    std::vector<unsigned char> buf(std::max<size_t>((size_t)kSplitLdsMax, count_lds));
     sim::g_lds = buf.data();
     sim::g_lds_bytes = buf.size();
     std::vector<uint8_t> col_int(C, 0), col_ident(C, 0);
@@ -167,16 +167,34 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                 count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
     const WorkItem* it = gen_items.data();
+    // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's room for
+    // its counters and, when they fit, every column's keys in slots; the
+    // hand-over kernel's the larger of the counters and the reference table
+    auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
+    size_t split = 12u * (size_t)J.G;
+    if (split > (size_t)kLdsBudget) split = 16;
+    uint64_t slots = 0;
+    for (int g = 0; g < J.G; ++g) slots += J.group_rows[g];
+    const size_t staged = a16(split) + sizeof(typename KeyOf<V>::K) * (size_t)slots;
+    const uint32_t split_lds =
+        (split > 16u && staged <= (size_t)kSplitLdsMax) ? a16(staged) : a16(split);
+    const size_t hist = 4u * (size_t)(kHistMaxBins + 1);
+    const size_t sort = (sizeof(typename KeyOf<V>::K) == 8 ? 8u * (size_t)kRefKeysF64
+                                                           : 4u * (size_t)kRefKeysF32) +
+                        4u * (size_t)(kMaxBuckets + 1);
+    const uint32_t both_lds = std::max(a16(split), a16(std::max(hist, sort)));
     if (n_static == 0) {
         // hand-overs only: the combined kernel, on a grid of at most 3
         // workgroups so that each takes several queue items
         const unsigned hg = std::min(nc, 3u);
         for (unsigned b = 0; b < hg; ++b)
-            run_block(b, kColBlock, hg, [&] { handover_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+            run_block(b, kColBlock, hg, [&] { handover_kernel<V>(jobs, it, &gen_n, both_lds); },
+                      timeout_s);
     } else {
         const unsigned ng = n_static + nc;
         for (unsigned b = 0; b < ng; ++b)
-            run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n); }, timeout_s);
+            run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n, split_lds); },
+                      timeout_s);
         for (unsigned b = 0; b < ng; ++b)
             run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
         for (unsigned b = 0; b < ng; ++b)

@@ -19,6 +19,7 @@ enum : uint32_t {
 // Launch geometry shared by the per-column kernels.
 constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
 constexpr int kSplitBlock = 1024;           // threads per split workgroup (16 waves)
+constexpr int kSplitLdsMax = 152 * 1024;    // dynamic LDS a split workgroup may take
 constexpr int kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup, at most
 constexpr int kHistMaxBins = 8192;          // histogram path: widest integer range
 constexpr int kMaxBuckets = 4096;           // sort path: bucket index entries (pow2)
@@ -275,12 +276,15 @@ struct PipelineLaunch {
     bool f64;
     uint32_t count_lds;    // dynamic LDS bytes of the count kernel
     int count_block;       // its workgroup size, 256, 512 or 1024
+    uint64_t slots;        // rows in a group (a column's key slots)
+    uint32_t split_lds_max;   // dynamic LDS the split kernel may take
 };
 // ev (optional, 5 events): recorded before the count kernel and after the
 // count, split, reference and target kernels, for per-kernel timing.
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,
                                hipEvent_t* ev = nullptr);
-// Allow the count kernel `bytes` of dynamic LDS (once per process).
+// Allow the count kernel `bytes` of dynamic LDS and the split kernel
+// kSplitLdsMax (once per process).
 hipError_t prepare_count_kernel(uint32_t bytes);
 // Column metadata of a resident matrix: ok[c] = 1 when every stored value of
 // column c is non-finite or an integer in [-32768, 32767]; range[2c..2c+1] =

@@ -302,7 +302,8 @@ struct SplitState {
 };
 
 template <typename V, int NT>
-__device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
+__device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* items, uint32_t k,
+                                           uint32_t lds_bytes) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -352,10 +353,18 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t slots = block_excl_scan(cur, G, st.scan);
     const bool fixed = __builtin_amdgcn_readfirstlane((int64_t)slots <= p1 - p0 ? 1 : 0) != 0;
     K* keys = (K*)J.keys + p0;
+    // in slots, the keys gather in LDS when the workgroup has room for them
+    // all, and go to HBM afterwards in one coalesced copy (scattered 4-byte
+    // stores to HBM write whole partial lines)
+    const uint32_t cnt_bytes = (12u * (uint32_t)G + 15u) & ~15u;
+    const bool staged = fixed && __builtin_amdgcn_readfirstlane(
+        (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * slots <= lds_bytes ? 1 : 0) != 0;
+    K* stage = (K*)(smem + cnt_bytes);
 
     // counts and value statistics (and, in slots, the keys)
+    // fixed_tag: 0 scatter later, 1 keys to their slots in HBM, 2 in LDS
     auto count_pass = [&](auto fixed_tag) {
-        constexpr bool kFixed = decltype(fixed_tag)::value;
+        constexpr int kFixed = decltype(fixed_tag)::value;
         bool li = true;
         int lmin = INT_MAX, lmax = INT_MIN;
         K kmn = KT::kSentinel, kmx = 0;
@@ -365,12 +374,15 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
             const uint32_t i = atomicAdd(&cnt_nz[g], 1u);
             const K k = KT::key(v);
-            if (kFixed) {
+            if (kFixed == 1) {
                 // a group holding more entries than rows (repeated row
                 // indices) is an error of the column; its keys stay inside
                 // the column's range
                 const uint32_t pos = cur[g] + i;
                 if ((int64_t)pos < p1 - p0) keys[pos] = k;
+            } else if (kFixed == 2) {
+                const uint32_t pos = cur[g] + i;
+                if (pos < slots) stage[pos] = k;
             }
             int iv;
             if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
@@ -396,8 +408,9 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
     };
-    if (fixed) count_pass(std::true_type{});
-    else count_pass(std::false_type{});
+    if (staged) count_pass(std::integral_constant<int, 2>{});
+    else if (fixed) count_pass(std::integral_constant<int, 1>{});
+    else count_pass(std::integral_constant<int, 0>{});
     __syncthreads();
 
     // errors of this column (mannwhitneyu.cpp:936-1038), counts out
@@ -458,14 +471,21 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             const uint32_t pos = atomicAdd(&cur[g], 1u);
             keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
         });
-    } else if (hist) {
-        // the slots hold keys; a histogram plan reads bins: each group's
-        // keys (integers, as the statistics found them) become v - vmin
-        const int w = tid / kWave, nw = nt / kWave;
-        for (int g = w; g < G; g += nw) {
-            const uint32_t o = cur[g], m = min(cnt_nz[g], J.group_rows[g]);
-            for (uint32_t i = (uint32_t)lane; i < m; i += kWave)
-                keys[o + i] = (K)(uint32_t)((int)KT::value(keys[o + i]) - vmin);
+    } else {
+        K* slot_keys = staged ? stage : keys;
+        if (hist) {
+            // the slots hold keys; a histogram plan reads bins: each group's
+            // keys (integers, as the statistics found them) become v - vmin
+            const int w = tid / kWave, nw = nt / kWave;
+            for (int g = w; g < G; g += nw) {
+                const uint32_t o = cur[g], m = min(cnt_nz[g], J.group_rows[g]);
+                for (uint32_t i = (uint32_t)lane; i < m; i += kWave)
+                    slot_keys[o + i] = (K)(uint32_t)((int)KT::value(slot_keys[o + i]) - vmin);
+            }
+        }
+        if (staged) {   // the whole slot range out, coalesced
+            __syncthreads();
+            for (uint32_t i = (uint32_t)tid; i < slots; i += (uint32_t)nt) keys[i] = stage[i];
         }
     }
 }
@@ -479,9 +499,10 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
 template <typename V>
 __global__ __launch_bounds__(kSplitBlock) void split_kernel(const JobDesc* jobs,
                                                             const WorkItem* items,
-                                                            const uint32_t* n_items) {
+                                                            const uint32_t* n_items,
+                                                            uint32_t lds_bytes) {
     const uint32_t n = uniform_u32(*n_items);
-    if (blockIdx.x < n) split_item<V, kSplitBlock>(jobs, items, blockIdx.x);
+    if (blockIdx.x < n) split_item<V, kSplitBlock>(jobs, items, blockIdx.x, lds_bytes);
 }
 
 // Rank table of the reference slice seen by the target phase.
@@ -1039,10 +1060,11 @@ void target_kernel(const JobDesc* jobs, const WorkItem* items, const uint32_t* n
 template <typename V>
 __global__ __launch_bounds__(kColBlock) void handover_kernel(const JobDesc* jobs,
                                                              const WorkItem* items,
-                                                             const uint32_t* n_items) {
+                                                             const uint32_t* n_items,
+                                                             uint32_t lds_bytes) {
     const uint32_t n = uniform_u32(*n_items);
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        split_item<V, kColBlock>(jobs, items, k);
+        split_item<V, kColBlock>(jobs, items, k, lds_bytes);
         __threadfence();
         __syncthreads();
         ref_item<V>(jobs, items, k);
@@ -1862,6 +1884,12 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
             hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;
     }
+    const void* split_fns[] = {(const void*)split_kernel<float>, (const void*)split_kernel<double>};
+    for (const void* f : split_fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)kSplitLdsMax);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
@@ -1894,20 +1922,31 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     mark(1);
     size_t split = 12u * (size_t)L.G;
     if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
-    const unsigned split_lds = align16(split);
+    // room for the columns' keys in slots (every group's rows), when it fits:
+    // the kernel gathers them in LDS and writes them out coalesced
+    const size_t staged = (size_t)align16(split) + (L.f64 ? 8u : 4u) * (size_t)L.slots;
+    const unsigned split_lds =
+        (split > 16u && staged <= (size_t)L.split_lds_max) ? align16(staged) : align16(split);
     const unsigned tab_lds = table_lds_bytes(L.f64);
     auto sort_path = [&](auto v_tag) {
         using V = typename decltype(v_tag)::type;
-        split_kernel<V><<<sort_grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        split_kernel<V><<<sort_grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n,
+                                                                 split_lds);
         mark(2);
         ref_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
         mark(3);
         target_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
     };
-    const unsigned both_lds = split_lds > tab_lds ? split_lds : tab_lds;
+    // the hand-over kernel keeps to the table's LDS (no staging)
+    const unsigned both_split = align16(split);
+    const unsigned both_lds = both_split > tab_lds ? both_split : tab_lds;
     if (loop) {   // hand-overs only: the three phases in one launch
-        if (L.f64) handover_kernel<double><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        else handover_kernel<float><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
+        if (L.f64)
+            handover_kernel<double><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items,
+                                                                           L.gen_n, both_lds);
+        else
+            handover_kernel<float><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items,
+                                                                          L.gen_n, both_lds);
         mark(2);
         mark(3);
     } else if (L.f64) {

@@ -160,6 +160,28 @@ def test_float_sort_path(mode):
     assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
 
 
+@pytest.mark.parametrize("grouped", [20000, 45000])
+def test_dense_split_slots(grouped):
+    """Dense columns split in one pass into slots: 20,000 grouped rows keep
+    their keys in LDS and write them out coalesced, 45,000 (180 KB of keys)
+    write each key to its slot in HBM; explicit zeros leave slot gaps."""
+    say(f"dense split slots {grouped}")
+    rng = np.random.default_rng(20)
+    R, C, T = 46000, 6, 40
+    X = np.round(rng.normal(0, 1.5, (C, R)), 1).astype(np.float32)
+    X[rng.random((C, R)) < 0.05] = 0.0
+    indices = np.tile(np.arange(R, dtype=np.int64), C)
+    indptr = np.arange(C + 1, dtype=np.int64) * R
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:6000]] = 0        # within the 8,192 reference keys kept on chip
+    gid[perm[6000:grouped]] = rng.integers(1, T + 1, grouped - 6000)
+    w = dict(data=X.reshape(-1), indices=indices, indptr=indptr, group_id=gid, n_targets=T,
+             n_rows=R)
+    for mode in ("min", "none"):
+        assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+
+
 @pytest.mark.parametrize("mode", ["min", "none"])
 def test_float_windows_large_reference(mode):
     """Small targets against a reference of ~5,800 stored values per column:

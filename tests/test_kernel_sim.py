@@ -164,6 +164,21 @@ def test_sim_sort_path_dense_rows_out_of_order(mode):
     run_both(data, rows, ip, gid, T, R, mode)
 
 
+def test_sim_sort_path_dense_slots_beyond_lds():
+    # 45,000 grouped rows: the slots (180 KB of keys) do not fit a workgroup's
+    # LDS, so the one-pass split writes each key to its slot in HBM
+    rng = np.random.default_rng(41)
+    R, C, T = 46000, 2, 40
+    rows, indptr = _dense(R, C)
+    data = np.round(rng.normal(0, 1.5, R * C), 1).astype(np.float32)
+    data[rng.random(R * C) < 0.05] = 0.0
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:6000]] = 0        # within the 8,192 reference keys kept on chip
+    gid[perm[6000:45000]] = rng.integers(1, T + 1, 39000)
+    run_both(data, rows, indptr, gid, T, R, "min")
+
+
 def test_sim_sort_path_dense_slots_histogram_plan():
     # grouped values all small integers, one non-integer in a row of no group:
     # the column takes the sort path, whose split picks a histogram plan after
