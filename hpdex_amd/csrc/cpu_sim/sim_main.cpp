@@ -167,17 +167,12 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                 count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
         }, timeout_s);
     const WorkItem* it = gen_items.data();
-    // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's room for
-    // its counters and, when they fit, every column's keys in slots; the
-    // hand-over kernel's the larger of the counters and the reference table
+    // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most;
+    // the hand-over kernel's the larger of the counters and the reference table
     auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
     size_t split = 12u * (size_t)J.G;
     if (split > (size_t)kLdsBudget) split = 16;
-    uint64_t slots = 0;
-    for (int g = 0; g < J.G; ++g) slots += J.group_rows[g];
-    const size_t staged = a16(split) + sizeof(typename KeyOf<V>::K) * (size_t)slots;
-    const uint32_t split_lds =
-        (split > 16u && staged <= (size_t)kSplitLdsMax) ? a16(staged) : a16(split);
+    const uint32_t split_lds = split > 16u ? (uint32_t)kSplitLdsMax : a16(split);
     const size_t hist = 4u * (size_t)(kHistMaxBins + 1);
     const size_t sort = (sizeof(typename KeyOf<V>::K) == 8 ? 8u * (size_t)kRefKeysF64
                                                            : 4u * (size_t)kRefKeysF32) +

@@ -495,8 +495,6 @@ PipelineLaunch pipeline_of(const JobState& js, bool f64) {
     L.f64 = f64;
     L.count_lds = js.count_lds;
     L.count_block = js.count_block;
-    L.slots = 0;
-    for (const uint32_t r : js.group_rows) L.slots += r;
     L.split_lds_max = g_count_ready ? (uint32_t)kSplitLdsMax : (uint32_t)kLdsBudget;
     return L;
 }

@@ -276,7 +276,6 @@ struct PipelineLaunch {
     bool f64;
     uint32_t count_lds;    // dynamic LDS bytes of the count kernel
     int count_block;       // its workgroup size, 256, 512 or 1024
-    uint64_t slots;        // rows in a group (a column's key slots)
     uint32_t split_lds_max;   // dynamic LDS the split kernel may take
 };
 // ev (optional, 5 events): recorded before the count kernel and after the

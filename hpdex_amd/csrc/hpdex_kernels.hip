@@ -353,12 +353,14 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t slots = block_excl_scan(cur, G, st.scan);
     const bool fixed = __builtin_amdgcn_readfirstlane((int64_t)slots <= p1 - p0 ? 1 : 0) != 0;
     K* keys = (K*)J.keys + p0;
-    // in slots, the keys gather in LDS when the workgroup has room for them
-    // all, and go to HBM afterwards in one coalesced copy (scattered 4-byte
-    // stores to HBM write whole partial lines)
+    // the keys gather in LDS when the workgroup has room for the column's key
+    // range (its slots, or its entries when it is scattered by offsets) and
+    // go to HBM afterwards in one coalesced copy: scattered 4-byte stores to
+    // HBM write whole partial lines
     const uint32_t cnt_bytes = (12u * (uint32_t)G + 15u) & ~15u;
-    const bool staged = fixed && __builtin_amdgcn_readfirstlane(
-        (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * slots <= lds_bytes ? 1 : 0) != 0;
+    const uint64_t span = fixed ? (uint64_t)slots : (uint64_t)(p1 - p0);
+    const bool staged = __builtin_amdgcn_readfirstlane(
+        (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * span <= lds_bytes ? 1 : 0) != 0;
     K* stage = (K*)(smem + cnt_bytes);
 
     // counts and value statistics (and, in slots, the keys)
@@ -408,7 +410,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
     };
-    if (staged) count_pass(std::integral_constant<int, 2>{});
+    if (fixed && staged) count_pass(std::integral_constant<int, 2>{});
     else if (fixed) count_pass(std::integral_constant<int, 1>{});
     else count_pass(std::integral_constant<int, 0>{});
     __syncthreads();
@@ -466,11 +468,17 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const int32_t vmin = st.vmin;
     if (!fixed) {
         // scatter nonzero values into the groups' key ranges
+        K* dst = staged ? stage : keys;
         grouped([&](uint32_t g, V v) {
             if (!KT::finite(v) || v == V(0)) return;
             const uint32_t pos = atomicAdd(&cur[g], 1u);
-            keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
+            dst[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
         });
+        if (staged) {   // the column's key range out, coalesced
+            __syncthreads();
+            const uint32_t n = (uint32_t)(p1 - p0);
+            for (uint32_t i = (uint32_t)tid; i < n; i += (uint32_t)nt) keys[i] = stage[i];
+        }
     } else {
         K* slot_keys = staged ? stage : keys;
         if (hist) {
@@ -1922,11 +1930,10 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     mark(1);
     size_t split = 12u * (size_t)L.G;
     if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
-    // room for the columns' keys in slots (every group's rows), when it fits:
-    // the kernel gathers them in LDS and writes them out coalesced
-    const size_t staged = (size_t)align16(split) + (L.f64 ? 8u : 4u) * (size_t)L.slots;
-    const unsigned split_lds =
-        (split > 16u && staged <= (size_t)L.split_lds_max) ? align16(staged) : align16(split);
+    // the most LDS the split kernel may take: one 1024-thread workgroup per CU
+    // fits in any case, and a column whose key range fits in what is left
+    // gathers its keys there and writes them out coalesced
+    const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
     const unsigned tab_lds = table_lds_bytes(L.f64);
     auto sort_path = [&](auto v_tag) {
         using V = typename decltype(v_tag)::type;

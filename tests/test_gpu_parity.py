@@ -160,6 +160,24 @@ def test_float_sort_path(mode):
     assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
 
 
+def test_split_scatter_beyond_lds():
+    """Columns of ~50,000 entries scattered by group offsets (their groups hold
+    more rows than they store): 200 KB of keys do not fit a workgroup's LDS,
+    so the scatter writes to HBM; a 5,000-entry column stages in LDS."""
+    say("split scatter beyond lds")
+    rng = np.random.default_rng(21)
+    R, T = 200_000, 30
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.25, 0.25, 0.025]))
+    data = np.round(rng.normal(0, 1.5, len(rows)), 2).astype(np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:25000]] = 0       # ~6,250 stored reference values per wide column
+    gid[perm[25000:150000]] = rng.integers(1, T + 1, 125000)
+    w = dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R)
+    for mode in ("min", "none"):
+        assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
+
+
 @pytest.mark.parametrize("grouped", [20000, 45000])
 def test_dense_split_slots(grouped):
     """Dense columns split in one pass into slots: 20,000 grouped rows keep
