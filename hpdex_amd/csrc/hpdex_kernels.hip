@@ -39,9 +39,10 @@
 // every value (min), above every value (max) or at 0 (mix, and none -- where
 // only explicit zeros count).
 //
-// The split/reference/target kernels use 256-thread workgroups and at most
-// 48 KB of LDS; the count kernel 256 threads and a tuned LDS size (40 KB,
-// four workgroups per CU).
+// The reference and target kernels use 256-thread workgroups and about 48 KB
+// of LDS; the split kernel 1024 threads, one workgroup per CU, with up to
+// 152 KB of LDS to gather a column's keys; the count kernel 256 threads and a
+// tuned LDS size (40 KB, four workgroups per CU).
 #include <climits>
 #include <type_traits>
 
