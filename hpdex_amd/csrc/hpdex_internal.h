@@ -123,8 +123,9 @@ constexpr uint32_t kPackBitmapWords = 2048u;
 // are all integers of one range [lo, lo + bins) (the columns the count kernel
 // takes) keeps each entry also as one 32-bit word (row << vb) | code, with
 // code = the value's rank among the column's distinct nonzero values (the
-// count kernel's bin), pack_zero_code for an explicit zero and pack_skip_code
-// for a non-finite value, and a row outside [0, R) stored as R.  The count
+// count kernel's bin), the number of those values (at least 1: the count
+// kernel's zero column) for an explicit zero and pack_skip_code for a
+// non-finite value, and a row outside [0, R) stored as R.  The count
 // kernel then reads one word per entry instead of a row and a value.  The row
 // takes rb = pack_row_bits(R) bits (R itself included), the code vb = 32 - rb.
 __host__ __device__ inline uint32_t pack_row_bits(uint32_t R) {
@@ -132,9 +133,8 @@ __host__ __device__ inline uint32_t pack_row_bits(uint32_t R) {
     while (b < 32u && (R >> b) != 0u) ++b;
     return b;
 }
-__host__ __device__ inline uint32_t pack_zero_code(uint32_t vb) { return (1u << vb) - 2u; }
 __host__ __device__ inline uint32_t pack_skip_code(uint32_t vb) { return (1u << vb) - 1u; }
-// bins (0 or less: no nonzero value) leave both special codes free
+// bins (0 or less: no nonzero value) and the zero code leave the skip code free
 __host__ __device__ inline bool pack_fits(uint32_t vb, int64_t bins) {
     return vb >= 2u && vb <= 30u && bins <= (int64_t)(1u << vb) - 2;
 }

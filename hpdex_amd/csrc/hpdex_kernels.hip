@@ -1205,7 +1205,6 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     const bool packed = uniform_u32(item.flags & kCountPacked) != 0u;
     const uint32_t pvb = uniform_u32(J.pack_vb);
     const uint32_t pmask = (1u << pvb) - 1u;
-    const uint32_t pzero = pack_zero_code(pvb);
     const int32_t* col_rows = J.rows + p0;
     const V* col_vals = vals + p0;
     const uint32_t* col_packed = J.packed + p0;
@@ -1296,12 +1295,13 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
             uint32_t b;
             bool zero, inb;
             if (kPk) {
-                // the code: value - vmin, the zero code, or the skip code of
-                // a non-finite value (neither counted nor out of range)
+                // the code: a value's bin, NB for an explicit zero (the zero
+                // column), or the skip code of a non-finite value -- so the
+                // counter is row * rs + code for whatever is counted, and no
+                // code lies outside the bins
                 b = r[u] & pmask;
-                zero = b == pzero;
-                inb = b < NB;
-                bad |= b < pzero && !inb;
+                zero = false;
+                inb = b <= NB;
             } else {
                 const V vv = v[u];
                 // |v| <= 32768 covers every integer the column metadata
@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
             }
             // row < 2^16 whenever the entry is counted (a garbage row only
             // feeds a word that is not used)
-            const uint32_t q = __umul24(row, rs) + (zero ? NB : b);
+            const uint32_t q = __umul24(row, rs) + (kPk ? b : (zero ? NB : b));
             const uint32_t word = (mine && (zero || inb)) ? cell_off + (kWide ? q : (q >> 1)) : dummy;
             atomicAdd(&lds_words[word], kWide ? 1u : 1u + (q & 1u) * 65535u);
         }
@@ -1660,7 +1660,9 @@ __global__ __launch_bounds__(256) void pack_columns_kernel(const V* vals, const 
         dense[c] = total;
         zneg[c] = base >= 0 ? 0u : ((int64_t)(-(int64_t)base) < span ? rank((uint32_t)(-base)) : total);
     }
-    const uint32_t zc = pack_zero_code(vb), sc = pack_skip_code(vb);
+    // an explicit zero takes the code one past the last bin (the count
+    // kernel's zero column: a column with no nonzero value has one bin)
+    const uint32_t zc = total > 0u ? total : 1u, sc = pack_skip_code(vb);
     for (int64_t p = p0 + tid; p < p1; p += nt) {
         const V v = vals[p];
         const int32_t r = rows[p];
