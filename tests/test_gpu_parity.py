@@ -494,3 +494,40 @@ def test_histogram_workload_full_size():
     g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
     assert_parity(g, o)
     pb.close(); pp.close(); m.close()
+
+
+def test_float_workload_full_size():
+    """The 400,000 x 18,000 log1p(CP10k) workload (every column on the sort
+    path, split staged in LDS, target windows): batched == plain bit for bit,
+    structural properties everywhere, oracle parity on a column sample that
+    includes the densest genes."""
+    say("float workload generate")
+    w = synthetic.float_case()
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    say("float workload batched")
+    pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1, stats=True)
+    pb.run(2)
+    U1, P, tie, n1, n2 = pb.results(with_stats=True)
+    assert pb.info()["split_columns"] == w["n_cols"]
+    say("float workload plain")
+    pp = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=0, stats=True)
+    pp.run(1)
+    res_plain = pp.results(with_stats=True)
+    for a, b in zip((U1, P, tie, n1, n2), res_plain):
+        assert np.array_equal(a, b)
+    assert np.all(U1 >= 0) and np.all(U1 <= n1 * n2)
+    assert np.array_equal(2 * U1, np.round(2 * U1))
+    assert np.all((P >= 0) & (P <= 1))
+    assert np.all(n1 == np.count_nonzero(w["group_id"] == 0))
+    sizes = np.bincount(w["group_id"][w["group_id"] > 0], minlength=w["n_targets"] + 1)[1:]
+    assert np.array_equal(n2.reshape(-1, w["n_targets"])[0], sizes.astype(float))
+    nnz = np.diff(w["indptr"])
+    cols = np.unique(np.concatenate([np.argsort(nnz)[-8:], np.arange(0, w["n_cols"], 997)]))
+    sub = synthetic.subset_columns(w, cols)
+    say(f"float workload oracle on {len(cols)} columns")
+    o = oracle_stats(sub, "min")
+    T = w["n_targets"]
+    g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
+    assert_parity(g, o)
+    pb.close(); pp.close(); m.close()
