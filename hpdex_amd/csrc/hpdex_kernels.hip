@@ -7,18 +7,18 @@
 //              column from HBM once, keeps it in LDS, bins the reference
 //              slice into a histogram and ranks every target entry against
 //              its prefix, then writes U1, the tie term and the asymptotic
-//              p-value                           (north-star subsystems a, c-e)
+//              p-value                     (group split, ranks, U, p-value)
 //
 //   split      every other column (and any the count kernel hands over):
 //              count stored values per group, exclusive scan, scatter each
 //              group's nonzero values as order-preserving keys (or histogram
-//              bins) into a per-column key range in HBM           (a)
+//              bins) into a per-column key range in HBM    (group split)
 //   reference  sort the reference slice once per gene (register bitonic
 //              runs + LDS merges) and build its rank table -- a bucket index
-//              over the sorted keys, or the histogram prefix -- in HBM (b, d)
+//              over the sorted keys, or the histogram prefix -- in HBM (sort)
 //   target     load the column's rank table into LDS; each wave sorts one
 //              target slice in registers, ranks it against the table and
-//              writes U1, sizes and the asymptotic p          (c, e)
+//              writes U1, sizes and the asymptotic p   (ranks, U, p-value)
 //
 // Ranking never materialises a merge: a target value v needs only
 // lt(v) = #reference values below v and eq(v) = #reference values equal to v:
