@@ -296,6 +296,13 @@ def main():
                          "split/reference/target kernels)")
     ap.add_argument("--count-block", type=int, default=None,
                     help="tuning: workgroup size of the count kernel (256, 512, 1024)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every candidate tuning (histogram/sort cutover, count kernel LDS "
+                         "and workgroup size) on this workload before the timed steps and run "
+                         "them under the fastest")
+    ap.add_argument("--tuning", default=None,
+                    help="apply a tuning JSON written by hpdex_amd.autotune.save (with "
+                         "--autotune: write the chosen tuning there)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the streaming copy kernel over this many MiB (3 launches "
                          "after one warm launch) and print its GB/s: the calibration run for "
@@ -328,6 +335,19 @@ def main():
                            method=1 if o["method"] == "exact" else 2)
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], R)
     log(f"resident matrix: {m.device_bytes() / 1e9:.2f} GB")
+    tuned = None
+    if args.autotune:
+        from hpdex_amd import autotune
+        t0 = time.perf_counter()
+        res = autotune.autotune(m, w["group_id"], T, opts, iters=10, log=log)
+        tuned = {"tuning": res["best"], "ms": res["ms"],
+                 "candidates": len(res["trials"]), "seconds": time.perf_counter() - t0}
+        log(f"autotune: {tuned}")
+        if args.tuning:
+            autotune.save(res, args.tuning)
+    elif args.tuning:
+        from hpdex_amd import autotune
+        tuned = {"tuning": autotune.load(args.tuning), "from": args.tuning}
     pb = _capi.Plan(m, w["group_id"], T, opts, form=1)
     log("warmup (batched)")
     pb.run(args.warmup)
@@ -425,6 +445,8 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
     }
+    if tuned is not None:
+        line["autotune"] = tuned
     line.update(extra)
     print(json.dumps(line), flush=True)
 

@@ -210,7 +210,9 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
  * those of a normal run.                                                    */
 int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size_t err_len);
 
-/* Tunable choices of the device path (set by the autotuner). */
+/* Tunable choices of the device path.  hpdex_amd.autotune times candidate
+ * settings on a caller's job and applies the fastest through
+ * hpdex_set_tuning; the defaults are its choice on integer-count matrices.  */
 typedef struct hpdex_tuning {
     int32_t hist_max_bins;     /* widest integer value range a column may have to use
                                   the histogram path (0 = never, at most 8192)        */

@@ -1,0 +1,112 @@
+"""The autotuner's search logic on the CPU: the candidate set, the choice of
+the fastest trial and the JSON round trip (no device calls)."""
+import json
+
+import pytest
+
+from hpdex_amd import autotune
+
+
+def test_candidates_default_space():
+    c = autotune.candidates()
+    # 2 histogram cutovers x 3 LDS sizes x 3 workgroup sizes, plus the
+    # sort path alone (its LDS and workgroup size do not matter)
+    assert len(c) == 2 * 3 * 3 + 1
+    assert {"hist_max_bins": 0} in c
+    assert len({tuple(sorted(x.items())) for x in c}) == len(c)
+    for x in c:
+        if x["hist_max_bins"]:
+            assert set(x) == {"hist_max_bins", "count_lds_bytes", "count_block"}
+            assert x["count_block"] in (256, 512, 1024)
+            assert x["count_lds_bytes"] % 16 == 0 and 16384 <= x["count_lds_bytes"] <= 155648
+
+
+def test_candidates_custom_space():
+    c = autotune.candidates({"count_block": (256, 512)})
+    assert c == [{"count_block": 256}, {"count_block": 512}]
+
+
+def test_best_picks_least_ms_first_on_ties():
+    trials = [{"count_block": 256, "ms": 0.3}, {"count_block": 512, "ms": 0.2},
+              {"count_block": 1024, "ms": 0.2}, {"count_block": 128, "ms": None}]
+    assert autotune.best(trials)["count_block"] == 512
+    with pytest.raises(ValueError):
+        autotune.best([{"ms": None}])
+
+
+def test_save_load_round_trip(tmp_path, monkeypatch):
+    applied = []
+    monkeypatch.setattr(autotune, "apply", lambda t: applied.append(dict(t)))
+    res = {"best": {"hist_max_bins": 2048, "size_ordered": 1, "count_lds_bytes": 53248,
+                    "count_block": 512, "count_packed": 1},
+           "ms": 0.25, "trials": [{"count_block": 512, "ms": 0.25}]}
+    p = tmp_path / "tuning.json"
+    autotune.save(res, p)
+    assert json.loads(p.read_text())["tuning"] == res["best"]
+    assert autotune.load(p) == res["best"]
+    assert applied == [res["best"]]
+
+
+class _FakePlan:
+    """Stands in for a device plan: results and timing follow the tuning."""
+    tuning = {}
+    results_of = staticmethod(lambda t: (None, None))
+
+    def __init__(self, *a, **k):
+        self.t = dict(_FakePlan.tuning)
+
+    def run(self, iters=1):
+        return 0.0, 1.0 / self.t.get("count_block", 1)
+
+    def results(self):
+        return _FakePlan.results_of(self.t)
+
+    def info(self):
+        return {"count_columns": 1, "count_blocks": 1}
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def fake_device(monkeypatch):
+    import numpy as np
+    state = {"count_block": 256}
+    monkeypatch.setattr(autotune, "current", lambda: dict(state))
+    monkeypatch.setattr(autotune, "apply",
+                        lambda t: (state.update(t), setattr(_FakePlan, "tuning", dict(state))))
+    monkeypatch.setattr(autotune._capi, "Plan", _FakePlan)
+    same = (np.arange(4.0), np.full(4, 0.5))
+    _FakePlan.results_of = staticmethod(lambda t: same)
+    return state
+
+
+def test_autotune_keeps_infeasible_candidates_out(fake_device):
+    def results_of(t):
+        if t["count_block"] == 1024:
+            raise autotune._capi.HpdexError("column 3: too large")
+        import numpy as np
+        return np.arange(4.0), np.full(4, 0.5)
+    _FakePlan.results_of = staticmethod(results_of)
+    res = autotune.autotune(None, None, 1, None, space={"count_block": (256, 1024, 512)})
+    assert [t["ms"] for t in res["trials"]] == [1 / 256, None, 1 / 512]
+    assert res["trials"][1]["error"] == "column 3: too large"
+    assert res["best"]["count_block"] == 512 and fake_device["count_block"] == 512
+
+
+def test_autotune_rejects_a_candidate_that_changes_results(fake_device):
+    import numpy as np
+    _FakePlan.results_of = staticmethod(
+        lambda t: (np.arange(4.0) + (t["count_block"] == 512), np.full(4, 0.5)))
+    with pytest.raises(autotune.TuningMismatch):
+        autotune.autotune(None, None, 1, None, space={"count_block": (256, 512)})
+    assert fake_device["count_block"] == 256      # the tuning in force is restored
+
+
+def test_autotune_keeps_the_tuning_in_force_within_the_margin(fake_device):
+    # 260 is 1.5 % faster than the 256 in force: within the 2 % margin
+    res = autotune.autotune(None, None, 1, None, space={"count_block": (256, 260)})
+    assert res["best"]["count_block"] == 256 and fake_device["count_block"] == 256
+    # with no margin the faster one is taken
+    res = autotune.autotune(None, None, 1, None, space={"count_block": (256, 260)}, margin=0.0)
+    assert res["best"]["count_block"] == 260

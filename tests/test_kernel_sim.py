@@ -254,7 +254,7 @@ def run_count(data, indices, indptr, gid, T, R, mode, count_lds, f64=False, hist
 
 
 @pytest.mark.parametrize("pack", [True, False])
-@pytest.mark.parametrize("block", [256, 512])
+@pytest.mark.parametrize("block", [256, 512, 1024])
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
 def test_sim_count_kernel_modes(mode, block, pack):
     # small signed integers with explicit zeros: one workgroup per column, read
