@@ -299,7 +299,10 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="time every candidate tuning (histogram/sort cutover, count kernel LDS "
                          "and workgroup size) on this workload before the timed steps and run "
-                         "them under the fastest")
+                         "them under the fastest (the default, unless --no-autotune, "
+                         "--count-lds, --count-block or --tuning is given)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the timed steps under the built-in tuning")
     ap.add_argument("--tuning", default=None,
                     help="apply a tuning JSON written by hpdex_amd.autotune.save (with "
                          "--autotune: write the chosen tuning there)")
@@ -336,15 +339,22 @@ def main():
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], R)
     log(f"resident matrix: {m.device_bytes() / 1e9:.2f} GB")
     tuned = None
-    if args.autotune:
+    manual = args.count_lds is not None or args.count_block is not None or args.tuning
+    if args.autotune or not (args.no_autotune or manual):
         from hpdex_amd import autotune
         t0 = time.perf_counter()
-        res = autotune.autotune(m, w["group_id"], T, opts, iters=10, log=log)
-        tuned = {"tuning": res["best"], "ms": res["ms"],
-                 "candidates": len(res["trials"]), "seconds": time.perf_counter() - t0}
-        log(f"autotune: {tuned}")
-        if args.tuning:
-            autotune.save(res, args.tuning)
+        try:
+            res = autotune.autotune(m, w["group_id"], T, opts, log=log)
+        except Exception as e:       # the tuning in force was restored: time under it
+            tuned = {"error": str(e), "tuning": autotune.current()}
+            log(f"autotune failed, timing the tuning in force: {e}")
+        else:
+            n_ok = sum(t["ms"] is not None for t in res["trials"])
+            tuned = {"tuning": res["best"], "ms": res["ms"], "candidates": len(res["trials"]),
+                     "runnable": n_ok, "seconds": time.perf_counter() - t0}
+            log(f"autotune: {tuned}")
+            if args.tuning:
+                autotune.save(res, args.tuning)
     elif args.tuning:
         from hpdex_amd import autotune
         tuned = {"tuning": autotune.load(args.tuning), "from": args.tuning}
