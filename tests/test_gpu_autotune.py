@@ -72,7 +72,7 @@ def test_autotune_candidates_agree_and_match_oracle(mode, restore_tuning):
     np.testing.assert_allclose(P, o[1], rtol=1e-12, atol=0)
 
 
-def test_autotune_skips_a_candidate_the_job_cannot_run(restore_tuning):
+def test_autotune_skips_a_candidate_the_job_cannot_run(restore_tuning, tmp_path):
     # one integer column whose 12,000 reference entries span values 1..3000:
     # under a 2,048-bin cutover it takes the sort path, where the reference
     # slice exceeds the 8,192 keys sorted on chip (the plan reports the
@@ -94,6 +94,12 @@ def test_autotune_skips_a_candidate_the_job_cannot_run(restore_tuning):
         bad, good = res["trials"]
         assert bad["ms"] is None and "column 0" in bad["error"]
         assert good["ms"] > 0 and res["best"]["hist_max_bins"] == 8192
+        # the saved choice is what the library holds after a load
+        autotune.save(res, tmp_path / "tuning.json")
+        autotune.apply({"hist_max_bins": 2048, "count_block": 512})
+        assert autotune.current()["hist_max_bins"] == 2048
+        assert autotune.load(tmp_path / "tuning.json") == res["best"]
+        assert autotune.current() == res["best"]
         p = _capi.Plan(m, gid, T, opts, form=1)
         try:
             p.run(1)
