@@ -34,7 +34,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def load_workload(name):
+_WORKLOAD_ARRAYS = ("data", "indices", "indptr", "group_id")
+
+
+def generate_workload(name):
     from hpdex_amd import synthetic
     if name == "histogram":
         return synthetic.histogram_case()
@@ -42,9 +45,36 @@ def load_workload(name):
         return synthetic.small_case()
     if name == "float":
         return synthetic.float_case()
+    if name == "float-mix":
+        return synthetic.float_case(zero_handling="mix")
     if name == "exact":
         return synthetic.exact_case()
     raise SystemExit(f"unknown workload {name}")
+
+
+def load_workload(name, cache=None):
+    """The seeded workload; with `cache` (a directory) its arrays are read
+    from .npy files there when a previous run saved them (the generators are
+    deterministic, so this only saves the generation time of repeated runs,
+    e.g. one per profiler pass), else generated and saved there."""
+    if not cache:
+        return generate_workload(name)
+    d = os.path.join(cache, name)
+    meta = os.path.join(d, "meta.json")
+    if os.path.exists(meta):
+        with open(meta) as f:
+            w = json.load(f)
+        for k in _WORKLOAD_ARRAYS:
+            w[k] = np.load(os.path.join(d, k + ".npy"))
+        return w
+    w = generate_workload(name)
+    os.makedirs(d, exist_ok=True)
+    for k in _WORKLOAD_ARRAYS:
+        np.save(os.path.join(d, k + ".npy"), w[k])
+    with open(meta + ".tmp", "w") as f:
+        json.dump({k: v for k, v in w.items() if k not in _WORKLOAD_ARRAYS}, f)
+    os.replace(meta + ".tmp", meta)
+    return w
 
 
 def traffic_from_profiles(kernel, workload):
@@ -279,7 +309,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="histogram")
+    ap.add_argument("--workload", default="histogram",
+                    help="histogram (default), small, float, float-mix, exact")
+    ap.add_argument("--workload-cache", default=None,
+                    help="directory to keep the generated workload's arrays in (.npy) and read "
+                         "them from on later runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--stream-limit-gb", type=float, default=0.0,
@@ -326,7 +360,7 @@ def main():
     ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
     log(f"generating workload {args.workload}")
     t0 = time.perf_counter()
-    w = load_workload(args.workload)
+    w = load_workload(args.workload, args.workload_cache)
     o = w["opts"]
     C, T, R = w["n_cols"], w["n_targets"], w["n_rows"]
     nnz = int(w["indptr"][-1])
@@ -343,8 +377,12 @@ def main():
     if args.autotune or not (args.no_autotune or manual):
         from hpdex_amd import autotune
         t0 = time.perf_counter()
+        # knobs given by hand are kept: the sweep covers the others only
+        space = {k: v for k, v in autotune.DEFAULT_SPACE.items()
+                 if not (k == "count_lds_bytes" and args.count_lds is not None)
+                 and not (k == "count_block" and args.count_block is not None)}
         try:
-            res = autotune.autotune(m, w["group_id"], T, opts, log=log)
+            res = autotune.autotune(m, w["group_id"], T, opts, space=space, log=log)
         except Exception as e:       # the tuning in force was restored: time under it
             tuned = {"error": str(e), "tuning": autotune.current()}
             log(f"autotune failed, timing the tuning in force: {e}")

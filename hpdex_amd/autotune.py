@@ -99,17 +99,25 @@ def autotune(matrix, group_id, n_targets, opts, space=None, iters=20, warmup=3, 
     candidate's U1 or p differ from the first runnable candidate's (the
     tuning in force is restored first)."""
     base = current()
+    cands = candidates(space)
+    if not cands:
+        raise ValueError("the tuning space has no candidate")
     trials, ref = [], None
     try:
-        for cand in candidates(space):
-            apply({**base, **cand})
+        for cand in cands:
             try:
+                # a tuning the library refuses (e.g. LDS this device does not
+                # give) is a candidate that cannot run, not a failed sweep
+                apply({**base, **cand})
                 plan = _capi.Plan(matrix, group_id, n_targets, opts, form=1)
             except _capi.HpdexError as e:
                 trials.append({**cand, "ms": None, "error": str(e)})
+                if log:
+                    log(f"autotune {cand}: not runnable ({e})")
                 continue
             try:
-                plan.run(warmup * (2 if not trials else 1))
+                # the first candidate that runs gets a second warm-up
+                plan.run(warmup * (2 if ref is None else 1))
                 U1, P = plan.results()
                 ms = min(plan.run(iters)[1] for _ in range(max(1, repeats)))
                 info = plan.info()

@@ -4,6 +4,9 @@ extension module ``hpdex_amd.backend.bin.kernel``.
 
 Outputs are 1-D read-only float64 arrays of length C*n_targets, gene-major
 (index c*n_targets + g-1), exactly as the reference returns them.
+
+Portions reproduce the reference's text (its interface): Copyright (c) 2025
+Wang Zhongqi, MIT License -- the full notice is in the repository's NOTICE file.
 """
 from typing import Literal
 

@@ -3,11 +3,13 @@
 // What the reference does per gene column (src/cpp/mannwhitneyu.cpp:903-1043)
 // runs here as one of two launch paths over the columns of a job:
 //
-//   count      integer-valued columns (UMI counts): one workgroup reads the
-//              column from HBM once, keeps it in LDS, bins the reference
-//              slice into a histogram and ranks every target entry against
-//              its prefix, then writes U1, the tie term and the asymptotic
-//              p-value                     (group split, ranks, U, p-value)
+//   count      integer-valued columns (UMI counts): one workgroup (or one per
+//              slice of the targets) reads the column from HBM once and
+//              counts every entry into a (group row, value bin) counter in
+//              LDS; a scan of the reference row gives lt/eq per bin, per-
+//              target sums over the bins give 2U and the tie term, then U1,
+//              the tie term and the asymptotic p-value are written
+//                                          (group split, ranks, U, p-value)
 //
 //   split      every other column (and any the count kernel hands over):
 //              count stored values per group, exclusive scan, scatter each

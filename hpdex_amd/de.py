@@ -15,6 +15,11 @@ The row labels keep the reference's layout: target/feature labels and fold
 changes are target-major, while p_value, u_statistic and fdr are gene-major as
 the backend returns them (de.py:160-196), so rows mix pairs exactly as the
 reference's do whenever there is more than one target.
+
+Steps 0-2 (metric and key checks, reference handling, the min_samples filter,
+the group-id map) and the label/DataFrame assembly reproduce the reference's
+text (de.py:53-103, 160-196): Copyright (c) 2025 Wang Zhongqi, MIT License --
+the full notice is in the repository's NOTICE file.
 """
 from __future__ import annotations
 

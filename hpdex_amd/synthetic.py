@@ -245,7 +245,9 @@ def serving_workload(seed: int = 1005, n_matrices: int = 4, n_jobs: int = 64,
         R = pool[k]["n_rows"]
         n_ref = R // 10
         T = int(rng.integers(5, 501))
-        sizes = _target_sizes(rng, T, 200, 0.9, 1, R, R - n_ref)
+        # at least two cells per target: a group of one has no Mann-Whitney
+        # test ("Sample too small", src/cpp/mannwhitneyu.cpp:980-984)
+        sizes = _target_sizes(rng, T, 200, 0.9, 2, R, R - n_ref)
         gid = _assign_groups(rng, R, n_ref, sizes)
         jobs.append(dict(matrix=k, group_id=gid, n_targets=T,
                          resident=bool(rng.random() < resident_frac)))

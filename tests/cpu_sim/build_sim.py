@@ -12,8 +12,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CSRC = os.path.dirname(HERE)
-ROOT = os.path.dirname(os.path.dirname(CSRC))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "hpdex_amd", "csrc")
 OUT = os.path.join(ROOT, "_build", "cpu_sim")
 LIB = os.path.join(OUT, "libhpdex_cpusim.so")
 

@@ -180,6 +180,40 @@ def cases():
                     group_id=gid, n_targets=T, zero_handling="mix", tie_correction=True,
                     use_continuity=False, alternative="two-sided", method="exact", U1=U1, P=P,
                     tie=TIE))
+    out += negative_zero_cases()
+    return out
+
+
+def negative_zero_cases():
+    """Stored explicit zeros as both +0.0 and -0.0, beside negative values, in
+    every zero mode, for integer values (the count path on the device) and
+    continuous ones (the sort path).  The reference compares values with '<'
+    (src/cpp/mannwhitneyu.cpp:639,674), so -0.0 and +0.0 are one value: the
+    same zero block in min/max/mix and one tie run in none, as scipy ranks
+    them.  A generator of its own keeps the earlier fixtures unchanged."""
+    rng = np.random.default_rng(20261204)
+    ints = lambda n: rng.integers(-5, 6, n)
+    floats = lambda n: np.round(rng.normal(0.0, 1.5, n), 1)
+    out = []
+    for mode in ("none", "min", "max", "mix"):
+        for kind, vals in (("int", ints), ("float", floats)):
+            R, C, T = 80, 6, 3
+            gid = rng.integers(-1, T + 1, R).astype(np.int32)
+            gid[:2] = 0
+            for g in range(1, T + 1):
+                gid[2 * g:2 * g + 2] = g
+            data, indices, indptr = random_csc(rng, R, C, 0.6, vals, 0.25, 0.02)
+            zero = data == 0.0
+            data[zero & (rng.random(len(data)) < 0.5)] = -0.0
+            if mode == "none":
+                data, indices, indptr = _ensure_none_sizes(rng, data, indices, indptr, R, gid, T)
+            assert np.any(np.signbit(data) & (data == 0.0)) and np.any(~np.signbit(data) & (data == 0.0))
+            U1, P, TIE = expected(data, indices, indptr, R, gid, T, mode, True, True,
+                                  "two-sided", "asymptotic")
+            out.append(dict(name=f"negzero_{mode}_{kind}", data=data, indices=indices,
+                            indptr=indptr, n_rows=R, group_id=gid, n_targets=T,
+                            zero_handling=mode, tie_correction=True, use_continuity=True,
+                            alternative="two-sided", method="asymptotic", U1=U1, P=P, tie=TIE))
     return out
 
 

@@ -110,3 +110,38 @@ def test_autotune_keeps_the_tuning_in_force_within_the_margin(fake_device):
     # with no margin the faster one is taken
     res = autotune.autotune(None, None, 1, None, space={"count_block": (256, 260)}, margin=0.0)
     assert res["best"]["count_block"] == 260
+
+
+def test_autotune_records_a_tuning_the_library_refuses(fake_device, monkeypatch):
+    """set_tuning refusing a candidate (e.g. LDS the device does not give) is
+    one candidate that cannot run, recorded with its error; the sweep goes on,
+    and the first candidate that runs gets the doubled warm-up."""
+    warmups = []
+    orig_run = _FakePlan.run
+
+    def run(self, iters=1):
+        warmups.append(iters)
+        return orig_run(self, iters)
+
+    monkeypatch.setattr(_FakePlan, "run", run)
+    state = fake_device
+
+    def apply(t):
+        if t.get("count_block") == 1024:
+            raise autotune._capi.HpdexError("this device does not give the count kernel its LDS")
+        state.update(t)
+        _FakePlan.tuning = dict(state)
+
+    monkeypatch.setattr(autotune, "apply", apply)
+    res = autotune.autotune(None, None, 1, None, space={"count_block": (1024, 256, 512)},
+                            warmup=3, iters=5, repeats=1)
+    assert res["trials"][0]["ms"] is None and "LDS" in res["trials"][0]["error"]
+    assert [t["ms"] for t in res["trials"][1:]] == [1 / 256, 1 / 512]
+    # warm-up of the first runnable candidate doubled (6), the next one's not (3)
+    assert warmups == [6, 5, 3, 5]
+    assert res["best"]["count_block"] == 512
+
+
+def test_autotune_empty_space_is_an_error(fake_device):
+    with pytest.raises(ValueError, match="no candidate"):
+        autotune.autotune(None, None, 1, None, space={"count_block": ()})

@@ -2,6 +2,7 @@
 the golden fixtures.  U1 and the tie term must be bit-identical; asymptotic
 and exact p within 1e-12 relative of the FP64 oracle; group means within
 1e-6 relative."""
+import os
 import sys
 
 import numpy as np
@@ -32,10 +33,14 @@ def gpu_stats(w, mode, alt="two-sided", method="asymptotic", tie=True, cc=True):
                          opts, n_rows=w["n_rows"], with_stats=True)
 
 
+ORACLE_THREADS = min(16, os.cpu_count() or 1)
+
+
 def oracle_stats(w, mode, alt="two-sided", method="asymptotic", tie=True, cc=True):
     return oracle.mwu(w["data"], w["indices"], w["indptr"], w["group_id"], w["n_targets"],
                       n_rows=w["n_rows"], tie_correction=tie, use_continuity=cc,
-                      zero_handling=mode, alternative=alt, method=method, with_stats=True)
+                      zero_handling=mode, alternative=alt, method=method, with_stats=True,
+                      threads=ORACLE_THREADS)
 
 
 def assert_parity(g, o, p_rtol=P_RTOL):
@@ -82,6 +87,56 @@ def test_small_config_and_wrapper():
     assert np.array_equal(U1, exp["U1"]) and rel_close(P, exp["P"])
     gm = group_mean(M, w["group_id"], 5, include_zeros=True)
     assert rel_close(gm, exp["group_mean"], rtol=1e-6, atol=0)
+
+
+def test_kernel_module_default_arguments():
+    """kernel.mannwhitneyu with every default of the extension module
+    (module.cpp:215-228: tie correction on, continuity off, fast_norm=True,
+    zero_handling none, two-sided, asymptotic): fast_norm runs the precise
+    erfc path, so p matches the oracle's precise p to 1e-12."""
+    say("kernel module defaults")
+    from hpdex_amd.backend.bin import kernel
+    w = synthetic.small_case()
+    U1, P = kernel.mannwhitneyu(w["data"], w["indices"], w["indptr"], w["group_id"], 4)
+    oU1, oP = oracle.mwu(w["data"], w["indices"], w["indptr"], w["group_id"], 4,
+                         tie_correction=True, use_continuity=False, zero_handling="none")
+    assert np.array_equal(U1, oU1)
+    assert rel_close(P, oP, rtol=P_RTOL)
+    U1f, Pf = kernel.mannwhitneyu(w["data"], w["indices"], w["indptr"], w["group_id"], 4,
+                                  fast_norm=False)
+    assert np.array_equal(U1f, U1) and np.array_equal(Pf, P)
+
+
+NEGZERO = [c for c in CASES if c["name"].startswith("negzero")]
+
+
+@pytest.mark.parametrize("case", NEGZERO, ids=[c["name"] for c in NEGZERO])
+def test_negative_zero_float32_paths(case, restore_tuning):
+    """Stored -0.0 beside +0.0 and negative values, as float32: integer
+    columns on the count path (packed entries and rows + values) and on the
+    sort path's histogram branch, continuous columns on the sort path; every
+    route equals the oracle and the scipy-derived golden expectation."""
+    say(f"negative zero {case['name']}")
+    d32 = case["data"].astype(np.float32)
+    assert np.any((d32 == 0) & np.signbit(d32))
+    w = dict(data=d32, indices=case["indices"], indptr=case["indptr"],
+             group_id=case["group_id"], n_targets=case["n_targets"], n_rows=case["n_rows"])
+    mode = case["zero_handling"]
+    o = oracle_stats(w, mode)
+    assert np.array_equal(o[0], case["U1"]) and np.array_equal(o[2], case["tie"])
+    routes = [(1, 40448), (0, 40448), (1, 0)] if case["name"].endswith("int") else [(1, 40448)]
+    for packed, lds in routes:
+        _capi.set_tuning(count_packed=packed, count_lds_bytes=lds)
+        m, p = _plan_of(w, mode)
+        p.run(1)
+        info = p.info()
+        if case["name"].endswith("int"):
+            assert info["count_columns"] == (len(case["indptr"]) - 1 if lds else 0)
+        else:
+            assert info["split_columns"] == len(case["indptr"]) - 1
+        assert_parity(p.results(with_stats=True), o)
+        p.close(); m.close()
+    assert_parity(gpu_stats(w, mode), o)
 
 
 @pytest.mark.parametrize("k", range(4))
@@ -463,8 +518,8 @@ def test_plans_plain_batched_bit_identical():
 
 def test_histogram_workload_full_size():
     """The 150,000 x 8,000 integer-count workload: batched == plain bit for
-    bit, structural properties everywhere, oracle parity on a column sample
-    that includes the densest genes."""
+    bit, structural properties everywhere, and oracle parity on every one of
+    the 2.4M (gene, target) pairs."""
     say("histogram workload generate")
     w = synthetic.histogram_case()
     opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
@@ -485,27 +540,27 @@ def test_histogram_workload_full_size():
     assert np.all(n1 == 15000)
     sizes = np.bincount(w["group_id"][w["group_id"] > 0], minlength=w["n_targets"] + 1)[1:]
     assert np.array_equal(n2.reshape(-1, w["n_targets"])[0], sizes.astype(float))
-    nnz = np.diff(w["indptr"])
-    cols = np.unique(np.concatenate([np.argsort(nnz)[-40:], np.arange(0, w["n_cols"], 97)]))
-    sub = synthetic.subset_columns(w, cols)
-    say(f"histogram workload oracle on {len(cols)} columns")
-    o = oracle_stats(sub, "min")
-    T = w["n_targets"]
-    g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
-    assert_parity(g, o)
+    say("histogram workload oracle on every column")
+    assert_parity((U1, P, tie, n1, n2), oracle_stats(w, "min"))
     pb.close(); pp.close(); m.close()
 
 
-def test_float_workload_full_size():
-    """The 400,000 x 18,000 log1p(CP10k) workload (every column on the sort
-    path, split staged in LDS, target windows): batched == plain bit for bit,
-    structural properties everywhere, oracle parity on a column sample that
-    includes the densest genes."""
+@pytest.fixture(scope="module")
+def float_workload():
     say("float workload generate")
-    w = synthetic.float_case()
-    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    return synthetic.float_case()
+
+
+@pytest.mark.parametrize("mode", ["min", "mix"])
+def test_float_workload_full_size(float_workload, mode):
+    """The 400,000 x 18,000 log1p(CP10k) workload in zero_handling min and mix
+    (every column on the sort path, split staged in LDS, target windows):
+    batched == plain bit for bit, structural properties everywhere, oracle
+    parity on a column sample that includes the densest genes."""
+    w = float_workload
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=ZH[mode])
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
-    say("float workload batched")
+    say(f"float workload batched {mode}")
     pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1, stats=True)
     pb.run(2)
     U1, P, tie, n1, n2 = pb.results(with_stats=True)
@@ -526,8 +581,47 @@ def test_float_workload_full_size():
     cols = np.unique(np.concatenate([np.argsort(nnz)[-8:], np.arange(0, w["n_cols"], 997)]))
     sub = synthetic.subset_columns(w, cols)
     say(f"float workload oracle on {len(cols)} columns")
-    o = oracle_stats(sub, "min")
+    o = oracle_stats(sub, mode)
     T = w["n_targets"]
     g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
     assert_parity(g, o)
     pb.close(); pp.close(); m.close()
+
+
+def test_exact_workload_full_size():
+    """The dense 150,000 x 6,000 workload with 2,000 targets of 3-25 cells and
+    a 60-cell reference, exact p in FP64 (tie correction on, continuity off):
+    batched == plain bit for bit, structural properties on every pair, and
+    oracle parity (U1 and tie bit-exact, p within 1e-12 of the FP64 DP) on a
+    column sample -- every column holds every target."""
+    say("exact workload generate")
+    w = synthetic.exact_case()
+    opts = _capi.make_opts(tie_correction=True, use_continuity=False, zero_handling=1, method=1)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    say("exact workload batched")
+    pb = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1, stats=True)
+    pb.run(2)
+    U1, P, tie, n1, n2 = pb.results(with_stats=True)
+    say("exact workload plain")
+    pp = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=0, stats=True)
+    pp.run(1)
+    for a, b in zip((U1, P, tie, n1, n2), pp.results(with_stats=True)):
+        assert np.array_equal(a, b)
+    pb.close(); pp.close(); m.close()
+    T = w["n_targets"]
+    sizes = np.bincount(w["group_id"][w["group_id"] > 0], minlength=T + 1)[1:]
+    assert np.all(n1 == 60)
+    assert np.array_equal(n2.reshape(-1, T), np.broadcast_to(sizes.astype(float), (w["n_cols"], T)))
+    assert np.all(U1 >= 0) and np.all(U1 <= n1 * n2)
+    assert np.array_equal(2 * U1, np.round(2 * U1))
+    assert np.all((P > 0) & (P <= 1))
+    # float32 normals are nearly tie-free: few pairs carry a tie term, each
+    # a sum of t^3 - t over runs of t >= 2 (at least 6)
+    assert np.all(tie >= 0) and np.array_equal(tie, np.round(tie))
+    assert np.all((tie == 0) | (tie >= 6)) and np.mean(tie > 0) < 0.05
+    cols = np.arange(0, w["n_cols"], 40)
+    sub = synthetic.subset_columns(w, cols)
+    say(f"exact workload oracle on {len(cols)} columns")
+    o = oracle_stats(sub, "min", method="exact", cc=False)
+    g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
+    assert_parity(g, o)

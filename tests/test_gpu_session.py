@@ -174,3 +174,56 @@ def test_session_reports_job_errors():
     a = _alone(m, gid, 4, opts)
     assert np.array_equal(U1, a[0]) and np.array_equal(P, a[1])
     sess.close()
+
+
+def test_serving_workload_distribution():
+    """Jobs drawn from the serving distribution (cells U[5k, 200k], genes
+    U[2k, 20k], density U[3 %, 12 %], NB(LogNormal(0, 1), r = 0.7) + 1 counts,
+    a reference of 10 % of the cells, 5-500 targets, 30 % re-querying a
+    resident matrix) submitted at once into a 4-slot session: each job equals
+    the oracle (U1 bit for bit, p within 1e-12) on a sample of its genes that
+    includes its densest ones, and a few jobs equal the same job run alone."""
+    import os
+    say("serving workload generate")
+    pool, jobs = synthetic.serving_workload(seed=1105, n_matrices=3, n_jobs=12)
+    opts = _capi.make_opts(True, True, ZH["min"])
+    resident = {k: _capi.DeviceMatrix(pool[k]["data"], pool[k]["indices"], pool[k]["indptr"],
+                                      pool[k]["n_rows"])
+                for k in sorted({j["matrix"] for j in jobs if j["resident"]})}
+    assert resident, "the sample has resident re-queries"
+    sess = _capi.Session(budget_bytes=48 << 30, slots=4)
+    tickets = []
+    for j in jobs:
+        m = pool[j["matrix"]]
+        if j["resident"]:
+            tickets.append(sess.submit(j["group_id"], j["n_targets"], opts,
+                                       matrix=resident[j["matrix"]]))
+        else:
+            tickets.append(sess.submit(j["group_id"], j["n_targets"], opts, data=m["data"],
+                                       indices=m["indices"].astype(np.int32),
+                                       indptr=m["indptr"].astype(np.int32), n_rows=m["n_rows"]))
+    say(f"serving workload: {len(jobs)} jobs submitted")
+    results = [sess.wait(t)[:2] for t in tickets]
+    assert sess.peak()["jobs_in_flight"] >= 2
+    sess.close()
+    for m in resident.values():
+        m.close()
+    threads = min(16, os.cpu_count() or 1)
+    for k, (j, (U1, P)) in enumerate(zip(jobs, results)):
+        m = pool[j["matrix"]]
+        T, C = j["n_targets"], m["n_cols"]
+        nnz = np.diff(m["indptr"])
+        cols = np.unique(np.concatenate([np.argsort(nnz)[-5:], np.linspace(0, C - 1, 25)
+                                         .astype(np.int64)]))
+        sub = synthetic.subset_columns(dict(m, group_id=j["group_id"], n_targets=T), cols)
+        say(f"serving job {k}: {m['n_rows']} x {C}, T={T}, oracle on {len(cols)} genes")
+        oU1, oP = oracle.mwu(sub["data"], sub["indices"], sub["indptr"], j["group_id"], T,
+                             n_rows=m["n_rows"], tie_correction=True, use_continuity=True,
+                             zero_handling="min", threads=threads)
+        gU1 = U1.reshape(C, T)[cols].reshape(-1)
+        gP = P.reshape(C, T)[cols].reshape(-1)
+        assert np.array_equal(gU1, oU1), k
+        assert rel_close(gP, oP, rtol=1e-12), k
+        if k < 3:
+            a = _alone(m, j["group_id"], T, opts)
+            assert np.array_equal(U1, a[0]) and np.array_equal(P, a[1]), k

@@ -1,6 +1,8 @@
 """Streaming DE over gene runs of a CSC matrix held in memory-mapped files:
-every output equals the one-shot hpdex_de_csc on the whole matrix bit for
-bit, whatever the number of runs and of runs in flight."""
+every output equals the oracle (U1 bit for bit, p and FDR within 1e-12,
+fold changes within 1e-12 of the oracle's FP64 group means) and the one-shot
+hpdex_de_csc on the whole matrix bit for bit, whatever the number of runs and
+of runs in flight."""
 import sys
 from types import SimpleNamespace
 
@@ -9,6 +11,9 @@ import pandas as pd
 import pytest
 import scipy.sparse as sp
 
+import oracle
+import oracle.de as oracle_de
+from fixtures import rel_close
 from hpdex_amd import _capi, stream
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +59,18 @@ def test_stream_equals_one_shot(tmp_path, workers):
                         n_rows=X.shape[0])
     for g, w in zip(got, want):
         assert np.array_equal(g, w, equal_nan=True)
+    # against the oracle: the test (zero_handling min, tie + continuity, as
+    # de.py:147-158 calls it), fold changes from its group means, BH FDR
+    U1, P, fc, lfc, fdr = got
+    oU1, oP = oracle.mwu(X.data, X.indices, X.indptr, gid, T, n_rows=X.shape[0],
+                         tie_correction=True, use_continuity=True, zero_handling="min")
+    assert np.array_equal(U1, oU1)
+    assert rel_close(P, oP, rtol=1e-12)
+    means = oracle.group_mean(X.data, X.indices, X.indptr, gid, T + 1, include_zeros=True)
+    ofc, olfc = oracle_de.fold_change(means, X.shape[1], T)
+    assert rel_close(fc, ofc, rtol=1e-12, atol=1e-15)
+    assert rel_close(lfc, olfc, rtol=1e-12, atol=1e-12)
+    assert rel_close(fdr, oracle_de.fdr_bh(oP), rtol=1e-12)
 
 
 def test_stream_driver_target_major(tmp_path):

@@ -1,5 +1,5 @@
 """The gfx950 kernel source executed on the host (one thread per lane; see
-hpdex_amd/csrc/cpu_sim) against the oracle: U1, tie term and sample sizes
+tests/cpu_sim) against the oracle: U1, tie term and sample sizes
 bit-identical, p identical.  CPU only; covers the histogram and sort paths,
 every zero mode, reference slices above one register run, target slices
 split into several register runs, and 8-byte keys."""
@@ -9,7 +9,7 @@ import pytest
 import oracle
 from fixtures import mwu_cases
 from hpdex_amd import synthetic
-from hpdex_amd.csrc.cpu_sim import sim
+from cpu_sim import sim
 
 ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
 ALT = {"less": 0, "greater": 1, "two-sided": 2}
