@@ -199,7 +199,8 @@ def stream_bench(w, limit_gb, workers=3):
             f = os.path.join(d, name + ".npy")
             np.save(f, w[name])
             arrs.append(np.load(f, mmap_mode="r"))
-        runs = stream.plan_gene_runs(arrs[2], T, int(limit_gb * 2**30) // workers)
+        runs = stream.plan_gene_runs(arrs[2], T, int(limit_gb * 2**30) // workers, R,
+                                     arrs[0].dtype)
         t0 = time.perf_counter()
         got = stream.stream_de(*arrs, R, w["group_id"], T, memory_limit_gb=limit_gb,
                                num_workers=workers)

@@ -31,6 +31,7 @@ EXPORTED = (
     "hpdex_session_create", "hpdex_session_submit", "hpdex_session_wait", "hpdex_session_peak",
     "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc", "hpdex_matrix_create_from",
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
+    "hpdex_de_csc_device_bytes",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -136,6 +137,8 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(sz)]
         L.hpdex_matrix_download.argtypes = [vp, vp, vp, vp, cp, sz]
         L.hpdex_de_matrix.argtypes = [vp, vp, sz, ci, ci, ctypes.c_double, vp, vp, vp, vp, vp, cp, sz]
+        L.hpdex_de_csc_device_bytes.argtypes = [ci, sz, sz, sz, sz]
+        L.hpdex_de_csc_device_bytes.restype = sz
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
                      "hpdex_matrix_create", "hpdex_plan_create", "hpdex_plan_run",
                      "hpdex_plan_results", "hpdex_get_tuning", "hpdex_set_tuning",
@@ -236,6 +239,13 @@ def de_csc(data, indices, indptr, group_id, n_targets, n_rows=None, tie_correcti
     if rc != 0:
         raise HpdexError(err.value.decode())
     return tuple(o[:n] for o in outs)
+
+
+def de_csc_device_bytes(dtype, n_rows, n_cols, nnz, n_targets) -> int:
+    """Upper bound of the device bytes one de_csc call on a matrix of this
+    shape allocates (hpdex_de_csc_device_bytes; computed on the host)."""
+    return int(lib().hpdex_de_csc_device_bytes(DTYPE_CODES[np.dtype(dtype)], int(n_rows),
+                                                int(n_cols), int(nnz), int(n_targets)))
 
 
 def _index_array(a):

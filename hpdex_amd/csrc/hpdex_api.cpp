@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -646,6 +647,70 @@ int run_exact_none(JobState& js, hipStream_t s, char* err, size_t err_len) {
     return exact_launch(js, s, true, err, err_len);
 }
 
+size_t up256(size_t n) { return ((n ? n : 16) + 255) & ~(size_t)255; }
+
+size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, int index_bits,
+                        bool pack) {
+    const bool f64 = dtype_uses_double(dtype);
+    const size_t vsz = f64 ? 8 : 4, ib = index_bits == 32 ? 4 : 8;
+    // values and rows with their padding, column pointers, value ranges, the
+    // in-order flags and the classification's staging flags
+    size_t f = up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
+    f += up256(2 * C * 4) + 2 * up256(C);
+    if (layout == HPDEX_LAYOUT_CSR) {
+        // raw values, column indices and row pointers, the transpose workspace
+        f += up256(nnz * dtype_size(dtype)) + up256(nnz * ib) + up256((R + 1) * ib);
+        f += up256(csr_workspace_bytes(nnz));
+    } else {
+        if (dtype != HPDEX_FLOAT32 && dtype != HPDEX_FLOAT64) f += up256(nnz * dtype_size(dtype));
+        if (ib == 8) f += up256(nnz * 8);   // int64 rows staged for the conversion
+    }
+    if (pack) f += up256(nnz * 4) + 3 * up256(C * 4);
+    return f;
+}
+
+size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const hpdex_mwu_opts& o,
+                     const uint32_t* group_rows, bool stats) {
+    const size_t vsz = f64 ? 8 : 4, G = T + 1, np = C * T;
+    // job_setup: labels (32-bit and packed at <= 16 bits), group sizes, U1 and P,
+    // error block, the tie term and sample sizes when kept, sort-path workspace
+    size_t f = up256(R * 4) + up256(label_words(R, 16) * 4) + up256(G * 4) + 2 * up256(np * 8) +
+               up256(32);
+    if (stats) f += up256(np * 8);
+    if (stats || (o.method == 1 && o.zero_handling == 0)) f += 2 * up256(np * 8);
+    f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));
+    f += up256(C * (size_t)kTabStride * 4);
+    // job_upload_desc: descriptor, count-kernel items, sort-path queue
+    f += up256(sizeof(JobDesc)) + up256(C * kMaxSlices * sizeof(CountItem)) +
+         up256(C * sizeof(WorkItem)) + up256(16);
+    // exact-null tables of min/max/mix: sizes are the group row counts
+    if (o.method == 1 && o.zero_handling != 0 && group_rows) {
+        std::set<std::pair<int32_t, int32_t>> pairs;
+        for (size_t g = 1; g <= T; ++g) pairs.emplace((int32_t)group_rows[0], (int32_t)group_rows[g]);
+        size_t tab = 0, max_cells = 0;
+        for (const auto& pr : pairs) {
+            const size_t U = (size_t)pr.first * pr.second;
+            tab += 1 + 2 * (U + 1);
+            const size_t s_ = (size_t)std::min(pr.first, pr.second);
+            max_cells = std::max(max_cells, (s_ + 1) * (U + 1));
+        }
+        f += up256(pairs.size() * sizeof(ExactTable)) + up256(tab * 8);
+        f += up256(pairs.size() * 2 * max_cells * 8) + up256(T * 4);
+    }
+    return f;
+}
+
+size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T) {
+    hpdex_mwu_opts o{};
+    o.zero_handling = 1;
+    o.alternative = 2;
+    o.method = 2;
+    const size_t G = T + 1, np = C * T;
+    return matrix_footprint(dtype, R, C, nnz, HPDEX_LAYOUT_CSC, 64, false) +
+           job_footprint(R, C, nnz, dtype_uses_double(dtype), T, o, nullptr, false) +
+           up256(C * G * 8) + 3 * up256(np * 8) + up256(fdr_workspace_bound((int64_t)np));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -826,6 +891,8 @@ int hpdex_fdr_bh(const double* p, size_t n, double* fdr_out, char* err, size_t e
     HIPCHK(dp.alloc(n * 8));
     HIPCHK(df.alloc(n * 8));
     const size_t wsb = fdr_workspace_bytes((int64_t)n);
+    if (wsb > fdr_workspace_bound((int64_t)n))
+        return set_err(err, err_len, "FDR workspace above its stated bound");
     HIPCHK(ws.alloc(wsb));
     HIPCHK(hipMemcpyAsync(dp.p, p, n * 8, hipMemcpyHostToDevice, st.s));
     HIPCHK(launch_fdr_bh(dp.as<double>(), (int64_t)n, df.as<double>(), ws.p, wsb, st.s));
@@ -869,6 +936,9 @@ static int de_run(hpdex_matrix& m, const int32_t* group_id, size_t n_targets, in
     // de.py:172-184: BH FDR over every p-value
     HIPCHK(fdr.alloc(np * 8));
     const size_t wsb = fdr_workspace_bytes((int64_t)np);
+    // de_footprint counts the bound (it is computed without the device)
+    if (wsb > fdr_workspace_bound((int64_t)np))
+        return set_err(err, err_len, "FDR workspace above its stated bound");
     HIPCHK(ws.alloc(wsb));
     HIPCHK(launch_fdr_bh(js.P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, s));
     const size_t b = np * 8;
@@ -993,6 +1063,12 @@ int hpdex_matrix_create(int dtype, const void* data, const int64_t* indices,
 }
 
 void hpdex_matrix_destroy(hpdex_matrix* m) { delete m; }
+
+size_t hpdex_de_csc_device_bytes(int dtype, size_t n_rows, size_t n_cols, size_t nnz,
+                                 size_t n_targets) {
+    if (!valid_dtype(dtype) || n_targets < 1) return 0;
+    return de_footprint(dtype, n_rows, n_cols, nnz, n_targets);
+}
 
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
     if (!m) return 0;

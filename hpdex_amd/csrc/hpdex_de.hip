@@ -117,6 +117,18 @@ size_t fdr_workspace_bytes(int64_t n) {
     return 2 * a(8 * (size_t)n) + 2 * a(4 * (size_t)n) + 2 * a(8 * (size_t)n) + 256 + a(tmp);
 }
 
+// The same layout with rocPRIM's scratch bounded by 16 bytes per p-value
+// plus 4 MiB instead of asked for (its size query needs the device): a radix
+// sort of 8-byte keys with 4-byte values keeps at most one spare copy of both
+// (12 bytes per value) and per-block digit counts; launch_fdr_bh refuses a
+// workspace query above this bound.
+size_t fdr_workspace_bound(int64_t n) {
+    if (n <= 1) return 0;
+    auto a = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 2 * a(8 * (size_t)n) + 2 * a(4 * (size_t)n) + 2 * a(8 * (size_t)n) + 256 +
+           a(16 * (size_t)n + ((size_t)4 << 20));
+}
+
 hipError_t launch_fdr_bh(const double* p, int64_t n, double* fdr, void* ws, size_t ws_bytes,
                          hipStream_t s) {
     if (n <= 0) return hipSuccess;

@@ -184,4 +184,23 @@ int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1,
 bool opts_ok(const hpdex_mwu_opts* o);
 int run_exact_none(JobState& js, hipStream_t s, char* err, size_t err_len);
 
+// Device bytes, as BumpBlock::take rounds every DevBuf (256 bytes, at least
+// one 256-byte unit), of:
+//   matrix_footprint  -- upload_matrix / upload_matrix_from for a CSC (int32 or
+//                        int64 indices) or CSR source, staging included, with
+//                        packed entries when `pack`;
+//   job_footprint     -- job_setup + job_upload_desc + the exact-null tables of
+//                        zero_handling min/max/mix (group_rows: G row counts;
+//                        may be null when the job is asymptotic).  The tables
+//                        of exact p in mode none depend on the data and are
+//                        not included;
+//   de_footprint      -- everything one hpdex_de_csc call allocates.
+// Upper bounds: count-kernel items are counted at kMaxSlices per column.
+size_t up256(size_t n);
+size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, int index_bits,
+                        bool pack);
+size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const hpdex_mwu_opts& o,
+                     const uint32_t* group_rows, bool stats);
+size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T);
+
 }  // namespace hpdex_host

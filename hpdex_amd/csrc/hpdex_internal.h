@@ -313,6 +313,8 @@ hipError_t launch_fold_change(const double* means, int64_t C, int32_t T, double 
 // Benjamini-Hochberg FDR of n p-values (n < 2^32) in a workspace of
 // fdr_workspace_bytes(n); NaN everywhere if a p-value is outside [0, 1].
 size_t fdr_workspace_bytes(int64_t n);
+// An upper bound of fdr_workspace_bytes(n) computed without the device.
+size_t fdr_workspace_bound(int64_t n);
 hipError_t launch_fdr_bh(const double* p, int64_t n, double* fdr, void* ws, size_t ws_bytes,
                          hipStream_t s);
 

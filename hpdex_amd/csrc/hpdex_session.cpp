@@ -88,8 +88,6 @@ struct JobRec {
     std::string err;
 };
 
-size_t up256(size_t n) { return ((n ? n : 16) + 255) & ~(size_t)255; }
-
 // The host matrix a job brings, as hpdex_matrix_create_from takes it.
 hpdex_matrix_src source_of(const hpdex_job& j) {
     hpdex_matrix_src src{};
@@ -105,51 +103,16 @@ hpdex_matrix_src source_of(const hpdex_job& j) {
     return src;
 }
 
-// Upper bound of what one job allocates through DevBuf (every allocation is
-// rounded to 256 bytes, as BumpBlock::take rounds it).
-size_t job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows) {
+// Upper bound of what one job allocates through DevBuf: its matrix, if it
+// brings one, and its workspace (the library's own footprint formulas).
+size_t session_job_footprint(const hpdex_job& j, const std::vector<uint32_t>& group_rows) {
     const hpdex_matrix* m = j.matrix;
     const size_t R = m ? m->R : j.n_rows, C = m ? m->C : j.n_cols, nnz = m ? m->nnz : j.nnz;
     const bool f64 = m ? m->f64 : dtype_uses_double(j.dtype);
-    const size_t vsz = f64 ? 8 : 4;
-    const size_t T = j.n_targets, G = T + 1, np = C * T;
     size_t f = 0;
-    if (!m) {   // upload_matrix_from
-        f += up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
-        f += up256(2 * C * 4) + 2 * up256(C);
-        const size_t ib = j.index_bits == 32 ? 4 : 8;
-        if (j.layout == HPDEX_LAYOUT_CSR) {
-            // raw values, column indices, row pointers, transpose workspace
-            f += up256(nnz * dtype_size(j.dtype)) + up256(nnz * ib) + up256((R + 1) * ib);
-            f += up256(csr_workspace_bytes(nnz));
-        } else {
-            if (j.dtype != HPDEX_FLOAT32 && j.dtype != HPDEX_FLOAT64)
-                f += up256(nnz * dtype_size(j.dtype));
-            if (ib == 8) f += up256(nnz * 8);   // int64 rows staged for the conversion
-        }
-    }
-    // job_setup
-    f += up256(R * 4) + up256(label_words(R, 16) * 4) + up256(G * 4) + 2 * up256(np * 8) + up256(32);
-    f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));
-    f += up256(C * (size_t)kTabStride * 4);
-    // job_upload_desc
-    f += up256(sizeof(JobDesc)) + up256(C * kMaxSlices * sizeof(CountItem)) +
-         up256(C * sizeof(WorkItem)) + up256(16);
-    // exact-null tables of min/max/mix: sizes are the group row counts
-    if (j.opts.method == 1) {
-        std::set<std::pair<int32_t, int32_t>> pairs;
-        for (size_t g = 1; g <= T; ++g) pairs.emplace((int32_t)group_rows[0], (int32_t)group_rows[g]);
-        size_t tab = 0, max_cells = 0;
-        for (const auto& pr : pairs) {
-            const size_t U = (size_t)pr.first * pr.second;
-            tab += 1 + 2 * (U + 1);
-            const size_t s_ = (size_t)std::min(pr.first, pr.second);
-            max_cells = std::max(max_cells, (s_ + 1) * (U + 1));
-        }
-        f += up256(pairs.size() * sizeof(ExactTable)) + up256(tab * 8);
-        f += up256(pairs.size() * 2 * max_cells * 8) + up256(T * 4);
-    }
-    return f;
+    if (!m)
+        f += matrix_footprint(j.dtype, R, C, nnz, j.layout, j.index_bits == 32 ? 32 : 64, false);
+    return f + job_footprint(R, C, nnz, f64, j.n_targets, j.opts, group_rows.data(), false);
 }
 
 }  // namespace
@@ -341,7 +304,7 @@ int hpdex_session_submit(hpdex_session* S, const hpdex_job* job, int64_t* ticket
     }
     auto rec = std::make_shared<JobRec>();
     rec->job = j;
-    rec->footprint = job_footprint(j, group_rows);
+    rec->footprint = session_job_footprint(j, group_rows);
     if (rec->footprint > S->o.hbm_budget_bytes)
         return set_err(err, err_len,
                        "job needs " + std::to_string(rec->footprint) +

@@ -32,37 +32,39 @@ logger = logging.getLogger(__name__)
 
 COLUMNS = ["target", "feature", "p_value", "u_statistic", "fold_change", "log2_fold_change", "fdr"]
 
-# Bytes one stored value costs while its run is in flight: the host copy of
-# its value and int64 row index, and on the device its value, int32 row, the
-# int64 staging of the upload and a sort-path key (a run's matrix is used
-# once, so it gets no packed entries).
-BYTES_PER_VALUE = 40
-# Bytes per (gene, target) pair and per (gene, group) in flight: five float64
-# outputs on the host, their device buffers and sort-path counters, group means.
-BYTES_PER_PAIR = 96
-BYTES_PER_GROUP = 16
+# Host bytes of a run in flight: its values read from the source and their
+# int64 row indices and column pointers (copies made for the call), and the
+# five float64 outputs of every (gene, target) pair the call returns.
+HOST_BYTES_PER_VALUE_ROW = 8
+HOST_BYTES_PER_PAIR = 40
 
 
-def run_bytes(n_values, n_genes, n_targets):
+def run_bytes(n_values, n_genes, n_targets, n_rows=0, dtype=np.float32):
     """Upper bound of the memory a run of n_genes genes holding n_values stored
-    values takes while in flight."""
-    return (n_values * BYTES_PER_VALUE + n_genes * n_targets * BYTES_PER_PAIR +
-            n_genes * (n_targets + 1) * BYTES_PER_GROUP)
+    values takes while in flight: its host copies and outputs, plus the device
+    memory of its hpdex_de_csc call as the library bounds it
+    (hpdex_de_csc_device_bytes: the upload, the workspace of every gene and
+    of the n_rows labels, group means, fold changes and FDR)."""
+    dev = _capi.de_csc_device_bytes(dtype, n_rows, n_genes, n_values, n_targets)
+    host = (n_values * (np.dtype(dtype).itemsize + HOST_BYTES_PER_VALUE_ROW) +
+            (n_genes + 1) * 8 + n_genes * n_targets * HOST_BYTES_PER_PAIR)
+    return dev + host
 
 
-def plan_gene_runs(indptr, n_targets, budget_bytes):
+def plan_gene_runs(indptr, n_targets, budget_bytes, n_rows=0, dtype=np.float32):
     """Consecutive gene ranges [c0, c1), each within budget_bytes (greedy, in
     gene order); MemoryError naming the first gene that does not fit alone."""
     indptr = np.asarray(indptr, dtype=np.int64)
     C = len(indptr) - 1
     runs, c0, vals = [], 0, 0
+    cost = lambda nv, ng: run_bytes(nv, ng, n_targets, n_rows, dtype)
     for c in range(C):
         n = int(indptr[c + 1] - indptr[c])
-        if run_bytes(n, 1, n_targets) > budget_bytes:
-            raise MemoryError(f"Gene {c} requires {run_bytes(n, 1, n_targets) / 2**30:.2f}GB but "
+        if cost(n, 1) > budget_bytes:
+            raise MemoryError(f"Gene {c} requires {cost(n, 1) / 2**30:.2f}GB but "
                               f"the budget per run is {budget_bytes / 2**30:.2f}GB; consider "
                               "increasing memory_limit_gb or reducing num_workers")
-        if c > c0 and run_bytes(vals + n, c + 1 - c0, n_targets) > budget_bytes:
+        if c > c0 and cost(vals + n, c + 1 - c0) > budget_bytes:
             runs.append((c0, c))
             c0, vals = c, 0
         vals += n
@@ -82,7 +84,7 @@ def stream_de(data, indices, indptr, n_rows, group_id, n_targets, memory_limit_g
     indptr = np.asarray(indptr)
     C = len(indptr) - 1
     budget = int(memory_limit_gb * 2**30) // num_workers
-    runs = plan_gene_runs(indptr, T, budget)
+    runs = plan_gene_runs(indptr, T, budget, n_rows, np.asarray(data[:0]).dtype)
     logger.info(f"streaming {C} genes in {len(runs)} runs, {num_workers} in flight")
     group_id = np.ascontiguousarray(group_id, dtype=np.int32)
     U1 = np.empty(C * T)

@@ -105,6 +105,15 @@ int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int6
                  int threads, double* U1_out, double* P_out, double* fc_out,
                  double* log2fc_out, double* fdr_out, char* err, size_t err_len);
 
+/* Upper bound of the device memory one hpdex_de_csc call allocates for a
+ * matrix of this shape (upload and staging, the test's workspace, group
+ * means, fold changes, FDR), in bytes; 0 for an invalid dtype or n_targets 0.
+ * Computed on the host: needs no device.  No counterpart in the reference;
+ * hpdex_amd.stream sizes its gene runs under memory_limit_gb with it, as the
+ * reference's streaming driver budgets its batches (src/hpdex/stream.py:161-262). */
+size_t hpdex_de_csc_device_bytes(int dtype, size_t n_rows, size_t n_cols, size_t nnz,
+                                 size_t n_targets);
+
 /* ---------------- device-resident matrices and batched plans ----------------
  * No counterpart in the reference (its API is one call per job).  They let a
  * caller keep a CSC matrix in HBM and run several target sets against it.   */
