@@ -794,42 +794,68 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace hpdex_host {
+
+// The one-shot call on stream s (the caller checked the arguments and ran
+// ensure_init): upload, the launch sequence, exact p, copy out.  Device
+// memory comes from DevBuf (hipMalloc, or the calling thread's BumpBlock).
+int mwu_oneshot(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                size_t n_targets, const hpdex_mwu_opts* opts, double* U1_out, double* P_out,
+                double* tie_out, double* n1_out, double* n2_out, hipStream_t s, char* err,
+                size_t err_len) {
+    const Tuning tu = tuning();
+    hpdex_matrix m;
+    m.pack = false;   // used once: no packed entries
+    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, s, err, err_len))
+        return 1;
+    JobState js;
+    const bool stats = tie_out || n1_out || n2_out;
+    if (job_setup(js, &m, group_id, n_targets, opts, stats, s, err, err_len)) return 1;
+    const JobDesc J = make_desc(js, &m, tu);
+    if (job_upload_desc(js, J, m, tu, false, s, err, err_len)) return 1;
+    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), s));
+    if (job_errors(js, s, err, err_len)) return 1;
+    if (opts->method == 1) {
+        if (opts->zero_handling == 0) {
+            if (run_exact_none(js, s, err, err_len)) return 1;
+        } else {
+            if (exact_plan(js, nullptr, nullptr, s, err, err_len)) return 1;
+            if (exact_launch(js, s, true, err, err_len)) return 1;
+        }
+    }
+    return copy_out(js, U1_out, P_out, tie_out, n1_out, n2_out, s, err, err_len);
+}
+
+// Arguments of the one-shot calls, checked as the reference checks them.
+static int oneshot_checks(int dtype, size_t n_targets, const hpdex_mwu_opts* opts, char* err,
+                          size_t err_len) {
+    if (n_targets < 1)
+        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
+    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
+    if (!opts_ok(opts)) return set_err(err, err_len, "invalid option block");
+    return 0;
+}
+
+}  // namespace hpdex_host
+
+extern "C" {
+
 int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
                         const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
                         const int32_t* group_id, size_t n_targets, const hpdex_mwu_opts* opts,
                         int threads, double* U1_out, double* P_out, double* tie_out,
                         double* n1_out, double* n2_out, char* err, size_t err_len) {
     (void)threads;
-    if (n_targets < 1)
-        return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
-    if (!valid_dtype(dtype)) return set_err(err, err_len, "Unsupported dtype for data");
-    if (!opts_ok(opts)) return set_err(err, err_len, "invalid option block");
+    if (oneshot_checks(dtype, n_targets, opts, err, err_len)) return 1;
     if (n_cols == 0) return 0;
     if (ensure_init(err, err_len)) return 1;
-    const Tuning tu = tuning();
     Stream st;
     HIPCHK(st.create());
-    hpdex_matrix m;
-    m.pack = false;   // used once: no packed entries
-
-    if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
-        return 1;
-    JobState js;
-    const bool stats = tie_out || n1_out || n2_out;
-    if (job_setup(js, &m, group_id, n_targets, opts, stats, st.s, err, err_len)) return 1;
-    const JobDesc J = make_desc(js, &m, tu);
-    if (job_upload_desc(js, J, m, tu, false, st.s, err, err_len)) return 1;
-    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), st.s));
-    if (job_errors(js, st.s, err, err_len)) return 1;
-    if (opts->method == 1) {
-        if (opts->zero_handling == 0) {
-            if (run_exact_none(js, st.s, err, err_len)) return 1;
-        } else {
-            if (exact_plan(js, nullptr, nullptr, st.s, err, err_len)) return 1;
-            if (exact_launch(js, st.s, true, err, err_len)) return 1;
-        }
-    }
-    return copy_out(js, U1_out, P_out, tie_out, n1_out, n2_out, st.s, err, err_len);
+    return mwu_oneshot(dtype, data, indices, indptr, n_rows, n_cols, nnz, group_id, n_targets,
+                       opts, U1_out, P_out, tie_out, n1_out, n2_out, st.s, err, err_len);
 }
 
 int hpdex_mwu_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
