@@ -84,7 +84,7 @@ def test_stream_driver_target_major(tmp_path):
     say("stream driver")
     df = stream.parallel_differential_expression(adata, "guide", "ctrl",
                                                  groups=[f"KO{i}" for i in range(1, T + 1)],
-                                                 memory_limit_gb=0.002, num_workers=2)
+                                                 memory_limit_gb=0.012, num_workers=2)
     assert list(df.columns) == stream.COLUMNS and len(df) == T * X.shape[1]
     U1, P, fc, lfc, fdr = _capi.de_csc(X.data, X.indices.astype(np.int64),
                                        X.indptr.astype(np.int64), gid, T, n_rows=X.shape[0])
