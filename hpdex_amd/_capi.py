@@ -31,7 +31,7 @@ EXPORTED = (
     "hpdex_session_create", "hpdex_session_submit", "hpdex_session_wait", "hpdex_session_peak",
     "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc", "hpdex_matrix_create_from",
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
-    "hpdex_de_csc_device_bytes",
+    "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -137,6 +137,9 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(sz)]
         L.hpdex_matrix_download.argtypes = [vp, vp, vp, vp, cp, sz]
         L.hpdex_de_matrix.argtypes = [vp, vp, sz, ci, ci, ctypes.c_double, vp, vp, vp, vp, vp, cp, sz]
+        L.hpdex_mwu_csc_noalloc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz,
+                                            ctypes.POINTER(MwuOpts), ci, vp, vp, cp, sz]
+        L.hpdex_mwu_csc_noalloc.restype = ci
         L.hpdex_de_csc_device_bytes.argtypes = [ci, sz, sz, sz, sz]
         L.hpdex_de_csc_device_bytes.restype = sz
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
@@ -190,6 +193,26 @@ def mwu_csc(data, indices, indptr, group_id, n_targets, opts: MwuOpts, n_rows=No
         rc = lib().hpdex_mwu_csc(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr), R, C,
                                  len(data), _p(group_id), int(n_targets), ctypes.byref(opts),
                                  threads, _p(outs[0]), _p(outs[1]), err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return tuple(o[:np_] for o in outs)
+
+
+def mwu_csc_noalloc(data, indices, indptr, group_id, n_targets, opts: MwuOpts, n_rows=None,
+                    threads=-1):
+    """Call hpdex_mwu_csc_noalloc (no device allocation: the library's fixed
+    scratch) on host arrays.  Returns (U1, P)."""
+    data, indices, indptr = _csc_args(data, indices, indptr)
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    R = len(group_id) if n_rows is None else int(n_rows)
+    C = len(indptr) - 1
+    np_ = C * n_targets if n_targets > 0 else 0
+    outs = [np.empty(max(np_, 1)) for _ in range(2)]
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().hpdex_mwu_csc_noalloc(DTYPE_CODES[data.dtype], _p(data), _p(indices), _p(indptr),
+                                     R, C, len(data), _p(group_id), int(n_targets),
+                                     ctypes.byref(opts), threads, _p(outs[0]), _p(outs[1]), err,
+                                     len(err))
     if rc != 0:
         raise HpdexError(err.value.decode())
     return tuple(o[:np_] for o in outs)

@@ -858,6 +858,97 @@ int hpdex_mwu_csc_stats(int dtype, const void* data, const int64_t* indices,
                        opts, U1_out, P_out, tie_out, n1_out, n2_out, st.s, err, err_len);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// hpdex_mwu_csc_noalloc: the one-shot call with every device buffer carved
+// from this file-scope array instead of hipMalloc.  Each GPU holds its own
+// copy of the array, at its own address, until the process ends.  In host
+// code `scratch` names a host-side stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocDevice {
+    void* ptr = nullptr;        // device address of `scratch` on this device
+    hipStream_t s = nullptr;    // the stream its calls run on
+    bool ready = false;
+};
+// Calls share their device's scratch: the lock runs them one at a time.
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// Bind the calling thread's DevBuf allocations to a block for one scope.
+struct BlockScope {
+    explicit BlockScope(BumpBlock* b) { t_block = b; }
+    ~BlockScope() { t_block = nullptr; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hpdex_mwu_csc_noalloc(int dtype, const void* data, const int64_t* indices,
+                          const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                          const int32_t* group_id, size_t n_targets, const hpdex_mwu_opts* opts,
+                          int threads, double* U1_out, double* P_out, char* err,
+                          size_t err_len) {
+    (void)threads;
+    if (oneshot_checks(dtype, n_targets, opts, err, err_len)) return 1;
+    if (n_cols == 0) return 0;
+    if (ensure_init(err, err_len)) return 1;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return set_err(err, err_len, "hpdex_mwu_csc_noalloc: device index out of range");
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.ready) {   // set-up on this device's first call, with the device current
+        HIPCHK(hipGetSymbolAddress(&d.ptr, HIP_SYMBOL(scratch)));
+        HIPCHK(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking));
+        d.ready = true;
+    }
+    // everything the call carves from the scratch, before anything runs
+    if (n_rows > (size_t)INT32_MAX) return set_err(err, err_len, "more than 2^31-1 rows is unsupported");
+    std::vector<uint32_t> group_rows(n_targets + 1, 0u);
+    for (size_t r = 0; r < n_rows; ++r) {
+        const int g = group_id[r];
+        if (g >= 0 && (size_t)g <= n_targets) ++group_rows[g];
+    }
+    const size_t need =
+        matrix_footprint(dtype, n_rows, n_cols, nnz, HPDEX_LAYOUT_CSC, 64, false) +
+        job_footprint(n_rows, n_cols, nnz, dtype_uses_double(dtype), n_targets, *opts,
+                      group_rows.data(), false);
+    if (need > kNoAllocScratchBytes)
+        return set_err(err, err_len,
+                       "hpdex_mwu_csc_noalloc: the job needs " + std::to_string(need) +
+                           " bytes of device memory, more than its scratch of " +
+                           std::to_string(kNoAllocScratchBytes) + " bytes; use hpdex_mwu_csc");
+    BumpBlock blk;
+    blk.base = (char*)d.ptr;
+    blk.cap = kNoAllocScratchBytes;
+    int rc;
+    {
+        BlockScope scope(&blk);
+        rc = mwu_oneshot(dtype, data, indices, indptr, n_rows, n_cols, nnz, group_id, n_targets,
+                         opts, U1_out, P_out, nullptr, nullptr, nullptr, d.s, err, err_len);
+    }
+    // nothing of this call may still run when the next one reuses the scratch
+    const hipError_t se = hipStreamSynchronize(d.s);
+    if (rc && blk.exhausted)   // exact p in mode none: tables sized by the data
+        return set_err(err, err_len,
+                       "hpdex_mwu_csc_noalloc: the job's exact-null tables do not fit its "
+                       "scratch of " + std::to_string(kNoAllocScratchBytes) +
+                           " bytes; use hpdex_mwu_csc");
+    if (rc) return rc;
+    if (se != hipSuccess) return set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(se));
+    return 0;
+}
+
 int hpdex_mwu_csc(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
                   size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
                   size_t n_targets, const hpdex_mwu_opts* opts, int threads, double* U1_out,

@@ -55,9 +55,13 @@ struct BumpBlock {
     char* base = nullptr;
     size_t cap = 0;
     size_t used = 0;
+    bool exhausted = false;   // a take() did not fit
     void* take(size_t n) {
         n = (n + 255) & ~(size_t)255;
-        if (used + n > cap) return nullptr;
+        if (used + n > cap) {
+            exhausted = true;
+            return nullptr;
+        }
         void* p = base + used;
         used += n;
         return p;
@@ -202,5 +206,11 @@ size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, i
 size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const hpdex_mwu_opts& o,
                      const uint32_t* group_rows, bool stats);
 size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T);
+
+int mwu_oneshot(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
+                size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
+                size_t n_targets, const hpdex_mwu_opts* opts, double* U1_out, double* P_out,
+                double* tie_out, double* n1_out, double* n2_out, hipStream_t s, char* err,
+                size_t err_len);
 
 }  // namespace hpdex_host

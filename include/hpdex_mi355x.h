@@ -65,6 +65,25 @@ int hpdex_mwu_csc(int dtype, const void* data, const int64_t* indices, const int
                   size_t n_targets, const hpdex_mwu_opts* opts, int threads,
                   double* U1_out, double* P_out, char* err, size_t err_len);
 
+/* hpdex_mwu_csc without any device allocation: no counterpart in the
+ * reference.  For callers that cannot call hipMalloc (tight serving loops,
+ * harnesses that forbid allocation).  Same arguments, checks, messages and
+ * results as hpdex_mwu_csc; every device buffer the call needs (the uploaded
+ * matrix and the job's workspace) is carved from a fixed 64 MiB scratch
+ * array that each GPU the library runs on holds until the process ends.  A
+ * job whose bound, computed from its shape before anything runs (its values
+ * and rows, about 34 KiB of tables per column, 16 bytes per pair), exceeds
+ * the scratch is refused with an error naming both sizes; so is exact p in
+ * mode none whose data-sized null tables do not fit.  Calls on one device share its
+ * scratch and are serialised by a lock; the scratch keeps the last call's
+ * contents, and no kernel reads a byte of it that the same call has not
+ * written first.                                                            */
+int hpdex_mwu_csc_noalloc(int dtype, const void* data, const int64_t* indices,
+                          const int64_t* indptr, size_t n_rows, size_t n_cols, size_t nnz,
+                          const int32_t* group_id, size_t n_targets, const hpdex_mwu_opts* opts,
+                          int threads, double* U1_out, double* P_out, char* err,
+                          size_t err_len);
+
 /* hpdex_mwu_csc plus the per-pair statistics the reference keeps internally
  * (tie_arr, n1_arr, n2_arr of src/cpp/mannwhitneyu.cpp:859-861): tie_out is
  * the tie term sum(t^3 - t) (0 when tie_correction is off), n1_out/n2_out the
