@@ -422,8 +422,6 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
 int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const Tuning& tu,
                     bool size_ordered, hipStream_t s, char* err, size_t err_len) {
     js.J = J;
-    HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
-    HIPCHK(hipMemcpyAsync(js.d_job.p, &J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
     const bool count_ok = tu.count_lds > 0 && js.G >= 2 && js.G <= kCountMaxGroups && js.gid_bits > 0;
     const uint32_t T = (uint32_t)js.T;
     std::vector<CountItem> count_items;
@@ -466,6 +464,34 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     }
     js.n_count = (int)count_items.size();
     js.n_gen_static = (int)gen_items.size();
+    // spill space for the sort-path columns whose reference slice may pass
+    // the on-chip sort (its stored values: at most the column's entries and
+    // the reference's rows)
+    {
+        const size_t cap = m.f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+        std::vector<int64_t> off;
+        int64_t total = 0;
+        for (const WorkItem& wi : gen_items) {
+            const size_t b = spill_keys_of(m.h_indptr[wi.col + 1] - m.h_indptr[wi.col],
+                                           js.group_rows[0], cap);
+            if (b == 0) continue;
+            if (off.empty()) off.assign(m.C, -1);
+            off[wi.col] = total;
+            total += (int64_t)b;
+        }
+        js.J.spill = nullptr;
+        js.J.spill_off = nullptr;
+        if (!off.empty()) {
+            HIPCHK(js.spill.alloc((size_t)total * (m.f64 ? 8 : 4)));
+            HIPCHK(js.spill_off.alloc(m.C * 8));
+            HIPCHK(hipMemcpyAsync(js.spill_off.p, off.data(), m.C * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));   // off is a host temporary
+            js.J.spill = js.spill.p;
+            js.J.spill_off = js.spill_off.as<int64_t>();
+        }
+    }
+    HIPCHK(js.d_job.alloc(sizeof(JobDesc)));
+    HIPCHK(hipMemcpyAsync(js.d_job.p, &js.J, sizeof(JobDesc), hipMemcpyHostToDevice, s));
     js.count_lds = (uint32_t)tu.count_lds;
     js.count_block = tu.count_block;
     HIPCHK(js.d_count_items.alloc(count_items.size() * sizeof(CountItem)));
@@ -606,8 +632,9 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
         const std::string why =
             (status & kStatusGroups)
                 ? "more target groups than the on-chip group counters hold (5,460)"
-                : "the reference slice holds more distinct-valued entries than the on-chip "
-                  "sort takes (8,192 for 4-byte values, 4,096 for 8-byte values)";
+                : "the reference slice passes the on-chip sort (8,192 4-byte or 4,096 8-byte "
+                  "keys) and the job reserved no spill space for it (a column handed over "
+                  "by the count kernel)";
         return set_err(err, err_len, "column " + std::to_string(h[1]) + ": " + why);
     }
     return 0;
@@ -669,8 +696,22 @@ size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, i
     return f;
 }
 
+size_t spill_keys_of(int64_t len, uint32_t gr0, size_t cap) {
+    const size_t b = std::min((size_t)len, (size_t)gr0);
+    return b > cap ? b : 0;
+}
+
+size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64) {
+    const size_t cap = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+    if (gr0 <= cap) return 0;
+    if (!indptr) return std::min(nnz, C * (size_t)gr0);
+    size_t t = 0;
+    for (size_t c = 0; c < C; ++c) t += spill_keys_of(indptr[c + 1] - indptr[c], gr0, cap);
+    return t;
+}
+
 size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const hpdex_mwu_opts& o,
-                     const uint32_t* group_rows, bool stats) {
+                     const uint32_t* group_rows, bool stats, size_t spill_keys) {
     const size_t vsz = f64 ? 8 : 4, G = T + 1, np = C * T;
     // job_setup: labels (32-bit and packed at <= 16 bits), group sizes, U1 and P,
     // error block, the tie term and sample sizes when kept, sort-path workspace
@@ -683,6 +724,7 @@ size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const h
     // job_upload_desc: descriptor, count-kernel items, sort-path queue
     f += up256(sizeof(JobDesc)) + up256(C * kMaxSlices * sizeof(CountItem)) +
          up256(C * sizeof(WorkItem)) + up256(16);
+    if (spill_keys) f += up256(spill_keys * vsz) + up256(C * 8);
     // exact-null tables of min/max/mix: sizes are the group row counts
     if (o.method == 1 && o.zero_handling != 0 && group_rows) {
         std::set<std::pair<int32_t, int32_t>> pairs;
@@ -707,7 +749,9 @@ size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T) {
     o.method = 2;
     const size_t G = T + 1, np = C * T;
     return matrix_footprint(dtype, R, C, nnz, HPDEX_LAYOUT_CSC, 64, false) +
-           job_footprint(R, C, nnz, dtype_uses_double(dtype), T, o, nullptr, false) +
+           job_footprint(R, C, nnz, dtype_uses_double(dtype), T, o, nullptr, false,
+                         spill_keys_bound(nullptr, C, nnz, (uint32_t)std::min(R, (size_t)UINT32_MAX),
+                                          dtype_uses_double(dtype))) +
            up256(C * G * 8) + 3 * up256(np * 8) + up256(fdr_workspace_bound((int64_t)np));
 }
 
@@ -922,7 +966,9 @@ int hpdex_mwu_csc_noalloc(int dtype, const void* data, const int64_t* indices,
     const size_t need =
         matrix_footprint(dtype, n_rows, n_cols, nnz, HPDEX_LAYOUT_CSC, 64, false) +
         job_footprint(n_rows, n_cols, nnz, dtype_uses_double(dtype), n_targets, *opts,
-                      group_rows.data(), false);
+                      group_rows.data(), false,
+                      spill_keys_bound(indptr, n_cols, nnz, group_rows[0],
+                                       dtype_uses_double(dtype)));
     if (need > kNoAllocScratchBytes)
         return set_err(err, err_len,
                        "hpdex_mwu_csc_noalloc: the job needs " + std::to_string(need) +

@@ -144,6 +144,7 @@ struct JobState {
     uint32_t gid_bits = 0;
     size_t gid_words = 0;
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
+    DevBuf spill, spill_off;                        // reference slices beyond the chip
     DevBuf d_job;                                   // descriptor
     DevBuf d_count_items, d_gen_items, d_gen_n;     // work queues of the two paths
     JobDesc J{};                                    // host copy of the descriptor
@@ -204,7 +205,12 @@ size_t up256(size_t n);
 size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, int index_bits,
                         bool pack);
 size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const hpdex_mwu_opts& o,
-                     const uint32_t* group_rows, bool stats);
+                     const uint32_t* group_rows, bool stats, size_t spill_keys);
+// Spill keys of a column of len entries whose reference has gr0 rows (0 when
+// its reference slice fits the on-chip sort of cap keys), and their bound over
+// a matrix (indptr null: min(nnz, C * gr0), for a matrix not yet in CSC).
+size_t spill_keys_of(int64_t len, uint32_t gr0, size_t cap);
+size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64);
 size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T);
 
 int mwu_oneshot(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,

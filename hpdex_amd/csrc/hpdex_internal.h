@@ -12,7 +12,7 @@ enum ZeroMode : int32_t { kZeroNone = 0, kZeroMin = 1, kZeroMax = 2, kZeroMix = 
 
 // Status bits a kernel raises when a column does not fit its on-chip plan.
 enum : uint32_t {
-    kStatusCapacity = 1u,   // a reference slice exceeded the workgroup's LDS budget
+    kStatusCapacity = 1u,   // a reference slice beyond the on-chip sort had no spill space
     kStatusGroups = 2u,     // too many groups for the LDS counters
 };
 
@@ -72,6 +72,9 @@ struct JobDesc {
     uint32_t* goff;              // C*G start of each group's keys, from indptr[c]
     ColPlan* plan;               // C
     uint32_t* reftab;            // C*kTabStride reference tables
+    void* spill;                 // keys (as `keys`) for reference slices beyond the on-chip
+                                 // sort: the merge passes' second buffer
+    const int64_t* spill_off;    // C: column c's first spill key, -1 = none (or null)
     // outputs (gene-major, c*T + g-1); tie, n1o and n2o may be null (not kept)
     double* U1;
     double* P;

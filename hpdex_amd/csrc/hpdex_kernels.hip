@@ -597,6 +597,63 @@ __device__ __forceinline__ void sort_ref(K* rk, uint32_t P) {
     }
 }
 
+// Sort the n keys a[0..n) in HBM (n above the cap keys an LDS sort takes):
+// runs of cap keys sorted in LDS (rk, cap keys, cap a power of two), then
+// merge passes of doubling width between a and b (n keys of spill space);
+// the result ends in a.  Each thread merges one slice of a pair's output,
+// found by a binary search along the merge path; equal keys take the left
+// run's first.  Every barrier sits under workgroup-uniform control (n, cap).
+template <typename K>
+__device__ void sort_ref_global(K* a, K* b, uint32_t n, K* rk, uint32_t cap, K sentinel) {
+    const uint32_t tid = (uint32_t)tid_x(), nt = blockDim.x;
+    for (uint32_t q0 = 0; q0 < n; q0 += cap) {
+        const uint32_t m = min(cap, n - q0);
+        uint32_t P = 64;
+        while (P < m) P <<= 1;
+        for (uint32_t x = tid; x < P; x += nt) rk[x] = x < m ? a[q0 + x] : sentinel;
+        __syncthreads();
+        sort_ref<K>(rk, P);
+        for (uint32_t x = tid; x < m; x += nt) a[q0 + x] = rk[x];
+        __syncthreads();
+    }
+    K* src = a;
+    K* dst = b;
+    for (uint32_t w = cap; w < n; w <<= 1) {
+        for (uint32_t i = 0; i < n; i += 2u * w) {
+            const uint32_t nA = min(w, n - i);
+            const uint32_t nB = n - i > w ? min(w, n - i - w) : 0u;
+            const K* A = src + i;
+            const K* B = A + nA;
+            const uint32_t tot = nA + nB, per = (tot + nt - 1u) / nt;
+            const uint32_t d0 = min(tid * per, tot), d1 = min(d0 + per, tot);
+            if (d0 < d1) {
+                // the first d0 outputs take ia keys of A: the least ia with
+                // B[d0 - ia - 1] < A[ia]
+                uint32_t lo = d0 > nB ? d0 - nB : 0u, hi = min(d0, nA);
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (!(B[d0 - mid - 1u] < A[mid])) lo = mid + 1u;
+                    else hi = mid;
+                }
+                uint32_t ia = lo, ib = d0 - lo;
+                K ka = ia < nA ? A[ia] : sentinel, kb = ib < nB ? B[ib] : sentinel;
+                for (uint32_t d = d0; d < d1; ++d) {
+                    const bool takeA = ib >= nB || (ia < nA && !(kb < ka));
+                    dst[i + d] = takeA ? ka : kb;
+                    if (takeA) { ++ia; ka = ia < nA ? A[ia] : sentinel; }
+                    else { ++ib; kb = ib < nB ? B[ib] : sentinel; }
+                }
+            }
+        }
+        __syncthreads();
+        K* t = src; src = dst; dst = t;
+    }
+    if (src != a) {
+        for (uint32_t x = tid; x < n; x += nt) a[x] = src[x];
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // reference: sort the reference slice once per gene and build its rank table
 // (histogram prefix or bucket index) plus its own tie sum.
@@ -625,8 +682,33 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
     const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
     uint64_t ta = 0;
-    if (!hist && na > cap) {
+    // a reference slice beyond the on-chip sort: sorted in HBM with the
+    // column's spill space, its bucket index built from the sorted keys there
+    const int64_t spill_at =
+        (!hist && na > cap) ? uniform_i64(J.spill_off ? J.spill_off[c] : -1) : -1;
+    if (!hist && na > cap && spill_at < 0) {
         if (tid == 0) raise_status(J, kStatusCapacity, c);
+    } else if (!hist && na > cap) {
+        K* rk = (K*)smem;
+        uint32_t* bidx = (uint32_t*)(rk + cap);
+        sort_ref_global<K>(base, (K*)J.spill + spill_at, na, rk, cap, KT::kSentinel);
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
+        __syncthreads();
+        for (uint32_t x = tid; x < na; x += nt)
+            atomicAdd(&bidx[(uint32_t)(((uint64_t)base[x] - pl.kmin) >> pl.shift)], 1u);
+        __syncthreads();
+        block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
+        for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
+        RefView<K> rv;
+        rv.cum = nullptr; rv.keys = base; rv.bidx = bidx; rv.kmin = pl.kmin; rv.kmax = pl.kmax;
+        rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        for (uint32_t x = tid; x < na; x += nt) {
+            if (x > 0 && base[x - 1] == base[x]) continue;   // not the first of its run
+            uint32_t lt, eq;
+            rv.lookup(base[x], lt, eq);
+            ta += tie_f(eq);
+        }
+        ta = block_sum_u64(ta, st.red);
     } else if (hist) {
         uint32_t* cum = (uint32_t*)smem;
         for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = 0u;
@@ -663,7 +745,7 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         }
         ta = block_sum_u64(ta, st.red);
     }
-    if (tid == 0 && (hist || na <= cap)) J.plan[c].TA = ta;
+    if (tid == 0 && (hist || na <= cap || spill_at >= 0)) J.plan[c].TA = ta;
 }
 
 // workgroup b takes queue item b (the grid is sized for the queue's capacity)
@@ -999,14 +1081,21 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
     const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
-    // a reference too large for the chip: the reference kernel raised the status
-    if (!hist && na > cap) return;
+    // a reference beyond the on-chip sort: its sorted keys stay in HBM (read
+    // through L2 by the lookups), its bucket index comes to LDS; with no spill
+    // space the reference kernel raised the status
+    const bool big = !hist && na > cap;
+    if (big && uniform_i64(J.spill_off ? J.spill_off[c] : -1) < 0) return;
     RefView<K> rv;
     rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = hist;
     if (hist) {
         uint32_t* cum = (uint32_t*)smem;
         for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = tab[b];
         rv.cum = cum; rv.keys = nullptr; rv.bidx = nullptr;
+    } else if (big) {
+        uint32_t* bidx = (uint32_t*)((K*)smem + cap);
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = tab[b];
+        rv.cum = nullptr; rv.keys = (const K*)J.keys + J.indptr[c]; rv.bidx = bidx;
     } else {
         K* rk = (K*)smem;
         uint32_t* bidx = (uint32_t*)(rk + cap);

@@ -112,7 +112,19 @@ size_t session_job_footprint(const hpdex_job& j, const std::vector<uint32_t>& gr
     size_t f = 0;
     if (!m)
         f += matrix_footprint(j.dtype, R, C, nnz, j.layout, j.index_bits == 32 ? 32 : 64, false);
-    return f + job_footprint(R, C, nnz, f64, j.n_targets, j.opts, group_rows.data(), false);
+    // the column pointers bound the spill space: the resident matrix's, or
+    // the CSC's the job brings (a CSR job's columns are known only on the device)
+    std::vector<int64_t> ip;
+    const int64_t* indptr = m ? m->h_indptr.data() : nullptr;
+    if (!m && j.layout == HPDEX_LAYOUT_CSC) {
+        ip.resize(C + 1);
+        for (size_t c = 0; c <= C; ++c)
+            ip[c] = j.index_bits == 32 ? (int64_t)((const int32_t*)j.indptr)[c]
+                                       : ((const int64_t*)j.indptr)[c];
+        indptr = ip.data();
+    }
+    const size_t spill = spill_keys_bound(indptr, C, nnz, group_rows[0], f64);
+    return f + job_footprint(R, C, nnz, f64, j.n_targets, j.opts, group_rows.data(), false, spill);
 }
 
 }  // namespace

@@ -233,6 +233,18 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
+    // spill space of the columns whose reference slice may pass the on-chip
+    // sort (job_upload_desc), for every column: any may take the sort path
+    const size_t cap = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+    std::vector<int64_t> spill_off(C, -1);
+    size_t spill_n = 0;
+    for (int c = 0; c < C; ++c) {
+        const size_t b = std::min((size_t)(indptr[c + 1] - indptr[c]), (size_t)grows[0]);
+        if (b > cap) { spill_off[c] = (int64_t)spill_n; spill_n += b; }
+    }
+    std::vector<uint64_t> spill(spill_n + 1, 0xA5A5A5A5A5A5A5A5ull);
+    J.spill = spill.data();
+    J.spill_off = spill_off.data();
     std::vector<int32_t> range(2 * (size_t)C + 2);
     J.colrange = range.data();
     std::vector<int32_t> order(C);

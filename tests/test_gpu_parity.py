@@ -255,6 +255,38 @@ def test_dense_split_slots(grouped):
         assert_parity(gpu_stats(w, mode), oracle_stats(w, mode))
 
 
+@pytest.mark.parametrize("f64,n_ref", [(False, 60000), (True, 20000)], ids=["f32-60k", "f64-20k"])
+def test_reference_beyond_chip(f64, n_ref):
+    """Reference slices of up to 60,000 four-byte or 20,000 eight-byte keys
+    per column, beyond the on-chip sort (8,192 / 4,096 keys): sorted in HBM
+    (LDS-sorted runs and merge passes through the job's spill space) and
+    ranked from HBM, beside columns whose reference fits the chip; ties
+    inside the reference and against targets, negatives; every zero mode;
+    plain and batched plans bit-identical to the one-shot call."""
+    say(f"reference beyond chip f64={f64} n_ref={n_ref}")
+    rng = np.random.default_rng(23)
+    R, T = n_ref + 24000, 40
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([1.0, 0.97, 0.6, 0.25, 0.1]))
+    data = np.round(rng.normal(0, 3, len(rows)), 2).astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:n_ref]] = 0
+    gid[perm[n_ref:]] = rng.integers(1, T + 1, R - n_ref)
+    w = dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R)
+    for mode in ("min", "max", "mix", "none"):
+        o = oracle_stats(w, mode)
+        g = gpu_stats(w, mode)
+        assert_parity(g, o)
+        if mode == "min":
+            for form in (0, 1):
+                m, p = _plan_of(w, mode, form=form)
+                p.run(2)
+                assert p.info()["split_columns"] == 5
+                for a, b in zip(p.results(with_stats=True), g):
+                    assert np.array_equal(a, b)
+                p.close(); m.close()
+
+
 @pytest.mark.parametrize("mode", ["min", "none"])
 def test_float_windows_large_reference(mode):
     """Small targets against a reference of ~5,800 stored values per column:

@@ -422,3 +422,22 @@ def test_sim_count_kernel_wide_counters():
     d, r, ip, g = _rw(29, 70000, 1, 20, 0.96, lambda rg, n: rg.integers(1, 3001, n))
     assert ip[1] >= 65536
     run_count(d, r, ip, g, 20, 70000, "min", 76 * 1024, expect_counted=1, expect_blocks=7)
+
+
+@pytest.mark.parametrize("mode,f64", [("min", False), ("none", True)], ids=["min-f32", "none-f64"])
+def test_sim_reference_beyond_chip(mode, f64):
+    # reference slices above the on-chip sort (8,192 four-byte keys, 4,096
+    # eight-byte keys): ~10,000 values per column sorted in HBM (two or three
+    # LDS-sorted runs and merge passes through the spill space), ranked from
+    # HBM; ties inside the reference (values at 0.01) and against targets,
+    # negatives, and a column whose reference slice fits the chip beside them
+    rng = np.random.default_rng(35)
+    R, T = 14000, 4
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.95, 0.8, 0.3]))
+    data = np.round(rng.normal(0, 3, len(rows)), 2)
+    data = data.astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:11000]] = 0
+    gid[perm[11000:]] = rng.integers(1, T + 1, R - 11000)
+    run_both(data, rows, indptr, gid, T, R, mode, f64=f64)
