@@ -52,7 +52,7 @@ class MwuOpts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes",
-                                              "count_block", "count_packed")]
+                                              "count_block", "count_packed", "split_arrival")]
 
 
 class PlanInfo(ctypes.Structure):
@@ -476,8 +476,10 @@ def get_tuning() -> Tuning:
 
 
 def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None,
-               count_packed=None):
+               count_packed=None, split_arrival=None):
     t = get_tuning()
+    if split_arrival is not None:
+        t.split_arrival = int(split_arrival)
     if count_packed is not None:
         t.count_packed = int(count_packed)
     if count_block is not None:

@@ -38,7 +38,8 @@ DEFAULT_SPACE = {
     "count_block": (256, 512, 1024),
 }
 
-KNOBS = ("hist_max_bins", "size_ordered", "count_lds_bytes", "count_block", "count_packed")
+KNOBS = ("hist_max_bins", "size_ordered", "count_lds_bytes", "count_block", "count_packed",
+         "split_arrival")
 
 
 class TuningMismatch(_capi.HpdexError):

@@ -409,6 +409,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.tie_corr = js.opts.tie_correction ? 1 : 0;
     J.asym = (js.opts.method == 2) ? 1 : 0;
     J.hist_max = tu.hist_max;
+    J.split_arrival = tu.split_arrival;
     J.cc = js.opts.use_continuity ? 0.5 : 0.0;
     return J;
 }
@@ -809,6 +810,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     out->count_lds_bytes = t.count_lds;
     out->count_block = t.count_block;
     out->count_packed = t.count_packed;
+    out->split_arrival = t.split_arrival;
     return 0;
 }
 
@@ -827,6 +829,8 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
         return set_err(err, err_len, "count_block must be 256, 512 or 1024");
     if (t->count_packed != 0 && t->count_packed != 1)
         return set_err(err, err_len, "count_packed must be 0 or 1");
+    if (t->split_arrival != 0 && t->split_arrival != 1)
+        return set_err(err, err_len, "split_arrival must be 0 or 1");
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init && !g_count_ready && t->count_lds_bytes != 0)
         return set_err(err, err_len, "this device does not give the count kernel its LDS");
@@ -835,6 +839,7 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     g_tune.count_lds = t->count_lds_bytes;
     g_tune.count_block = t->count_block;
     g_tune.count_packed = t->count_packed;
+    g_tune.split_arrival = t->split_arrival;
     return 0;
 }
 

@@ -33,6 +33,7 @@ struct Tuning {
     int count_lds = kCountLdsDefault;
     int count_block = kCountBlockDefault;
     int count_packed = 1;
+    int split_arrival = 1;
 };
 
 extern std::mutex g_mu;

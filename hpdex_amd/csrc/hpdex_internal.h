@@ -27,6 +27,17 @@ constexpr int kRefKeysF32 = 8192;           // sort path: reference keys kept on
 constexpr int kRefKeysF64 = 4096;           // sort path: reference keys kept on chip (8 B keys)
 constexpr int kTabStride = (kHistMaxBins > kMaxBuckets ? kHistMaxBins : kMaxBuckets) + 1;
 
+// LDS of the reference-table and target kernels: the larger of a histogram
+// prefix (kHistMaxBins + 1 counts) and the on-chip sorted reference with its
+// bucket index, rounded to 16 bytes.
+__host__ __device__ constexpr uint32_t ref_table_lds(bool f64) {
+    const uint32_t hist = 4u * (uint32_t)(kHistMaxBins + 1);
+    const uint32_t keys = f64 ? 8u * (uint32_t)kRefKeysF64 : 4u * (uint32_t)kRefKeysF32;
+    const uint32_t sort = keys + 4u * (uint32_t)(kMaxBuckets + 1);
+    return ((hist > sort ? hist : sort) + 15u) & ~15u;
+}
+
+
 // Count kernel (integer-valued columns in one pass).
 constexpr int kCountBlockDefault = 256;     // threads per workgroup: 256, 512 or 1024
 constexpr int kCountLdsDefault = 40448;     // dynamic LDS per workgroup: four per CU
@@ -93,6 +104,8 @@ struct JobDesc {
     int32_t tie_corr;
     int32_t asym;                // 1: asymptotic p written with U1
     int32_t hist_max;            // largest integer range binned (0 = off)
+    int32_t split_arrival;       // 1: the split keeps a sparse column's entry groups in LDS
+                                 // and re-reads only its values to scatter
     double cc;                   // continuity correction, 0.5 or 0
 };
 

@@ -122,8 +122,8 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
     }
 }
 
-// Visit (group, value) of every entry of the column [p0, p0 + n) whose row is
-// in a group, for workgroups of NT threads.  Batches of NT*kUnroll entries are
+// Visit (group, value, entry index in the column) of every entry of the
+// column [p0, p0 + n) whose row is in a group, for workgroups of NT threads.  Batches of NT*kUnroll entries are
 // read as buffer loads whose resources end at the batch's last entry (lanes
 // past it read zeros and are masked) and software-pipelined: the next batch's
 // rows and values are issued after this batch's label gathers and before its
@@ -186,7 +186,7 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u)
-            if (g[u] < G) f(g[u], v[u]);
+            if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
     }
@@ -199,7 +199,7 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
     gather(r, g);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
-        if (g[u] < G) f(g[u], v[u]);
+        if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
 }
 
 __device__ inline void raise_status(const JobDesc& J, uint32_t bit, int64_t c) {
@@ -365,6 +365,18 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const bool staged = __builtin_amdgcn_readfirstlane(
         (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * span <= lds_bytes ? 1 : 0) != 0;
     K* stage = (K*)(smem + cnt_bytes);
+    // A column scattered by offsets whose entries' groups also fit LDS (16 bits
+    // each, after the staged keys) keeps them from its counting pass, so the
+    // scatter reads only the values again: no second pass over the rows and
+    // no second gather of the labels.  Entries that give no key keep 0xFFFF.
+    const uint32_t n_col = (uint32_t)(p1 - p0);
+    const uint32_t stage_bytes = ((uint32_t)sizeof(K) * n_col + 15u) & ~15u;
+    const bool arrival = __builtin_amdgcn_readfirstlane(
+        (!fixed && J.split_arrival &&
+         (uint64_t)cnt_bytes + stage_bytes + 2ull * n_col <= (uint64_t)lds_bytes) ? 1 : 0) != 0;
+    uint16_t* agrp = (uint16_t*)(smem + cnt_bytes + stage_bytes);
+    if (arrival)
+        for (uint32_t e = (uint32_t)tid; e < n_col; e += (uint32_t)nt) agrp[e] = 0xFFFFu;
 
     // counts and value statistics (and, in slots, the keys)
     // fixed_tag: 0 scatter later, 1 keys to their slots in HBM, 2 in LDS
@@ -374,12 +386,14 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         int lmin = INT_MAX, lmax = INT_MIN;
         K kmn = KT::kSentinel, kmx = 0;
         uint32_t neg = 0;
-        grouped([&](uint32_t g, V v) {
+        grouped([&](uint32_t g, V v, uint32_t e) {
             if (!KT::finite(v)) return;
             if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
             const uint32_t i = atomicAdd(&cnt_nz[g], 1u);
             const K k = KT::key(v);
-            if (kFixed == 1) {
+            if (kFixed == 0) {
+                if (arrival) agrp[e] = (uint16_t)g;
+            } else if (kFixed == 1) {
                 // a group holding more entries than rows (repeated row
                 // indices) is an error of the column; its keys stay inside
                 // the column's range
@@ -413,6 +427,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
     };
+    if (arrival) __syncthreads();   // the cleared groups before the pass writes them
     if (fixed && staged) count_pass(std::integral_constant<int, 2>{});
     else if (fixed) count_pass(std::integral_constant<int, 1>{});
     else count_pass(std::integral_constant<int, 0>{});
@@ -469,10 +484,33 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
 
     const uint32_t hist = st.hist;
     const int32_t vmin = st.vmin;
-    if (!fixed) {
+    if (arrival) {
+        // scatter from the recorded groups, the values read again (coalesced,
+        // four entries per thread in flight)
+        const V* col_vals = vals + p0;
+        constexpr uint32_t kA = sizeof(V) == 8 ? 2u : 4u;
+        for (uint32_t e0 = (uint32_t)tid; e0 < n_col; e0 += (uint32_t)nt * kA) {
+            uint32_t g[kA];
+            V v[kA];
+#pragma unroll
+            for (uint32_t u = 0; u < kA; ++u) {
+                const uint32_t e = e0 + u * (uint32_t)nt;
+                g[u] = e < n_col ? (uint32_t)agrp[e] : 0xFFFFu;
+                v[u] = g[u] != 0xFFFFu ? gl(col_vals)[e] : V(0);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kA; ++u) {
+                if (g[u] == 0xFFFFu) continue;
+                const uint32_t pos = atomicAdd(&cur[g[u]], 1u);
+                stage[pos] = hist ? (K)(uint32_t)((int)v[u] - vmin) : KT::key(v[u]);
+            }
+        }
+        __syncthreads();   // the column's key range out, coalesced
+        for (uint32_t i = (uint32_t)tid; i < n_col; i += (uint32_t)nt) keys[i] = stage[i];
+    } else if (!fixed) {
         // scatter nonzero values into the groups' key ranges
         K* dst = staged ? stage : keys;
-        grouped([&](uint32_t g, V v) {
+        grouped([&](uint32_t g, V v, uint32_t) {
             if (!KT::finite(v) || v == V(0)) return;
             const uint32_t pos = atomicAdd(&cur[g], 1u);
             dst[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
@@ -1968,12 +2006,7 @@ static inline unsigned align16(size_t b) { return (unsigned)((b + 15) & ~(size_t
 
 // LDS of the reference-table and target kernels: the larger of a full
 // histogram table and the on-chip sorted reference plus its bucket index.
-static unsigned table_lds_bytes(bool f64) {
-    const size_t hist = 4u * (size_t)(kHistMaxBins + 1);
-    const size_t keys = f64 ? 8u * (size_t)kRefKeysF64 : 4u * (size_t)kRefKeysF32;
-    const size_t sort = keys + 4u * (size_t)(kMaxBuckets + 1);
-    return align16(hist > sort ? hist : sort);
-}
+static unsigned table_lds_bytes(bool f64) { return ref_table_lds(f64); }
 
 hipError_t prepare_count_kernel(uint32_t bytes) {
     const void* fns[] = {(const void*)count_kernel<float, 256>, (const void*)count_kernel<float, 512>,

@@ -252,6 +252,11 @@ typedef struct hpdex_tuning {
     int32_t count_packed;      /* 1 = that kernel reads a matrix's packed entries (one
                                   32-bit word per entry, made when the matrix is
                                   created) where it has them; 0 = rows and values  */
+    int32_t split_arrival;     /* sort path: 1 = the split kernel keeps each entry's
+                                  group from its counting pass in LDS (columns whose
+                                  entries fit) and reads only the values again to
+                                  scatter them; 0 = counting and scattering passes
+                                  both read rows, values and group labels           */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

@@ -29,6 +29,7 @@ namespace {
 int sim_last_blocks = 0;   // count-kernel workgroups of the last sim_mwu call
 int sim_shrink_range = 0;  // test hook: record each column's range one value short
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
+int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
 int sim_last_packed = 0;   // count-kernel workgroups of the last call that read packed entries
 
 struct ThreadInfo { std::atomic<const char*>* last; };
@@ -233,6 +234,7 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
+    J.split_arrival = sim_split_arrival;
     // spill space of the columns whose reference slice may pass the on-chip
     // sort (job_upload_desc), for every column: any may take the sort path
     const size_t cap = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
@@ -261,3 +263,4 @@ extern "C" int sim_count_blocks() { return sim_last_blocks; }
 extern "C" void sim_set_shrink_range(int on) { sim_shrink_range = on; }
 extern "C" void sim_set_pack(int on) { sim_pack = on; }
 extern "C" int sim_packed_blocks() { return sim_last_packed; }
+extern "C" void sim_set_split_arrival(int on) { sim_split_arrival = on; }

@@ -337,7 +337,7 @@ def restore_tuning():
     t = _capi.get_tuning()
     yield
     _capi.set_tuning(t.hist_max_bins, t.size_ordered, t.count_lds_bytes, t.count_block,
-                     t.count_packed)
+                     t.count_packed, t.split_arrival)
 
 
 def _plan_of(w, mode, form=1, stats=True):

@@ -16,10 +16,10 @@ ALT = {"less": 0, "greater": 1, "two-sided": 2}
 
 
 def run_both(data, indices, indptr, gid, T, R, mode, alt="two-sided", tie=True, cc=True,
-             f64=False):
+             f64=False, **kw):
     U1, P, TIE, n1, n2, err, status = sim.mwu(data, indices, indptr, gid, T, R, mode=ZH[mode],
                                               alt=ALT[alt], tie=tie, cc=cc, f64=f64,
-                                              timeout_s=120)
+                                              timeout_s=120, **kw)
     assert err == 2 ** 64 - 1 and status == 0
     oU1, oP, oT, on1, on2 = oracle.mwu(data, indices, indptr, gid, T, n_rows=R,
                                        tie_correction=tie, use_continuity=cc,
@@ -83,8 +83,9 @@ def test_sim_sort_path_many_small_targets(mode):
     run_both(data, rows, indptr, gid, T, R, mode)
 
 
+@pytest.mark.parametrize("arrival", [True, False], ids=["groups-kept", "two-reads"])
 @pytest.mark.parametrize("mode", ["min", "none"])
-def test_sim_sort_path_sparse_columns(mode):
+def test_sim_sort_path_sparse_columns(mode, arrival):
     # columns holding under a quarter of the rows are split through the
     # bit-packed labels: ~5,200 entries (two whole batches of 2,048 and a
     # partial one), ~2,000 (one partial batch) and, in mode min (mode none
@@ -99,7 +100,7 @@ def test_sim_sort_path_sparse_columns(mode):
     perm = rng.permutation(R)
     gid[perm[:9000]] = 0
     gid[perm[9000:30000]] = rng.integers(1, T + 1, 21000)
-    run_both(data, rows, indptr, gid, T, R, mode)
+    run_both(data, rows, indptr, gid, T, R, mode, split_arrival=arrival)
 
 
 def _dense(R, C):
