@@ -32,18 +32,60 @@ HIP_FLAGS = [
 ]
 
 
+# Register budget of every kernel, checked on the compiler's resource report
+# at each build: spilled SGPRs go through VGPR lanes and spilled VGPRs to
+# scratch memory.  A kernel far beyond these (hundreds of SGPR spills) is the
+# profile of one that once stalled the device; none may come near it again.
+SGPR_SPILL_BUDGET = 48
+VGPR_SPILL_BUDGET = 8
+RESOURCE_REMARKS = "-Rpass-analysis=kernel-resource-usage"
+
+
 def _run(cmds):
     procs = [(c, subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
              for c in cmds]
-    failed = []
+    failed, outs = [], []
     for c, p in procs:
         out, _ = p.communicate()
         lines = [ln for ln in out.splitlines() if "packed-fp32-ops" not in ln]
+        outs.append("\n".join(lines))
         if p.returncode != 0:
             failed.append((c, "\n".join(lines)))
     if failed:
         msg = "\n\n".join(" ".join(c) + "\n" + o for c, o in failed)
         raise RuntimeError("build failed:\n" + msg)
+    return outs
+
+
+def kernel_resources(report: str) -> dict:
+    """{kernel: {"VGPRs": n, "SGPRs Spill": n, "VGPRs Spill": n, ...}} from a
+    compile's resource-usage remarks."""
+    import re
+    out, cur = {}, None
+    for ln in report.splitlines():
+        m = re.search(r"Function Name: (\S+)", ln)
+        if m:
+            cur = out.setdefault(m.group(1), {})
+            continue
+        m = re.search(r"remark:\s+([^:]+): (\d+)", ln)
+        if m and cur is not None:
+            cur[m.group(1).strip()] = int(m.group(2))
+    return out
+
+
+def check_resources(report: str) -> dict:
+    """Raise if a kernel spills more registers than the budgets allow."""
+    res = kernel_resources(report)
+    if not res:
+        raise RuntimeError("no kernel resource report in the compiler output")
+    over = [f"{k}: {v.get('SGPRs Spill', 0)} SGPR / {v.get('VGPRs Spill', 0)} VGPR spills"
+            for k, v in res.items()
+            if v.get("SGPRs Spill", 0) > SGPR_SPILL_BUDGET or
+            v.get("VGPRs Spill", 0) > VGPR_SPILL_BUDGET]
+    if over:
+        raise RuntimeError(f"kernels over the register budget ({SGPR_SPILL_BUDGET} SGPR, "
+                           f"{VGPR_SPILL_BUDGET} VGPR spills): " + "; ".join(over))
+    return res
 
 
 def ext_suffix() -> str:
@@ -59,14 +101,20 @@ def build(verbose: bool = False) -> None:
     s_obj = os.path.join(BUILD, "hpdex_session.o")
     d_obj = os.path.join(BUILD, "hpdex_de.o")
     i_obj = os.path.join(BUILD, "hpdex_ingest.o")
-    _run([
-        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_kernels.hip"), "-o", k_obj],
+    outs = _run([
+        [HIPCC, *HIP_FLAGS, RESOURCE_REMARKS, "-c", os.path.join(CSRC, "hpdex_kernels.hip"), "-o",
+         k_obj],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_de.hip"), "-o", d_obj],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "hpdex_ingest.hip"), "-o", i_obj],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "hpdex_api.cpp"), "-o", a_obj],
         [HIPCC, *HIP_FLAGS, "-pthread", "-x", "hip", "-c", os.path.join(CSRC, "hpdex_session.cpp"),
          "-o", s_obj],
     ])
+    res = check_resources(outs[0])
+    if verbose:
+        worst = max(res.items(), key=lambda kv: kv[1].get("SGPRs Spill", 0))
+        print(f"{len(res)} kernels within the register budget (most SGPR spills: "
+              f"{worst[1].get('SGPRs Spill', 0)})")
     tmp = LIB + ".tmp"
     _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", k_obj, d_obj, i_obj, a_obj,
            s_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so"]])

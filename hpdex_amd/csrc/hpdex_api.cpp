@@ -499,12 +499,10 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     if (!count_items.empty())
         HIPCHK(hipMemcpyAsync(js.d_count_items.p, count_items.data(),
                               count_items.size() * sizeof(CountItem), hipMemcpyHostToDevice, s));
-    // room for every column: the count kernel appends those it hands over
-    HIPCHK(js.d_gen_items.alloc(m.C * sizeof(WorkItem)));
+    HIPCHK(js.d_gen_items.alloc(gen_items.size() * sizeof(WorkItem)));
     if (!gen_items.empty())
         HIPCHK(hipMemcpyAsync(js.d_gen_items.p, gen_items.data(),
                               gen_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
-    HIPCHK(js.d_gen_n.alloc(16));
     HIPCHK(hipStreamSynchronize(s));   // the item vectors are host temporaries
     return 0;
 }
@@ -515,10 +513,8 @@ PipelineLaunch pipeline_of(const JobState& js, bool f64) {
     L.job = js.J;
     L.count_items = js.d_count_items.as<CountItem>();
     L.n_count = js.n_count;
-    L.n_count_cols = js.n_count_cols;
     L.gen_items = js.d_gen_items.as<WorkItem>();
-    L.gen_n = js.d_gen_n.as<uint32_t>();
-    L.n_gen_static = js.n_gen_static;
+    L.n_gen = js.n_gen_static;
     L.G = (int)js.G;
     L.f64 = f64;
     L.count_lds = js.count_lds;
@@ -633,9 +629,11 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
         const std::string why =
             (status & kStatusGroups)
                 ? "more target groups than the on-chip group counters hold (5,460)"
+            : (status & kStatusRange)
+                ? "a stored value outside the column's recorded value range (the resident "
+                  "matrix's metadata does not match its values)"
                 : "the reference slice passes the on-chip sort (8,192 4-byte or 4,096 8-byte "
-                  "keys) and the job reserved no spill space for it (a column handed over "
-                  "by the count kernel)";
+                  "keys) and the job reserved no spill space for it";
         return set_err(err, err_len, "column " + std::to_string(h[1]) + ": " + why);
     }
     return 0;
@@ -724,7 +722,7 @@ size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const h
     f += up256(C * (size_t)kTabStride * 4);
     // job_upload_desc: descriptor, count-kernel items, sort-path queue
     f += up256(sizeof(JobDesc)) + up256(C * kMaxSlices * sizeof(CountItem)) +
-         up256(C * sizeof(WorkItem)) + up256(16);
+         up256(C * sizeof(WorkItem));
     if (spill_keys) f += up256(spill_keys * vsz) + up256(C * 8);
     // exact-null tables of min/max/mix: sizes are the group row counts
     if (o.method == 1 && o.zero_handling != 0 && group_rows) {
@@ -1337,13 +1335,7 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
     out->count_packed_columns = p->js.n_count_packed;
     out->count_nnz = p->js.count_nnz;
     out->split_columns = p->js.n_gen_static;
-    out->handed_over = 0;
-    if (p->js.d_gen_n.p && p->m->C > 0) {
-        uint32_t n = 0;
-        HIPCHK(hipMemcpyAsync(&n, p->js.d_gen_n.p, 4, hipMemcpyDeviceToHost, p->st.s));
-        HIPCHK(hipStreamSynchronize(p->st.s));
-        out->handed_over = n >= (uint32_t)p->js.n_gen_static ? n - p->js.n_gen_static : 0;
-    }
+    out->handed_over = 0;   // the count kernel reports a column it cannot take as an error
     return 0;
 }
 

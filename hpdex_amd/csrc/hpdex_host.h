@@ -147,7 +147,7 @@ struct JobState {
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace
     DevBuf spill, spill_off;                        // reference slices beyond the chip
     DevBuf d_job;                                   // descriptor
-    DevBuf d_count_items, d_gen_items, d_gen_n;     // work queues of the two paths
+    DevBuf d_count_items, d_gen_items;              // work queues of the two paths
     JobDesc J{};                                    // host copy of the descriptor
     int n_count = 0, n_count_cols = 0, n_gen_static = 0;   // count workgroups, their columns
     int n_count_packed = 0;                                  // of them read packed

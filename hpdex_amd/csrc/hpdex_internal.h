@@ -14,6 +14,7 @@ enum ZeroMode : int32_t { kZeroNone = 0, kZeroMin = 1, kZeroMax = 2, kZeroMix = 
 enum : uint32_t {
     kStatusCapacity = 1u,   // a reference slice beyond the on-chip sort had no spill space
     kStatusGroups = 2u,     // too many groups for the LDS counters
+    kStatusRange = 4u,      // a count-path column held a value outside its recorded range
 };
 
 // Launch geometry shared by the per-column kernels.
@@ -273,21 +274,17 @@ hipError_t launch_dense_count(int dtype, bool f64, const void* dense, size_t R, 
 hipError_t launch_dense_compact(bool f64, size_t R, size_t C, const int64_t* indptr, void* vals_out,
                                 int32_t* rows_out, void* ws, hipStream_t s);
 
-// One launch sequence of the Mann-Whitney pass of one job.  Integer-valued
-// columns go to the count kernel (count_items, one workgroup per column and
-// target slice); the rest, and every column the count kernel hands over at
-// run time, to the split -> reference -> target kernels, which read their
-// items from gen_items[0 .. *gen_n).  gen_items holds the n_gen_static
-// host-routed columns first and has room for every counted column more.
+// One launch sequence of the Mann-Whitney pass of one job: the count kernel
+// over the integer-valued columns the host routed to it (count_items, one
+// workgroup per column and target slice), then the split -> reference ->
+// target kernels over the other columns (gen_items[0 .. n_gen)).
 struct PipelineLaunch {
     const JobDesc* jobs;       // device copy (split/reference/target kernels)
     JobDesc job;               // host copy, passed by value to the count kernel
     const CountItem* count_items; // workgroups of the count kernel (one job per launch)
     int n_count;                  // their number
-    int n_count_cols;             // columns among them (hand-overs: at most one each)
-    WorkItem* gen_items;
-    uint32_t* gen_n;
-    int n_gen_static;
+    const WorkItem* gen_items;
+    int n_gen;
     int G;                 // largest group count over the jobs (LDS sizing)
     bool f64;
     uint32_t count_lds;    // dynamic LDS bytes of the count kernel

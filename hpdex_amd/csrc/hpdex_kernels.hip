@@ -11,7 +11,7 @@
 //              the tie term and the asymptotic p-value are written
 //                                          (group split, ranks, U, p-value)
 //
-//   split      every other column (and any the count kernel hands over):
+//   split      every other column:
 //              count stored values per group, exclusive scan, scatter each
 //              group's nonzero values as order-preserving keys (or histogram
 //              bins) into a per-column key range in HBM    (group split)
@@ -257,10 +257,6 @@ __device__ inline void work_item(const WorkItem* items, uint32_t k, int& job, in
     col = (int64_t)__builtin_amdgcn_readfirstlane(items[k].col);
 }
 
-// Workgroups of the hand-over kernel (used when the host routed no column to
-// the sort path, so the queue holds only the count kernel's rare hand-overs):
-// at most this many, each taking several queue items
-constexpr unsigned kHandoverGrid = 256;
 template <typename T> struct TypeTag { using type = T; };
 
 // A workgroup-uniform value held in an SGPR.
@@ -548,10 +544,9 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
 template <typename V>
 __global__ __launch_bounds__(kSplitBlock) void split_kernel(const JobDesc* jobs,
                                                             const WorkItem* items,
-                                                            const uint32_t* n_items,
+                                                            uint32_t n_items,
                                                             uint32_t lds_bytes) {
-    const uint32_t n = uniform_u32(*n_items);
-    if (blockIdx.x < n) split_item<V, kSplitBlock>(jobs, items, blockIdx.x, lds_bytes);
+    if (blockIdx.x < n_items) split_item<V, kSplitBlock>(jobs, items, blockIdx.x, lds_bytes);
 }
 
 // Rank table of the reference slice seen by the target phase.
@@ -789,9 +784,8 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
 // workgroup b takes queue item b (the grid is sized for the queue's capacity)
 template <typename V>
 __global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
-                                                   const uint32_t* n_items) {
-    const uint32_t n = uniform_u32(*n_items);
-    if (blockIdx.x < n) ref_item<V>(jobs, items, blockIdx.x);
+                                                   uint32_t n_items) {
+    if (blockIdx.x < n_items) ref_item<V>(jobs, items, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1185,32 +1179,8 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
 // the cap spills 4 VGPRs).
 template <typename V>
 __global__ __launch_bounds__(kColBlock) __attribute__((amdgpu_waves_per_eu(3)))
-void target_kernel(const JobDesc* jobs, const WorkItem* items, const uint32_t* n_items) {
-    const uint32_t n = uniform_u32(*n_items);
-    if (blockIdx.x < n) target_item<V>(jobs, items, blockIdx.x);
-}
-
-// The count kernel's hand-overs when the host routed no column to the sort
-// path: split, reference table and targets of each queued column in one
-// workgroup, one launch instead of three (each phase reads only what the
-// previous one wrote for the same column; the workgroup's barrier and fence
-// order them).
-template <typename V>
-__global__ __launch_bounds__(kColBlock) void handover_kernel(const JobDesc* jobs,
-                                                             const WorkItem* items,
-                                                             const uint32_t* n_items,
-                                                             uint32_t lds_bytes) {
-    const uint32_t n = uniform_u32(*n_items);
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        split_item<V, kColBlock>(jobs, items, k, lds_bytes);
-        __threadfence();
-        __syncthreads();
-        ref_item<V>(jobs, items, k);
-        __threadfence();
-        __syncthreads();
-        target_item<V>(jobs, items, k);
-        __syncthreads();   // the next item reuses LDS
-    }
+void target_kernel(const JobDesc* jobs, const WorkItem* items, uint32_t n_items) {
+    if (blockIdx.x < n_items) target_item<V>(jobs, items, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1234,10 +1204,10 @@ __global__ __launch_bounds__(kColBlock) void handover_kernel(const JobDesc* jobs
 // the targets into slices of S (CountItem), one workgroup each: every slice
 // reads the whole column and counts the reference row and its own targets,
 // so the slices of a column run side by side rather than one after another.
-// Columns whose slices would be too many, and columns whose values leave the
-// recorded range (the matrix's metadata no longer matches), take the
-// split/reference/target kernels instead (the latter are appended to their
-// queue by the column's first slice).
+// Columns whose slices would be too many take the split/reference/target
+// kernels instead (the host routes them); a column holding a value outside
+// its recorded range (the matrix's metadata no longer matches its values) is
+// an error of the job (kStatusRange).
 // ---------------------------------------------------------------------------
 struct CountState {
     uint32_t bad;
@@ -1266,12 +1236,10 @@ struct PairCounters {
 template <typename V, int NT>
 __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
                                                    const CountItem* __restrict__ items,
-                                                   WorkItem* overflow, uint32_t* n_overflow,
                                                    uint32_t lds_bytes) {
     using KT = KeyOf<V>;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ CountState st;
-    const int job = 0;
     // every value that steers control flow around a barrier is read into an
     // SGPR (a branch on a VGPR would put the barriers under exec masks)
     const CountItem item = items[blockIdx.x];
@@ -1293,14 +1261,14 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
 
     // the same test on every slice of the column (rows of S + 1), so that they
     // all agree whether the column is theirs
-    const int handover = __builtin_amdgcn_readfirstlane(
+    // (the host routed the column with the same tuning and metadata: a
+    // column failing here would be a host/device disagreement, reported as
+    // an error rather than counted wrongly)
+    const int unfit = __builtin_amdgcn_readfirstlane(
         ((int64_t)NB > (int64_t)uniform_u32((uint32_t)J.hist_max) ||
          count_lds_need(S + 1u, NB, wide) > (uint64_t)lds_bytes) ? 1 : 0);
-    if (handover) {
-        if (t0 == 0u && tid == 0) {
-            const uint32_t k = atomicAdd(n_overflow, 1u);
-            overflow[k] = WorkItem{job, (int32_t)c};
-        }
+    if (unfit) {
+        if (t0 == 0u && tid == 0) raise_status(J, kStatusRange, c);
         return;
     }
 
@@ -1533,13 +1501,10 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     if ((uint32_t)tid + (uint32_t)nt < rows) grs[tid + nt] = gpre1;
     for (uint32_t x = (uint32_t)tid + 2u * (uint32_t)nt; x < rows; x += nt) grs[x] = grows[t0 + x];
     __syncthreads();
-    // a value outside the recorded range: the column goes to the split path
-    // (queued once, by its first slice)
+    // a value outside the recorded range: the matrix's metadata no longer
+    // matches its values -- an error of the job (raised once, by the first slice)
     if (__builtin_amdgcn_readfirstlane((int)st.bad)) {
-        if (t0 == 0u && tid == 0) {
-            const uint32_t k = atomicAdd(n_overflow, 1u);
-            overflow[k] = WorkItem{job, (int32_t)c};
-        }
+        if (t0 == 0u && tid == 0) raise_status(J, kStatusRange, c);
         return;
     }
 
@@ -1804,12 +1769,6 @@ __global__ __launch_bounds__(256) void pack_columns_kernel(const V* vals, const 
     }
 }
 
-// Start of every launch sequence: the overflow queue holds the host-routed
-// columns only.
-__global__ void reset_queue_kernel(uint32_t* n, uint32_t v) {
-    if (threadIdx.x == 0) *n = v;
-}
-
 __global__ void init_job_state_kernel(unsigned long long* err_key, uint32_t* status,
                                       unsigned long long* status_col) {
     if (threadIdx.x == 0) {
@@ -2029,20 +1988,12 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
 }
 
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_t* ev) {
-    const int n_gen = L.n_gen_static + L.n_count_cols;
-    if (n_gen <= 0) return hipSuccess;
-    // host-routed columns: one workgroup per queue slot; hand-overs only: a
-    // small grid of looping workgroups
-    const bool loop = L.n_gen_static == 0;
-    const unsigned sort_grid =
-        loop ? ((unsigned)n_gen < kHandoverGrid ? (unsigned)n_gen : kHandoverGrid) : (unsigned)n_gen;
+    if (L.n_count <= 0 && L.n_gen <= 0) return hipSuccess;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], s); };
-    reset_queue_kernel<<<1, 64, 0, s>>>(L.gen_n, (uint32_t)L.n_gen_static);
     mark(0);
     if (L.n_count > 0) {
-#define HPDEX_COUNT(V, NT)                                                                  \
-    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_items, L.gen_items, \
-                                                           L.gen_n, L.count_lds)
+#define HPDEX_COUNT(V, NT) \
+    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_items, L.count_lds)
         if (L.f64) {
             if (L.count_block == 256) HPDEX_COUNT(double, 256);
             else if (L.count_block == 1024) HPDEX_COUNT(double, 1024);
@@ -2055,38 +2006,28 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
 #undef HPDEX_COUNT
     }
     mark(1);
-    size_t split = 12u * (size_t)L.G;
-    if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
-    // the most LDS the split kernel may take: one 1024-thread workgroup per CU
-    // fits in any case, and a column whose key range fits in what is left
-    // gathers its keys there and writes them out coalesced
-    const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
-    const unsigned tab_lds = table_lds_bytes(L.f64);
-    auto sort_path = [&](auto v_tag) {
-        using V = typename decltype(v_tag)::type;
-        split_kernel<V><<<sort_grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, L.gen_n,
-                                                                 split_lds);
-        mark(2);
-        ref_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-        mark(3);
-        target_kernel<V><<<sort_grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, L.gen_n);
-    };
-    // the hand-over kernel keeps to the table's LDS (no staging)
-    const unsigned both_split = align16(split);
-    const unsigned both_lds = both_split > tab_lds ? both_split : tab_lds;
-    if (loop) {   // hand-overs only: the three phases in one launch
-        if (L.f64)
-            handover_kernel<double><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items,
-                                                                           L.gen_n, both_lds);
-        else
-            handover_kernel<float><<<sort_grid, kColBlock, both_lds, s>>>(L.jobs, L.gen_items,
-                                                                          L.gen_n, both_lds);
-        mark(2);
-        mark(3);
-    } else if (L.f64) {
-        sort_path(TypeTag<double>{});
+    if (L.n_gen > 0) {
+        size_t split = 12u * (size_t)L.G;
+        if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
+        // the most LDS the split kernel may take: one 1024-thread workgroup
+        // per CU fits in any case, and a column whose key range fits in what
+        // is left gathers its keys there and writes them out coalesced
+        const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
+        const unsigned tab_lds = table_lds_bytes(L.f64);
+        const unsigned grid = (unsigned)L.n_gen, n = (uint32_t)L.n_gen;
+        auto sort_path = [&](auto v_tag) {
+            using V = typename decltype(v_tag)::type;
+            split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
+            mark(2);
+            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            mark(3);
+            target_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+        };
+        if (L.f64) sort_path(TypeTag<double>{});
+        else sort_path(TypeTag<float>{});
     } else {
-        sort_path(TypeTag<float>{});
+        mark(2);
+        mark(3);
     }
     mark(4);
     return hipGetLastError();

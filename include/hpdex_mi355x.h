@@ -216,9 +216,9 @@ int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie
                        double* n1_out, double* n2_out, char* err, size_t err_len);
 
 /* Where the plan's columns go: the one-pass count kernel (integer-valued
- * columns) or the split/reference/target kernels; handed_over counts the
- * count-kernel columns that the last run passed on to the split kernels
- * (values outside the column's recorded range).  count_blocks is the count
+ * columns) or the split/reference/target kernels; handed_over is always 0
+ * (a count-kernel column holding a value outside its recorded range makes
+ * the run fail with an error naming the column).  count_blocks is the count
  * kernel's workgroups: one per column, or one per slice of targets when a
  * column's counters for every target do not fit one workgroup's LDS;
  * count_packed_columns the count-kernel columns read as packed entries.     */

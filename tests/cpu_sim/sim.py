@@ -57,14 +57,14 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     outs = [np.zeros(max(n, 1)) for _ in range(5)]
     err = np.zeros(1, dtype=np.uint64)
     status = np.zeros(1, dtype=np.uint32)
-    lib().sim_set_shrink_range(int(shrink_range))   # test hook: forces hand-overs
+    lib().sim_set_shrink_range(int(shrink_range))   # test hook: metadata that does not match
     lib().sim_set_pack(int(pack))   # packed entries for the count kernel where they fit
     lib().sim_set_split_arrival(int(split_arrival))
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
                   float(timeout_s))
-    mwu.last_counted = counted   # columns the count kernel finished itself
+    mwu.last_counted = counted   # columns routed to the count kernel
     mwu.last_blocks = lib().sim_count_blocks()   # its workgroups (column x target slice)
     mwu.last_packed = lib().sim_packed_blocks()   # of them, those that read packed entries
     return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

@@ -27,7 +27,8 @@ using namespace hpdex;
 namespace {
 
 int sim_last_blocks = 0;   // count-kernel workgroups of the last sim_mwu call
-int sim_shrink_range = 0;  // test hook: record each column's range one value short
+int sim_shrink_range = 0;  // test hook: record each column's range one value short, so the
+                           // count kernel meets values outside it (an error status)
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
 int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
 int sim_last_packed = 0;   // count-kernel workgroups of the last call that read packed entries
@@ -78,8 +79,8 @@ bool run_block(unsigned b, unsigned nt, unsigned grid, const std::function<void(
 
 // The launch sequence of launch_mwu_pipeline: column classification, the
 // count kernel over the integer-valued columns, then the split / reference /
-// target kernels over the queue (host-routed columns plus hand-overs).
-// Returns the number of columns the count kernel finished itself.
+// target kernels over the other columns.
+// Returns the number of columns routed to the count kernel.
 template <typename V>
 static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& order,
                         uint32_t count_lds, unsigned count_block, double timeout_s) {
@@ -156,52 +157,36 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     sim_last_blocks = (int)items.size();
     sim_last_packed = 0;
     for (const CountItem& x : items) sim_last_packed += (x.flags & kCountPacked) ? 1 : 0;
-    uint32_t gen_n = n_static;
     const unsigned nb = (unsigned)items.size();
     for (unsigned b = 0; b < nb; ++b)
         run_block(b, count_block, nb, [&] {
             if (count_block == 256)
-                count_kernel<V, 256>(J, items.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 256>(J, items.data(), count_lds);
             else if (count_block == 1024)
-                count_kernel<V, 1024>(J, items.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 1024>(J, items.data(), count_lds);
             else
-                count_kernel<V, 512>(J, items.data(), gen_items.data(), &gen_n, count_lds);
+                count_kernel<V, 512>(J, items.data(), count_lds);
         }, timeout_s);
     const WorkItem* it = gen_items.data();
-    // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most;
-    // the hand-over kernel's the larger of the counters and the reference table
+    // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most,
+    // the reference and target kernels' table
     auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
     size_t split = 12u * (size_t)J.G;
     if (split > (size_t)kLdsBudget) split = 16;
     const uint32_t split_lds = split > 16u ? (uint32_t)kSplitLdsMax : a16(split);
-    const size_t hist = 4u * (size_t)(kHistMaxBins + 1);
-    const size_t sort = (sizeof(typename KeyOf<V>::K) == 8 ? 8u * (size_t)kRefKeysF64
-                                                           : 4u * (size_t)kRefKeysF32) +
-                        4u * (size_t)(kMaxBuckets + 1);
-    const uint32_t both_lds = std::max(a16(split), a16(std::max(hist, sort)));
-    if (n_static == 0) {
-        // hand-overs only: the combined kernel, on a grid of at most 3
-        // workgroups so that each takes several queue items
-        const unsigned hg = std::min(nc, 3u);
-        for (unsigned b = 0; b < hg; ++b)
-            run_block(b, kColBlock, hg, [&] { handover_kernel<V>(jobs, it, &gen_n, both_lds); },
-                      timeout_s);
-    } else {
-        const unsigned ng = n_static + nc;
-        for (unsigned b = 0; b < ng; ++b)
-            run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, &gen_n, split_lds); },
-                      timeout_s);
-        for (unsigned b = 0; b < ng; ++b)
-            run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, &gen_n); }, timeout_s);
-        for (unsigned b = 0; b < ng; ++b)
-            run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, &gen_n); }, timeout_s);
-    }
-    return (int)(nc - (gen_n - n_static));
+    const unsigned ng = n_static;
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, ng, split_lds); }, timeout_s);
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, ng); }, timeout_s);
+    for (unsigned b = 0; b < ng; ++b)
+        run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, ng); }, timeout_s);
+    return (int)nc;
 }
 
 // count_lds: dynamic LDS of the count kernel (0 = every column takes the
 // split/reference/target kernels); block: its workgroup size (256, 512, 1024).  Columns are visited in reversed order.
-// Returns the number of columns the count kernel finished itself.
+// Returns the number of columns routed to the count kernel.
 extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int64_t* indptr,
                        int R, int C, const int32_t* gid, int T, int mode, int alt, int tie_corr,
                        int cc, int count_lds, int block, int hist_max, double* U1, double* P,

@@ -72,3 +72,22 @@ def test_kernel_module_defaults():
                  "SupportsIndex = 2, method", "layout: str = 'csc'", "show_progress: bool = False"):
         assert frag in doc, frag
     assert "include_zeros: bool = True" in kernel.group_mean.__doc__
+
+
+def test_register_budget_check_reads_the_compiler_report():
+    """The build's register check parses the compiler's resource remarks and
+    refuses a kernel over the spill budget (a report, not a device run)."""
+    from hpdex_amd import build
+    report = "\n".join([
+        "x.hip:1:1: remark: Function Name: _Z1kv [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:1:1: remark:     VGPRs: 100 [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:1:1: remark:     SGPRs Spill: 2 [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:1:1: remark:     VGPRs Spill: 0 [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:2:1: remark: Function Name: _Z1jv [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:2:1: remark:     SGPRs Spill: 280 [-Rpass-analysis=kernel-resource-usage]",
+    ])
+    res = build.kernel_resources(report)
+    assert res["_Z1kv"] == {"VGPRs": 100, "SGPRs Spill": 2, "VGPRs Spill": 0}
+    with pytest.raises(RuntimeError, match="_Z1jv: 280 SGPR"):
+        build.check_resources(report)
+    build.check_resources("\n".join(report.splitlines()[:4]))

@@ -378,26 +378,25 @@ def test_sim_count_kernel_extreme_values():
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("mode", ["min", "none"])
-def test_sim_count_kernel_hand_overs(mode, mixed):
+def test_sim_count_kernel_metadata_mismatch_is_an_error(mixed):
     # recorded ranges one value short of the data: every counted column meets
-    # a value outside its range and is handed to the sort path at run time --
-    # with no host-routed column (mixed False) the combined hand-over kernel
-    # takes them, otherwise the per-column kernels after the routed float one.
-    # The columns are read as rows and values: packed entries rank the values
-    # a column holds, so they cannot meet one outside their bins.
+    # a value outside its range, and the job reports the status (naming the
+    # lowest such column) instead of counting it wrongly; a column the host
+    # routed to the sort path beside them is still done.  The columns are
+    # read as rows and values: packed entries rank the values a column holds,
+    # so they cannot meet one outside their bins.
     d, r, ip, g = _rw(32, 3000, 5, 6, 0.5, lambda rg, n: rg.integers(-3, 12, n))
     if mixed:
         d = d.copy()
         d[ip[4]] = 0.5          # column 4 is not integer-valued: routed by the host
-    U1, P, TIE, n1, n2, err, status = sim.mwu(d, r, ip, g, 6, 3000, mode=ZH[mode],
+    U1, P, TIE, n1, n2, err, status = sim.mwu(d, r, ip, g, 6, 3000, mode=ZH["min"],
                                               shrink_range=True, timeout_s=120, pack=False)
-    assert err == 2 ** 64 - 1 and status == 0
-    assert sim.mwu.last_counted == 0
-    oU1, oP, oT, on1, on2 = oracle.mwu(d, r, ip, g, 6, n_rows=3000, zero_handling=mode,
-                                       use_continuity=True, with_stats=True)
-    assert np.array_equal(U1, oU1) and np.array_equal(TIE, oT)
-    assert np.array_equal(n1, on1) and np.array_equal(n2, on2) and np.array_equal(P, oP)
+    assert status == 4 and err == 2 ** 64 - 1   # kStatusRange
+    assert sim.mwu.last_counted == (4 if mixed else 5)
+    if mixed:
+        oU1, oP, oT, on1, on2 = oracle.mwu(d, r, ip, g, 6, n_rows=3000, zero_handling="min",
+                                           use_continuity=True, with_stats=True)
+        assert np.array_equal(U1[24:30], oU1[24:30]) and np.array_equal(TIE[24:30], oT[24:30])
 
 
 def test_sim_count_kernel_eight_byte_values():
