@@ -305,6 +305,209 @@ def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
     return out
 
 
+ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
+
+
+def opts_of(w):
+    from hpdex_amd import _capi
+    o = w["opts"]
+    return _capi.make_opts(tie_correction=o["tie_correction"], use_continuity=o["use_continuity"],
+                           zero_handling=ZH[o["zero_handling"]],
+                           method=1 if o["method"] == "exact" else 2)
+
+
+def describe(name, w):
+    o = w["opts"]
+    return (f"{name}: {w['n_rows']} cells x {w['n_cols']} genes, nnz {int(w['indptr'][-1])}, "
+            f"reference {int((w['group_id'] == 0).sum())} cells, {w['n_targets']} targets, "
+            f"zero_handling={o['zero_handling']}, method={o['method']}")
+
+
+def first_job(w):
+    """One job on a matrix that is not resident yet: upload (H2D of the int64
+    ABI arrays, column classification, packed entries, row conversion),
+    the plan (labels, routing, graph capture), one pass and the results back
+    in host memory -- what a single job on a fresh matrix costs."""
+    from hpdex_amd import _capi
+    opts = opts_of(w)
+    t0 = time.perf_counter()
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    t1 = time.perf_counter()
+    p = _capi.Plan(m, w["group_id"], w["n_targets"], opts, form=1)
+    t2 = time.perf_counter()
+    p.run(1)
+    U1, P = p.results()
+    t3 = time.perf_counter()
+    p.close()
+    m.close()
+    return {"first_job_ms": (t3 - t0) * 1e3, "matrix_upload_ms": (t1 - t0) * 1e3,
+            "plan_ms": (t2 - t1) * 1e3, "pass_and_results_ms": (t3 - t2) * 1e3}
+
+
+def measure(name, w, steps, warmup, plain=True, cpu=True, cpu_target_s=12.0, m=None):
+    """The workload on a resident matrix: `steps` replays of the batched form
+    (graph), the per-kernel times and the dominant kernel's roofline, the
+    plain form on the same resident matrix (launched eagerly), the plain
+    one-shot call from host buffers, and the CPU baseline sample."""
+    from hpdex_amd import _capi, synthetic
+    C, T, R = w["n_cols"], w["n_targets"], w["n_rows"]
+    opts = opts_of(w)
+    own = m is None
+    if own:
+        m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], R)
+    pb = _capi.Plan(m, w["group_id"], T, opts, form=1)
+    pb.run(warmup)
+    log(f"{name}: timing {steps} steps (batched)")
+    ms_total, graph_ms = pb.run(steps)
+    ms_step = ms_total / steps
+    U1b, Pb = pb.results()
+    tests = C * T
+    value = tests / (ms_step / 1e3)
+    vbytes = 4 if w["data"].dtype == np.float32 else 8
+    info = pb.info()
+    # roofline of the dominant kernel: its own mean duration from HIP events
+    # recorded around it on the plan's stream (plain launch sequence)
+    stages = pb.stage_times(max(3, steps // 2))
+    if stages["count"] >= max(stages["split"], stages["reference"], stages["target"]):
+        dom, dom_ms = "count_kernel", stages["count"]
+        B = synthetic.pass_bytes(info["count_nnz"], info["count_columns"], R, T, vbytes)
+    else:
+        dom = max(("split", "reference", "target"), key=lambda k: stages[k])
+        dom_ms = stages[dom]
+        dom = dom + "_kernel"
+        B = synthetic.algorithmic_bytes(w, vbytes)
+    achieved = B / (dom_ms / 1e3) / 1e9
+    step_B = synthetic.algorithmic_bytes(w, vbytes)
+    log(f"{name}: {ms_step:.3f} ms/step, {value:.3e} tests/s; kernels {stages}; "
+        f"dominant {dom} {achieved:.0f} GB/s algorithmic")
+    out = {"workload": describe(name, w), "value": value, "unit": "tests/s",
+           "ms_per_step": ms_step, "graph_ms": graph_ms, "tests_per_step": tests,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": traffic_from_profiles(dom, name), "kernel": dom,
+                        "algorithmic_bytes": B, "kernel_ms": dom_ms},
+           "step_roofline_frac": step_B / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "kernel_ms": stages, "routing": info}
+    if plain:
+        pp = _capi.Plan(m, w["group_id"], T, opts, form=0)
+        pp.run(2)
+        pms, pker = pp.run(max(3, steps // 5))
+        U1p, Pp = pp.results()
+        same = bool(np.array_equal(U1p, U1b) and np.array_equal(Pp, Pb))
+        pp.close()
+        t0 = time.perf_counter()
+        _capi.mwu_csc(w["data"], w["indices"], w["indptr"], w["group_id"], T, opts, n_rows=R)
+        e2e = time.perf_counter() - t0
+        out["plain"] = {"resident_ms": pker, "resident_tests_per_s": tests / (pker / 1e3),
+                        "batched_over_plain_resident": value / (tests / (pker / 1e3)),
+                        "per_call_ms": e2e * 1e3, "per_call_tests_per_s": tests / e2e,
+                        "per_call_note": "one-shot C-ABI call from host buffers: upload of "
+                                         "the int64 ABI arrays over PCIe, kernels, download",
+                        "bit_identical_to_batched": same}
+        log(f"{name}: plain resident {pker:.3f} ms (identical {same}); "
+            f"one-shot call {e2e * 1e3:.1f} ms")
+    pb.close()
+    if own:
+        m.close()
+    if cpu:
+        log(f"{name}: CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(w, target_s=cpu_target_s)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        log(f"{name}: CPU {out['cpu_baseline']['value']:.3e} tests/s")
+    return out
+
+
+def batch_bench(seed=1005, n_jobs=64, reps=5):
+    """Many jobs on resident matrices, batched against plain: the serving
+    workload's jobs (4 matrices of the serving distribution, each job a new
+    target set of 5-500 targets) with every matrix in HBM.  Plain: one plan
+    per job in the plain form, run one after another (each run launches its
+    kernels eagerly and waits).  Batched: the same plans as one hpdex_batch
+    (one queue of every job's count-kernel items, largest first, one graph).
+    Wall time of `reps` passes over all jobs; results bit-identical."""
+    from hpdex_amd import _capi, synthetic
+    pool, jobs = synthetic.serving_workload(seed=seed, n_jobs=n_jobs)
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True, zero_handling=1)
+    mats = [_capi.DeviceMatrix(m["data"], m["indices"], m["indptr"], m["n_rows"]) for m in pool]
+    plans = [_capi.Plan(mats[j["matrix"]], j["group_id"], j["n_targets"], opts, form=0)
+             for j in jobs]
+    tests = sum(pool[j["matrix"]]["n_cols"] * j["n_targets"] for j in jobs)
+    for p in plans:                      # warm-up
+        p.run(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in plans:
+            p.run(1)
+    t_plain = (time.perf_counter() - t0) / reps
+    want = [p.results() for p in plans]
+    b = _capi.Batch(plans)
+    b.run(2)
+    t0 = time.perf_counter()
+    b.run(reps)
+    t_batch = (time.perf_counter() - t0) / reps
+    same = all(np.array_equal(p.results()[0], w[0]) and np.array_equal(p.results()[1], w[1])
+               for p, w in zip(plans, want))
+    info = b.info()
+    b.close()
+    for p in plans:
+        p.close()
+    for m in mats:
+        m.close()
+    out = {"workload": f"{len(jobs)} jobs of the serving distribution on {len(pool)} resident "
+                       "matrices (" + ", ".join(f"{m['n_rows']}x{m['n_cols']}" for m in pool) +
+                       "), zero_handling=min, tie + continuity, asymptotic",
+           "tests": int(tests), "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3,
+           "plain_tests_per_s": tests / t_plain, "batched_tests_per_s": tests / t_batch,
+           "batched_over_plain": t_plain / t_batch, "bit_identical": bool(same),
+           "count_items": info["count_items"]}
+    log(f"batch: {len(jobs)} jobs, plain {t_plain * 1e3:.2f} ms, batched {t_batch * 1e3:.2f} ms "
+        f"({out['batched_over_plain']:.2f}x), identical {same}")
+    return out
+
+
+def batch_target_sets(w, n_sets, reps=5, seed=7):
+    """Several target sets against one resident matrix (the reference fixed,
+    the targets redrawn per set with the same sizes), batched against plain
+    as in batch_bench."""
+    from hpdex_amd import _capi
+    rng = np.random.default_rng(seed)
+    opts = opts_of(w)
+    m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], w["n_rows"])
+    gid0 = w["group_id"]
+    plans = []
+    for _ in range(n_sets):
+        gid = gid0.copy()
+        tgt = np.nonzero(gid0 > 0)[0]
+        gid[tgt] = gid0[tgt][rng.permutation(len(tgt))]   # same sizes, new members
+        plans.append(_capi.Plan(m, gid, w["n_targets"], opts, form=0))
+    tests = n_sets * w["n_cols"] * w["n_targets"]
+    for p in plans:
+        p.run(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in plans:
+            p.run(1)
+    t_plain = (time.perf_counter() - t0) / reps
+    want = [p.results() for p in plans]
+    b = _capi.Batch(plans)
+    b.run(2)
+    t0 = time.perf_counter()
+    b.run(reps)
+    t_batch = (time.perf_counter() - t0) / reps
+    same = all(np.array_equal(p.results()[0], x[0]) and np.array_equal(p.results()[1], x[1])
+               for p, x in zip(plans, want))
+    b.close()
+    for p in plans:
+        p.close()
+    m.close()
+    log(f"batch of {n_sets} target sets: plain {t_plain * 1e3:.2f} ms, batched "
+        f"{t_batch * 1e3:.2f} ms ({t_plain / t_batch:.2f}x), identical {same}")
+    return {"workload": f"{n_sets} target sets against one resident matrix: {describe('', w)}",
+            "tests": int(tests), "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3,
+            "plain_tests_per_s": tests / t_plain, "batched_tests_per_s": tests / t_batch,
+            "batched_over_plain": t_plain / t_batch, "bit_identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +520,11 @@ def main():
                          "them from on later runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the float (min, mix) and exact workloads reported beside the "
+                         "headline")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the many-jobs batched-vs-plain measurement")
     ap.add_argument("--stream-limit-gb", type=float, default=0.0,
                     help="also stream the workload's DE from memory-mapped files under this "
                          "memory limit (GB) and compare with the one-shot call")
@@ -332,9 +540,8 @@ def main():
     ap.add_argument("--count-block", type=int, default=None,
                     help="tuning: workgroup size of the count kernel (256, 512, 1024)")
     ap.add_argument("--autotune", action="store_true",
-                    help="time every candidate tuning (histogram/sort cutover, count kernel LDS "
-                         "and workgroup size) on this workload before the timed steps and run "
-                         "them under the fastest (the default, unless --no-autotune, "
+                    help="time every candidate tuning on this workload before the timed steps "
+                         "and run them under the fastest (the default, unless --no-autotune, "
                          "--count-lds, --count-block or --tuning is given)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the timed steps under the built-in tuning")
@@ -351,26 +558,24 @@ def main():
               "(genes would shard by column; see the design notes).", file=sys.stderr)
         sys.exit(2)
 
-    from hpdex_amd import _capi, synthetic
+    from hpdex_amd import _capi
     if args.copy_probe_mib:
         gbs = _capi.bandwidth_probe(args.copy_probe_mib << 20, iters=3)
         print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20, "gbs": gbs}), flush=True)
         return
     if args.count_lds is not None or args.count_block is not None:
         _capi.set_tuning(count_lds_bytes=args.count_lds, count_block=args.count_block)
-    ZH = {"none": 0, "min": 1, "max": 2, "mix": 3}
     log(f"generating workload {args.workload}")
     t0 = time.perf_counter()
     w = load_workload(args.workload, args.workload_cache)
-    o = w["opts"]
     C, T, R = w["n_cols"], w["n_targets"], w["n_rows"]
     nnz = int(w["indptr"][-1])
     log(f"generated in {time.perf_counter() - t0:.1f}s: {R} cells x {C} genes, nnz={nnz}, T={T}")
-    info = _capi.device_info()
-    log(f"device {info}")
-    opts = _capi.make_opts(tie_correction=o["tie_correction"], use_continuity=o["use_continuity"],
-                           zero_handling=ZH[o["zero_handling"]],
-                           method=1 if o["method"] == "exact" else 2)
+    log(f"device {_capi.device_info()}")
+    extra = {}
+    if not args.no_plain:
+        extra["first_job"] = first_job(w)
+        log(f"first job on a fresh matrix: {extra['first_job']}")
     m = _capi.DeviceMatrix(w["data"], w["indices"], w["indptr"], R)
     log(f"resident matrix: {m.device_bytes() / 1e9:.2f} GB")
     tuned = None
@@ -379,11 +584,11 @@ def main():
         from hpdex_amd import autotune
         t0 = time.perf_counter()
         # knobs given by hand are kept: the sweep covers the others only
-        space = {k: v for k, v in autotune.DEFAULT_SPACE.items()
+        space = {k: v for k, v in autotune.space_for(m).items()
                  if not (k == "count_lds_bytes" and args.count_lds is not None)
                  and not (k == "count_block" and args.count_block is not None)}
         try:
-            res = autotune.autotune(m, w["group_id"], T, opts, space=space, log=log)
+            res = autotune.autotune(m, w["group_id"], T, opts_of(w), space=space, log=log)
         except Exception as e:       # the tuning in force was restored: time under it
             tuned = {"error": str(e), "tuning": autotune.current()}
             log(f"autotune failed, timing the tuning in force: {e}")
@@ -397,52 +602,9 @@ def main():
     elif args.tuning:
         from hpdex_amd import autotune
         tuned = {"tuning": autotune.load(args.tuning), "from": args.tuning}
-    pb = _capi.Plan(m, w["group_id"], T, opts, form=1)
-    log("warmup (batched)")
-    pb.run(args.warmup)
-    log(f"timing {args.steps} steps (batched)")
-    ms_total, kernel_ms = pb.run(args.steps)
-    ms_step = ms_total / args.steps
-    U1b, Pb = pb.results()
-    tests = C * T
-    value = tests / (ms_step / 1e3)
-    vbytes = 4 if w["data"].dtype == np.float32 else 8
-    info = pb.info()
-    log(f"batched: {ms_step:.3f} ms/step, graph {kernel_ms:.3f} ms, {value:.3e} tests/s; "
-        f"routing {info}")
-    # roofline of the dominant kernel: its own mean duration from HIP events
-    # recorded around it on the plan's stream (plain launch sequence)
-    stages = pb.stage_times(max(3, args.steps // 2))
-    log(f"kernel times (ms): {stages}")
-    if stages["count"] >= max(stages["split"], stages["reference"], stages["target"]):
-        dom, dom_ms = "count_kernel", stages["count"]
-        B = synthetic.pass_bytes(info["count_nnz"], info["count_columns"], R, T, vbytes)
-    else:
-        dom = max(("split", "reference", "target"), key=lambda k: stages[k])
-        dom_ms = stages[dom]
-        dom = dom + "_kernel"
-        B = synthetic.algorithmic_bytes(w, vbytes)
-    achieved = B / (dom_ms / 1e3) / 1e9
-    log(f"dominant kernel {dom}: {dom_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
-
-    extra = {}
-    if not args.no_plain:
-        pp = _capi.Plan(m, w["group_id"], T, opts, form=0)
-        pp.run(2)
-        pms, pker = pp.run(max(3, args.steps // 5))
-        U1p, Pp = pp.results()
-        same = bool(np.array_equal(U1p, U1b) and np.array_equal(Pp, Pb))
-        pp.close()
-        log(f"plain kernel (resident): {pker:.3f} ms; identical to batched: {same}")
-        t0 = time.perf_counter()
-        _capi.mwu_csc(w["data"], w["indices"], w["indptr"], w["group_id"], T, opts, n_rows=R)
-        e2e = time.perf_counter() - t0
-        log(f"plain per-call (host buffers, incl. transfers): {e2e * 1e3:.1f} ms")
-        extra["plain"] = {"per_call_ms": e2e * 1e3, "per_call_tests_per_s": tests / e2e,
-                          "resident_kernel_ms": pker, "resident_tests_per_s": tests / (pker / 1e3),
-                          "batched_over_plain_per_call": value / (tests / e2e),
-                          "batched_over_plain_resident": value / (tests / (pker / 1e3)),
-                          "bit_identical_to_batched": same}
+    head = measure(args.workload, w, args.steps, args.warmup, plain=not args.no_plain,
+                   cpu=not args.no_cpu_baseline, m=m)
+    m.close()
     if not args.no_ingest and args.workload in ("histogram", "small"):
         log("ingest: CSR with int32 indices, host route vs device route")
         extra["ingest"] = ingest_bench(w)
@@ -455,43 +617,53 @@ def main():
         sb = extra["stream"]
         log(f"stream: {sb['runs']} runs, {sb['stream_ms']:.0f} ms vs one-shot {sb['one_shot_ms']:.0f} ms, "
             f"identical {sb['identical']}")
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("CPU baseline (oracle)")
-        cpu = cpu_baseline(w)
-        log(f"CPU baseline: {cpu['value']:.3e} tests/s on {cpu['cores']} threads")
-    pb.close()
-    m.close()
+    del w
+    if not args.no_configs and args.workload == "histogram":
+        extra["configs"] = {}
+        for name in ("float", "float-mix", "exact"):
+            log(f"workload {name}")
+            wx = load_workload(name, args.workload_cache)
+            extra["configs"][name] = measure(name, wx, max(3, args.steps // 5), 2,
+                                             plain=not args.no_plain,
+                                             cpu=not args.no_cpu_baseline, cpu_target_s=6.0)
+            del wx
+    if not args.no_batch:
+        from hpdex_amd import synthetic
+        log("many jobs on resident matrices: batched vs plain")
+        extra["batch"] = {"serving_mix": batch_bench(n_jobs=args.serving_jobs),
+                          "small_target_sets": batch_target_sets(synthetic.small_case(), 64)}
+        if args.workload == "histogram":
+            wh = load_workload("histogram", args.workload_cache)
+            extra["batch"]["histogram_target_sets"] = batch_target_sets(wh, 8)
+            del wh
     serving = None
     if not args.no_serving:
         log("serving workload")
         serving = serving_bench(n_jobs=args.serving_jobs, rate=args.serving_rate)
-    traffic = traffic_from_profiles(dom, args.workload)
     line = {
         "metric": "gene\u00d7target MWU tests/sec on 1 MI355X, batched vs plain; p50/p99 job latency",
-        "value": value,
+        "value": head["value"],
         "unit": "tests/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-keys/int64-U/f64-p",
         "data": "synthetic (seeded generator, hpdex_amd/synthetic.py)",
-        "config": {"workload": f"{args.workload}: {R} cells x {C} genes, nnz {nnz}, "
-                               f"reference {int((w['group_id'] == 0).sum())} cells, {T} targets, "
-                               f"zero_handling={o['zero_handling']}, method={o['method']}",
-                   "cells": R, "genes": C, "targets": T, "nnz": nnz,
-                   "tests_per_step": tests, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                     "algorithmic_bytes": B, "kernel_ms": dom_ms},
-        "kernel_ms": stages,
-        "routing": info,
-        "graph_ms": kernel_ms,
-        "cpu_baseline": cpu,
+        "config": {"workload": head["workload"], "cells": R, "genes": C, "targets": T,
+                   "nnz": nnz, "tests_per_step": head["tests_per_step"],
+                   "parallelism": "single GPU"},
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "batched_over_plain_resident": head.get("plain", {}).get("batched_over_plain_resident"),
+        "kernel_ms": head["kernel_ms"],
+        "routing": head["routing"],
+        "graph_ms": head["graph_ms"],
+        "step_roofline_frac": head["step_roofline_frac"],
+        "plain": head.get("plain"),
         "serving": serving,
     }
     if tuned is not None:

@@ -32,6 +32,7 @@ EXPORTED = (
     "hpdex_session_destroy", "hpdex_fdr_bh", "hpdex_de_csc", "hpdex_matrix_create_from",
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
+    "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -140,6 +141,15 @@ def lib() -> ctypes.CDLL:
         L.hpdex_mwu_csc_noalloc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz,
                                             ctypes.POINTER(MwuOpts), ci, vp, vp, cp, sz]
         L.hpdex_mwu_csc_noalloc.restype = ci
+        L.hpdex_batch_create.argtypes = [ctypes.POINTER(vp), sz, ctypes.POINTER(vp), cp, sz]
+        L.hpdex_batch_create.restype = ci
+        L.hpdex_batch_run.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_batch_run.restype = ci
+        L.hpdex_batch_get_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64)]
+        L.hpdex_batch_get_info.restype = ci
+        L.hpdex_batch_destroy.argtypes = [vp]
+        L.hpdex_batch_destroy.restype = None
         L.hpdex_de_csc_device_bytes.argtypes = [ci, sz, sz, sz, sz]
         L.hpdex_de_csc_device_bytes.restype = sz
         for name in ("hpdex_mwu_csc", "hpdex_mwu_csc_stats", "hpdex_group_mean_csc",
@@ -460,6 +470,46 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().hpdex_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several plans run as one launch sequence (hpdex_batch_*): one queue of
+    every plan's count-kernel items, largest first, then the sort-path
+    kernels over every plan's other columns, replayed from one hipGraph.
+    Results stay in each plan (Plan.results)."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_batch_create(arr, len(self.plans), ctypes.byref(h), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        self._h = h
+
+    def run(self, iters=1) -> float:
+        """Replay `iters` times and wait; returns the elapsed milliseconds."""
+        ms = ctypes.c_double()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_batch_run(self._h, int(iters), ctypes.byref(ms), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        return ms.value
+
+    def info(self) -> dict:
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        lib().hpdex_batch_get_info(self._h, ctypes.byref(a), ctypes.byref(b))
+        return {"count_items": a.value, "split_columns": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hpdex_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

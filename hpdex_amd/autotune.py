@@ -11,7 +11,14 @@ out several ways, and which is fastest depends on the matrix:
 * ``count_lds_bytes`` -- LDS per histogram workgroup: more LDS holds more
   target rows per workgroup (fewer slices re-reading a column) at fewer
   workgroups per CU;
-* ``count_block``    -- threads per histogram workgroup (256, 512, 1024).
+* ``count_block``    -- threads per histogram workgroup (256, 512, 1024);
+* ``split_arrival``  -- sort path: the split kernel keeps a sparse column's
+  entry groups in LDS and reads only its values again to scatter them (1), or
+  reads rows, values and labels twice (0).
+
+Candidates that lay the job out the same way as one already timed (the
+histogram knobs of a job with no histogram column, the split knob of a job
+with no sort-path column) are not timed again: they take that trial's time.
 
 ``autotune`` times every candidate on the caller's own resident matrix and
 group labels in the batched (graph-replayed) form, checks that every
@@ -36,7 +43,15 @@ DEFAULT_SPACE = {
     "hist_max_bins": (8192, 2048, 0),
     "count_lds_bytes": (40448, 52 * 1024, 152 * 1024),
     "count_block": (256, 512, 1024),
+    "split_arrival": (1, 0),
 }
+COUNT_KNOBS = ("count_lds_bytes", "count_block", "count_packed")
+SPLIT_KNOBS = ("split_arrival",)
+
+
+def space_for(matrix=None) -> dict:
+    """The search space for a job on `matrix` (every knob of DEFAULT_SPACE)."""
+    return dict(DEFAULT_SPACE)
 
 KNOBS = ("hist_max_bins", "size_ordered", "count_lds_bytes", "count_block", "count_packed",
          "split_arrival")
@@ -67,7 +82,7 @@ def candidates(space: dict | None = None) -> list[dict]:
     for vals in itertools.product(*(space[n] for n in names)):
         c = dict(zip(names, (int(v) for v in vals)))
         if c.get("hist_max_bins", 1) == 0:
-            c = {"hist_max_bins": 0}
+            c = {k: v for k, v in c.items() if k not in COUNT_KNOBS}
         key = tuple(sorted(c.items()))
         if key not in seen:
             seen.add(key)
@@ -103,25 +118,40 @@ def autotune(matrix, group_id, n_targets, opts, space=None, iters=20, warmup=3, 
     cands = candidates(space)
     if not cands:
         raise ValueError("the tuning space has no candidate")
-    trials, ref = [], None
+    trials, ref, timed = [], None, {}
     try:
         for cand in cands:
+            cur = {**base, **cand}
             try:
                 # a tuning the library refuses (e.g. LDS this device does not
                 # give) is a candidate that cannot run, not a failed sweep
-                apply({**base, **cand})
+                apply(cur)
                 plan = _capi.Plan(matrix, group_id, n_targets, opts, form=1)
             except _capi.HpdexError as e:
                 trials.append({**cand, "ms": None, "error": str(e)})
                 if log:
                     log(f"autotune {cand}: not runnable ({e})")
                 continue
+            info = plan.info()
+            # the knobs that shape this job's launches: the histogram knobs
+            # only with a histogram column, the split knob only with a
+            # sort-path column (and the routing the cutover gives)
+            eff = (info["count_columns"], info["count_blocks"],
+                   tuple(int(cur.get(k, 0)) for k in COUNT_KNOBS) if info["count_columns"] else (),
+                   tuple(int(cur.get(k, 0)) for k in SPLIT_KNOBS) if info["split_columns"] else ())
+            if eff in timed:
+                plan.close()
+                t = dict(timed[eff])
+                trials.append({**cand, "ms": t["ms"], "count_columns": t["count_columns"],
+                               "count_blocks": t["count_blocks"], "same_layout_as": t["index"]})
+                if log:
+                    log(f"autotune {cand}: same layout as candidate {t['index']}")
+                continue
             try:
                 # the first candidate that runs gets a second warm-up
                 plan.run(warmup * (2 if ref is None else 1))
                 U1, P = plan.results()
                 ms = min(plan.run(iters)[1] for _ in range(max(1, repeats)))
-                info = plan.info()
             except _capi.HpdexError as e:
                 trials.append({**cand, "ms": None, "error": str(e)})
                 if log:
@@ -133,6 +163,8 @@ def autotune(matrix, group_id, n_targets, opts, space=None, iters=20, warmup=3, 
                 ref = (U1.copy(), P.copy())
             elif not (np.array_equal(U1, ref[0]) and np.array_equal(P, ref[1], equal_nan=True)):
                 raise TuningMismatch(f"tuning {cand} changed the results")
+            timed[eff] = {"ms": ms, "count_columns": info["count_columns"],
+                          "count_blocks": info["count_blocks"], "index": len(trials)}
             trials.append({**cand, "ms": ms, "count_columns": info["count_columns"],
                            "count_blocks": info["count_blocks"]})
             if log:

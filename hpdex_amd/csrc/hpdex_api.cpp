@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <set>
@@ -362,12 +363,9 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
         HIPCHK(js.n2.alloc(np * 8));
     }
     HIPCHK(js.flags.alloc(32));
-    HIPCHK(js.keys.alloc(m->nnz * (m->f64 ? 8 : 4)));
-    HIPCHK(js.cnt.alloc(js.C * js.G * 4));
-    HIPCHK(js.zcnt.alloc(js.C * js.G * 4));
-    HIPCHK(js.goff.alloc(js.C * js.G * 4));
-    HIPCHK(js.plan.alloc(js.C * sizeof(ColPlan)));
-    HIPCHK(js.reftab.alloc(js.C * (size_t)kTabStride * 4));
+    // the sort path's workspace (keys, counts, offsets, column plans,
+    // reference tables) is allocated by job_upload_desc, for jobs that route
+    // a column there
     HIPCHK(launch_init_job_state(js.err_key(), js.status(), js.status_col(), s));
     return 0;
 }
@@ -465,6 +463,20 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     }
     js.n_count = (int)count_items.size();
     js.n_gen_static = (int)gen_items.size();
+    if (!gen_items.empty()) {   // the sort path's workspace
+        HIPCHK(js.keys.alloc(m.nnz * (m.f64 ? 8 : 4)));
+        HIPCHK(js.cnt.alloc(js.C * js.G * 4));
+        HIPCHK(js.zcnt.alloc(js.C * js.G * 4));
+        HIPCHK(js.goff.alloc(js.C * js.G * 4));
+        HIPCHK(js.plan.alloc(js.C * sizeof(ColPlan)));
+        HIPCHK(js.reftab.alloc(js.C * (size_t)kTabStride * 4));
+        js.J.keys = js.keys.p;
+        js.J.cnt = js.cnt.as<uint32_t>();
+        js.J.zcnt = js.zcnt.as<uint32_t>();
+        js.J.goff = js.goff.as<uint32_t>();
+        js.J.plan = js.plan.as<ColPlan>();
+        js.J.reftab = js.reftab.as<uint32_t>();
+    }
     // spill space for the sort-path columns whose reference slice may pass
     // the on-chip sort (its stored values: at most the column's entries and
     // the reference's rows)
@@ -503,7 +515,9 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     if (!gen_items.empty())
         HIPCHK(hipMemcpyAsync(js.d_gen_items.p, gen_items.data(),
                               gen_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));   // the item vectors are host temporaries
+    HIPCHK(hipStreamSynchronize(s));
+    js.h_count_items = std::move(count_items);
+    js.h_gen_items = std::move(gen_items);   // the item vectors are host temporaries
     return 0;
 }
 
@@ -1418,5 +1432,178 @@ int hpdex_plan_results(hpdex_plan* p, double* U1_out, double* P_out, double* tie
     if (job_errors(p->js, p->st.s, err, err_len)) return 1;
     return copy_out(p->js, U1_out, P_out, tie_out, n1_out, n2_out, p->st.s, err, err_len);
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: several plans run as one launch sequence
+// ---------------------------------------------------------------------------
+struct hpdex_batch {
+    std::vector<hpdex_plan*> plans;
+    Stream st;
+    DevBuf d_jobs[2], d_count[2], d_gen[2];   // [0] float matrices, [1] double
+    BatchLaunch L[2]{};
+    int64_t n_count_items = 0, n_gen_items = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~hpdex_batch() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }
+};
+
+namespace hpdex_host {
+
+// What an item of the count kernel costs, to order the queue: the entries it
+// reads plus the (row, bin) cells its per-target sums walk.
+static uint64_t count_item_cost(const CountItem& it, size_t T) {
+    const uint64_t rows = 1u + std::min<uint64_t>(it.S, T - it.t0);
+    return (uint64_t)it.n + rows * (uint64_t)it.nb / 4u;
+}
+
+// The batch's launch sequence: per value type, the persistent count kernel
+// over every plan's items and the sort-path kernels over every plan's other
+// columns; then each exact plan's p-value lookup.
+static int batch_enqueue(hpdex_batch* b, char* err, size_t err_len) {
+    const hipStream_t s = b->st.s;
+    for (int v = 0; v < 2; ++v)
+        if (b->L[v].n_count > 0 || b->L[v].n_gen > 0) HIPCHK(launch_mwu_batch(b->L[v], s));
+    for (hpdex_plan* p : b->plans)
+        if (p->js.opts.method == 1 && exact_launch(p->js, s, false, err, err_len)) return 1;
+    return 0;
+}
+
+}  // namespace hpdex_host
+
+extern "C" {
+
+int hpdex_batch_create(hpdex_plan* const* plans, size_t n_plans, hpdex_batch** out, char* err,
+                       size_t err_len) {
+    if (!out || (n_plans && !plans)) return set_err(err, err_len, "null argument");
+    *out = nullptr;
+    if (n_plans == 0) return set_err(err, err_len, "a batch needs at least one plan");
+    if (n_plans >= ((size_t)1 << (32 - kCountJobShift)))
+        return set_err(err, err_len, "too many plans in one batch");
+    if (ensure_init(err, err_len)) return 1;
+    std::unique_ptr<hpdex_batch> b(new hpdex_batch());
+    b->plans.assign(plans, plans + n_plans);
+    for (size_t i = 0; i < n_plans; ++i) {
+        if (!plans[i]) return set_err(err, err_len, "null plan in batch");
+        for (size_t k = 0; k < i; ++k)
+            if (plans[k] == plans[i]) return set_err(err, err_len, "a plan appears twice in the batch");
+        if (plans[i]->js.opts.method == 1 && plans[i]->js.opts.zero_handling == 0)
+            return set_err(err, err_len,
+                           "exact p with zero_handling none sizes its tables from the data: run "
+                           "that plan on its own");
+    }
+    if (b->st.create() != hipSuccess) return set_err(err, err_len, "stream creation failed");
+    const Tuning tu = tuning();
+    for (int v = 0; v < 2; ++v) {
+        std::vector<JobDesc> jobs;
+        std::vector<std::pair<uint64_t, CountItem>> ci;
+        std::vector<std::pair<int64_t, WorkItem>> gi;
+        uint32_t lds = 0;
+        int G = 2;
+        for (size_t i = 0; i < n_plans; ++i) {
+            const hpdex_plan* p = plans[i];
+            if ((int)p->m->f64 != v) continue;
+            const int32_t job = (int32_t)jobs.size();
+            jobs.push_back(p->js.J);
+            lds = std::max(lds, p->js.count_lds);
+            G = std::max(G, (int)p->js.G);
+            for (CountItem it : p->js.h_count_items) {
+                it.flags |= (uint32_t)job << kCountJobShift;
+                ci.emplace_back(count_item_cost(it, p->js.T), it);
+            }
+            for (WorkItem w : p->js.h_gen_items) {
+                const int64_t len = p->m->h_indptr[w.col + 1] - p->m->h_indptr[w.col];
+                w.job = job;
+                gi.emplace_back(len, w);
+            }
+        }
+        if (jobs.empty()) continue;
+        // the queue, largest first (a stable order: ties keep plan order)
+        std::stable_sort(ci.begin(), ci.end(),
+                         [](const auto& a, const auto& c) { return a.first > c.first; });
+        std::stable_sort(gi.begin(), gi.end(),
+                         [](const auto& a, const auto& c) { return a.first > c.first; });
+        std::vector<CountItem> items(ci.size());
+        for (size_t k = 0; k < ci.size(); ++k) items[k] = ci[k].second;
+        std::vector<WorkItem> gen(gi.size());
+        for (size_t k = 0; k < gi.size(); ++k) gen[k] = gi[k].second;
+        const hipStream_t s = b->st.s;
+        HIPCHK(b->d_jobs[v].alloc(jobs.size() * sizeof(JobDesc)));
+        HIPCHK(hipMemcpyAsync(b->d_jobs[v].p, jobs.data(), jobs.size() * sizeof(JobDesc),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(b->d_count[v].alloc(items.size() * sizeof(CountItem)));
+        if (!items.empty())
+            HIPCHK(hipMemcpyAsync(b->d_count[v].p, items.data(), items.size() * sizeof(CountItem),
+                                  hipMemcpyHostToDevice, s));
+        HIPCHK(b->d_gen[v].alloc(gen.size() * sizeof(WorkItem)));
+        if (!gen.empty())
+            HIPCHK(hipMemcpyAsync(b->d_gen[v].p, gen.data(), gen.size() * sizeof(WorkItem),
+                                  hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));   // the vectors are host temporaries
+        BatchLaunch& L = b->L[v];
+        L.jobs = b->d_jobs[v].as<JobDesc>();
+        L.count_items = b->d_count[v].as<CountItem>();
+        L.n_count = (int)items.size();
+        L.gen_items = b->d_gen[v].as<WorkItem>();
+        L.n_gen = (int)gen.size();
+        L.G = G;
+        L.f64 = v == 1;
+        L.count_lds = lds ? lds : (uint32_t)tu.count_lds;
+        L.count_block = tu.count_block;
+        L.split_lds_max = g_count_ready ? (uint32_t)kSplitLdsMax : (uint32_t)kLdsBudget;
+        b->n_count_items += L.n_count;
+        b->n_gen_items += L.n_gen;
+    }
+    // one graph for the whole sequence, replayed by hpdex_batch_run
+    if (hipStreamBeginCapture(b->st.s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return set_err(err, err_len, "graph capture failed to start");
+    const int rc = batch_enqueue(b.get(), err, err_len);
+    const hipError_t e = hipStreamEndCapture(b->st.s, &b->graph);
+    if (rc) return 1;
+    if (e != hipSuccess) return set_err(err, err_len, hipGetErrorString(e));
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+        return set_err(err, err_len, "graph instantiation failed");
+    *out = b.release();
+    return 0;
+}
+
+int hpdex_batch_run(hpdex_batch* b, int iters, double* ms_total, char* err, size_t err_len) {
+    if (!b) return set_err(err, err_len, "null batch");
+    if (iters < 1) iters = 1;
+    const hipStream_t s = b->st.s;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    (void)hipEventRecord(e0, s);
+    for (int it = 0; it < iters && !rc; ++it) {
+        const hipError_t e = hipGraphLaunch(b->exec, s);
+        if (e != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(e));
+    }
+    (void)hipEventRecord(e1, s);
+    const hipError_t se = hipStreamSynchronize(s);
+    if (!rc && se != hipSuccess) rc = set_err(err, err_len, hipGetErrorString(se));
+    if (!rc && ms_total) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *ms_total = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int hpdex_batch_get_info(hpdex_batch* b, int64_t* count_items, int64_t* split_columns) {
+    if (!b) return 1;
+    if (count_items) *count_items = b->n_count_items;
+    if (split_columns) *split_columns = b->n_gen_items;
+    return 0;
+}
+
+void hpdex_batch_destroy(hpdex_batch* b) { delete b; }
 
 }  // extern "C"

@@ -148,6 +148,8 @@ struct JobState {
     DevBuf spill, spill_off;                        // reference slices beyond the chip
     DevBuf d_job;                                   // descriptor
     DevBuf d_count_items, d_gen_items;              // work queues of the two paths
+    std::vector<CountItem> h_count_items;           // host copies (batched launches merge them)
+    std::vector<WorkItem> h_gen_items;
     JobDesc J{};                                    // host copy of the descriptor
     int n_count = 0, n_count_cols = 0, n_gen_static = 0;   // count workgroups, their columns
     int n_count_packed = 0;                                  // of them read packed

@@ -130,9 +130,11 @@ struct CountItem {
     int32_t col;
     uint16_t t0;
     uint16_t S;
-    uint32_t flags;   // kCountPacked: read the column's packed entries
+    uint32_t flags;   // kCountPacked: read the column's packed entries; bits 8.. its job
+                      // in a batched launch
 };
 constexpr uint32_t kCountPacked = 1u;
+constexpr uint32_t kCountJobShift = 8u;
 // words of the pack kernel's presence bitmap: every 16-bit integer value
 constexpr uint32_t kPackBitmapWords = 2048u;
 
@@ -291,6 +293,24 @@ struct PipelineLaunch {
     int count_block;       // its workgroup size, 256, 512 or 1024
     uint32_t split_lds_max;   // dynamic LDS the split kernel may take
 };
+// The batched launch of several jobs whose matrices share one value type:
+// the count kernel over count_items (every job's, ordered largest first; each
+// names its job), one workgroup per item, then the split -> reference ->
+// target kernels over gen_items (every job's other columns, largest first,
+// WorkItem.job naming the job).
+struct BatchLaunch {
+    const JobDesc* jobs;          // device array, indexed by an item's job
+    const CountItem* count_items;
+    int n_count;
+    const WorkItem* gen_items;
+    int n_gen;
+    int G;                        // largest group count over the jobs
+    bool f64;
+    uint32_t count_lds;
+    int count_block;
+    uint32_t split_lds_max;
+};
+hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s);
 // ev (optional, 5 events): recorded before the count kernel and after the
 // count, split, reference and target kernels, for per-kernel timing.
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,

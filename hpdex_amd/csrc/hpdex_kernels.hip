@@ -1230,19 +1230,18 @@ struct PairCounters {
     }
 };
 
-// J arrives by value in the kernel arguments, so its fields are read with
-// scalar loads into SGPRs; items lists the (column, target slice) workgroups
-// of this launch with their columns' extents and value ranges.
+// One (column, target slice) item of a job: the column's extents and value
+// range in `item_in`, the job's fields read with scalar loads into SGPRs
+// (J is a kernel argument, or a job of the batch's descriptor array at a
+// workgroup-uniform index).
 template <typename V, int NT>
-__global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
-                                                   const CountItem* __restrict__ items,
-                                                   uint32_t lds_bytes) {
+__device__ __forceinline__ void count_item(const JobDesc& J, const CountItem& item_in,
+                                           uint32_t lds_bytes, CountState& st) {
     using KT = KeyOf<V>;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ CountState st;
     // every value that steers control flow around a barrier is read into an
     // SGPR (a branch on a VGPR would put the barriers under exec masks)
-    const CountItem item = items[blockIdx.x];
+    const CountItem item = item_in;
     const int64_t c = (int64_t)uniform_u32((uint32_t)item.col);
     const uint32_t t0 = uniform_u32((uint32_t)item.t0);
     const uint32_t S = uniform_u32((uint32_t)item.S);
@@ -1656,6 +1655,32 @@ __global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
     }
 }
 
+// The plain launch: workgroup b takes item b of one job.
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void count_kernel(const JobDesc J,
+                                                   const CountItem* __restrict__ items,
+                                                   uint32_t lds_bytes) {
+    __shared__ CountState st;
+    count_item<V, NT>(J, items[blockIdx.x], lds_bytes, st);
+}
+
+// The batched launch: the items of several jobs (each names its job) in one
+// queue ordered largest first, one workgroup per item.  The hardware hands
+// out workgroups in queue order as CUs free up, so long columns start first
+// and short ones of any job fill in behind them -- the queue is drained by
+// the dispatcher rather than by a software loop (an item loop inside the
+// kernel keeps values of one item live across items: 178 to 256 VGPRs
+// against 106, one or two waves per SIMD instead of four).
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void count_queue_kernel(const JobDesc* __restrict__ jobs,
+                                                         const CountItem* __restrict__ items,
+                                                         uint32_t lds_bytes) {
+    __shared__ CountState st;
+    const CountItem it = items[blockIdx.x];
+    const uint32_t job = uniform_u32(it.flags >> kCountJobShift);
+    count_item<V, NT>(jobs[job], it, lds_bytes, st);
+}
+
 // Metadata of a resident matrix, computed once at upload: ok[c] = 1 when every
 // stored value of column c is non-finite or an integer in [-32768, 32767];
 // range[2c], range[2c+1] = smallest and largest stored nonzero finite value
@@ -1972,7 +1997,13 @@ hipError_t prepare_count_kernel(uint32_t bytes) {
                          (const void*)count_kernel<float, 1024>,
                          (const void*)count_kernel<double, 256>,
                          (const void*)count_kernel<double, 512>,
-                         (const void*)count_kernel<double, 1024>};
+                         (const void*)count_kernel<double, 1024>,
+                         (const void*)count_queue_kernel<float, 256>,
+                         (const void*)count_queue_kernel<float, 512>,
+                         (const void*)count_queue_kernel<float, 1024>,
+                         (const void*)count_queue_kernel<double, 256>,
+                         (const void*)count_queue_kernel<double, 512>,
+                         (const void*)count_queue_kernel<double, 1024>};
     for (const void* f : fns) {
         const hipError_t e =
             hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -2030,6 +2061,40 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
         mark(3);
     }
     mark(4);
+    return hipGetLastError();
+}
+
+hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
+    if (L.n_count > 0) {
+#define HPDEX_COUNTQ(V, NT)                                                                  \
+    count_queue_kernel<V, NT><<<(unsigned)L.n_count, NT, L.count_lds, s>>>(L.jobs, L.count_items, \
+                                                                          L.count_lds)
+        if (L.f64) {
+            if (L.count_block == 256) HPDEX_COUNTQ(double, 256);
+            else if (L.count_block == 1024) HPDEX_COUNTQ(double, 1024);
+            else HPDEX_COUNTQ(double, 512);
+        } else {
+            if (L.count_block == 256) HPDEX_COUNTQ(float, 256);
+            else if (L.count_block == 1024) HPDEX_COUNTQ(float, 1024);
+            else HPDEX_COUNTQ(float, 512);
+        }
+#undef HPDEX_COUNTQ
+    }
+    if (L.n_gen > 0) {
+        size_t split = 12u * (size_t)L.G;
+        if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
+        const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
+        const unsigned tab_lds = table_lds_bytes(L.f64);
+        const unsigned grid = (unsigned)L.n_gen, n = (uint32_t)L.n_gen;
+        auto sort_path = [&](auto v_tag) {
+            using V = typename decltype(v_tag)::type;
+            split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
+            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            target_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+        };
+        if (L.f64) sort_path(TypeTag<double>{});
+        else sort_path(TypeTag<float>{});
+    }
     return hipGetLastError();
 }
 

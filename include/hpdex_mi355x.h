@@ -238,6 +238,27 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
  * those of a normal run.                                                    */
 int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size_t err_len);
 
+/* ---------------- batches: several plans in one launch sequence ----------------
+ * No counterpart in the reference.  A batch runs several plans (jobs on
+ * resident matrices: different target sets on one matrix, or jobs on
+ * different matrices) as one launch sequence captured in a hipGraph: the
+ * count kernel over one queue of every plan's (column, target slice) items,
+ * ordered largest first and handed out to the CUs in that order, then the
+ * split/reference/target kernels over every plan's other columns (largest
+ * first), then the exact p lookups.  Each plan's results stay in the plan
+ * (hpdex_plan_results).  The plans are borrowed for the batch's life and
+ * must not run on their own while a batch run is in flight.  Plans of exact
+ * p in zero_handling none cannot join.                                      */
+typedef struct hpdex_batch hpdex_batch;
+int hpdex_batch_create(hpdex_plan* const* plans, size_t n_plans, hpdex_batch** out, char* err,
+                       size_t err_len);
+/* Replay the batch `iters` times back to back and wait; ms_total receives
+ * the elapsed milliseconds (HIP events on the batch's stream). */
+int hpdex_batch_run(hpdex_batch* b, int iters, double* ms_total, char* err, size_t err_len);
+/* The batch's count-kernel items (workgroups) and sort-path columns. */
+int hpdex_batch_get_info(hpdex_batch* b, int64_t* count_items, int64_t* split_columns);
+void hpdex_batch_destroy(hpdex_batch* b);
+
 /* Tunable choices of the device path.  hpdex_amd.autotune times candidate
  * settings on a caller's job and applies the fastest through
  * hpdex_set_tuning; the defaults are its choice on integer-count matrices.  */

@@ -9,14 +9,16 @@ from hpdex_amd import autotune
 
 def test_candidates_default_space():
     c = autotune.candidates()
-    # 2 histogram cutovers x 3 LDS sizes x 3 workgroup sizes, plus the
-    # sort path alone (its LDS and workgroup size do not matter)
-    assert len(c) == 2 * 3 * 3 + 1
-    assert {"hist_max_bins": 0} in c
+    # (2 histogram cutovers x 3 LDS sizes x 3 workgroup sizes, plus the sort
+    # path alone, whose LDS and workgroup size do not matter) x 2 split modes
+    assert len(c) == (2 * 3 * 3 + 1) * 2
+    assert {"hist_max_bins": 0, "split_arrival": 1} in c
+    assert {"hist_max_bins": 0, "split_arrival": 0} in c
     assert len({tuple(sorted(x.items())) for x in c}) == len(c)
     for x in c:
+        assert x["split_arrival"] in (0, 1)
         if x["hist_max_bins"]:
-            assert set(x) == {"hist_max_bins", "count_lds_bytes", "count_block"}
+            assert set(x) == {"hist_max_bins", "count_lds_bytes", "count_block", "split_arrival"}
             assert x["count_block"] in (256, 512, 1024)
             assert x["count_lds_bytes"] % 16 == 0 and 16384 <= x["count_lds_bytes"] <= 155648
 
@@ -50,6 +52,8 @@ def test_save_load_round_trip(tmp_path, monkeypatch):
 class _FakePlan:
     """Stands in for a device plan: results and timing follow the tuning."""
     tuning = {}
+    count_columns = 1
+    split_columns = 0
     results_of = staticmethod(lambda t: (None, None))
 
     def __init__(self, *a, **k):
@@ -62,7 +66,8 @@ class _FakePlan:
         return _FakePlan.results_of(self.t)
 
     def info(self):
-        return {"count_columns": 1, "count_blocks": 1}
+        return {"count_columns": _FakePlan.count_columns, "count_blocks": 1,
+                "split_columns": _FakePlan.split_columns}
 
     def close(self):
         pass
@@ -76,6 +81,8 @@ def fake_device(monkeypatch):
     monkeypatch.setattr(autotune, "apply",
                         lambda t: (state.update(t), setattr(_FakePlan, "tuning", dict(state))))
     monkeypatch.setattr(autotune._capi, "Plan", _FakePlan)
+    monkeypatch.setattr(_FakePlan, "count_columns", 1)
+    monkeypatch.setattr(_FakePlan, "split_columns", 0)
     same = (np.arange(4.0), np.full(4, 0.5))
     _FakePlan.results_of = staticmethod(lambda t: same)
     return state
@@ -145,3 +152,29 @@ def test_autotune_records_a_tuning_the_library_refuses(fake_device, monkeypatch)
 def test_autotune_empty_space_is_an_error(fake_device):
     with pytest.raises(ValueError, match="no candidate"):
         autotune.autotune(None, None, 1, None, space={"count_block": ()})
+
+
+def test_autotune_times_one_candidate_per_layout(fake_device, monkeypatch):
+    """A job with no histogram column: the histogram knobs do not change its
+    launches, so their candidates take the first one's time; the split knob
+    does and is timed."""
+    monkeypatch.setattr(_FakePlan, "count_columns", 0)
+    monkeypatch.setattr(_FakePlan, "split_columns", 5)
+    runs = []
+    orig = _FakePlan.run
+
+    def run(self, iters=1):
+        runs.append(dict(self.t))
+        return orig(self, iters)
+
+    monkeypatch.setattr(_FakePlan, "run", run)
+    res = autotune.autotune(None, None, 1, None, warmup=1, iters=1, repeats=1,
+                            space={"count_block": (256, 512), "split_arrival": (1, 0)})
+    timed = [t for t in res["trials"] if "same_layout_as" not in t]
+    assert [t["split_arrival"] for t in timed] == [1, 0]
+    assert len(res["trials"]) == 4
+    for t in res["trials"]:
+        if "same_layout_as" in t:
+            src = res["trials"][t["same_layout_as"]]
+            assert t["ms"] == src["ms"] and t["split_arrival"] == src["split_arrival"]
+    assert {r["split_arrival"] for r in runs} == {0, 1}
