@@ -266,6 +266,7 @@ def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
     tickets = [submit(sess, j) for j in jobs]
     sat_lat = [sess.wait(t)[2]["latency_ms"] for t in tickets]
     t_sat = time.perf_counter() - t0
+    batching = sess.batches()
     sess.close()
     # plain: one blocking C-ABI call per job (upload, kernels, download)
     plain = []
@@ -297,6 +298,7 @@ def serving_bench(n_jobs=64, rate=8.0, slots=4, budget_gb=48, seed=1005):
         "plain_p50_ms": pct(plain, 50), "plain_p99_ms": pct(plain, 99),
         "saturated_over_plain": (sum(tests) / t_sat) / (sum(tests) / t_plain),
         "peak_jobs_in_flight": peak["jobs_in_flight"],
+        "merged_launches": batching,
         "peak_arena_gb": peak["arena_bytes"] / 2 ** 30,
     }
     log(f"serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms at {rate}/s; "

@@ -33,6 +33,7 @@ EXPORTED = (
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
     "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
+    "hpdex_session_batches",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -78,7 +79,9 @@ class Job(ctypes.Structure):
 
 class JobTimes(ctypes.Structure):
     _fields_ = [("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
-                ("latency_ms", ctypes.c_double), ("footprint_bytes", ctypes.c_size_t)]
+                ("latency_ms", ctypes.c_double), ("footprint_bytes", ctypes.c_size_t),
+                ("upload_ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("kernels_ms", ctypes.c_double), ("finish_ms", ctypes.c_double)]
 
 
 class HpdexError(RuntimeError):
@@ -148,6 +151,8 @@ def lib() -> ctypes.CDLL:
         L.hpdex_batch_get_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int64)]
         L.hpdex_batch_get_info.restype = ci
+        L.hpdex_session_batches.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int64)] * 3
+        L.hpdex_session_batches.restype = ci
         L.hpdex_batch_destroy.argtypes = [vp]
         L.hpdex_batch_destroy.restype = None
         L.hpdex_de_csc_device_bytes.argtypes = [ci, sz, sz, sz, sz]
@@ -629,6 +634,12 @@ class Session:
         j, b = ctypes.c_int32(), ctypes.c_size_t()
         lib().hpdex_session_peak(self._h, ctypes.byref(j), ctypes.byref(b))
         return {"jobs_in_flight": j.value, "arena_bytes": b.value}
+
+    def batches(self):
+        """Merged kernel launches so far, the jobs they ran, the most in one."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        lib().hpdex_session_batches(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"launches": a.value, "jobs": b.value, "max_jobs": c.value}
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

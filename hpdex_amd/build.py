@@ -117,7 +117,7 @@ def build(verbose: bool = False) -> None:
               f"{worst[1].get('SGPRs Spill', 0)})")
     tmp = LIB + ".tmp"
     _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", k_obj, d_obj, i_obj, a_obj,
-           s_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so"]])
+           s_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so", "-Wl,--no-undefined"]])
     os.replace(tmp, LIB)
 
     import pybind11  # noqa: WPS433 (build-time only)

@@ -216,6 +216,24 @@ size_t spill_keys_of(int64_t len, uint32_t gr0, size_t cap);
 size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64);
 size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T);
 
+// Several jobs of one value type as one batched launch: their descriptors,
+// every job's count-kernel items (naming the job) and sort-path columns, each
+// list ordered largest first.
+struct MergedJobs {
+    std::vector<JobDesc> jobs;
+    std::vector<CountItem> count;
+    std::vector<WorkItem> gen;
+    uint32_t lds = 0;
+    int G = 2;
+};
+void merge_jobs(const std::vector<const JobState*>& jss, const std::vector<const hpdex_matrix*>& ms,
+                MergedJobs& out);
+// Put a merged batch on the device (reusing the buffers when they are large
+// enough) and describe its launch.
+int upload_merged(const MergedJobs& mj, bool f64, const Tuning& tu, DevBuf& d_jobs,
+                  DevBuf& d_count, DevBuf& d_gen, hipStream_t s, BatchLaunch& L, char* err,
+                  size_t err_len);
+
 int mwu_oneshot(int dtype, const void* data, const int64_t* indices, const int64_t* indptr,
                 size_t n_rows, size_t n_cols, size_t nnz, const int32_t* group_id,
                 size_t n_targets, const hpdex_mwu_opts* opts, double* U1_out, double* P_out,

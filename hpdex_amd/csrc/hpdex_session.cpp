@@ -1,5 +1,6 @@
 // hpdex_session.cpp -- serving session: DE jobs of different sizes admitted by
-// HBM footprint and run concurrently on one GPU (continuous batching).
+// HBM footprint and run concurrently on one GPU (continuous batching: the
+// work items of every job ready at the time run in one merged launch).
 //
 // One arena of device memory is allocated when the session starts.  Each job's
 // footprint -- an upper bound of every buffer it allocates (its uploaded matrix
@@ -7,15 +8,19 @@
 // -- is computed when it is submitted.  Worker threads, one stream each, take
 // jobs from a FIFO queue whenever the head job's footprint fits in the arena;
 // the job's buffers are then carved from its block (BumpBlock) with no
-// hipMalloc, its kernels run on the worker's stream beside the other workers'
-// jobs, and the block returns to the arena when the results are in the
-// caller's host buffers.  Jobs that name a resident matrix skip the upload:
+// hipMalloc; the worker uploads and sets the job up on its own stream, then
+// hands it to the launcher thread, which runs the kernels of every job
+// waiting at that moment as one merged launch (one queue of all their
+// count-kernel items, largest first, and one list of their sort-path
+// columns) while the next jobs upload; the worker then copies the results
+// out and the block returns to the arena.  Jobs that name a resident matrix skip the upload:
 // a new target set against cells already in HBM costs only its workspace.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <utility>
 #include <deque>
 #include <map>
 #include <memory>
@@ -83,6 +88,7 @@ struct JobRec {
     hpdex_job job{};
     size_t footprint = 0;
     Clock::time_point t_submit, t_start, t_end;
+    Clock::time_point t_uploaded, t_ready, t_ran;   // phases of the run (see hpdex_job_times)
     bool done = false;
     int rc = 0;
     std::string err;
@@ -143,12 +149,38 @@ struct hpdex_session {
     int in_flight = 0, peak_in_flight = 0;
     size_t peak_bytes = 0;
     std::vector<std::thread> workers;
+    // continuous batching: jobs set up and waiting for their kernels, and the
+    // launcher that runs every waiting job's work items as one merged launch
+    std::deque<struct ReadyJob*> ready;
+    std::condition_variable cv_ready;   // launcher: a job is ready (or the session stops)
+    std::condition_variable cv_ran;     // workers: a batch holding their job ended
+    std::thread launcher;
+    bool stop_launcher = false;
+    int64_t batches = 0, batched_jobs = 0, max_batch = 0;
 };
 
 namespace {
 
-// Everything one job does, on the worker's stream, with buffers from its block.
-void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
+}  // namespace
+
+// A job whose matrix, labels and work lists are on the device, waiting for
+// the launcher to run its kernels in the next merged launch.
+struct ReadyJob {
+    const JobState* js;
+    const hpdex_matrix* m;
+    bool ran = false;
+    int rc = 0;
+    std::string err;
+};
+
+namespace {
+
+// Everything one job does, on the worker's stream, with buffers from its
+// block: the upload (when it brings its matrix), labels, routing and exact
+// tables; then its kernels, in a launch the launcher merges with every other
+// job ready at the time; then the exact p lookup, the errors and the copy of
+// its results to the caller's buffers.
+void run_job(hpdex_session* S, JobRec& R, hipStream_t s, char* block, size_t cap) {
     BumpBlock blk;
     blk.base = block;
     blk.cap = cap;
@@ -172,6 +204,7 @@ void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
                 if (rc) break;
                 m = own.get();
             }
+            R.t_uploaded = R.t_ready = R.t_ran = Clock::now();
             if (m->C == 0) break;
             rc = job_setup(js, m, j.group_id, j.n_targets, &j.opts, false, s, err, err_len);
             if (rc) break;
@@ -183,9 +216,25 @@ void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
                 rc = exact_plan(js, nullptr, nullptr, s, err, err_len);
                 if (rc) break;
             }
-            const hipError_t e = launch_mwu_pipeline(pipeline_of(js, m->f64), s);
-            if (e != hipSuccess) {
-                rc = set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(e));
+            // everything the kernels read is on the device before they run on
+            // the launcher's stream
+            if (hipStreamSynchronize(s) != hipSuccess) {
+                rc = set_err(err, err_len, "HIP error before the job's kernels");
+                break;
+            }
+            ReadyJob rj;
+            rj.js = &js;
+            rj.m = m;
+            R.t_ready = Clock::now();
+            {
+                std::unique_lock<std::mutex> lk(S->mu);
+                S->ready.push_back(&rj);
+                S->cv_ready.notify_one();
+                S->cv_ran.wait(lk, [&] { return rj.ran; });
+            }
+            R.t_ran = Clock::now();
+            if (rj.rc) {
+                rc = set_err(err, err_len, rj.err);
                 break;
             }
             if (exact) {
@@ -204,6 +253,161 @@ void run_job(JobRec& R, hipStream_t s, char* block, size_t cap) {
     t_block = nullptr;
     R.rc = rc;
     if (rc) R.err = err;
+}
+
+// Pinned host staging of one merged launch's lists (allocated with
+// hipHostMalloc, grown as needed), so that their copies to the device are
+// asynchronous and the launcher can build the next launch meanwhile.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t fit(size_t bytes) {
+        if (p && n >= bytes) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        const hipError_t e = hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault);
+        if (e == hipSuccess) n = bytes ? bytes : 16;
+        return e;
+    }
+};
+
+// One merged launch in flight: its jobs, the event after its kernels, and the
+// device and pinned buffers its lists live in until it ends.
+struct InFlight {
+    std::vector<ReadyJob*> jobs;
+    hipEvent_t done = nullptr;
+    int rc = 0;
+    std::string err;
+};
+struct LaunchSlot {
+    DevBuf d_jobs[2], d_count[2], d_gen[2];
+    PinnedBuf h_jobs[2], h_count[2], h_gen[2];
+};
+
+// Copy one merged list to the device through its pinned staging, on s.
+template <typename T>
+hipError_t stage(const std::vector<T>& v, PinnedBuf& h, DevBuf& d, hipStream_t s) {
+    const size_t bytes = v.size() * sizeof(T);
+    hipError_t e = h.fit(bytes);
+    if (e != hipSuccess) return e;
+    if (!(d.p && d.n >= bytes) && (e = d.alloc(bytes ? bytes : 16)) != hipSuccess) return e;
+    if (!bytes) return hipSuccess;
+    std::memcpy(h.p, v.data(), bytes);
+    return hipMemcpyAsync(d.p, h.p, bytes, hipMemcpyHostToDevice, s);
+}
+
+// The launcher: whenever jobs are ready, take all of them and run their work
+// items as one merged launch per value type (the count kernel over one queue
+// of every job's items, largest first, then the sort-path kernels over every
+// job's other columns).  Launches go to one stream back to back; two may be
+// in flight, each with its own buffers, so the next one is built and its
+// lists copied while the current one runs; a launch's jobs are released when
+// its event has passed.
+void launcher(hpdex_session* S) {
+    Stream st;
+    const hipError_t se = st.create();
+    constexpr int kSlots = 2;
+    LaunchSlot slots[kSlots];
+    std::deque<std::pair<InFlight, int>> flight;   // with its slot
+    int next_slot = 0;
+    auto release = [&](InFlight& f) {
+        const hipError_t e = hipEventSynchronize(f.done);
+        if (!f.rc && e != hipSuccess) {
+            f.rc = 1;
+            f.err = std::string("HIP error: ") + hipGetErrorString(e);
+        }
+        (void)hipEventDestroy(f.done);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (ReadyJob* r : f.jobs) {
+                r->rc = f.rc;
+                if (f.rc) r->err = f.err;
+                r->ran = true;
+            }
+            S->batches += 1;
+            S->batched_jobs += (int64_t)f.jobs.size();
+            S->max_batch = std::max(S->max_batch, (int64_t)f.jobs.size());
+        }
+        S->cv_ran.notify_all();
+    };
+    for (;;) {
+        std::vector<ReadyJob*> batch;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (flight.empty())
+                S->cv_ready.wait(lk, [&] { return !S->ready.empty() || S->stop_launcher; });
+            if (S->ready.empty() && flight.empty()) return;   // stopping, nothing left
+            if (!S->ready.empty() && (int)flight.size() < kSlots) {
+                batch.assign(S->ready.begin(), S->ready.end());
+                S->ready.clear();
+            }
+        }
+        if (batch.empty()) {   // nothing new to launch: finish the oldest launch
+            release(flight.front().first);
+            flight.pop_front();
+            continue;
+        }
+        LaunchSlot& sl = slots[next_slot];
+        InFlight f;
+        f.jobs = std::move(batch);
+        char err[1024];
+        const size_t err_len = sizeof err;
+        err[0] = 0;
+        int rc = se != hipSuccess ? set_err(err, err_len, "HIP error: launcher stream") : 0;
+        const Tuning tu = tuning();
+        for (int v = 0; v < 2 && !rc; ++v) {
+            std::vector<const JobState*> jss;
+            std::vector<const hpdex_matrix*> ms;
+            for (ReadyJob* r : f.jobs)
+                if ((int)r->m->f64 == v) {
+                    jss.push_back(r->js);
+                    ms.push_back(r->m);
+                }
+            if (jss.empty()) continue;
+            MergedJobs mj;
+            merge_jobs(jss, ms, mj);
+            hipError_t e = stage(mj.jobs, sl.h_jobs[v], sl.d_jobs[v], st.s);
+            if (e == hipSuccess) e = stage(mj.count, sl.h_count[v], sl.d_count[v], st.s);
+            if (e == hipSuccess) e = stage(mj.gen, sl.h_gen[v], sl.d_gen[v], st.s);
+            BatchLaunch L{};
+            L.jobs = sl.d_jobs[v].as<JobDesc>();
+            L.count_items = sl.d_count[v].as<CountItem>();
+            L.n_count = (int)mj.count.size();
+            L.gen_items = sl.d_gen[v].as<WorkItem>();
+            L.n_gen = (int)mj.gen.size();
+            L.G = mj.G;
+            L.f64 = v == 1;
+            L.count_lds = mj.lds ? mj.lds : (uint32_t)tu.count_lds;
+            L.count_block = tu.count_block;
+            L.split_lds_max = g_count_ready ? (uint32_t)kSplitLdsMax : (uint32_t)kLdsBudget;
+            if (e == hipSuccess) e = launch_mwu_batch(L, st.s);
+            if (e != hipSuccess)
+                rc = set_err(err, err_len, std::string("HIP error: ") + hipGetErrorString(e));
+        }
+        f.rc = rc;
+        if (rc) f.err = err;
+        if (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(f.done, st.s) != hipSuccess) {
+            (void)hipStreamSynchronize(st.s);
+            if (!f.rc) {
+                f.rc = 1;
+                f.err = "HIP error: launcher event";
+            }
+            if (f.done) (void)hipEventDestroy(f.done);
+            (void)hipEventCreateWithFlags(&f.done, hipEventDisableTiming);   // passes at once
+        }
+        flight.emplace_back(std::move(f), next_slot);
+        next_slot = (next_slot + 1) % kSlots;
+        // a full pipeline: the oldest launch must end before its slot is reused
+        if ((int)flight.size() == kSlots) {
+            release(flight.front().first);
+            flight.pop_front();
+        }
+    }
 }
 
 void worker(hpdex_session* S) {
@@ -230,12 +434,13 @@ void worker(hpdex_session* S) {
             S->peak_in_flight = std::max(S->peak_in_flight, S->in_flight);
             S->peak_bytes = std::max(S->peak_bytes, S->arena.used());
             J->t_start = Clock::now();
+            J->t_uploaded = J->t_ready = J->t_ran = J->t_start;
         }
         if (se != hipSuccess) {
             J->rc = 1;
             J->err = std::string("HIP error: ") + hipGetErrorString(se) + " (stream creation)";
         } else {
-            run_job(*J, st.s, (char*)S->base + off, J->footprint);
+            run_job(S, *J, st.s, (char*)S->base + off, J->footprint);
         }
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -269,6 +474,7 @@ int hpdex_session_create(const hpdex_session_opts* o, hpdex_session** out, char*
     S->o = *o;
     HIPCHK(hipMalloc(&S->base, o->hbm_budget_bytes));
     S->arena.reset(o->hbm_budget_bytes);
+    S->launcher = std::thread(launcher, S.get());
     for (int i = 0; i < o->slots; ++i) S->workers.emplace_back(worker, S.get());
     *out = S.release();
     return 0;
@@ -351,6 +557,10 @@ int hpdex_session_wait(hpdex_session* S, int64_t ticket, hpdex_job_times* times,
         times->run_ms = ms_between(J->t_start, J->t_end);
         times->latency_ms = ms_between(J->t_submit, J->t_end);
         times->footprint_bytes = J->footprint;
+        times->upload_ms = ms_between(J->t_start, J->t_uploaded);
+        times->setup_ms = ms_between(J->t_uploaded, J->t_ready);
+        times->kernels_ms = ms_between(J->t_ready, J->t_ran);
+        times->finish_ms = ms_between(J->t_ran, J->t_end);
     }
     if (J->rc) return set_err(err, err_len, J->err);
     return 0;
@@ -364,6 +574,15 @@ int hpdex_session_peak(hpdex_session* S, int32_t* jobs_in_flight, size_t* arena_
     return 0;
 }
 
+int hpdex_session_batches(hpdex_session* S, int64_t* launches, int64_t* jobs, int64_t* max_jobs) {
+    if (!S) return 1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (launches) *launches = S->batches;
+    if (jobs) *jobs = S->batched_jobs;
+    if (max_jobs) *max_jobs = S->max_batch;
+    return 0;
+}
+
 void hpdex_session_destroy(hpdex_session* S) {
     if (!S) return;
     {
@@ -371,7 +590,13 @@ void hpdex_session_destroy(hpdex_session* S) {
         S->stop = true;
     }
     S->cv_admit.notify_all();
-    for (auto& t : S->workers) t.join();
+    for (auto& t : S->workers) t.join();   // their last jobs still need the launcher
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop_launcher = true;
+    }
+    S->cv_ready.notify_all();
+    S->launcher.join();
     if (S->base) (void)hipFree(S->base);
     delete S;
 }

@@ -289,8 +289,8 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);
  * own stream.  Submitted jobs wait in FIFO order and are admitted as soon as a
  * slot is free and the job's footprint (an upper bound of everything it
  * allocates: its uploaded matrix, if any, and its workspace) fits in the
- * arena; admitted jobs run concurrently and their kernels overlap on the
- * device.  A job either names a resident matrix (hpdex_matrix_create; a new
+ * arena; admitted jobs upload and set up concurrently, and the kernels of
+ * every job set up at the time run in one merged launch (hpdex_session_batches).  A job either names a resident matrix (hpdex_matrix_create; a new
  * target set against cells already in HBM) or carries a host CSC that is
  * uploaded into its arena block and dropped when it ends.  All pointers in a
  * job are borrowed until hpdex_session_wait returns for it.                  */
@@ -320,6 +320,12 @@ typedef struct hpdex_job_times {
     double run_ms;             /* admitted -> results in the host buffers */
     double latency_ms;         /* submit -> results in the host buffers */
     size_t footprint_bytes;    /* arena bytes the job held */
+    /* run_ms in phases: */
+    double upload_ms;          /* the matrix the job brings, to the device and classified (0
+                                  for a resident matrix) */
+    double setup_ms;           /* labels, routing, work lists and exact tables */
+    double kernels_ms;         /* set up -> the merged launch holding its kernels ended */
+    double finish_ms;          /* exact p lookup, errors, results to the host buffers */
 } hpdex_job_times;
 int hpdex_session_create(const hpdex_session_opts* o, hpdex_session** out, char* err,
                          size_t err_len);
@@ -333,6 +339,11 @@ int hpdex_session_wait(hpdex_session* s, int64_t ticket, hpdex_job_times* times,
                        size_t err_len);
 /* Largest number of jobs that ran at once, and most arena bytes in use, so far. */
 int hpdex_session_peak(hpdex_session* s, int32_t* jobs_in_flight, size_t* arena_bytes);
+/* Continuous batching so far: merged kernel launches, the jobs they ran, and
+ * the most jobs one launch held.  Every job's kernels run in the launch the
+ * session's launcher makes of all jobs set up (uploaded, labels and work
+ * lists on the device) at the time; jobs set up while it runs form the next. */
+int hpdex_session_batches(hpdex_session* s, int64_t* launches, int64_t* jobs, int64_t* max_jobs);
 /* Wait for every queued job, then free the arena and the workers. */
 void hpdex_session_destroy(hpdex_session* s);
 

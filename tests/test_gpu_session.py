@@ -93,6 +93,10 @@ def test_session_mix_matches_jobs_alone():
     peak = sess.peak()
     assert 1 <= peak["jobs_in_flight"] <= 4
     assert peak["arena_bytes"] <= 4 << 30
+    # every job's kernels ran in one of the launcher's merged launches
+    bt = sess.batches()
+    assert bt["jobs"] == len(tickets) and 1 <= bt["launches"] <= len(tickets)
+    assert 1 <= bt["max_jobs"] <= 4
     sess.close()
     for m in resident.values():
         m.close()
@@ -205,6 +209,9 @@ def test_serving_workload_distribution():
     say(f"serving workload: {len(jobs)} jobs submitted")
     results = [sess.wait(t)[:2] for t in tickets]
     assert sess.peak()["jobs_in_flight"] >= 2
+    bt = sess.batches()
+    assert bt["jobs"] == len(jobs) and bt["launches"] <= len(jobs)
+    say(f"serving workload: {bt}")
     sess.close()
     for m in resident.values():
         m.close()
