@@ -33,7 +33,7 @@ EXPORTED = (
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
     "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
-    "hpdex_session_batches",
+    "hpdex_session_batches", "hpdex_pden_matrix",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -50,6 +50,13 @@ class MwuOpts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "ref_sorted", "tar_sorted", "tie_correction", "use_continuity", "fast_norm",
         "zero_handling", "alternative", "method")]
+
+
+class PdenOpts(ctypes.Structure):
+    _fields_ = [("tie_correction", ctypes.c_int32), ("continuity_correction", ctypes.c_int32),
+                ("use_asymptotic", ctypes.c_int32), ("use_hist", ctypes.c_int32),
+                ("max_bins", ctypes.c_int64), ("chunk_genes", ctypes.c_int64),
+                ("clip_value", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -141,6 +148,9 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(sz)]
         L.hpdex_matrix_download.argtypes = [vp, vp, vp, vp, cp, sz]
         L.hpdex_de_matrix.argtypes = [vp, vp, sz, ci, ci, ctypes.c_double, vp, vp, vp, vp, vp, cp, sz]
+        L.hpdex_pden_matrix.argtypes = [vp, vp, sz, ctypes.POINTER(PdenOpts), vp, vp, vp, vp, vp,
+                                        cp, sz]
+        L.hpdex_pden_matrix.restype = ci
         L.hpdex_mwu_csc_noalloc.argtypes = [ci, vp, vp, vp, sz, sz, sz, vp, sz,
                                             ctypes.POINTER(MwuOpts), ci, vp, vp, cp, sz]
         L.hpdex_mwu_csc_noalloc.restype = ci
@@ -313,6 +323,30 @@ def de_matrix(m, group_id, n_targets, tie_correction=True, use_continuity=True, 
     if rc != 0:
         raise HpdexError(err.value.decode())
     return tuple(o[:n] for o in outs)
+
+
+def pden_matrix(m, group_id, n_targets, tie_correction=True, continuity_correction=True,
+                use_asymptotic=None, use_hist=False, max_bins=100_000, chunk_genes=None,
+                clip_value=20.0):
+    """hpdex_pden_matrix on a DeviceMatrix: (U_target, P, fc, log2fc, fdr),
+    each n_targets x n_cols target-major, flattened."""
+    group_id = np.ascontiguousarray(group_id, dtype=np.int32)
+    n = m.n_cols * int(n_targets)
+    outs = [np.empty(max(n, 1)) for _ in range(5)]
+    o = PdenOpts()
+    o.tie_correction = int(bool(tie_correction))
+    o.continuity_correction = int(bool(continuity_correction))
+    o.use_asymptotic = -1 if use_asymptotic is None else int(bool(use_asymptotic))
+    o.use_hist = int(bool(use_hist))
+    o.max_bins = int(max_bins)
+    o.chunk_genes = int(chunk_genes) if chunk_genes else max(m.n_cols, 1)
+    o.clip_value = float(clip_value)
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().hpdex_pden_matrix(m.handle, _p(group_id), int(n_targets), ctypes.byref(o),
+                                 *[_p(x) for x in outs], err, len(err))
+    if rc != 0:
+        raise HpdexError(err.value.decode())
+    return tuple(x[:n] for x in outs)
 
 
 class DeviceMatrix:

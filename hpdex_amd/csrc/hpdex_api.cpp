@@ -408,6 +408,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.asym = (js.opts.method == 2) ? 1 : 0;
     J.hist_max = tu.hist_max;
     J.split_arrival = tu.split_arrival;
+    J.min_n = 2u;
     J.cc = js.opts.use_continuity ? 0.5 : 0.0;
     return J;
 }
@@ -1174,6 +1175,157 @@ int hpdex_de_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
     HIPCHK(st.create());
     return de_run(*m, group_id, n_targets, tie_correction, use_continuity, clip_value, U1_out, P_out,
                   fc_out, log2fc_out, fdr_out, st.s, err, err_len);
+}
+
+// The numba pipeline (src/hpdex/backend/de_.py:508-658 with the kernels of
+// pykernel_.py) on a resident matrix: the Mann-Whitney pass under "mix"
+// (the pipeline densifies X, so zeros rank at their true position and n is
+// the group's row count) with the raw tie sums kept and no small-sample
+// check; group sums for the fold change; the per-(target, gene chunk) method
+// choice; the exact tables the chosen chunks need; target U and p, FDR.
+static int pden_run(hpdex_matrix& m, const int32_t* group_id, size_t T, const hpdex_pden_opts& po,
+                    double* U_out, double* P_out, double* fc_out, double* log2fc_out,
+                    double* fdr_out, hipStream_t s, char* err, size_t err_len) {
+    const size_t C = m.C, R = m.R, G = T + 1, np = C * T;
+    const Tuning tu = tuning();
+    hpdex_mwu_opts o{};
+    o.tie_correction = 1;   // the raw tie sums: the p step applies tie_correction itself
+    o.use_continuity = po.continuity_correction ? 1 : 0;
+    o.zero_handling = 3;
+    o.alternative = 2;
+    o.method = 1;           // U and ties only: the pass writes no p
+    JobState js;
+    if (job_setup(js, &m, group_id, T, &o, true, s, err, err_len)) return 1;
+    JobDesc J = make_desc(js, &m, tu);
+    J.asym = 0;
+    J.min_n = 0u;
+    if (job_upload_desc(js, J, m, tu, false, s, err, err_len)) return 1;
+    DevBuf sums, fc, lfc;
+    HIPCHK(sums.alloc(C * G * 8));
+    HIPCHK(launch_pden_group_sums(m.vals.p, m.f64, m.dtype == HPDEX_FLOAT32,
+                                  m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
+                                  js.gid.as<int32_t>(), (int64_t)C, (int32_t)R, (int32_t)G,
+                                  sums.as<double>(), s));
+    HIPCHK(fc.alloc(np * 8));
+    HIPCHK(lfc.alloc(np * 8));
+    HIPCHK(launch_pden_fold_change(sums.as<double>(), js.grows.as<uint32_t>(), (int64_t)C,
+                                   (int32_t)T, po.clip_value, m.dtype == HPDEX_FLOAT32,
+                                   fc.as<double>(), lfc.as<double>(), s));
+    HIPCHK(launch_mwu_pipeline(pipeline_of(js, m.f64), s));
+    if (job_errors(js, s, err, err_len)) return 1;
+    // the method of each (target, gene chunk)
+    const int64_t chunk = po.chunk_genes;
+    const int64_t nchunk = ((int64_t)C + chunk - 1) / chunk;
+    DevBuf kmin, kmax, tflag, dexact;
+    const bool hist = po.use_hist != 0;
+    HIPCHK(kmin.alloc(hist ? (size_t)nchunk * G * 8 : 16));
+    HIPCHK(kmax.alloc(hist ? (size_t)nchunk * G * 8 : 16));
+    HIPCHK(tflag.alloc((size_t)nchunk * T * 4));
+    HIPCHK(dexact.alloc((size_t)nchunk * T));
+    HIPCHK(launch_pden_decide(m.vals.p, m.f64, m.rows.as<int32_t>(), m.indptr.as<int64_t>(),
+                              js.gid.as<int32_t>(), js.grows.as<uint32_t>(), (int64_t)C, (int32_t)R,
+                              (int32_t)T, js.tie.as<double>(), chunk, po.use_asymptotic, hist,
+                              (double)po.max_bins, kmin.as<unsigned long long>(),
+                              kmax.as<unsigned long long>(), tflag.as<uint32_t>(),
+                              dexact.as<uint8_t>(), s));
+    std::vector<uint8_t> ex((size_t)nchunk * T);
+    HIPCHK(hipMemcpyAsync(ex.data(), dexact.p, ex.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // exact tables of the targets with an exact chunk, one per distinct size
+    std::vector<ExactTable> tabs;
+    std::vector<int32_t> tidx(T, -1);
+    std::map<int32_t, int32_t> by_n;
+    int64_t off = 0;
+    size_t max_cells = 0;
+    for (size_t t = 0; t < T; ++t) {
+        bool any = false;
+        for (int64_t k = 0; k < nchunk && !any; ++k) any = ex[t * nchunk + k] != 0;
+        if (!any) continue;
+        const int32_t nt = (int32_t)js.group_rows[t + 1], nr = (int32_t)js.group_rows[0];
+        auto it = by_n.find(nt);
+        if (it != by_n.end()) {
+            tidx[t] = it->second;
+            continue;
+        }
+        const int64_t U = (int64_t)nr * nt;
+        const size_t cells = ((size_t)std::min(nr, nt) + 1) * (size_t)(U + 1);
+        if (cells > kExactMaxCells)
+            return set_err(err, err_len,
+                           "exact null table too large for n_ref=" + std::to_string(nr) +
+                               ", n_tar=" + std::to_string(nt) + "; use use_asymptotic=True");
+        max_cells = std::max(max_cells, cells);
+        ExactTable tb;
+        tb.offset = off;
+        tb.n1 = nr;
+        tb.n2 = nt;
+        off += 1 + 2 * (U + 1);
+        tidx[t] = (int32_t)tabs.size();
+        by_n[nt] = tidx[t];
+        tabs.push_back(tb);
+    }
+    DevBuf d_tabs, tabbuf, tscratch, d_tidx;
+    HIPCHK(d_tidx.alloc(T * 4));
+    HIPCHK(hipMemcpyAsync(d_tidx.p, tidx.data(), T * 4, hipMemcpyHostToDevice, s));
+    if (!tabs.empty()) {
+        HIPCHK(d_tabs.alloc(tabs.size() * sizeof(ExactTable)));
+        HIPCHK(hipMemcpyAsync(d_tabs.p, tabs.data(), tabs.size() * sizeof(ExactTable),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(tabbuf.alloc((size_t)off * 8));
+        HIPCHK(tscratch.alloc(tabs.size() * 2 * max_cells * 8));
+        HIPCHK(launch_exact_tables(d_tabs.as<ExactTable>(), (int)tabs.size(), tabbuf.as<double>(),
+                                   tscratch.as<double>(), 2 * max_cells, s));
+    }
+    DevBuf U, P, fdr, ws;
+    HIPCHK(U.alloc(np * 8));
+    HIPCHK(P.alloc(np * 8));
+    PdenJob PJ{};
+    PJ.U1 = js.U1.as<double>();
+    PJ.tie = js.tie.as<double>();
+    PJ.group_rows = js.grows.as<uint32_t>();
+    PJ.exact = dexact.as<uint8_t>();
+    PJ.tidx = d_tidx.as<int32_t>();
+    PJ.tables = d_tabs.as<ExactTable>();
+    PJ.tabbuf = tabbuf.as<double>();
+    PJ.U_out = U.as<double>();
+    PJ.P_out = P.as<double>();
+    PJ.C = (int64_t)C;
+    PJ.chunk = chunk;
+    PJ.nchunk = nchunk;
+    PJ.T = (int32_t)T;
+    PJ.tie_corr = po.tie_correction ? 1 : 0;
+    PJ.cc = po.continuity_correction ? 0.5 : 0.0;
+    HIPCHK(launch_pden_finish(PJ, s));
+    // BH over every p-value (de_.py:655-656, _compute_fdr)
+    HIPCHK(fdr.alloc(np * 8));
+    const size_t wsb = fdr_workspace_bytes((int64_t)np);
+    HIPCHK(ws.alloc(wsb ? wsb : 16));
+    HIPCHK(launch_fdr_bh(P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, s));
+    const size_t b = np * 8;
+    if (U_out) HIPCHK(hipMemcpyAsync(U_out, U.p, b, hipMemcpyDeviceToHost, s));
+    if (P_out) HIPCHK(hipMemcpyAsync(P_out, P.p, b, hipMemcpyDeviceToHost, s));
+    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, s));
+    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, s));
+    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int hpdex_pden_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                      const hpdex_pden_opts* o, double* U_out, double* P_out, double* fc_out,
+                      double* log2fc_out, double* fdr_out, char* err, size_t err_len) {
+    if (!m || !o) return set_err(err, err_len, "null argument");
+    if (de_checks(n_targets, m->C, err, err_len)) return 1;
+    if (o->use_asymptotic < -1 || o->use_asymptotic > 1)
+        return set_err(err, err_len, "use_asymptotic must be -1 (None), 0 or 1");
+    if (o->chunk_genes < 1) return set_err(err, err_len, "chunk_genes must be at least 1");
+    if (m->C == 0) return 0;
+    if (n_targets * m->C > (size_t)UINT32_MAX)
+        return set_err(err, err_len, "more than 2^32-1 (gene, target) pairs");
+    if (ensure_init(err, err_len)) return 1;
+    Stream st;
+    HIPCHK(st.create());
+    return pden_run(*m, group_id, n_targets, *o, U_out, P_out, fc_out, log2fc_out, fdr_out, st.s,
+                    err, err_len);
 }
 
 int hpdex_matrix_create_from(const hpdex_matrix_src* src, hpdex_matrix** out, char* err,

@@ -107,6 +107,8 @@ struct JobDesc {
     int32_t hist_max;            // largest integer range binned (0 = off)
     int32_t split_arrival;       // 1: the split keeps a sparse column's entry groups in LDS
                                  // and re-reads only its values to scatter
+    uint32_t min_n;              // a group with fewer observations raises "Sample too small"
+                                 // (2; 0 for the numba pipeline, which has no such check)
     double cc;                   // continuity correction, 0.5 or 0
 };
 
@@ -249,6 +251,27 @@ struct ExactJob {
     int32_t alt;
 };
 
+// The numba pipeline's p-values and target U (src/hpdex/backend/pykernel_.py),
+// from the Mann-Whitney pass's reference U and raw tie sums (gene-major), into
+// target-major outputs.
+struct PdenJob {
+    const double* U1;            // C*T reference U, c*T + t
+    const double* tie;           // C*T sums of t^3 - t over tie runs (zero block included)
+    const uint32_t* group_rows;  // G
+    const uint8_t* exact;        // T*nchunk: 1 = the (target, gene chunk) takes the exact p
+    const int32_t* tidx;         // T: exact table of the target, -1 = none
+    const ExactTable* tables;
+    const double* tabbuf;
+    double* U_out;               // T*C target U, t*C + c
+    double* P_out;               // T*C
+    int64_t C;
+    int64_t chunk;               // genes per decision chunk
+    int64_t nchunk;
+    int32_t T;
+    int32_t tie_corr;
+    double cc;                   // 0.5 or 0
+};
+
 // ---- launchers (hpdex_kernels.hip) ----
 hipError_t launch_rows_i64_to_i32(const int64_t* in, int32_t* out, size_t n, int64_t R,
                                   hipStream_t s);
@@ -350,5 +373,29 @@ size_t fdr_workspace_bytes(int64_t n);
 size_t fdr_workspace_bound(int64_t n);
 hipError_t launch_fdr_bh(const double* p, int64_t n, double* fdr, void* ws, size_t ws_bytes,
                          hipStream_t s);
+
+// ---- the numba pipeline, pden (hpdex_de.hip) ----
+// sums[c*G + g] of every (column, group), added in row order in float32
+// (acc_f32: a float32 matrix) or float64, as numpy's axis-0 mean adds them
+hipError_t launch_pden_group_sums(const void* vals, bool f64, bool acc_f32, const int32_t* rows,
+                                  const int64_t* indptr, const int32_t* gid, int64_t C, int32_t R,
+                                  int32_t G, double* sums, hipStream_t s);
+// fc/log2fc[t*C + c] from those sums over group_rows[g] rows, in float32
+// arithmetic as numpy does for a float32 matrix (mean_f32) or in float64
+// (src/hpdex/backend/de_.py:189-247, 640-652)
+hipError_t launch_pden_fold_change(const double* sums, const uint32_t* group_rows, int64_t C,
+                                   int32_t T, double clip, bool mean_f32, double* fc,
+                                   double* log2fc, hipStream_t s);
+// The per-(target, gene chunk) choice between the exact and the asymptotic p
+// (pykernel_.py:258-262, 486-490; the histogram kernel's fallback of
+// pykernel_.py:467-481 when use_hist), into exact[T*nchunk].  kmin/kmax:
+// nchunk*G workspace (use_hist only); tflag: T*nchunk workspace.
+hipError_t launch_pden_decide(const void* vals, bool f64, const int32_t* rows,
+                              const int64_t* indptr, const int32_t* gid,
+                              const uint32_t* group_rows, int64_t C, int32_t R, int32_t T,
+                              const double* tie, int64_t chunk, int force, bool use_hist,
+                              double max_bins, unsigned long long* kmin, unsigned long long* kmax,
+                              uint32_t* tflag, uint8_t* exact, hipStream_t s);
+hipError_t launch_pden_finish(const PdenJob& P, hipStream_t s);
 
 }  // namespace hpdex

@@ -437,7 +437,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             const uint32_t nz = cnt_nz[g], ez = cnt_ez[g], gr = J.group_rows[g];
             if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
             const uint32_t n = (J.mode == kZeroNone) ? nz + ez : gr;
-            if (n < 2) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
+            if (n < J.min_n) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
             J.cnt[b + g] = nz;
             J.zcnt[b + g] = ez;
             if (!fixed) cur[g] = nz;
@@ -1629,7 +1629,7 @@ __device__ __forceinline__ void count_item(const JobDesc& J, const CountItem& it
         const uint32_t nz = cnz[x], ez = cez[x], gr = grs[x];
         if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
         const uint32_t ns = (mode == kZeroNone) ? nz + ez : gr;
-        if (ns < 2u) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
+        if (ns < J.min_n) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0u ? G : G + g)));
         if (x == 0u) continue;
         uint32_t zb, n1, n2;
         if (mode == kZeroNone) {

@@ -185,6 +185,32 @@ int hpdex_de_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
                     int tie_correction, int use_continuity, double clip_value, double* U1_out,
                     double* P_out, double* fc_out, double* log2fc_out, double* fdr_out, char* err,
                     size_t err_len);
+/* The numba pipeline, hpdex.pden (src/hpdex/backend/de_.py:508-658 with
+ * rank_sum_chunk_kernel_float / _hist of src/hpdex/backend/pykernel_.py:220-
+ * 509), on a resident matrix built from the AnnData X: zeros rank at their
+ * true position (the pipeline densifies X, de_.py:177-180), n is the group's
+ * row count, and no group is too small for the test.  Every output is
+ * target-major (t*n_cols + c, the row order of de_.py's DataFrame): U_out is
+ * the TARGET group's U (pykernel_.py:74-76, 394-396), P_out its two-sided p
+ * by pykernel_.py:84-147 or the exact 2 sf[rint(max U)] (:281-287), chosen
+ * per (target, chunk of chunk_genes genes) as each task chooses it
+ * (:267-271, 489-493; de_.py:113-124 and 291-333 make the tasks);
+ * fc/log2fc by de_.py:189-247 on numpy's means (:640-652); fdr the BH of all
+ * p-values (de_.py:250-288, 656).                                            */
+typedef struct hpdex_pden_opts {
+    int32_t tie_correction;
+    int32_t continuity_correction;
+    int32_t use_asymptotic;    /* -1: None (auto per chunk), 0: exact, 1: asymptotic */
+    int32_t use_hist;          /* the histogram kernel runs (prefer_hist_if_int, an integer X,
+                                  num_workers > 1): a chunk whose value range vmax - vmin + 1
+                                  passes max_bins takes the asymptotic p (pykernel_.py:467-481) */
+    int64_t max_bins;
+    int64_t chunk_genes;       /* genes per task (de_.py:291-340), >= 1 */
+    double clip_value;
+} hpdex_pden_opts;
+int hpdex_pden_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets,
+                      const hpdex_pden_opts* o, double* U_out, double* P_out, double* fc_out,
+                      double* log2fc_out, double* fdr_out, char* err, size_t err_len);
 void hpdex_matrix_destroy(hpdex_matrix* m);
 /* Bytes of HBM the resident matrix occupies. */
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);

@@ -218,7 +218,7 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.U1 = U1; J.P = P; J.tie = tie; J.n1o = n1; J.n2o = n2;
     J.err_key = err_key; J.status = status; J.status_col = &status_col;
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
-    J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0;
+    J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0; J.min_n = 2u;
     J.split_arrival = sim_split_arrival;
     // spill space of the columns whose reference slice may pass the on-chip
     // sort (job_upload_desc), for every column: any may take the sort path
