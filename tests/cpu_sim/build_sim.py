@@ -41,23 +41,37 @@ def _device_text(path, cut_marker=None):
 
 
 def build(sanitize=True, compiler="g++", opt="-O1", lib=None):
+    """Build under an exclusive lock (parallel test workers may ask at once);
+    the library is written beside its target and renamed over it."""
+    import fcntl
     os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, ".build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        return _build(sanitize, compiler, opt, lib)
+
+
+def _build(sanitize, compiler, opt, lib):
+    target = lib or LIB
+    tag = f".{os.getpid()}"
     dev = _device_text(os.path.join(CSRC, "hpdex_device.h"))
     ker = _device_text(os.path.join(CSRC, "hpdex_kernels.hip"),
                        "// ---------------------------------------------------------------------------\n// launchers")
     ker = ker.replace('#include "hpdex_device.h"', dev) + "\n}  // namespace hpdex\n"
-    with open(os.path.join(OUT, "hpdex_kernels_device.inc"), "w") as f:
+    inc = os.path.join(OUT, "hpdex_kernels_device.inc")
+    with open(inc + tag, "w") as f:
         f.write(ker)
+    os.replace(inc + tag, inc)
     flags = [opt, "-g", "-std=c++20", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
              "-Wno-unknown-pragmas", "-Wno-attributes"]
     if sanitize:
         flags += ["-fsanitize=address,undefined", "-fno-omit-frame-pointer"]
     cmd = [compiler, *flags, "-I", HERE, "-I", OUT, "-I", CSRC,
-           os.path.join(HERE, "sim_main.cpp"), "-o", lib or LIB]
+           os.path.join(HERE, "sim_main.cpp"), "-o", target + tag]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-6000:])
-    return lib or LIB
+    os.replace(target + tag, target)
+    return target
 
 
 if __name__ == "__main__":

@@ -20,9 +20,16 @@ def lib(path=None):
         srcs = [os.path.join(build_sim.CSRC, f) for f in ("hpdex_kernels.hip", "hpdex_device.h",
                                                           "hpdex_internal.h")]
         srcs += [os.path.join(build_sim.HERE, f) for f in ("sim_main.cpp", "sim_runtime.h")]
-        if not os.path.exists(path) or any(os.path.getmtime(x) > os.path.getmtime(path)
-                                           for x in srcs):
-            build_sim.build(sanitize=False, opt="-O2", lib=path)
+        def stale():
+            return not os.path.exists(path) or any(os.path.getmtime(x) > os.path.getmtime(path)
+                                                   for x in srcs)
+        if stale():
+            import fcntl
+            os.makedirs(build_sim.OUT, exist_ok=True)
+            with open(os.path.join(build_sim.OUT, ".lib.lock"), "w") as lk:
+                fcntl.flock(lk, fcntl.LOCK_EX)
+                if stale():   # another worker may have built it meanwhile
+                    build_sim.build(sanitize=False, opt="-O2", lib=path)
         L = ctypes.CDLL(path)
         vp, ci = ctypes.c_void_p, ctypes.c_int
         L.sim_mwu.argtypes = [ci, vp, vp, vp, ci, ci, vp, ci, ci, ci, ci, ci, ci, ci, ci,
