@@ -38,6 +38,15 @@ __host__ __device__ constexpr uint32_t ref_table_lds(bool f64) {
     return ((hist > sort ? hist : sort) + 15u) & ~15u;
 }
 
+// Target kernel: per wave, a table of kTieSlots (key, target, count) words
+// that finds a key's position among the equal keys of its target.
+constexpr uint32_t kTieSlots = 128;
+constexpr uint64_t kTieEmpty = ~0ull;
+constexpr uint32_t kTieCountBits = 26;
+constexpr uint64_t kTieCountMask = (1ull << kTieCountBits) - 1ull;
+__host__ __device__ constexpr uint32_t target_lds(bool f64) {
+    return ref_table_lds(f64) + (uint32_t)(kColBlock / 64) * kTieSlots * 8u;
+}
 
 // Count kernel (integer-valued columns in one pass).
 constexpr int kCountBlockDefault = 256;     // threads per workgroup: 256, 512 or 1024
@@ -60,6 +69,8 @@ struct ColPlan {
     uint32_t hist;         // 1 = histogram path, 0 = sort path
     uint32_t nb;           // histogram bins, or reference buckets (sort path)
     uint32_t shift;        // sort path: bucket = (key - kmin) >> shift
+    uint32_t maxb;         // sort path, on-chip reference: keys in its fullest bucket
+    uint32_t pad;
 };
 
 // One Mann-Whitney job: a device-resident CSC matrix, its group labels and

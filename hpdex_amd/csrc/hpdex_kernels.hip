@@ -59,6 +59,11 @@ namespace hpdex {
 #define HPDEX_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ __forceinline__ HPDEX_GLOBAL T* gl(T* p) { return (HPDEX_GLOBAL T*)p; }
+// The same for LDS: a pointer known to be in LDS gives ds_read/ds_write
+// instead of flat accesses that wait on both counters.
+#define HPDEX_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ HPDEX_LDS T* lds(T* p) { return (HPDEX_LDS T*)p; }
 
 // Buffer loads (gfx9 family): word 3 of a raw buffer resource, and the
 // cache-policy bits of the loads (default policy: the non-temporal hint made
@@ -231,8 +236,8 @@ __device__ inline int wave_max_i(int v) {
 }
 
 // first index in a[lo, hi) with a[i] >= v
-template <typename K>
-__device__ inline uint32_t lower_bound_k(const K* a, uint32_t lo, uint32_t hi, K v) {
+template <typename K, typename P>
+__device__ inline uint32_t lower_bound_k(P a, uint32_t lo, uint32_t hi, K v) {
     uint32_t n = hi - lo;
     while (n > 0) {
         const uint32_t h = n >> 1;
@@ -241,8 +246,8 @@ __device__ inline uint32_t lower_bound_k(const K* a, uint32_t lo, uint32_t hi, K
     return lo;
 }
 // first index in a[lo, hi) with a[i] > v
-template <typename K>
-__device__ inline uint32_t upper_bound_k(const K* a, uint32_t lo, uint32_t hi, K v) {
+template <typename K, typename P>
+__device__ inline uint32_t upper_bound_k(P a, uint32_t lo, uint32_t hi, K v) {
     uint32_t n = hi - lo;
     while (n > 0) {
         const uint32_t h = n >> 1;
@@ -275,12 +280,12 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t x) {
 __device__ inline void pair_sizes(const JobDesc& J, int64_t c, uint32_t g, uint32_t& za,
                                   uint32_t& zb, uint32_t& n1, uint32_t& n2) {
     const int64_t b = c * J.G;
-    const uint32_t nza = J.cnt[b], nzb = J.cnt[b + g];
+    const uint32_t nza = gl(J.cnt)[b], nzb = gl(J.cnt)[b + g];
     if (J.mode == kZeroNone) {
-        za = J.zcnt[b]; zb = J.zcnt[b + g];
+        za = gl(J.zcnt)[b]; zb = gl(J.zcnt)[b + g];
         n1 = nza + za; n2 = nzb + zb;
     } else {
-        const uint32_t gr0 = J.group_rows[0], grg = J.group_rows[g];
+        const uint32_t gr0 = gl(J.group_rows)[0], grg = gl(J.group_rows)[g];
         za = gr0 > nza ? gr0 - nza : 0u;
         zb = grg > nzb ? grg - nzb : 0u;
         n1 = gr0; n2 = grg;
@@ -455,6 +460,8 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         pl.kmin = st.kmin;
         pl.kmax = st.kmax;
         pl.TA = 0;
+        pl.maxb = 0;
+        pl.pad = 0;
         const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
         if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
             pl.hist = 1;
@@ -549,30 +556,77 @@ __global__ __launch_bounds__(kSplitBlock) void split_kernel(const JobDesc* jobs,
     if (blockIdx.x < n_items) split_item<V, kSplitBlock>(jobs, items, blockIdx.x, lds_bytes);
 }
 
+// Search of a sorted run a[base, base + n) in LDS with a fixed number of
+// halvings (iters >= ceil(log2(n)), the same in every lane, so no lane waits
+// on another's loop): the first index whose key is not below v (kUpper: the
+// first above v).  Reads stay in a[0 .. last]; a halving of a run of one
+// key or none reads a clamped index and leaves base alone.
+template <bool kUpper, typename K>
+__device__ __forceinline__ uint32_t bound_fixed(const HPDEX_LDS K* a, uint32_t base, uint32_t n,
+                                                K v, uint32_t iters, uint32_t last) {
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t half = n >> 1;
+        const K x = a[min(base + half - 1u, last)];
+        const bool right = kUpper ? !(v < x) : (x < v);
+        base += right ? half : 0u;
+        n -= half;
+    }
+    const K x = a[min(base, last)];
+    const bool right = kUpper ? !(v < x) : (x < v);
+    return base + ((n > 0u && right) ? 1u : 0u);
+}
+
 // Rank table of the reference slice seen by the target phase.
 template <typename K>
 struct RefView {
-    const uint32_t* cum;   // histogram path: exclusive prefix of bin counts
-    const K* keys;         // sort path: sorted reference keys
-    const uint32_t* bidx;  // sort path: bucket -> first key index
+    const uint32_t* cum;     // histogram path: exclusive prefix of bin counts (LDS)
+    const K* keys;           // sort path, reference beyond the on-chip sort: sorted keys in HBM
+    const HPDEX_LDS K* lkeys;  // sort path: sorted keys in LDS
+    const uint32_t* bidx;    // sort path: bucket -> first key index (LDS), nb + 1 entries
     uint64_t kmin, kmax;
-    uint32_t shift, na, hist;
+    uint32_t shift, na, hist, big, nb;
+    uint32_t iters;          // halvings that cover the fullest bucket
 
+    // lt = reference keys below v, eq = reference keys equal to v.  kWave:
+    // every lane of the wave calls it together (the upper-bound search is
+    // then skipped when no lane's key is present in the reference).
+    template <bool kWave = false>
     __device__ inline void lookup(K v, uint32_t& lt, uint32_t& eq) const {
         if (hist) {
             const uint32_t a = cum[(uint32_t)v], b = cum[(uint32_t)v + 1];
             lt = a; eq = b - a;
             return;
         }
-        if (na == 0 || (uint64_t)v < kmin) { lt = 0; eq = 0; return; }
-        if ((uint64_t)v > kmax) { lt = na; eq = 0; return; }
-        const uint32_t bkt = (uint32_t)(((uint64_t)v - kmin) >> shift);
-        const uint32_t lo = bidx[bkt], hi = bidx[bkt + 1];
-        const uint32_t l = lower_bound_k<K>(keys, lo, hi, v);
-        const uint32_t u = upper_bound_k<K>(keys, l, hi, v);
+        if (big) {
+            if (na == 0 || (uint64_t)v < kmin) { lt = 0; eq = 0; return; }
+            if ((uint64_t)v > kmax) { lt = na; eq = 0; return; }
+            const uint32_t bkt = (uint32_t)(((uint64_t)v - kmin) >> shift);
+            const uint32_t lo = bidx[bkt], hi = bidx[bkt + 1];
+            const uint32_t l = lower_bound_k<K>(gl(keys), lo, hi, v);
+            const uint32_t u = upper_bound_k<K>(gl(keys), l, hi, v);
+            lt = l; eq = u - l;
+            return;
+        }
+        // below the smallest key: bucket 0, whose keys are all above v;
+        // above the largest: the empty range at bidx[nb] = na
+        const bool below = v < (K)kmin, above = v > (K)kmax;
+        const uint32_t bkt = below ? 0u : above ? nb : (uint32_t)((K)(v - (K)kmin) >> shift);
+        const uint32_t lo = bidx[bkt], hi = bidx[above ? bkt : bkt + 1u];
+        const uint32_t last = na ? na - 1u : 0u;
+        const uint32_t l = bound_fixed<false, K>(lkeys, lo, hi - lo, v, iters, last);
+        // the upper bound only where the key is present (rare for float data)
+        const bool hit = l < hi && lkeys[min(l, last)] == v;
+        uint32_t u = l;
+        if (!kWave || __ballot(hit)) {
+            const uint32_t uu = bound_fixed<true, K>(lkeys, l, hi - l, v, iters, last);
+            u = hit ? uu : l;
+        }
         lt = l; eq = u - l;
     }
 };
+
+// Halvings that take a run of up to maxb keys to one.
+__device__ inline uint32_t halvings(uint32_t maxb) { return maxb <= 1u ? 0u : 32u - (uint32_t)__clz(maxb - 1u); }
 
 // Sort reference keys rk[0..P) (P a power of two >= 64, padded with the
 // sentinel) in LDS: registers for runs of up to kChunk, LDS merge stages above.
@@ -693,8 +747,23 @@ __device__ void sort_ref_global(K* a, K* b, uint32_t n, K* rk, uint32_t cap, K s
 // ---------------------------------------------------------------------------
 struct RefState {
     uint32_t scan[20];
+    uint32_t maxb;
     uint64_t red[16];
 };
+
+// Keys in the fullest bucket of a bucket index bidx[0 .. nb] (LDS), for
+// every thread of the workgroup.
+__device__ inline uint32_t fullest_bucket(const uint32_t* bidx, uint32_t nb, RefState& st) {
+    const uint32_t tid = (uint32_t)tid_x(), nt = blockDim.x;
+    if (tid == 0) st.maxb = 0u;
+    __syncthreads();
+    uint32_t m = 0;
+    for (uint32_t b = tid; b < nb; b += nt) m = max(m, bidx[b + 1] - bidx[b]);
+    m = wave_max_u32(m);
+    if (lane_id() == 0) atomicMax(&st.maxb, m);
+    __syncthreads();
+    return st.maxb;
+}
 
 template <typename V>
 __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
@@ -715,6 +784,7 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
     const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
     uint64_t ta = 0;
+    uint32_t maxb = 0;   // keys in the fullest bucket of an on-chip reference
     // a reference slice beyond the on-chip sort: sorted in HBM with the
     // column's spill space, its bucket index built from the sorted keys there
     const int64_t spill_at =
@@ -733,8 +803,9 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
         RefView<K> rv;
-        rv.cum = nullptr; rv.keys = base; rv.bidx = bidx; rv.kmin = pl.kmin; rv.kmax = pl.kmax;
-        rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.cum = nullptr; rv.keys = base; rv.lkeys = nullptr; rv.bidx = bidx;
+        rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.big = 1; rv.nb = pl.nb; rv.iters = 0;
         for (uint32_t x = tid; x < na; x += nt) {
             if (x > 0 && base[x - 1] == base[x]) continue;   // not the first of its run
             uint32_t lt, eq;
@@ -768,9 +839,11 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         __syncthreads();
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
+        maxb = fullest_bucket(bidx, pl.nb, st);
         RefView<K> rv;
-        rv.cum = nullptr; rv.keys = rk; rv.bidx = bidx; rv.kmin = pl.kmin; rv.kmax = pl.kmax;
-        rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
+        rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.big = 0; rv.nb = pl.nb; rv.iters = halvings(maxb);
         for (uint32_t x = tid; x < na; x += nt) {
             uint32_t lt, eq;
             rv.lookup(rk[x], lt, eq);
@@ -778,7 +851,10 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         }
         ta = block_sum_u64(ta, st.red);
     }
-    if (tid == 0 && (hist || na <= cap || spill_at >= 0)) J.plan[c].TA = ta;
+    if (tid == 0 && (hist || na <= cap || spill_at >= 0)) {
+        J.plan[c].TA = ta;
+        J.plan[c].maxb = maxb;
+    }
 }
 
 // workgroup b takes queue item b (the grid is sized for the queue's capacity)
@@ -826,14 +902,14 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
 #pragma unroll
     for (int i = 0; i < KK; ++i) {
         const uint32_t e = (uint32_t)(i * kWave + lane);
-        v[i] = e < m ? base[e] : KeyOf<V>::kSentinel;
+        v[i] = e < m ? gl(base)[e] : KeyOf<V>::kSentinel;
     }
     wave_sort<KK, K>(v);
     if (store) {
 #pragma unroll
         for (int i = 0; i < KK; ++i) {
             const uint32_t e = (uint32_t)(i * kWave + lane);
-            if (e < m) base[e] = v[i];
+            if (e < m) gl(base)[e] = v[i];
         }
         __threadfence();
     }
@@ -858,8 +934,8 @@ __device__ __forceinline__ void target_chunk(TargetCtx<typename KeyOf<V>::K>& t,
             uint32_t prior = 0;
             for (int qq = 0; qq < q; ++qq) {
                 const K* pc = seg + (size_t)qq * kChunk;
-                const uint32_t l = lower_bound_k<K>(pc, 0, kChunk, v[i]);
-                const uint32_t u = upper_bound_k<K>(pc, l, kChunk, v[i]);
+                const uint32_t l = lower_bound_k<K>(gl(pc), 0, kChunk, v[i]);
+                const uint32_t u = upper_bound_k<K>(gl(pc), l, kChunk, v[i]);
                 prior += u - l;
             }
             const uint64_t jj = (uint64_t)(e - h + 1) + prior;
@@ -884,13 +960,13 @@ __device__ __forceinline__ TargetResult process_target(const JobDesc& J, int64_t
                                                        const RefView<typename KeyOf<V>::K>& ref) {
     using K = typename KeyOf<V>::K;
     const int64_t b = c * J.G;
-    const uint32_t nzb = J.cnt[b + g];
-    K* seg = (K*)J.keys + J.indptr[c] + J.goff[b + g];
+    const uint32_t nzb = gl(J.cnt)[b + g];
+    K* seg = (K*)J.keys + gl(J.indptr)[c] + gl(J.goff)[b + g];
     uint32_t za, zb, n1, n2;
     pair_sizes(J, c, g, za, zb, n1, n2);
     TargetCtx<K> t;
     t.ref = ref;
-    t.mode = J.mode;
+    t.mode = (int32_t)uniform_u32((uint32_t)J.mode);
     t.vmin = pl.vmin;
     t.za = za;
     t.S = 0;
@@ -951,22 +1027,26 @@ template <typename V>
 __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t c, uint32_t g0,
                                                        uint32_t ng, const ColPlan& pl,
                                                        const RefView<typename KeyOf<V>::K>& ref,
-                                                       const typename KeyOf<V>::K* colkeys) {
+                                                       const typename KeyOf<V>::K* colkeys,
+                                                       unsigned long long* htab) {
     using K = typename KeyOf<V>::K;
     static_assert(sizeof(K) == 4, "target windows compare 32-bit keys across lanes");
     const uint32_t lane = (uint32_t)lane_id();
     const int64_t b = c * J.G;
     const uint32_t gj = g0 + lane;
     const bool own = lane < ng;
-    const uint32_t sz = own ? J.cnt[b + gj] : 0u;
-    const uint32_t off = own ? J.goff[b + gj] : 0u;
+    const uint32_t nzj = own ? gl(J.cnt)[b + gj] : 0u;
+    const uint32_t off = own ? gl(J.goff)[b + gj] : 0u;
+    // a target of more than 64 keys is ranked alone (process_target) after
+    // the windows, which take it as empty
+    const uint32_t sz = nzj <= (uint32_t)kWave ? nzj : 0u;
     uint32_t zaj = 0, zb = 0, n1 = 0, n2 = 0;
     if (own) pair_sizes(J, c, gj, zaj, zb, n1, n2);
     // the reference's zero block, the same for every target
-    const uint32_t nza = uniform_u32(J.cnt[b]);
-    const uint32_t za = uniform_u32((J.mode == kZeroNone)
-                                        ? J.zcnt[b]
-                                        : (J.group_rows[0] > nza ? J.group_rows[0] - nza : 0u));
+    const uint32_t nza = uniform_u32(gl(J.cnt)[b]);
+    const uint32_t gr0 = uniform_u32(gl(J.group_rows)[0]);
+    const uint32_t za = uniform_u32((J.mode == kZeroNone) ? gl(J.zcnt)[b]
+                                                          : (gr0 > nza ? gr0 - nza : 0u));
     const uint32_t inc = wave_incl_scan_dpp(sz, [](uint32_t a, uint32_t x) { return a + x; });
     // a window's per-key terms sum below 2^32 when 64 keys of 2*lt' + eq
     // (at most 2*(nza + za) + nza) and of 3x(x - 1) (x = eq + position, at
@@ -976,25 +1056,31 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
                         192ull * xm * xm < (1ull << 32);
     TargetCtx<K> t;
     t.ref = ref;
-    t.mode = J.mode;
+    t.mode = (int32_t)uniform_u32((uint32_t)J.mode);
     t.vmin = pl.vmin;
     t.za = za;
     uint64_t S = 0, TT = 0;
     bool big = false;
     TargetResult mine{};
-    uint32_t j0 = 0;
-    while (j0 < ng) {
+    // The window that starts at target j0: its end j1 (j1 == j0: target j0
+    // alone has more than 64 keys), its m keys, and each lane's owner (the
+    // target of its key), position of the owner's first key, and key.  The
+    // loop plans window w + 1, issuing its key loads, before it ranks window
+    // w: one window's loads are in flight while the one before is ranked.
+    struct Win {
+        uint32_t j0, j1, m, first, owner;
+        K key;
+    };
+    auto plan = [&](uint32_t j0) -> Win {
+        Win w;
+        w.j0 = j0; w.j1 = j0; w.m = 0; w.first = 0; w.owner = 0; w.key = K(0);
+        if (j0 >= ng) return w;
         const uint32_t base = j0 ? readlane_u32(inc, j0 - 1u) : 0u;
+        // at least target j0: no target takes more than 64 window keys
         const uint64_t fit = __ballot(lane >= j0 && own && inc - base <= (uint32_t)kWave);
         const uint32_t j1 = j0 + (uint32_t)__builtin_popcountll(fit);
-        if (j1 == j0) {   // one target of more than 64 keys
-            const TargetResult r = process_target<V>(J, c, g0 + j0, pl, ref);
-            if (lane == j0) { mine = r; big = true; }
-            ++j0;
-            continue;
-        }
-        const uint32_t m = readlane_u32(inc, j1 - 1u) - base;   // keys in the window
-        const bool valid = lane < m;
+        w.j1 = j1;
+        w.m = readlane_u32(inc, j1 - 1u) - base;   // keys in the window
         // target j's keys take positions [excl_j, excl_j + sz_j) of the window
         const uint32_t excl = inc - sz - base;
         const bool starts = lane >= j0 && lane < j1 && sz > 0u;
@@ -1008,33 +1094,60 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         const uint64_t tmask = __ballot(starts);
         const uint64_t upto = (lane == (uint32_t)kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
         const uint64_t sm = smask & upto;
-        const uint32_t first = sm ? (uint32_t)(63 - __clzll(sm)) : 0u;   // its first key
+        w.first = sm ? (uint32_t)(63 - __clzll(sm)) : 0u;   // its first key
         const uint32_t k = (uint32_t)__builtin_popcountll(sm);
         // with no empty target in the window the k-th starting target is j0 + k - 1
         const uint64_t wmask = (j1 == 64u ? ~0ull : ((1ull << j1) - 1ull)) & ~((1ull << j0) - 1ull);
-        const uint32_t owner = tmask == wmask ? j0 + k - 1u : nth_set_bit(tmask, k);
+        w.owner = tmask == wmask ? j0 + k - 1u : nth_set_bit(tmask, k);
         // the lane's key: position lane - first of the owner's keys (a
         // group's keys need not end where the next group's start)
-        const uint32_t at = shfl_u32(off, (int)owner) + lane - first;
-        const K key = valid ? colkeys[at] : K(0);
-        // the key's position among the equal keys of its target at lower
-        // lanes, plus one (any order of equal keys gives the same sums): the
-        // keys of lanes 1 .. r back, r = its rank in its segment, compared
-        // through a chain of one-lane shifts as long as the longest segment
-        const uint32_t r = valid ? lane - first : 0u;
-        const uint32_t rmax = readlane_u32(
-            wave_incl_scan_dpp(r, [](uint32_t a, uint32_t x) { return a > x ? a : x; }),
-            (uint32_t)kWave - 1u);
-        uint32_t jj = 1u;
-        uint32_t kd = (uint32_t)key;
-        for (uint32_t d = 1; d <= rmax; ++d) {
-            kd = wave_shr1_u32(kd);
-            jj += (d <= r && kd == (uint32_t)key) ? 1u : 0u;
+        const uint32_t at = shfl_u32(off, (int)w.owner) + lane - w.first;
+        w.key = lane < w.m ? gl(colkeys)[at] : K(0);
+        return w;
+    };
+    Win w = plan(0u);
+    {
+        // the first window's keys in registers before the loop, so that the
+        // loop waits for no key but the next window's, at its end
+        K k0 = w.key;
+        asm volatile("" : "+v"(k0));
+        w.key = k0;
+    }
+    while (w.j0 < ng) {
+        const Win nx = plan(w.j1);
+        const uint32_t j0 = w.j0, j1 = w.j1;
+        const uint32_t base = j0 ? readlane_u32(inc, j0 - 1u) : 0u;
+        const bool valid = lane < w.m;
+        const uint32_t excl = inc - sz - base;
+        const bool starts = lane >= j0 && lane < j1 && sz > 0u;
+        const uint32_t owner = w.owner;
+        const K key = w.key;
+        // every lane ranks (a lane past the window ranks key 0 and drops it)
+        uint32_t lt, eq;
+        t.ref.template lookup<true>(key, lt, eq);
+        // the key's position among the equal keys of its target, plus one
+        // (any order of equal keys gives the same sums): the (key, owner)
+        // pair is placed in the wave's hash table (linear probing, compare
+        // and swap), then a counter beside the pair is incremented; the
+        // count before the increment is the position.  At most 64 pairs in
+        // kTieSlots > 64 slots: every lane finds its pair's slot or an empty
+        // one within kTieSlots probes.
+        const uint64_t tag = ((uint64_t)(uint32_t)key << 32) | ((uint64_t)owner << kTieCountBits);
+        uint32_t slot = (((uint32_t)key * 0x9E3779B1u) ^ (owner * 0x85EBCA77u)) >> 25;
+        bool placed = !valid;
+        for (uint32_t probe = 0; probe < kTieSlots; ++probe) {
+            if (__ballot(!placed) == 0ull) break;
+            if (!placed) {
+                const uint64_t old = atomicCAS(&htab[slot], (unsigned long long)kTieEmpty,
+                                               (unsigned long long)tag);
+                if (old == kTieEmpty || (old & ~kTieCountMask) == tag) placed = true;
+                else slot = (slot + 1u) & (kTieSlots - 1u);
+            }
         }
+        uint32_t jj = 1u;
+        if (valid) jj += (uint32_t)(atomicAdd(&htab[slot], 1ull) & kTieCountMask);
         uint64_t s_e = 0, t_e = 0;
         if (valid) {
-            uint32_t lt, eq;
-            t.ref.lookup(key, lt, eq);
             const uint64_t ltz = (uint64_t)lt + (zero_below<V>(t, key) ? za : 0u);
             s_e = 2ull * ltz + eq;
             const uint64_t x = (uint64_t)eq + jj;
@@ -1073,7 +1186,16 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         };
         if (narrow) sums(TypeTag<uint32_t>{});
         else sums(TypeTag<uint64_t>{});
-        j0 = j1;
+        // the table empty again for the next window (the wave's LDS
+        // operations complete in order: after every lane's increment)
+        if (valid) htab[slot] = kTieEmpty;
+        w = nx;
+    }
+    // the targets of more than 64 keys, one at a time
+    for (uint64_t bigm = __ballot(own && nzj > (uint32_t)kWave); bigm; bigm &= bigm - 1ull) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(bigm);
+        const TargetResult r = process_target<V>(J, c, g0 + j, pl, ref);
+        if (lane == j) { mine = r; big = true; }
     }
     if (own && !big) {
         uint32_t ltz;
@@ -1117,38 +1239,44 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     // through L2 by the lookups), its bucket index comes to LDS; with no spill
     // space the reference kernel raised the status
     const bool big = !hist && na > cap;
-    if (big && uniform_i64(J.spill_off ? J.spill_off[c] : -1) < 0) return;
+    if (big && uniform_i64(J.spill_off ? gl(J.spill_off)[c] : -1) < 0) return;
     RefView<K> rv;
     rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = hist;
+    rv.big = big ? 1u : 0u; rv.nb = pl.nb; rv.iters = 0; rv.lkeys = nullptr;
     if (hist) {
         uint32_t* cum = (uint32_t*)smem;
-        for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = tab[b];
+        for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = gl(tab)[b];
         rv.cum = cum; rv.keys = nullptr; rv.bidx = nullptr;
     } else if (big) {
         uint32_t* bidx = (uint32_t*)((K*)smem + cap);
-        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = tab[b];
-        rv.cum = nullptr; rv.keys = (const K*)J.keys + J.indptr[c]; rv.bidx = bidx;
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = gl(tab)[b];
+        rv.cum = nullptr; rv.keys = (const K*)J.keys + gl(J.indptr)[c]; rv.bidx = bidx;
     } else {
         K* rk = (K*)smem;
         uint32_t* bidx = (uint32_t*)(rk + cap);
-        const K* base = (const K*)J.keys + J.indptr[c];
-        for (uint32_t x = tid; x < na; x += nt) rk[x] = base[x];
-        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = tab[b];
-        rv.cum = nullptr; rv.keys = rk; rv.bidx = bidx;
+        const K* base = (const K*)J.keys + gl(J.indptr)[c];
+        for (uint32_t x = tid; x < na; x += nt) rk[x] = gl(base)[x];
+        for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = gl(tab)[b];
+        rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
+        rv.iters = halvings(uniform_u32(pl.maxb));
     }
+    // each wave's tie-position table (target_windows), all slots empty
+    unsigned long long* htab = (unsigned long long*)(smem + ref_table_lds(sizeof(K) == 8)) +
+                               (size_t)(tid / kWave) * kTieSlots;
+    for (uint32_t i = (uint32_t)lane; i < kTieSlots; i += (uint32_t)kWave) htab[i] = kTieEmpty;
     if (tid == 0) st.next_t = 1u;
     __syncthreads();
     if constexpr (sizeof(K) == 4) {
         // a wave claims 64 consecutive targets, one per lane, and ranks the
         // small ones several at a time (target_windows)
-        const K* colkeys = (const K*)J.keys + J.indptr[c];
+        const K* colkeys = (const K*)J.keys + gl(J.indptr)[c];
         for (;;) {
             uint32_t g0 = 0;
             if (lane == 0) g0 = atomicAdd(&st.next_t, (uint32_t)kWave);
             g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
             if (g0 >= (uint32_t)J.G) break;
             const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
-            const TargetResult r = target_windows<V>(J, c, g0, ng, pl, rv, colkeys);
+            const TargetResult r = target_windows<V>(J, c, g0, ng, pl, rv, colkeys, htab);
             flush_targets(J, r, (int)ng);
         }
     } else {
@@ -2052,7 +2180,7 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
             mark(2);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             mark(3);
-            target_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            target_kernel<V><<<grid, kColBlock, target_lds(L.f64), s>>>(L.jobs, L.gen_items, n);
         };
         if (L.f64) sort_path(TypeTag<double>{});
         else sort_path(TypeTag<float>{});
@@ -2090,7 +2218,7 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
             using V = typename decltype(v_tag)::type;
             split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
-            target_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            target_kernel<V><<<grid, kColBlock, target_lds(L.f64), s>>>(L.jobs, L.gen_items, n);
         };
         if (L.f64) sort_path(TypeTag<double>{});
         else sort_path(TypeTag<float>{});

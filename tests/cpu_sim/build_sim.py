@@ -28,6 +28,7 @@ def _device_text(path, cut_marker=None):
     # address spaces, clang vector types and non-temporal loads have no g++
     # spelling: plain pointers, structs with the same members, plain loads
     s = s.replace("#define HPDEX_GLOBAL __attribute__((address_space(1)))", "#define HPDEX_GLOBAL")
+    s = s.replace("#define HPDEX_LDS __attribute__((address_space(3)))", "#define HPDEX_LDS")
     s = re.sub(r"typedef (\w+) (\w+)x4 __attribute__\(\(ext_vector_type\(4\)\)\);",
                r"struct \2x4 { \1 x, y, z, w; };", s)
     s = re.sub(r"typedef (\w+) (\w+)x2 __attribute__\(\(ext_vector_type\(2\)\)\);",

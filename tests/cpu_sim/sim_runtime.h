@@ -206,6 +206,12 @@ inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
 template <typename T> inline T sim_atomic_add(T* p, T v) { return std::atomic_ref<T>(*p).fetch_add(v); }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return sim_atomic_add(p, v); }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { return sim_atomic_add(p, v); }
+// compare and swap: the old value (the swap happened when it equals cmp)
+inline unsigned long long atomicCAS(unsigned long long* p, unsigned long long cmp,
+                                    unsigned long long v) {
+    std::atomic_ref<unsigned long long>(*p).compare_exchange_strong(cmp, v);
+    return cmp;
+}
 inline double atomicAdd(double* p, double v) {
     std::atomic_ref<double> a(*p);
     double old = a.load();
@@ -226,6 +232,7 @@ template <typename T> inline T sim_atomic_max(T* p, T v) {
 }
 inline int atomicMin(int* p, int v) { return sim_atomic_min(p, v); }
 inline int atomicMax(int* p, int v) { return sim_atomic_max(p, v); }
+inline unsigned atomicMax(unsigned* p, unsigned v) { return sim_atomic_max(p, v); }
 inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v) { return sim_atomic_min(p, v); }
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) { return sim_atomic_max(p, v); }
 inline unsigned atomicAnd(unsigned* p, unsigned v) { return std::atomic_ref<unsigned>(*p).fetch_and(v); }
