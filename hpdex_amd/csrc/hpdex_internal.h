@@ -19,6 +19,7 @@ enum : uint32_t {
 
 // Launch geometry shared by the per-column kernels.
 constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
+constexpr int kTargetBlock = 512;           // threads per target-window workgroup (8 waves)
 constexpr int kSplitBlock = 1024;           // threads per split workgroup (16 waves)
 constexpr int kSplitLdsMax = 152 * 1024;    // dynamic LDS a split workgroup may take
 constexpr int kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup, at most
@@ -45,7 +46,7 @@ constexpr uint64_t kTieEmpty = ~0ull;
 constexpr uint32_t kTieCountBits = 26;
 constexpr uint64_t kTieCountMask = (1ull << kTieCountBits) - 1ull;
 __host__ __device__ constexpr uint32_t target_lds(bool f64) {
-    return ref_table_lds(f64) + (uint32_t)(kColBlock / 64) * kTieSlots * 8u;
+    return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kTieSlots * 8u;
 }
 
 // Count kernel (integer-valued columns in one pass).
@@ -70,7 +71,8 @@ struct ColPlan {
     uint32_t nb;           // histogram bins, or reference buckets (sort path)
     uint32_t shift;        // sort path: bucket = (key - kmin) >> shift
     uint32_t maxb;         // sort path, on-chip reference: keys in its fullest bucket
-    uint32_t pad;
+    uint32_t bigt;         // 4-byte keys: the column has targets of more than 64 keys,
+                           // left by the window pass to the sorted pass
 };
 
 // One Mann-Whitney job: a device-resident CSC matrix, its group labels and

@@ -461,7 +461,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         pl.kmax = st.kmax;
         pl.TA = 0;
         pl.maxb = 0;
-        pl.pad = 0;
+        pl.bigt = 0;
         const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
         if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
             pl.hist = 1;
@@ -1003,7 +1003,7 @@ __device__ __forceinline__ TargetResult process_target(const JobDesc& J, int64_t
 // Outputs of up to 64 targets held one per lane: written, and their FP64
 // asymptotic p computed, by the lanes in parallel.
 __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResult& r, int cnt) {
-    if (lane_id() < cnt) {
+    if (lane_id() < cnt && r.idx >= 0) {
         gl(J.U1)[r.idx] = r.U1;
         if (J.tie) gl(J.tie)[r.idx] = r.tie;
         if (J.n1o) { gl(J.n1o)[r.idx] = (double)r.n1; gl(J.n2o)[r.idx] = (double)r.n2; }
@@ -1028,7 +1028,7 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
                                                        uint32_t ng, const ColPlan& pl,
                                                        const RefView<typename KeyOf<V>::K>& ref,
                                                        const typename KeyOf<V>::K* colkeys,
-                                                       unsigned long long* htab) {
+                                                       unsigned long long* htab, bool& any_big) {
     using K = typename KeyOf<V>::K;
     static_assert(sizeof(K) == 4, "target windows compare 32-bit keys across lanes");
     const uint32_t lane = (uint32_t)lane_id();
@@ -1062,6 +1062,7 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
     uint64_t S = 0, TT = 0;
     bool big = false;
     TargetResult mine{};
+    mine.idx = -1;   // nothing to write (a lane past the claim or a big target)
     // The window that starts at target j0: its end j1 (j1 == j0: target j0
     // alone has more than 64 keys), its m keys, and each lane's owner (the
     // target of its key), position of the owner's first key, and key.  The
@@ -1191,12 +1192,9 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         if (valid) htab[slot] = kTieEmpty;
         w = nx;
     }
-    // the targets of more than 64 keys, one at a time
-    for (uint64_t bigm = __ballot(own && nzj > (uint32_t)kWave); bigm; bigm &= bigm - 1ull) {
-        const uint32_t j = (uint32_t)__builtin_ctzll(bigm);
-        const TargetResult r = process_target<V>(J, c, g0 + j, pl, ref);
-        if (lane == j) { mine = r; big = true; }
-    }
+    // targets of more than 64 keys: ranked by the sorted pass
+    big = own && nzj > (uint32_t)kWave;
+    any_big = __ballot(big) != 0ull;
     if (own && !big) {
         uint32_t ltz;
         if (J.mode == kZeroMin) ltz = 0;
@@ -1217,9 +1215,17 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
 
 struct TargetState {
     uint32_t next_t;
+    uint32_t any_big;
 };
 
-template <typename V>
+// The target phase of one column.  kWindows (4-byte keys): a wave claims 64
+// consecutive targets, one per lane, and ranks those of at most 64 keys
+// several at a time (target_windows); a column with larger targets is
+// flagged for the sorted pass.  Otherwise (the sorted pass): a wave claims
+// 64 targets and ranks each of them alone (process_target) -- every target
+// for 8-byte keys, the targets of more than 64 keys of a flagged column for
+// 4-byte keys.
+template <typename V, bool kWindows>
 __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using K = typename KeyOf<V>::K;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1230,6 +1236,8 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     const JobDesc J = jobs[job];
     const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
     const ColPlan pl = J.plan[c];
+    // the sorted pass of 4-byte keys: only columns the window pass flagged
+    if (!kWindows && sizeof(K) == 4 && uniform_u32(pl.bigt) == 0u) return;
     const uint32_t* tab = J.reftab + (size_t)c * kTabStride;
     // uniform decisions (kept in SGPRs so the barrier below is not under an exec mask)
     const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
@@ -1260,55 +1268,74 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
         rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
         rv.iters = halvings(uniform_u32(pl.maxb));
     }
-    // each wave's tie-position table (target_windows), all slots empty
-    unsigned long long* htab = (unsigned long long*)(smem + ref_table_lds(sizeof(K) == 8)) +
-                               (size_t)(tid / kWave) * kTieSlots;
-    for (uint32_t i = (uint32_t)lane; i < kTieSlots; i += (uint32_t)kWave) htab[i] = kTieEmpty;
-    if (tid == 0) st.next_t = 1u;
-    __syncthreads();
-    if constexpr (sizeof(K) == 4) {
-        // a wave claims 64 consecutive targets, one per lane, and ranks the
-        // small ones several at a time (target_windows)
+    if (tid == 0) { st.next_t = 1u; st.any_big = 0u; }
+    if constexpr (kWindows) {
+        static_assert(sizeof(K) == 4, "target windows take 4-byte keys");
+        // each wave's tie-position table (target_windows), all slots empty
+        unsigned long long* htab = (unsigned long long*)(smem + ref_table_lds(false)) +
+                                   (size_t)(tid / kWave) * kTieSlots;
+        for (uint32_t i = (uint32_t)lane; i < kTieSlots; i += (uint32_t)kWave) htab[i] = kTieEmpty;
+        __syncthreads();
         const K* colkeys = (const K*)J.keys + gl(J.indptr)[c];
+        bool any_big = false;
         for (;;) {
             uint32_t g0 = 0;
             if (lane == 0) g0 = atomicAdd(&st.next_t, (uint32_t)kWave);
             g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
             if (g0 >= (uint32_t)J.G) break;
             const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
-            const TargetResult r = target_windows<V>(J, c, g0, ng, pl, rv, colkeys, htab);
+            bool claim_big = false;
+            const TargetResult r =
+                target_windows<V>(J, c, g0, ng, pl, rv, colkeys, htab, claim_big);
+            any_big = any_big || claim_big;
             flush_targets(J, r, (int)ng);
         }
+        if (any_big && lane == 0) st.any_big = 1u;
+        __syncthreads();
+        if (tid == 0 && st.any_big) J.plan[c].bigt = 1u;
     } else {
-        // lane k of the wave keeps the k-th finished target until 64 are pending
-        TargetResult mine{};
-        int pending = 0;
+        __syncthreads();
+        // lane j of the wave keeps target g0 + j's result
         for (;;) {
-            uint32_t g = 0;
-            if (lane == 0) g = atomicAdd(&st.next_t, 1u);
-            g = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g, 0));
-            if (g >= (uint32_t)J.G) break;
-            const TargetResult r = process_target<V>(J, c, g, pl, rv);
-            if (lane == pending) mine = r;
-            if (++pending == kWave) {
-                flush_targets(J, mine, pending);
-                pending = 0;
+            uint32_t g0 = 0;
+            if (lane == 0) g0 = atomicAdd(&st.next_t, (uint32_t)kWave);
+            g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
+            if (g0 >= (uint32_t)J.G) break;
+            const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
+            const bool own = (uint32_t)lane < ng;
+            const uint32_t nzj = own ? gl(J.cnt)[c * J.G + g0 + (uint32_t)lane] : 0u;
+            TargetResult mine{};
+            mine.idx = -1;
+            for (uint64_t m = __ballot(own && (sizeof(K) == 8 || nzj > (uint32_t)kWave)); m;
+                 m &= m - 1ull) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(m);
+                const TargetResult r = process_target<V>(J, c, g0 + j, pl, rv);
+                if ((uint32_t)lane == j) mine = r;
             }
+            flush_targets(J, mine, (int)ng);
         }
-        flush_targets(J, mine, pending);
     }
 }
 
-// workgroup b takes queue item b (the grid is sized for the queue's capacity).
-// At most 168 VGPRs, so that three waves share a SIMD: the table LDS
-// (49 KB) fits three workgroups per CU, and the third wave hides the
-// latency of the per-target gathers (6.0 -> 4.4 ms on 2,000 small targets
-// x 6,000 genes, against 2 waves at the 178 VGPRs the kernel takes freely;
-// the cap spills 4 VGPRs).
+// The window pass of 4-byte keys: workgroup b takes queue item b (the grid
+// is sized for the queue's capacity).  512 threads at four waves per SIMD
+// (at most 128 VGPRs): the reference table and eight tie tables (57 KB of
+// LDS) fit two workgroups per CU, 16 waves to hide the latency of the
+// LDS searches and key loads.
+template <typename V>
+__global__ __launch_bounds__(kTargetBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void target_window_kernel(const JobDesc* jobs, const WorkItem* items, uint32_t n_items) {
+    if (blockIdx.x < n_items) target_item<V, true>(jobs, items, blockIdx.x);
+}
+
+// The sorted pass (every target of 8-byte keys; the targets of more than 64
+// keys of 4-byte keys): workgroup b takes queue item b.  At most 168 VGPRs,
+// so that three waves share a SIMD: the table LDS (49 KB) fits three
+// workgroups per CU.
 template <typename V>
 __global__ __launch_bounds__(kColBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void target_kernel(const JobDesc* jobs, const WorkItem* items, uint32_t n_items) {
-    if (blockIdx.x < n_items) target_item<V>(jobs, items, blockIdx.x);
+    if (blockIdx.x < n_items) target_item<V, false>(jobs, items, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2120,6 +2147,17 @@ static inline unsigned align16(size_t b) { return (unsigned)((b + 15) & ~(size_t
 // histogram table and the on-chip sorted reference plus its bucket index.
 static unsigned table_lds_bytes(bool f64) { return ref_table_lds(f64); }
 
+// The target phase of the sort-path columns: 4-byte keys take the window
+// pass, then the sorted pass over the columns it flagged; 8-byte keys the
+// sorted pass over every column.
+template <typename V>
+static void launch_target_pass(const JobDesc* jobs, const WorkItem* items, uint32_t n,
+                               hipStream_t s) {
+    if constexpr (sizeof(V) == 4)
+        target_window_kernel<V><<<n, kTargetBlock, target_lds(false), s>>>(jobs, items, n);
+    target_kernel<V><<<n, kColBlock, table_lds_bytes(sizeof(V) == 8), s>>>(jobs, items, n);
+}
+
 hipError_t prepare_count_kernel(uint32_t bytes) {
     const void* fns[] = {(const void*)count_kernel<float, 256>, (const void*)count_kernel<float, 512>,
                          (const void*)count_kernel<float, 1024>,
@@ -2180,7 +2218,7 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
             mark(2);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             mark(3);
-            target_kernel<V><<<grid, kColBlock, target_lds(L.f64), s>>>(L.jobs, L.gen_items, n);
+            launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         };
         if (L.f64) sort_path(TypeTag<double>{});
         else sort_path(TypeTag<float>{});
@@ -2218,7 +2256,7 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
             using V = typename decltype(v_tag)::type;
             split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
-            target_kernel<V><<<grid, kColBlock, target_lds(L.f64), s>>>(L.jobs, L.gen_items, n);
+            launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         };
         if (L.f64) sort_path(TypeTag<double>{});
         else sort_path(TypeTag<float>{});

@@ -179,6 +179,10 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
         run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, ng, split_lds); }, timeout_s);
     for (unsigned b = 0; b < ng; ++b)
         run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, ng); }, timeout_s);
+    if constexpr (sizeof(V) == 4)
+        for (unsigned b = 0; b < ng; ++b)
+            run_block(b, kTargetBlock, ng, [&] { target_window_kernel<V>(jobs, it, ng); },
+                      timeout_s);
     for (unsigned b = 0; b < ng; ++b)
         run_block(b, kColBlock, ng, [&] { target_kernel<V>(jobs, it, ng); }, timeout_s);
     return (int)nc;
