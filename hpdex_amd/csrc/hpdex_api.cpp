@@ -643,7 +643,7 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
     if (status) {
         const std::string why =
             (status & kStatusGroups)
-                ? "more target groups than the on-chip group counters hold (5,460)"
+                ? "more target groups than the on-chip group counters hold (5,428)"
             : (status & kStatusRange)
                 ? "a stored value outside the column's recorded value range (the resident "
                   "matrix's metadata does not match its values)"

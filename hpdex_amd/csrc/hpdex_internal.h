@@ -20,6 +20,14 @@ enum : uint32_t {
 // Launch geometry shared by the per-column kernels.
 constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
 constexpr int kTargetBlock = 512;           // threads per target-window workgroup (8 waves)
+// Split kernel counters: per group a 64-bit word (explicit zeros in the high
+// half, nonzero values in the low) and a 32-bit offset, plus kSplitDummy
+// words that entries outside every group hit (one per lane of a half-wave,
+// so those no-op atomics meet in no bank).
+constexpr uint32_t kSplitDummy = 32;
+__host__ __device__ constexpr uint64_t split_counter_bytes(uint64_t G) {
+    return (12u * (G + kSplitDummy) + 15u) & ~(uint64_t)15u;
+}
 constexpr int kSplitBlock = 1024;           // threads per split workgroup (16 waves)
 constexpr int kSplitLdsMax = 152 * 1024;    // dynamic LDS a split workgroup may take
 constexpr int kLdsBudget = 64 * 1024;       // dynamic LDS per workgroup, at most

@@ -128,17 +128,106 @@ __device__ __forceinline__ void for_each_entry(const int32_t* __restrict__ rows,
 }
 
 // Visit (group, value, entry index in the column) of every entry of the
-// column [p0, p0 + n) whose row is in a group, for workgroups of NT threads.  Batches of NT*kUnroll entries are
+// column [p0, p0 + n) whose row is in a group, for workgroups of NT threads
+// (kBatch: f takes a whole batch, kUnroll entries per thread, entries outside
+// every group with a group of G or above).  Batches of NT*kUnroll entries are
 // read as buffer loads whose resources end at the batch's last entry (lanes
-// past it read zeros and are masked) and software-pipelined: the next batch's
-// rows and values are issued after this batch's label gathers and before its
-// visits, so they are in flight while this batch is visited (loads complete in
-// order: waiting for the gathers leaves them outstanding).  kPacked gathers the
-// labels from the bit-packed array (rows out of range read its sentinel at
-// index R): a sparse column's gathers then touch as few cache lines as
-// possible.  Otherwise the 32-bit labels are gathered: for a column holding a
-// large share of the rows they are coalesced and unpacking would only add work.
-template <typename V, int NT, bool kPacked, bool kInOrder = false, typename F>
+// past it read zeros and are masked) in a two-stage software pipeline: while
+// batch b is visited, the labels of batch b + 1 are being gathered and the
+// rows and values of batch b + 2 loaded (loads complete in order: waiting for
+// one batch leaves the later ones outstanding).  kPacked gathers the labels
+// from the bit-packed array (rows out of range read its sentinel at index R):
+// a sparse column's gathers then touch as few cache lines as possible.
+// Otherwise the 32-bit labels are gathered: for a column holding a large
+// share of the rows they are coalesced and unpacking would only add work.
+template <typename V, int NT, bool kPacked, bool kBatch = false, int U = kUnroll, typename F>
+__device__ __forceinline__ void for_each_grouped_pipelined(const JobDesc& J, int64_t p0, int64_t n,
+                                                           F&& f) {
+    const uint32_t tid = (uint32_t)tid_x();
+    const uint32_t R = (uint32_t)J.R, G = (uint32_t)J.G;
+    const uint32_t lbits = J.gid_bits, lmask = (1u << lbits) - 1u;
+    const __amdgpu_buffer_rsrc_t gid_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)J.gidp, (short)0, kPacked ? J.gid_words * 4u : 0u, kBufferWord3);
+    const int32_t* col_rows = J.rows + p0;
+    const V* col_vals = (const V*)J.vals + p0;
+    constexpr uint32_t step = (uint32_t)NT * U;
+    // rows and values of the batch at ib
+    auto load = [&](int64_t ib, uint32_t (&r)[U], V (&v)[U]) {
+        const int64_t left = n - ib;
+        const uint32_t m = left < (int64_t)step ? (uint32_t)left : step;
+        const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_rows + ib), (short)0, m * 4u, kBufferWord3);
+        const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(col_vals + ib), (short)0, m * (uint32_t)sizeof(V), kBufferWord3);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = tid + (uint32_t)(u * NT);
+            r[u] = buffer_load_u32(rows_rsrc, e * 4u, 0u);
+            v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
+        }
+    };
+    // labels of the batch at ib; lanes past the column's end take none
+    auto gather = [&](int64_t ib, const uint32_t (&r)[U], uint32_t (&g)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = ib + (int64_t)(tid + (uint32_t)(u * NT)) < n;
+            const uint32_t ru = in ? r[u] : 0xFFFFFFFFu;
+            if (kPacked) {
+                const uint32_t pos = min(ru, R) * lbits;
+                const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
+                g[u] = (uint32_t)(w2 >> (pos & 31u)) & lmask;
+            } else {
+                g[u] = ru < R ? (uint32_t)gl(J.gid)[ru] : 0xFFFFFFFFu;
+            }
+        }
+    };
+    auto visit = [&](int64_t ib, const uint32_t (&g)[U], const V (&v)[U]) {
+        if constexpr (kBatch) {
+            f(g, v, (uint32_t)ib + tid);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
+        }
+    };
+    if (n <= 0) return;
+    uint32_t r[U], g[U];
+    V v[U];
+    load(0, r, v);
+    gather(0, r, g);
+    int64_t ib = 0;
+    if ((int64_t)step < n) {
+        uint32_t r2[U];
+        V v2[U];
+        load(step, r2, v2);
+        for (;;) {   // batch ib with the batch at ib + step after it
+            uint32_t g2[U];
+            gather(ib + step, r2, g2);
+            const bool more = ib + 2 * (int64_t)step < n;
+            uint32_t r3[U];
+            V v3[U];
+            if (more) load(ib + 2 * step, r3, v3);
+            __builtin_amdgcn_sched_barrier(0);
+            visit(ib, g, v);
+            ib += step;
+#pragma unroll
+            for (int u = 0; u < U; ++u) { g[u] = g2[u]; v[u] = v2[u]; }
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) { r2[u] = r3[u]; v2[u] = v3[u]; }
+        }
+    }
+    visit(ib, g, v);   // the column's last batch
+}
+
+// The one-stage reader: the next batch's rows and values are issued after
+// this batch's label gathers and before its visits.  kInOrder (the column
+// stores rows 0 .. R-1 in order): an entry's row is its position, only its
+// value is read; its label gathers are coalesced (for such a column, labels
+// loaded with the values one stage earlier, in smaller batches, were slower:
+// 1.64 -> 1.89 ms on a dense 150,000 x 6,000 split).
+template <typename V, int NT, bool kPacked, bool kInOrder = false, bool kBatch = false,
+          int U = kUnroll, typename F>
 __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, int64_t n, F&& f) {
     const uint32_t tid = (uint32_t)tid_x();
     const uint32_t R = (uint32_t)J.R, G = (uint32_t)J.G;
@@ -147,8 +236,8 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
         (void*)J.gidp, (short)0, kPacked ? J.gid_words * 4u : 0u, kBufferWord3);
     const int32_t* col_rows = J.rows + p0;
     const V* col_vals = (const V*)J.vals + p0;
-    constexpr uint32_t step = (uint32_t)NT * kUnroll;
-    auto load = [&](int64_t ib, uint32_t (&r)[kUnroll], V (&v)[kUnroll]) {
+    constexpr uint32_t step = (uint32_t)NT * U;
+    auto load = [&](int64_t ib, uint32_t (&r)[U], V (&v)[U]) {
         const int64_t left = n - ib;
         const uint32_t m = left < (int64_t)step ? (uint32_t)left : step;
         const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -156,7 +245,7 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
         const __amdgpu_buffer_rsrc_t vals_rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(col_vals + ib), (short)0, m * (uint32_t)sizeof(V), kBufferWord3);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             const uint32_t e = tid + (uint32_t)(u * NT);
             // kInOrder: the column stores rows 0 .. R-1 in order, so an
             // entry's row is its position (R < 2^31 keeps it in 32 bits)
@@ -164,9 +253,9 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
             v[u] = buffer_load_value<V>(vals_rsrc, e * (uint32_t)sizeof(V), 0u);
         }
     };
-    auto gather = [&](const uint32_t (&r)[kUnroll], uint32_t (&g)[kUnroll]) {
+    auto gather = [&](const uint32_t (&r)[U], uint32_t (&g)[U]) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (kPacked) {
                 const uint32_t pos = min(r[u], R) * lbits;
                 const uint64_t w2 = buffer_load_u64(gid_rsrc, (pos >> 5) * 4u);
@@ -177,35 +266,44 @@ __device__ __forceinline__ void for_each_grouped(const JobDesc& J, int64_t p0, i
         }
     };
     if (n <= 0) return;
-    uint32_t r[kUnroll];
-    V v[kUnroll];
+    uint32_t r[U];
+    V v[U];
     load(0, r, v);
     int64_t ib = 0;
     for (; ib + (int64_t)step < n; ib += step) {   // a whole batch with another after it
-        uint32_t g[kUnroll];
+        uint32_t g[U];
         gather(r, g);
         __builtin_amdgcn_sched_barrier(0);
-        uint32_t r2[kUnroll];
-        V v2[kUnroll];
+        uint32_t r2[U];
+        V v2[U];
         load(ib + step, r2, v2);
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (kBatch) {
+            f(g, v, (uint32_t)ib + tid);
+        } else {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
-            if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
+            for (int u = 0; u < U; ++u)
+                if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
+        }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
+        for (int u = 0; u < U; ++u) { r[u] = r2[u]; v[u] = v2[u]; }
     }
     // the last batch: lanes past the column's end masked
     const uint32_t m = (uint32_t)(n - ib);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
         if (tid + (uint32_t)(u * NT) >= m) r[u] = 0xFFFFFFFFu;
-    uint32_t g[kUnroll];
+    uint32_t g[U];
     gather(r, g);
+    if constexpr (kBatch) {
+        f(g, v, (uint32_t)ib + tid);
+    } else {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-        if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
+        for (int u = 0; u < U; ++u)
+            if (g[u] < G) f(g[u], v[u], (uint32_t)ib + tid + (uint32_t)(u * NT));
+    }
 }
+
 
 __device__ inline void raise_status(const JobDesc& J, uint32_t bit, int64_t c) {
     atomicOr(J.status, bit);
@@ -310,6 +408,8 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
                                            uint32_t lds_bytes) {
     using KT = KeyOf<V>;
     using K = typename KT::K;
+    // entries per thread per batch of the two-stage column reader
+    constexpr int kSplitUnroll = sizeof(V) == 4 ? 4 : 2;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ SplitState st;
     int job;
@@ -321,13 +421,16 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t R = (uint32_t)J.R;
     const V* vals = (const V*)J.vals;
     const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
-    if (12 * G > kLdsBudget) {
+    if (split_counter_bytes((uint64_t)G) > (uint64_t)kLdsBudget) {
         if (tid == 0) raise_status(J, kStatusGroups, c);
         return;
     }
-    uint32_t* cnt_nz = (uint32_t*)smem;
-    uint32_t* cnt_ez = cnt_nz + G;
-    uint32_t* cur = cnt_ez + G;
+    // per group: explicit zeros (high half) and nonzero values (low half),
+    // then the offsets; kSplitDummy extra words of each take the no-op
+    // counts of entries outside every group
+    unsigned long long* cnt = (unsigned long long*)smem;
+    uint32_t* cur = (uint32_t*)(cnt + G + kSplitDummy);
+    const uint32_t dummy = (uint32_t)G + ((uint32_t)lane & (kSplitDummy - 1u));
     // the column's entries in a group: bit-packed labels for a sparse column
     // (its rows are far apart, so 32-bit label gathers would touch a cache
     // line per entry), 32-bit labels for a column holding a quarter of the
@@ -337,10 +440,20 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // a column storing rows 0 .. R-1 in order (a dense matrix held as CSC):
     // rows from positions, so only its values are read
     const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
+    // sparse columns through the two-stage reader (the label gathers of one
+    // batch overlap the visits of the one before: 2.06 -> 1.98 ms on a
+    // 400,000 x 18,000 split at 5 %), columns in row order through the
+    // one-stage reader with rows from positions
     auto grouped = [&](auto&& f) {
-        if (packed) for_each_grouped<V, NT, true>(J, p0, p1 - p0, f);
-        else if (in_order) for_each_grouped<V, NT, false, true>(J, p0, p1 - p0, f);
-        else for_each_grouped<V, NT, false>(J, p0, p1 - p0, f);
+        if (packed) for_each_grouped_pipelined<V, NT, true, false, kSplitUnroll>(J, p0, p1 - p0, f);
+        else if (in_order) for_each_grouped<V, NT, false, true, false, kUnroll>(J, p0, p1 - p0, f);
+        else for_each_grouped_pipelined<V, NT, false, false, kSplitUnroll>(J, p0, p1 - p0, f);
+    };
+    // every entry of a batch (group G or above: none)
+    auto grouped_batch = [&](auto&& f) {
+        if (packed) for_each_grouped_pipelined<V, NT, true, true, kSplitUnroll>(J, p0, p1 - p0, f);
+        else if (in_order) for_each_grouped<V, NT, false, true, true, kUnroll>(J, p0, p1 - p0, f);
+        else for_each_grouped_pipelined<V, NT, false, true, kSplitUnroll>(J, p0, p1 - p0, f);
     };
 
     // Slots: group g's keys at the exclusive prefix of the group sizes.  When
@@ -348,7 +461,10 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // entries as there are grouped rows, e.g. a dense one), every stored value
     // goes to its slot in the counting pass itself and the column is read once;
     // otherwise the counts are scanned into offsets and a second pass scatters.
-    for (int g = tid; g < G; g += nt) { cnt_nz[g] = 0; cnt_ez[g] = 0; cur[g] = J.group_rows[g]; }
+    for (int g = tid; g < G + (int)kSplitDummy; g += nt) {
+        cnt[g] = 0ull;
+        cur[g] = g < G ? J.group_rows[g] : 0u;
+    }
     if (tid == 0) {
         st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
         st.kmin = ~0ull; st.kmax = 0;
@@ -361,7 +477,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // range (its slots, or its entries when it is scattered by offsets) and
     // go to HBM afterwards in one coalesced copy: scattered 4-byte stores to
     // HBM write whole partial lines
-    const uint32_t cnt_bytes = (12u * (uint32_t)G + 15u) & ~15u;
+    const uint32_t cnt_bytes = (uint32_t)split_counter_bytes((uint64_t)G);
     const uint64_t span = fixed ? (uint64_t)slots : (uint64_t)(p1 - p0);
     const bool staged = __builtin_amdgcn_readfirstlane(
         (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * span <= lds_bytes ? 1 : 0) != 0;
@@ -379,7 +495,10 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     if (arrival)
         for (uint32_t e = (uint32_t)tid; e < n_col; e += (uint32_t)nt) agrp[e] = 0xFFFFu;
 
-    // counts and value statistics (and, in slots, the keys)
+    // counts and value statistics (and, in slots, the keys), a batch of
+    // kUnroll entries per thread at a time with no branch around the
+    // counters: every entry adds to its group's word or to a dummy word, so
+    // the batch's LDS atomics are issued back to back and waited for once
     // fixed_tag: 0 scatter later, 1 keys to their slots in HBM, 2 in LDS
     auto count_pass = [&](auto fixed_tag) {
         constexpr int kFixed = decltype(fixed_tag)::value;
@@ -387,30 +506,45 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         int lmin = INT_MAX, lmax = INT_MIN;
         K kmn = KT::kSentinel, kmx = 0;
         uint32_t neg = 0;
-        grouped([&](uint32_t g, V v, uint32_t e) {
-            if (!KT::finite(v)) return;
-            if (v == V(0)) { atomicAdd(&cnt_ez[g], 1u); return; }
-            const uint32_t i = atomicAdd(&cnt_nz[g], 1u);
-            const K k = KT::key(v);
-            if (kFixed == 0) {
-                if (arrival) agrp[e] = (uint16_t)g;
-            } else if (kFixed == 1) {
-                // a group holding more entries than rows (repeated row
-                // indices) is an error of the column; its keys stay inside
-                // the column's range
-                const uint32_t pos = cur[g] + i;
-                if ((int64_t)pos < p1 - p0) keys[pos] = k;
-            } else if (kFixed == 2) {
-                const uint32_t pos = cur[g] + i;
-                if (pos < slots) stage[pos] = k;
+        grouped_batch([&](const auto& g, const auto& v, uint32_t e0) {
+            constexpr int U = (int)(sizeof(g) / sizeof(g[0]));
+            bool nz[U];
+            uint32_t gi[U], pos[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = g[u] < (uint32_t)G && KT::finite(v[u]);
+                nz[u] = in && !(v[u] == V(0));
+                gi[u] = in ? g[u] : dummy;
             }
-            int iv;
-            if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
-            else li = false;
-            if (g == 0) {
-                kmn = k < kmn ? k : kmn;
-                kmx = k > kmx ? k : kmx;
-                neg += (v < V(0)) ? 1u : 0u;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                pos[u] = (uint32_t)atomicAdd(&cnt[gi[u]], nz[u] ? 1ull : (1ull << 32));
+            if constexpr (kFixed != 0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) pos[u] += cur[nz[u] ? gi[u] : dummy];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!nz[u]) continue;
+                const K k = KT::key(v[u]);
+                if (kFixed == 0) {
+                    if (arrival) agrp[e0 + (uint32_t)(u * NT)] = (uint16_t)g[u];
+                } else if (kFixed == 1) {
+                    // a group holding more entries than rows (repeated row
+                    // indices) is an error of the column; its keys stay
+                    // inside the column's range
+                    if ((int64_t)pos[u] < p1 - p0) keys[pos[u]] = k;
+                } else {
+                    if (pos[u] < slots) stage[pos[u]] = k;
+                }
+                int iv;
+                if (KT::small_int(v[u], iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+                else li = false;
+                if (g[u] == 0) {
+                    kmn = k < kmn ? k : kmn;
+                    kmx = k > kmx ? k : kmx;
+                    neg += (v[u] < V(0)) ? 1u : 0u;
+                }
             }
         });
         const bool wall = __all(li ? 1 : 0);
@@ -439,7 +573,8 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         const uint64_t ebase = (uint64_t)c * (uint64_t)(2 * G + 1);
         const int64_t b = c * G;
         for (int g = tid; g < G; g += nt) {
-            const uint32_t nz = cnt_nz[g], ez = cnt_ez[g], gr = J.group_rows[g];
+            const uint32_t nz = (uint32_t)cnt[g], ez = (uint32_t)(cnt[g] >> 32),
+                           gr = J.group_rows[g];
             if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
             const uint32_t n = (J.mode == kZeroNone) ? nz + ez : gr;
             if (n < J.min_n) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
@@ -455,7 +590,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // plan of the column
     if (tid == 0) {
         ColPlan pl;
-        pl.na = cnt_nz[0];
+        pl.na = (uint32_t)cnt[0];
         pl.refneg = st.refneg;
         pl.kmin = st.kmin;
         pl.kmax = st.kmax;
@@ -501,11 +636,16 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
                 g[u] = e < n_col ? (uint32_t)agrp[e] : 0xFFFFu;
                 v[u] = g[u] != 0xFFFFu ? gl(col_vals)[e] : V(0);
             }
+            // the offsets taken back to back (entries with no key take a
+            // dummy word), then the keys stored
+            uint32_t pos[kA];
+#pragma unroll
+            for (uint32_t u = 0; u < kA; ++u)
+                pos[u] = atomicAdd(&cur[g[u] != 0xFFFFu ? g[u] : dummy], 1u);
 #pragma unroll
             for (uint32_t u = 0; u < kA; ++u) {
                 if (g[u] == 0xFFFFu) continue;
-                const uint32_t pos = atomicAdd(&cur[g[u]], 1u);
-                stage[pos] = hist ? (K)(uint32_t)((int)v[u] - vmin) : KT::key(v[u]);
+                stage[pos[u]] = hist ? (K)(uint32_t)((int)v[u] - vmin) : KT::key(v[u]);
             }
         }
         __syncthreads();   // the column's key range out, coalesced
@@ -530,7 +670,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             // keys (integers, as the statistics found them) become v - vmin
             const int w = tid / kWave, nw = nt / kWave;
             for (int g = w; g < G; g += nw) {
-                const uint32_t o = cur[g], m = min(cnt_nz[g], J.group_rows[g]);
+                const uint32_t o = cur[g], m = min((uint32_t)cnt[g], J.group_rows[g]);
                 for (uint32_t i = (uint32_t)lane; i < m; i += kWave)
                     slot_keys[o + i] = (K)(uint32_t)((int)KT::value(slot_keys[o + i]) - vmin);
             }
@@ -2204,7 +2344,7 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     }
     mark(1);
     if (L.n_gen > 0) {
-        size_t split = 12u * (size_t)L.G;
+        size_t split = (size_t)split_counter_bytes((uint64_t)L.G);
         if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
         // the most LDS the split kernel may take: one 1024-thread workgroup
         // per CU fits in any case, and a column whose key range fits in what
@@ -2247,7 +2387,7 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
 #undef HPDEX_COUNTQ
     }
     if (L.n_gen > 0) {
-        size_t split = 12u * (size_t)L.G;
+        size_t split = (size_t)split_counter_bytes((uint64_t)L.G);
         if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
         const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
         const unsigned tab_lds = table_lds_bytes(L.f64);

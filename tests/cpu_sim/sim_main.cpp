@@ -171,7 +171,7 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most,
     // the reference and target kernels' table
     auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
-    size_t split = 12u * (size_t)J.G;
+    size_t split = (size_t)split_counter_bytes((uint64_t)J.G);
     if (split > (size_t)kLdsBudget) split = 16;
     const uint32_t split_lds = split > 16u ? (uint32_t)kSplitLdsMax : a16(split);
     const unsigned ng = n_static;
