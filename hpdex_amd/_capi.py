@@ -61,7 +61,8 @@ class PdenOpts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes",
-                                              "count_block", "count_packed", "split_arrival")]
+                                              "count_block", "count_packed", "split_arrival",
+                                              "ref_lds_keys")]
 
 
 class PlanInfo(ctypes.Structure):
@@ -565,8 +566,10 @@ def get_tuning() -> Tuning:
 
 
 def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None,
-               count_packed=None, split_arrival=None):
+               count_packed=None, split_arrival=None, ref_lds_keys=None):
     t = get_tuning()
+    if ref_lds_keys is not None:
+        t.ref_lds_keys = int(ref_lds_keys)
     if split_arrival is not None:
         t.split_arrival = int(split_arrival)
     if count_packed is not None:

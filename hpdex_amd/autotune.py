@@ -14,10 +14,13 @@ out several ways, and which is fastest depends on the matrix:
 * ``count_block``    -- threads per histogram workgroup (256, 512, 1024);
 * ``split_arrival``  -- sort path: the split kernel keeps a sparse column's
   entry groups in LDS and reads only its values again to scatter them (1), or
-  reads rows, values and labels twice (0).
+  reads rows, values and labels twice (0);
+* ``ref_lds_keys``   -- sort path, the LDS/global sort cutover: a reference
+  slice of at most this many values is sorted in LDS and ranked from LDS, a
+  larger one is sorted in HBM (merge passes) and ranked through L2.
 
 Candidates that lay the job out the same way as one already timed (the
-histogram knobs of a job with no histogram column, the split knob of a job
+histogram knobs of a job with no histogram column, the sort-path knobs of a job
 with no sort-path column) are not timed again: they take that trial's time.
 
 ``autotune`` times every candidate on the caller's own resident matrix and
@@ -44,9 +47,10 @@ DEFAULT_SPACE = {
     "count_lds_bytes": (40448, 52 * 1024, 152 * 1024),
     "count_block": (256, 512, 1024),
     "split_arrival": (1, 0),
+    "ref_lds_keys": (8192, 2048, 512),
 }
 COUNT_KNOBS = ("count_lds_bytes", "count_block", "count_packed")
-SPLIT_KNOBS = ("split_arrival",)
+SORT_KNOBS = ("split_arrival", "ref_lds_keys")
 
 
 def space_for(matrix=None) -> dict:
@@ -54,7 +58,7 @@ def space_for(matrix=None) -> dict:
     return dict(DEFAULT_SPACE)
 
 KNOBS = ("hist_max_bins", "size_ordered", "count_lds_bytes", "count_block", "count_packed",
-         "split_arrival")
+         "split_arrival", "ref_lds_keys")
 
 
 class TuningMismatch(_capi.HpdexError):
@@ -134,11 +138,11 @@ def autotune(matrix, group_id, n_targets, opts, space=None, iters=20, warmup=3, 
                 continue
             info = plan.info()
             # the knobs that shape this job's launches: the histogram knobs
-            # only with a histogram column, the split knob only with a
+            # only with a histogram column, the sort-path knobs only with a
             # sort-path column (and the routing the cutover gives)
             eff = (info["count_columns"], info["count_blocks"],
                    tuple(int(cur.get(k, 0)) for k in COUNT_KNOBS) if info["count_columns"] else (),
-                   tuple(int(cur.get(k, 0)) for k in SPLIT_KNOBS) if info["split_columns"] else ())
+                   tuple(int(cur.get(k, 0)) for k in SORT_KNOBS) if info["split_columns"] else ())
             if eff in timed:
                 plan.close()
                 t = dict(timed[eff])

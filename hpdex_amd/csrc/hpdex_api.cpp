@@ -408,6 +408,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.asym = (js.opts.method == 2) ? 1 : 0;
     J.hist_max = tu.hist_max;
     J.split_arrival = tu.split_arrival;
+    J.ref_cap = (uint32_t)ref_lds_cap(m->f64, tu);
     J.min_n = 2u;
     J.cc = js.opts.use_continuity ? 0.5 : 0.0;
     return J;
@@ -482,7 +483,7 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
     // the on-chip sort (its stored values: at most the column's entries and
     // the reference's rows)
     {
-        const size_t cap = m.f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+        const size_t cap = js.J.ref_cap;
         std::vector<int64_t> off;
         int64_t total = 0;
         for (const WorkItem& wi : gen_items) {
@@ -715,8 +716,14 @@ size_t spill_keys_of(int64_t len, uint32_t gr0, size_t cap) {
     return b > cap ? b : 0;
 }
 
-size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64) {
-    const size_t cap = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+size_t ref_lds_cap(bool f64, const Tuning& tu) {
+    const size_t hw = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+    return std::min(hw, (size_t)tu.ref_lds_keys);
+}
+
+size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64,
+                        const Tuning& tu) {
+    const size_t cap = ref_lds_cap(f64, tu);
     if (gr0 <= cap) return 0;
     if (!indptr) return std::min(nnz, C * (size_t)gr0);
     size_t t = 0;
@@ -765,7 +772,7 @@ size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T) {
     return matrix_footprint(dtype, R, C, nnz, HPDEX_LAYOUT_CSC, 64, false) +
            job_footprint(R, C, nnz, dtype_uses_double(dtype), T, o, nullptr, false,
                          spill_keys_bound(nullptr, C, nnz, (uint32_t)std::min(R, (size_t)UINT32_MAX),
-                                          dtype_uses_double(dtype))) +
+                                          dtype_uses_double(dtype), tuning())) +
            up256(C * G * 8) + 3 * up256(np * 8) + up256(fdr_workspace_bound((int64_t)np));
 }
 
@@ -824,6 +831,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     out->count_block = t.count_block;
     out->count_packed = t.count_packed;
     out->split_arrival = t.split_arrival;
+    out->ref_lds_keys = t.ref_lds_keys;
     return 0;
 }
 
@@ -844,6 +852,9 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
         return set_err(err, err_len, "count_packed must be 0 or 1");
     if (t->split_arrival != 0 && t->split_arrival != 1)
         return set_err(err, err_len, "split_arrival must be 0 or 1");
+    if (t->ref_lds_keys < 64 || t->ref_lds_keys > kRefKeysF32 ||
+        (t->ref_lds_keys & (t->ref_lds_keys - 1)) != 0)
+        return set_err(err, err_len, "ref_lds_keys must be a power of two in [64, 8192]");
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init && !g_count_ready && t->count_lds_bytes != 0)
         return set_err(err, err_len, "this device does not give the count kernel its LDS");
@@ -853,6 +864,7 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     g_tune.count_block = t->count_block;
     g_tune.count_packed = t->count_packed;
     g_tune.split_arrival = t->split_arrival;
+    g_tune.ref_lds_keys = t->ref_lds_keys;
     return 0;
 }
 
@@ -986,7 +998,7 @@ int hpdex_mwu_csc_noalloc(int dtype, const void* data, const int64_t* indices,
         job_footprint(n_rows, n_cols, nnz, dtype_uses_double(dtype), n_targets, *opts,
                       group_rows.data(), false,
                       spill_keys_bound(indptr, n_cols, nnz, group_rows[0],
-                                       dtype_uses_double(dtype)));
+                                       dtype_uses_double(dtype), tuning()));
     if (need > kNoAllocScratchBytes)
         return set_err(err, err_len,
                        "hpdex_mwu_csc_noalloc: the job needs " + std::to_string(need) +

@@ -34,6 +34,7 @@ struct Tuning {
     int count_block = kCountBlockDefault;
     int count_packed = 1;
     int split_arrival = 1;
+    int ref_lds_keys = kRefKeysF32;
 };
 
 extern std::mutex g_mu;
@@ -213,7 +214,10 @@ size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const h
 // its reference slice fits the on-chip sort of cap keys), and their bound over
 // a matrix (indptr null: min(nnz, C * gr0), for a matrix not yet in CSC).
 size_t spill_keys_of(int64_t len, uint32_t gr0, size_t cap);
-size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64);
+size_t spill_keys_bound(const int64_t* indptr, size_t C, size_t nnz, uint32_t gr0, bool f64,
+                        const Tuning& tu);
+// reference keys a column's slice may hold and still be sorted in LDS
+size_t ref_lds_cap(bool f64, const Tuning& tu);
 size_t de_footprint(int dtype, size_t R, size_t C, size_t nnz, size_t T);
 
 // Several jobs of one value type as one batched launch: their descriptors,

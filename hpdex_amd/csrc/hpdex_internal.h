@@ -128,6 +128,9 @@ struct JobDesc {
     int32_t hist_max;            // largest integer range binned (0 = off)
     int32_t split_arrival;       // 1: the split keeps a sparse column's entry groups in LDS
                                  // and re-reads only its values to scatter
+    uint32_t ref_cap;            // sort path: reference slices of at most this many keys
+                                 // are sorted in LDS, larger ones in HBM (a power of two,
+                                 // at most kRefKeysF32 / kRefKeysF64)
     uint32_t min_n;              // a group with fewer observations raises "Sample too small"
                                  // (2; 0 for the numba pipeline, which has no such check)
     double cc;                   // continuity correction, 0.5 or 0

@@ -922,7 +922,8 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     // uniform decisions (kept in SGPRs so no barrier below sits under an exec mask)
     const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
-    const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
+    // the LDS/global sort cutover (a power of two, at most the on-chip capacity)
+    const uint32_t cap = uniform_u32(J.ref_cap);
     uint64_t ta = 0;
     uint32_t maxb = 0;   // keys in the fullest bucket of an on-chip reference
     // a reference slice beyond the on-chip sort: sorted in HBM with the
@@ -1382,7 +1383,8 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     // uniform decisions (kept in SGPRs so the barrier below is not under an exec mask)
     const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
-    const uint32_t cap = sizeof(K) == 4 ? (uint32_t)kRefKeysF32 : (uint32_t)kRefKeysF64;
+    // the LDS/global sort cutover (a power of two, at most the on-chip capacity)
+    const uint32_t cap = uniform_u32(J.ref_cap);
     // a reference beyond the on-chip sort: its sorted keys stay in HBM (read
     // through L2 by the lookups), its bucket index comes to LDS; with no spill
     // space the reference kernel raised the status

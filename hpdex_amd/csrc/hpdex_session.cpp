@@ -129,7 +129,7 @@ size_t session_job_footprint(const hpdex_job& j, const std::vector<uint32_t>& gr
                                        : ((const int64_t*)j.indptr)[c];
         indptr = ip.data();
     }
-    const size_t spill = spill_keys_bound(indptr, C, nnz, group_rows[0], f64);
+    const size_t spill = spill_keys_bound(indptr, C, nnz, group_rows[0], f64, tuning());
     return f + job_footprint(R, C, nnz, f64, j.n_targets, j.opts, group_rows.data(), false, spill);
 }
 

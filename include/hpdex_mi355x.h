@@ -304,6 +304,10 @@ typedef struct hpdex_tuning {
                                   entries fit) and reads only the values again to
                                   scatter them; 0 = counting and scattering passes
                                   both read rows, values and group labels           */
+    int32_t ref_lds_keys;      /* sort path, the LDS/global sort cutover: a column's
+                                  reference slice of at most this many values is
+                                  sorted in LDS, a larger one in HBM (a power of two
+                                  in [64, 8192]; 8-byte values: at most 4096 on chip) */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

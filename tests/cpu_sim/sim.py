@@ -42,6 +42,8 @@ def lib(path=None):
         L.sim_set_pack.restype = None
         L.sim_set_split_arrival.argtypes = [ci]
         L.sim_set_split_arrival.restype = None
+        L.sim_set_ref_cap.argtypes = [ci]
+        L.sim_set_ref_cap.restype = None
         L.sim_packed_blocks.restype = ci
         _lib = L
     return _lib
@@ -49,7 +51,7 @@ def lib(path=None):
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
         count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0,
-        shrink_range=False, pack=True, split_arrival=True):
+        shrink_range=False, pack=True, split_arrival=True, ref_cap=8192):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels)."""
     # laid out like a resident matrix: PAD padding entries (value 0, row -1)
@@ -67,6 +69,7 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     lib().sim_set_shrink_range(int(shrink_range))   # test hook: metadata that does not match
     lib().sim_set_pack(int(pack))   # packed entries for the count kernel where they fit
     lib().sim_set_split_arrival(int(split_arrival))
+    lib().sim_set_ref_cap(int(ref_cap))   # the LDS/global sort cutover of the reference
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,

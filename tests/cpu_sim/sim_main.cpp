@@ -31,6 +31,7 @@ int sim_shrink_range = 0;  // test hook: record each column's range one value sh
                            // count kernel meets values outside it (an error status)
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
 int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
+uint32_t sim_ref_cap = 8192; // the LDS/global sort cutover (tuning ref_lds_keys)
 int sim_last_packed = 0;   // count-kernel workgroups of the last call that read packed entries
 
 struct ThreadInfo { std::atomic<const char*>* last; };
@@ -224,9 +225,11 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.C = C; J.R = R; J.G = G; J.T = T; J.mode = mode; J.alt = alt; J.tie_corr = tie_corr;
     J.asym = 1; J.hist_max = hist_max; J.cc = cc ? 0.5 : 0.0; J.min_n = 2u;
     J.split_arrival = sim_split_arrival;
+    J.ref_cap = std::min<uint32_t>(f64 ? (uint32_t)kRefKeysF64 : (uint32_t)kRefKeysF32,
+                                   sim_ref_cap);
     // spill space of the columns whose reference slice may pass the on-chip
     // sort (job_upload_desc), for every column: any may take the sort path
-    const size_t cap = f64 ? (size_t)kRefKeysF64 : (size_t)kRefKeysF32;
+    const size_t cap = J.ref_cap;
     std::vector<int64_t> spill_off(C, -1);
     size_t spill_n = 0;
     for (int c = 0; c < C; ++c) {
@@ -253,3 +256,4 @@ extern "C" void sim_set_shrink_range(int on) { sim_shrink_range = on; }
 extern "C" void sim_set_pack(int on) { sim_pack = on; }
 extern "C" int sim_packed_blocks() { return sim_last_packed; }
 extern "C" void sim_set_split_arrival(int on) { sim_split_arrival = on; }
+extern "C" void sim_set_ref_cap(int keys) { sim_ref_cap = (uint32_t)keys; }

@@ -11,14 +11,17 @@ def test_candidates_default_space():
     c = autotune.candidates()
     # (2 histogram cutovers x 3 LDS sizes x 3 workgroup sizes, plus the sort
     # path alone, whose LDS and workgroup size do not matter) x 2 split modes
-    assert len(c) == (2 * 3 * 3 + 1) * 2
-    assert {"hist_max_bins": 0, "split_arrival": 1} in c
-    assert {"hist_max_bins": 0, "split_arrival": 0} in c
+    # x 3 LDS/global sort cutovers
+    assert len(c) == (2 * 3 * 3 + 1) * 2 * 3
+    assert {"hist_max_bins": 0, "split_arrival": 1, "ref_lds_keys": 8192} in c
+    assert {"hist_max_bins": 0, "split_arrival": 0, "ref_lds_keys": 512} in c
     assert len({tuple(sorted(x.items())) for x in c}) == len(c)
     for x in c:
         assert x["split_arrival"] in (0, 1)
+        assert x["ref_lds_keys"] in (8192, 2048, 512)
         if x["hist_max_bins"]:
-            assert set(x) == {"hist_max_bins", "count_lds_bytes", "count_block", "split_arrival"}
+            assert set(x) == {"hist_max_bins", "count_lds_bytes", "count_block", "split_arrival",
+                              "ref_lds_keys"}
             assert x["count_block"] in (256, 512, 1024)
             assert x["count_lds_bytes"] % 16 == 0 and 16384 <= x["count_lds_bytes"] <= 155648
 

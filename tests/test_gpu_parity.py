@@ -287,6 +287,37 @@ def test_reference_beyond_chip(f64, n_ref):
                 p.close(); m.close()
 
 
+@pytest.mark.parametrize("f64", [False, True], ids=["f32", "f64"])
+def test_reference_cutover_tuning(f64, restore_tuning):
+    """The LDS/global sort cutover as a tuning (ref_lds_keys 8192, 1024, 64):
+    reference slices above it are sorted in HBM and ranked through L2, the
+    rest on chip; results bit-identical across cutovers and equal to the
+    oracle, the tuning refused outside powers of two in [64, 8192]."""
+    say(f"reference cutover tuning f64={f64}")
+    rng = np.random.default_rng(43)
+    R, T = 12000, 30
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([1.0, 0.7, 0.3, 0.1, 0.02]))
+    data = np.round(rng.normal(0, 2, len(rows)), 2).astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:2500]] = 0
+    gid[perm[2500:]] = rng.integers(1, T + 1, R - 2500)
+    w = dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R)
+    o = oracle_stats(w, "min")
+    first = None
+    for cap in (8192, 1024, 64):
+        _capi.set_tuning(ref_lds_keys=cap)
+        g = gpu_stats(w, "min")
+        assert_parity(g, o)
+        if first is None:
+            first = g
+        for a, b in zip(g, first):
+            assert np.array_equal(a, b)
+    for bad in (0, 32, 100, 16384):
+        with pytest.raises(_capi.HpdexError, match="ref_lds_keys"):
+            _capi.set_tuning(ref_lds_keys=bad)
+
+
 @pytest.mark.parametrize("mode", ["min", "none"])
 def test_float_windows_large_reference(mode):
     """Small targets against a reference of ~5,800 stored values per column:
@@ -337,7 +368,7 @@ def restore_tuning():
     t = _capi.get_tuning()
     yield
     _capi.set_tuning(t.hist_max_bins, t.size_ordered, t.count_lds_bytes, t.count_block,
-                     t.count_packed, t.split_arrival)
+                     t.count_packed, t.split_arrival, t.ref_lds_keys)
 
 
 def _plan_of(w, mode, form=1, stats=True):

@@ -441,3 +441,22 @@ def test_sim_reference_beyond_chip(mode, f64):
     gid[perm[:11000]] = 0
     gid[perm[11000:]] = rng.integers(1, T + 1, R - 11000)
     run_both(data, rows, indptr, gid, T, R, mode, f64=f64)
+
+
+@pytest.mark.parametrize("cap,f64", [(64, False), (256, False), (128, True)],
+                         ids=["cap64-f32", "cap256-f32", "cap128-f64"])
+def test_sim_reference_cutover(cap, f64):
+    # the LDS/global sort cutover as a tuning (ref_lds_keys): reference
+    # slices of a few hundred values, above a low cutover, sorted in HBM and
+    # ranked through it, beside slices at or below it ranked from LDS; ties,
+    # negatives and targets both smaller and larger than 64 keys
+    rng = np.random.default_rng(41)
+    R, T = 3000, 12
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.5, 0.2, 0.05]))
+    data = np.round(rng.normal(0, 2, len(rows)), 1).astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:700]] = 0
+    gid[perm[700:]] = rng.integers(1, T + 1, R - 700)
+    for mode in ("min", "none"):
+        run_both(data, rows, indptr, gid, T, R, mode, f64=f64, ref_cap=cap)
