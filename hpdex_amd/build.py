@@ -101,6 +101,7 @@ def build(verbose: bool = False) -> None:
     s_obj = os.path.join(BUILD, "hpdex_session.o")
     d_obj = os.path.join(BUILD, "hpdex_de.o")
     i_obj = os.path.join(BUILD, "hpdex_ingest.o")
+    g_obj = os.path.join(BUILD, "hpdex_stage.o")
     outs = _run([
         [HIPCC, *HIP_FLAGS, RESOURCE_REMARKS, "-c", os.path.join(CSRC, "hpdex_kernels.hip"), "-o",
          k_obj],
@@ -109,6 +110,8 @@ def build(verbose: bool = False) -> None:
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "hpdex_api.cpp"), "-o", a_obj],
         [HIPCC, *HIP_FLAGS, "-pthread", "-x", "hip", "-c", os.path.join(CSRC, "hpdex_session.cpp"),
          "-o", s_obj],
+        [HIPCC, *HIP_FLAGS, "-pthread", "-x", "hip", "-c", os.path.join(CSRC, "hpdex_stage.cpp"),
+         "-o", g_obj],
     ])
     res = check_resources(outs[0])
     if verbose:
@@ -117,7 +120,7 @@ def build(verbose: bool = False) -> None:
               f"{worst[1].get('SGPRs Spill', 0)})")
     tmp = LIB + ".tmp"
     _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", k_obj, d_obj, i_obj, a_obj,
-           s_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so", "-Wl,--no-undefined"]])
+           s_obj, g_obj, "-o", tmp, "-Wl,-soname,libhpdex_mi355x.so", "-Wl,--no-undefined"]])
     os.replace(tmp, LIB)
 
     import pybind11  # noqa: WPS433 (build-time only)

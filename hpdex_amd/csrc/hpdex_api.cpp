@@ -119,19 +119,26 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     HIPCHK(m->indptr.alloc((C + 1) * 8));
     if (nnz) {
         if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
-            HIPCHK(hipMemcpyAsync(m->vals.p, data, nnz * vsz, hipMemcpyHostToDevice, s));
+            HIPCHK(upload_bytes(m->vals.p, data, nnz * vsz, s));
         } else {
             DevBuf raw;
             HIPCHK(raw.alloc(nnz * dtype_size(dtype)));
-            HIPCHK(hipMemcpyAsync(raw.p, data, nnz * dtype_size(dtype), hipMemcpyHostToDevice, s));
+            HIPCHK(upload_bytes(raw.p, data, nnz * dtype_size(dtype), s));
             HIPCHK(launch_convert_values(dtype, raw.p, m->vals.p, nnz, s));
             HIPCHK(hipStreamSynchronize(s));
         }
-        DevBuf idx64;
-        HIPCHK(idx64.alloc(nnz * 8));
-        HIPCHK(hipMemcpyAsync(idx64.p, indices, nnz * 8, hipMemcpyHostToDevice, s));
-        HIPCHK(launch_rows_i64_to_i32(idx64.as<int64_t>(), m->rows.as<int32_t>(), nnz,
-                                      (int64_t)R, s));
+        // the int64 rows become int32 on the host while they are staged (half
+        // the bytes over PCIe), or on the device after a plain copy
+        const hipError_t e = upload_rows_i64(m->rows.as<int32_t>(), indices, nnz, (int64_t)R, s);
+        if (e == hipErrorNotSupported) {
+            DevBuf idx64;
+            HIPCHK(idx64.alloc(nnz * 8));
+            HIPCHK(hipMemcpyAsync(idx64.p, indices, nnz * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(launch_rows_i64_to_i32(idx64.as<int64_t>(), m->rows.as<int32_t>(), nnz,
+                                          (int64_t)R, s));
+        } else {
+            HIPCHK(e);
+        }
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (C + 1) * 8, hipMemcpyHostToDevice, s));
@@ -220,7 +227,7 @@ int widen_indptr(const void* p, int bits, size_t n, size_t nnz, std::vector<int6
 static hipError_t upload_raw(DevBuf& b, const void* host, size_t bytes, hipStream_t s) {
     hipError_t e = b.alloc(bytes);
     if (e != hipSuccess || bytes == 0) return e;
-    return hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s);
+    return upload_bytes(b.p, host, bytes, s);
 }
 
 int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t s, char* err,
@@ -258,14 +265,14 @@ int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t
         HIPCHK(alloc_out(nnz));
         if (nnz) {
             if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
-                HIPCHK(hipMemcpyAsync(m->vals.p, src->data, nnz * vsz, hipMemcpyHostToDevice, s));
+                HIPCHK(upload_bytes(m->vals.p, src->data, nnz * vsz, s));
             } else {
                 DevBuf raw;
                 HIPCHK(upload_raw(raw, src->data, nnz * dtype_size(dtype), s));
                 HIPCHK(launch_convert_values(dtype, raw.p, m->vals.p, nnz, s));
                 HIPCHK(hipStreamSynchronize(s));
             }
-            HIPCHK(hipMemcpyAsync(m->rows.p, src->indices, nnz * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(upload_bytes(m->rows.p, src->indices, nnz * 4, s));
             HIPCHK(launch_rows_i32_check(m->rows.as<int32_t>(), nnz, (int64_t)R, s));
         }
         HIPCHK(hipMemcpyAsync(m->indptr.p, m->h_indptr.data(), (C + 1) * 8, hipMemcpyHostToDevice, s));
@@ -662,11 +669,11 @@ int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1,
         return set_err(err, err_len,
                        "the tie term and sample sizes are kept only by plans created with "
                        "HPDEX_PLAN_STATS");
-    if (U1 && b) HIPCHK(hipMemcpyAsync(U1, js.U1.p, b, hipMemcpyDeviceToHost, s));
-    if (P && b) HIPCHK(hipMemcpyAsync(P, js.P.p, b, hipMemcpyDeviceToHost, s));
-    if (tie && b) HIPCHK(hipMemcpyAsync(tie, js.tie.p, b, hipMemcpyDeviceToHost, s));
-    if (n1 && b) HIPCHK(hipMemcpyAsync(n1, js.n1.p, b, hipMemcpyDeviceToHost, s));
-    if (n2 && b) HIPCHK(hipMemcpyAsync(n2, js.n2.p, b, hipMemcpyDeviceToHost, s));
+    if (U1 && b) HIPCHK(download_bytes(U1, js.U1.p, b, s));
+    if (P && b) HIPCHK(download_bytes(P, js.P.p, b, s));
+    if (tie && b) HIPCHK(download_bytes(tie, js.tie.p, b, s));
+    if (n1 && b) HIPCHK(download_bytes(n1, js.n1.p, b, s));
+    if (n2 && b) HIPCHK(download_bytes(n2, js.n2.p, b, s));
     HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
@@ -976,6 +983,11 @@ int hpdex_mwu_csc_noalloc(int dtype, const void* data, const int64_t* indices,
     if (n_cols == 0) return 0;
     if (ensure_init(err, err_len)) return 1;
     std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    // host copies without pinned staging (its buffers are allocations too)
+    struct PlainCopies {
+        PlainCopies() { t_plain_copies = true; }
+        ~PlainCopies() { t_plain_copies = false; }
+    } plain_copies;
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= kNoAllocMaxDevices)
@@ -1135,11 +1147,11 @@ static int de_run(hpdex_matrix& m, const int32_t* group_id, size_t n_targets, in
     HIPCHK(ws.alloc(wsb));
     HIPCHK(launch_fdr_bh(js.P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, s));
     const size_t b = np * 8;
-    if (U1_out) HIPCHK(hipMemcpyAsync(U1_out, js.U1.p, b, hipMemcpyDeviceToHost, s));
-    if (P_out) HIPCHK(hipMemcpyAsync(P_out, js.P.p, b, hipMemcpyDeviceToHost, s));
-    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, s));
-    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, s));
-    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, s));
+    if (U1_out) HIPCHK(download_bytes(U1_out, js.U1.p, b, s));
+    if (P_out) HIPCHK(download_bytes(P_out, js.P.p, b, s));
+    if (fc_out) HIPCHK(download_bytes(fc_out, fc.p, b, s));
+    if (log2fc_out) HIPCHK(download_bytes(log2fc_out, lfc.p, b, s));
+    if (fdr_out) HIPCHK(download_bytes(fdr_out, fdr.p, b, s));
     HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
@@ -1313,11 +1325,11 @@ static int pden_run(hpdex_matrix& m, const int32_t* group_id, size_t T, const hp
     HIPCHK(ws.alloc(wsb ? wsb : 16));
     HIPCHK(launch_fdr_bh(P.as<double>(), (int64_t)np, fdr.as<double>(), ws.p, wsb, s));
     const size_t b = np * 8;
-    if (U_out) HIPCHK(hipMemcpyAsync(U_out, U.p, b, hipMemcpyDeviceToHost, s));
-    if (P_out) HIPCHK(hipMemcpyAsync(P_out, P.p, b, hipMemcpyDeviceToHost, s));
-    if (fc_out) HIPCHK(hipMemcpyAsync(fc_out, fc.p, b, hipMemcpyDeviceToHost, s));
-    if (log2fc_out) HIPCHK(hipMemcpyAsync(log2fc_out, lfc.p, b, hipMemcpyDeviceToHost, s));
-    if (fdr_out) HIPCHK(hipMemcpyAsync(fdr_out, fdr.p, b, hipMemcpyDeviceToHost, s));
+    if (U_out) HIPCHK(download_bytes(U_out, U.p, b, s));
+    if (P_out) HIPCHK(download_bytes(P_out, P.p, b, s));
+    if (fc_out) HIPCHK(download_bytes(fc_out, fc.p, b, s));
+    if (log2fc_out) HIPCHK(download_bytes(log2fc_out, lfc.p, b, s));
+    if (fdr_out) HIPCHK(download_bytes(fdr_out, fdr.p, b, s));
     HIPCHK(hipStreamSynchronize(s));
     return 0;
 }

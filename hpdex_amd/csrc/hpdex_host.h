@@ -71,6 +71,19 @@ struct BumpBlock {
 };
 extern thread_local BumpBlock* t_block;
 
+// Host-to-device copies of caller memory (hpdex_stage.cpp): large copies go
+// through pinned staging buffers filled by host threads.  t_plain_copies
+// (set by the calls that may allocate nothing) makes them plain
+// hipMemcpyAsync calls; upload_rows_i64 then returns hipErrorNotSupported
+// and the caller converts the rows on the device.
+extern thread_local bool t_plain_copies;
+hipError_t upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+// device to host, returning when the bytes are in dst (the stream's earlier
+// work included)
+hipError_t download_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+// int64 row indices to int32 on the way (rows outside [0, R) become -1)
+hipError_t upload_rows_i64(int32_t* dst, const int64_t* src, size_t n, int64_t R, hipStream_t s);
+
 struct DevBuf {
     void* p = nullptr;
     size_t n = 0;
