@@ -33,7 +33,8 @@ EXPORTED = (
     "hpdex_matrix_shape", "hpdex_matrix_download", "hpdex_de_matrix",
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
     "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
-    "hpdex_session_batches", "hpdex_pden_matrix",
+    "hpdex_session_batches", "hpdex_pden_matrix", "hpdex_matrix_ref_cache_info",
+    "hpdex_matrix_ref_cache_clear",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -62,13 +63,13 @@ class PdenOpts(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("hist_max_bins", "size_ordered", "count_lds_bytes",
                                               "count_block", "count_packed", "split_arrival",
-                                              "ref_lds_keys")]
+                                              "ref_lds_keys", "ref_cache")]
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "count_columns", "count_nnz", "split_columns", "handed_over", "count_blocks",
-        "count_packed_columns")]
+        "count_packed_columns", "ref_cached_columns")]
 
 
 class SessionOpts(ctypes.Structure):
@@ -119,6 +120,10 @@ def lib() -> ctypes.CDLL:
         L.hpdex_matrix_destroy.restype = None
         L.hpdex_matrix_device_bytes.argtypes = [vp]
         L.hpdex_matrix_device_bytes.restype = sz
+        L.hpdex_matrix_ref_cache_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                                  ctypes.POINTER(sz)]
+        L.hpdex_matrix_ref_cache_clear.argtypes = [vp]
+        L.hpdex_matrix_ref_cache_clear.restype = None
         L.hpdex_plan_create.argtypes = [vp, vp, sz, ctypes.POINTER(MwuOpts), ci,
                                         ctypes.POINTER(vp), cp, sz]
         L.hpdex_plan_destroy.argtypes = [vp]
@@ -437,6 +442,16 @@ class DeviceMatrix:
     def device_bytes(self) -> int:
         return int(lib().hpdex_matrix_device_bytes(self._h))
 
+    def ref_cache_info(self) -> dict:
+        """The reference tables the matrix keeps: columns with tables and
+        their device bytes (both 0 when it keeps none)."""
+        cols, nbytes = ctypes.c_int64(), ctypes.c_size_t()
+        lib().hpdex_matrix_ref_cache_info(self._h, ctypes.byref(cols), ctypes.byref(nbytes))
+        return {"columns": int(cols.value), "bytes": int(nbytes.value)}
+
+    def ref_cache_clear(self):
+        lib().hpdex_matrix_ref_cache_clear(self._h)
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().hpdex_matrix_destroy(self._h)
@@ -566,8 +581,10 @@ def get_tuning() -> Tuning:
 
 
 def set_tuning(hist_max_bins=None, size_ordered=None, count_lds_bytes=None, count_block=None,
-               count_packed=None, split_arrival=None, ref_lds_keys=None):
+               count_packed=None, split_arrival=None, ref_lds_keys=None, ref_cache=None):
     t = get_tuning()
+    if ref_cache is not None:
+        t.ref_cache = int(ref_cache)
     if ref_lds_keys is not None:
         t.ref_lds_keys = int(ref_lds_keys)
     if split_arrival is not None:

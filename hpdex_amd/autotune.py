@@ -58,7 +58,7 @@ def space_for(matrix=None) -> dict:
     return dict(DEFAULT_SPACE)
 
 KNOBS = ("hist_max_bins", "size_ordered", "count_lds_bytes", "count_block", "count_packed",
-         "split_arrival", "ref_lds_keys")
+         "split_arrival", "ref_lds_keys", "ref_cache")
 
 
 class TuningMismatch(_capi.HpdexError):
@@ -125,7 +125,10 @@ def autotune(matrix, group_id, n_targets, opts, space=None, iters=20, warmup=3, 
     trials, ref, timed = [], None, {}
     try:
         for cand in cands:
-            cur = {**base, **cand}
+            # every candidate sorts its own reference: a matrix's kept tables
+            # (filled by the first candidate's run) would time later
+            # candidates without that sort
+            cur = {**base, **cand, "ref_cache": 0}
             try:
                 # a tuning the library refuses (e.g. LDS this device does not
                 # give) is a candidate that cannot run, not a failed sweep

@@ -325,6 +325,78 @@ int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t
     return set_err(err, err_len, "unknown layout");
 }
 
+std::vector<uint64_t> reference_rows(const int32_t* group_id, size_t R) {
+    std::vector<uint64_t> bits((R + 63) / 64, 0ull);
+    for (size_t r = 0; r < R; ++r)
+        if (group_id[r] == 0) bits[r >> 6] |= 1ull << (r & 63);
+    return bits;
+}
+
+namespace {
+// DevBuf allocations in this scope take device memory of their own
+// (hipMalloc) even on a session worker, whose job buffers come from its block.
+struct OwnAllocations {
+    BumpBlock* saved = t_block;
+    OwnAllocations() { t_block = nullptr; }
+    ~OwnAllocations() { t_block = saved; }
+};
+}  // namespace
+
+int refcache_fill(JobState& js, const hpdex_matrix& m, hipStream_t s, char* err, size_t err_len) {
+    if (!js.rc_fill) return 0;
+    js.rc_fill = false;
+    unsigned long long h[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, js.flags.p, 24, hipMemcpyDeviceToHost, s));
+    std::vector<ColPlan> plans(js.C);
+    if (js.C) HIPCHK(hipMemcpyAsync(plans.data(), js.plan.p, js.C * sizeof(ColPlan),
+                                    hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h[0] != ~0ull || (uint32_t)h[2] != 0u) return 0;   // a failed job leaves no tables
+    auto e = std::make_shared<RefCacheEntry>();
+    e->ref_rows = js.ref_rows;
+    e->valid.assign(js.C, 0);
+    std::vector<RefCacheCol> cols(js.C);
+    std::memset(cols.data(), 0, cols.size() * sizeof(RefCacheCol));
+    std::vector<int32_t> list;
+    int64_t koff = 0, toff = 0;
+    for (const WorkItem& wi : js.h_gen_items) {
+        const ColPlan& pl = plans[wi.col];
+        if (pl.hist) continue;   // histogram tables depend on the targets' values too
+        RefCacheCol& rc = cols[wi.col];
+        rc.koff = koff;
+        rc.toff = toff;
+        rc.TA = pl.TA;
+        rc.na = pl.na;
+        rc.nb = pl.nb;
+        koff += pl.na;
+        toff += (int64_t)pl.nb + 1;
+        list.push_back(wi.col);
+        e->valid[wi.col] = 1;
+    }
+    if (list.empty()) return 0;
+    const size_t ksz = m.f64 ? 8 : 4;
+    DevBuf d_list;
+    {
+        OwnAllocations own;
+        HIPCHK(e->cols.alloc(js.C * sizeof(RefCacheCol)));
+        HIPCHK(e->keys.alloc((size_t)koff * ksz));
+        HIPCHK(e->tab.alloc((size_t)toff * 4));
+        HIPCHK(d_list.alloc(list.size() * 4));
+    }
+    HIPCHK(hipMemcpyAsync(e->cols.p, cols.data(), js.C * sizeof(RefCacheCol),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_refcache_fill(js.d_job.as<JobDesc>(), m.f64, d_list.as<int32_t>(),
+                                (uint32_t)list.size(), e->cols.as<RefCacheCol>(), e->keys.p,
+                                e->tab.as<uint32_t>(), s));
+    HIPCHK(hipStreamSynchronize(s));
+    e->n_cols = (int64_t)list.size();
+    e->bytes = e->cols.n + e->keys.n + e->tab.n;
+    std::lock_guard<std::mutex> lk(m.rc_mu);
+    m.rc = std::move(e);
+    return 0;
+}
+
 int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size_t T,
               const hpdex_mwu_opts* o, bool stats, hipStream_t s, char* err, size_t err_len) {
     if (T < 1) return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
@@ -335,6 +407,11 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     js.opts = *o;
     js.stats = stats;
     js.group_rows.assign(js.G, 0u);
+    js.rc_use.reset();
+    js.rc_fill = false;
+    js.rc_cols = 0;
+    js.ref_rows.clear();
+    if (m->resident) js.ref_rows = reference_rows(group_id, m->R);
     for (size_t r = 0; r < m->R; ++r) {
         const int g = group_id[r];
         if (g >= 0 && (size_t)g < js.G) ++js.group_rows[g];
@@ -485,6 +562,24 @@ int job_upload_desc(JobState& js, const JobDesc& J, const hpdex_matrix& m, const
         js.J.goff = js.goff.as<uint32_t>();
         js.J.plan = js.plan.as<ColPlan>();
         js.J.reftab = js.reftab.as<uint32_t>();
+        // the matrix's reference tables for these reference rows, if it has
+        // them; otherwise this job fills them after its first run
+        if (tu.ref_cache && m.resident) {
+            std::shared_ptr<RefCacheEntry> e;
+            {
+                std::lock_guard<std::mutex> lk(m.rc_mu);
+                e = m.rc;
+            }
+            if (e && e->ref_rows == js.ref_rows) {
+                js.J.rc = e->cols.as<RefCacheCol>();
+                js.J.rc_keys = e->keys.p;
+                js.J.rc_tab = e->tab.as<uint32_t>();
+                for (const WorkItem& wi : gen_items) js.rc_cols += e->valid[wi.col];
+                js.rc_use = std::move(e);
+            } else {
+                js.rc_fill = true;
+            }
+        }
     }
     // spill space for the sort-path columns whose reference slice may pass
     // the on-chip sort (its stored values: at most the column's entries and
@@ -839,6 +934,7 @@ int hpdex_get_tuning(hpdex_tuning* out) {
     out->count_packed = t.count_packed;
     out->split_arrival = t.split_arrival;
     out->ref_lds_keys = t.ref_lds_keys;
+    out->ref_cache = t.ref_cache;
     return 0;
 }
 
@@ -862,6 +958,8 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     if (t->ref_lds_keys < 64 || t->ref_lds_keys > kRefKeysF32 ||
         (t->ref_lds_keys & (t->ref_lds_keys - 1)) != 0)
         return set_err(err, err_len, "ref_lds_keys must be a power of two in [64, 8192]");
+    if (t->ref_cache != 0 && t->ref_cache != 1)
+        return set_err(err, err_len, "ref_cache must be 0 or 1");
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init && !g_count_ready && t->count_lds_bytes != 0)
         return set_err(err, err_len, "this device does not give the count kernel its LDS");
@@ -872,6 +970,7 @@ int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len) {
     g_tune.count_packed = t->count_packed;
     g_tune.split_arrival = t->split_arrival;
     g_tune.ref_lds_keys = t->ref_lds_keys;
+    g_tune.ref_cache = t->ref_cache;
     return 0;
 }
 
@@ -889,7 +988,8 @@ int mwu_oneshot(int dtype, const void* data, const int64_t* indices, const int64
                 size_t err_len) {
     const Tuning tu = tuning();
     hpdex_matrix m;
-    m.pack = false;   // used once: no packed entries
+    m.pack = false;   // used once: no packed entries, no reference tables kept
+    m.resident = false;
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, s, err, err_len))
         return 1;
     JobState js;
@@ -1061,7 +1161,8 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
-    m.pack = false;   // used once: no packed entries
+    m.pack = false;   // used once: no packed entries, no reference tables kept
+    m.resident = false;
 
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
@@ -1179,7 +1280,8 @@ int hpdex_de_csc(int dtype, const void* data, const int64_t* indices, const int6
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
-    m.pack = false;   // used once: no packed entries
+    m.pack = false;   // used once: no packed entries, no reference tables kept
+    m.resident = false;
 
     if (upload_matrix(&m, dtype, data, indices, indptr, n_rows, n_cols, nnz, st.s, err, err_len))
         return 1;
@@ -1426,6 +1528,24 @@ size_t hpdex_de_csc_device_bytes(int dtype, size_t n_rows, size_t n_cols, size_t
     return de_footprint(dtype, n_rows, n_cols, nnz, n_targets);
 }
 
+int hpdex_matrix_ref_cache_info(const hpdex_matrix* m, int64_t* columns, size_t* bytes) {
+    if (!m) return 1;
+    std::shared_ptr<RefCacheEntry> e;
+    {
+        std::lock_guard<std::mutex> lk(m->rc_mu);
+        e = m->rc;
+    }
+    if (columns) *columns = e ? e->n_cols : 0;
+    if (bytes) *bytes = e ? e->bytes : 0;
+    return 0;
+}
+
+void hpdex_matrix_ref_cache_clear(hpdex_matrix* m) {
+    if (!m) return;
+    std::lock_guard<std::mutex> lk(m->rc_mu);
+    m->rc.reset();
+}
+
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
     if (!m) return 0;
     return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n + m->colident.n + m->packed.n;
@@ -1526,6 +1646,7 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
     out->count_nnz = p->js.count_nnz;
     out->split_columns = p->js.n_gen_static;
     out->handed_over = 0;   // the count kernel reports a column it cannot take as an error
+    out->ref_cached_columns = p->js.rc_cols;
     return 0;
 }
 
@@ -1599,6 +1720,8 @@ int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms
         if (job_errors(p->js, s, err, err_len)) return 1;
         rc = run_exact_none(p->js, s, err, err_len);
     }
+    // the matrix keeps this job's sorted reference after its first run
+    if (!rc) rc = refcache_fill(p->js, *p->m, s, err, err_len);
     return rc;
 }
 
@@ -1814,6 +1937,9 @@ int hpdex_batch_run(hpdex_batch* b, int iters, double* ms_total, char* err, size
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    // each plan's matrix keeps its sorted reference after the first run
+    for (hpdex_plan* p : b->plans)
+        if (!rc) rc = refcache_fill(p->js, *p->m, p->st.s, err, err_len);
     return rc;
 }
 

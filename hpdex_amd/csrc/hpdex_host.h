@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -35,6 +36,7 @@ struct Tuning {
     int count_packed = 1;
     int split_arrival = 1;
     int ref_lds_keys = kRefKeysF32;
+    int ref_cache = 1;
 };
 
 extern std::mutex g_mu;
@@ -119,6 +121,23 @@ struct Stream {
     hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
 };
 
+// A resident matrix's reference tables for one set of reference rows
+// (RefCacheCol per column, the sorted keys and the bucket indexes), copied
+// from a finished job and read by later jobs with the same reference rows.
+// Immutable once installed; a job that reads it holds a reference, so a
+// replaced entry stays on the device until its last reader ends.  Its
+// buffers are the matrix's (hipMalloc'd), not a job's.
+struct RefCacheEntry {
+    std::vector<uint64_t> ref_rows;   // bit r: row r is in the reference group
+    std::vector<uint8_t> valid;       // host copy: column c has tables
+    DevBuf cols, keys, tab;
+    int64_t n_cols = 0;               // columns with tables
+    size_t bytes = 0;
+};
+
+// Bitmask of the rows of group 0 (the reference) in R labels.
+std::vector<uint64_t> reference_rows(const int32_t* group_id, size_t R);
+
 bool valid_dtype(int d);
 size_t dtype_size(int d);
 int check_indptr(const int64_t* indptr, size_t C, size_t nnz, char* err, size_t err_len);
@@ -145,6 +164,11 @@ struct hpdex_matrix {
     std::vector<uint8_t> col_packed;   // 1: column c has packed entries
     std::vector<uint32_t> h_dense;     // packed columns: distinct nonzero values (their bins)
     std::vector<uint32_t> h_zneg;      // packed columns: distinct values below zero
+    // the sorted reference of the last sort-path job whose reference rows had
+    // no tables yet (resident matrices only: those created through the C-ABI)
+    bool resident = true;
+    mutable std::mutex rc_mu;
+    mutable std::shared_ptr<hpdex_host::RefCacheEntry> rc;
 };
 
 namespace hpdex_host {
@@ -176,6 +200,12 @@ struct JobState {
     DevBuf d_tabs, tabbuf, tscratch, d_tidx;
     size_t scratch_per_table = 0;
     bool exact_by_pair = false;
+    // the matrix's reference tables this job reads (rc_use), or, when it has
+    // none for these reference rows, a fill of them after the job's first run
+    std::shared_ptr<RefCacheEntry> rc_use;
+    std::vector<uint64_t> ref_rows;
+    bool rc_fill = false;
+    int64_t rc_cols = 0;   // sort-path columns whose tables the cache holds
 
     unsigned long long* err_key() const { return flags.as<unsigned long long>(); }
     unsigned long long* status_col() const { return flags.as<unsigned long long>() + 1; }
@@ -205,6 +235,11 @@ int copy_out(const JobState& js, double* U1, double* P, double* tie, double* n1,
              hipStream_t s, char* err, size_t err_len);
 bool opts_ok(const hpdex_mwu_opts* o);
 int run_exact_none(JobState& js, hipStream_t s, char* err, size_t err_len);
+// After a job's launch sequence has completed on s: when the job was set up
+// to fill its matrix's reference cache (js.rc_fill), copy its sort-path
+// columns' reference tables into a new entry (device memory of the matrix)
+// and install it.  The fill is skipped when the job recorded an error.
+int refcache_fill(JobState& js, const hpdex_matrix& m, hipStream_t s, char* err, size_t err_len);
 
 // Device bytes, as BumpBlock::take rounds every DevBuf (256 bytes, at least
 // one 256-byte unit), of:

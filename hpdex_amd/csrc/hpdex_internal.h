@@ -83,6 +83,22 @@ struct ColPlan {
                            // left by the window pass to the sorted pass
 };
 
+// A resident matrix's cached reference tables, per column (sort path, not the
+// histogram branch): the sorted reference keys at rc_keys[koff ..
+// koff + na), the bucket index at rc_tab[toff .. toff + nb], the reference's
+// own tie sum and its fullest bucket.  They depend only on the column's values
+// and the set of reference rows, so a later job on the same matrix with the
+// same reference rows reads them instead of sorting the reference again.
+struct RefCacheCol {
+    int64_t koff;
+    int64_t toff;
+    uint64_t TA;
+    uint32_t na;
+    uint32_t nb;
+    uint32_t maxb;
+    uint32_t valid;   // 1: the entries above hold this column's tables
+};
+
 // One Mann-Whitney job: a device-resident CSC matrix, its group labels and
 // options, the per-job workspace of the kernel pipeline, and its outputs.
 struct JobDesc {
@@ -108,6 +124,11 @@ struct JobDesc {
     void* spill;                 // keys (as `keys`) for reference slices beyond the on-chip
                                  // sort: the merge passes' second buffer
     const int64_t* spill_off;    // C: column c's first spill key, -1 = none (or null)
+    // the matrix's cached reference tables for this job's reference rows (or
+    // null): a valid column skips the reference sort and is ranked from them
+    const RefCacheCol* rc;
+    const void* rc_keys;
+    const uint32_t* rc_tab;
     // outputs (gene-major, c*T + g-1); tie, n1o and n2o may be null (not kept)
     double* U1;
     double* P;
@@ -362,6 +383,12 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s);
 // count, split, reference and target kernels, for per-kernel timing.
 hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,
                                hipEvent_t* ev = nullptr);
+// Copy a finished job's reference tables of columns cols[0 .. n) into a
+// matrix's cache (rc[c] holds their offsets, na, nb and TA; the kernel adds
+// the fullest bucket and marks the entry valid).  `job` is the job's
+// descriptor on the device; it must have run to completion on s.
+hipError_t launch_refcache_fill(const JobDesc* job, bool f64, const int32_t* cols, uint32_t n,
+                                RefCacheCol* rc, void* rc_keys, uint32_t* rc_tab, hipStream_t s);
 // Allow the count kernel `bytes` of dynamic LDS and the split kernel
 // kSplitLdsMax (once per process).
 hipError_t prepare_count_kernel(uint32_t bytes);

@@ -905,6 +905,15 @@ __device__ inline uint32_t fullest_bucket(const uint32_t* bidx, uint32_t nb, Ref
     return st.maxb;
 }
 
+// Whether column c's reference tables come from the matrix's cache (the same
+// reference rows give the same count and bucket count: checked, so a stale
+// entry is never used).  Uniform over the workgroup.
+__device__ __forceinline__ bool ref_cached(const JobDesc& J, int64_t c, uint32_t na, uint32_t nb) {
+    if (!J.rc) return false;
+    const RefCacheCol& e = J.rc[c];
+    return uniform_u32(e.valid) == 1u && uniform_u32(e.na) == na && uniform_u32(e.nb) == nb;
+}
+
 template <typename V>
 __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using KT = KeyOf<V>;
@@ -924,6 +933,14 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
     // the LDS/global sort cutover (a power of two, at most the on-chip capacity)
     const uint32_t cap = uniform_u32(J.ref_cap);
+    // tables the matrix keeps for these reference rows: nothing to sort
+    if (!hist && ref_cached(J, c, na, pl.nb)) {
+        if (tid == 0) {
+            J.plan[c].TA = J.rc[c].TA;
+            J.plan[c].maxb = J.rc[c].maxb;
+        }
+        return;
+    }
     uint64_t ta = 0;
     uint32_t maxb = 0;   // keys in the fullest bucket of an on-chip reference
     // a reference slice beyond the on-chip sort: sorted in HBM with the
@@ -1003,6 +1020,42 @@ template <typename V>
 __global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
                                                    uint32_t n_items) {
     if (blockIdx.x < n_items) ref_item<V>(jobs, items, blockIdx.x);
+}
+
+// Keep a finished job's reference tables in a matrix's cache: workgroup b
+// copies column cols[b]'s sorted reference keys (group 0's keys at the start
+// of the column's key range, sorted there by the reference kernel) and its
+// bucket index to the offsets the host placed in rc[c], then records the
+// fullest bucket and marks the entry valid.  The host filled koff, toff, na,
+// nb and TA from the job's column plans.
+template <typename K>
+__global__ __launch_bounds__(kColBlock) void refcache_fill_kernel(const JobDesc* job,
+                                                             const int32_t* cols, uint32_t n,
+                                                             RefCacheCol* rc, K* rc_keys,
+                                                             uint32_t* rc_tab) {
+    __shared__ uint32_t maxb;
+    if (blockIdx.x >= n) return;
+    const uint32_t tid = (uint32_t)tid_x(), nt = blockDim.x;
+    const int64_t c = cols[blockIdx.x];
+    const RefCacheCol e = rc[c];
+    const K* src = (const K*)job->keys + job->indptr[c];
+    const uint32_t* tab = job->reftab + (size_t)c * kTabStride;
+    for (uint32_t x = tid; x < e.na; x += nt) rc_keys[e.koff + x] = src[x];
+    if (tid == 0) maxb = 0u;
+    __syncthreads();
+    uint32_t m = 0;
+    for (uint32_t b = tid; b <= e.nb; b += nt) {
+        const uint32_t lo = tab[b];
+        rc_tab[e.toff + b] = lo;
+        if (b < e.nb) m = max(m, tab[b + 1] - lo);
+    }
+    m = wave_max_u32(m);
+    if (lane_id() == 0) atomicMax(&maxb, m);
+    __syncthreads();
+    if (tid == 0) {
+        rc[c].maxb = maxb;
+        rc[c].valid = 1u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1379,17 +1432,22 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     const ColPlan pl = J.plan[c];
     // the sorted pass of 4-byte keys: only columns the window pass flagged
     if (!kWindows && sizeof(K) == 4 && uniform_u32(pl.bigt) == 0u) return;
-    const uint32_t* tab = J.reftab + (size_t)c * kTabStride;
     // uniform decisions (kept in SGPRs so the barrier below is not under an exec mask)
     const uint32_t na = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.na);
     const uint32_t hist = (uint32_t)__builtin_amdgcn_readfirstlane((int)pl.hist);
     // the LDS/global sort cutover (a power of two, at most the on-chip capacity)
     const uint32_t cap = uniform_u32(J.ref_cap);
+    // the sorted reference and its table: the column's own (the reference
+    // kernel sorted group 0's keys in place), or the matrix's cached ones
+    const bool cached = !hist && ref_cached(J, c, na, pl.nb);
+    const uint32_t* tab = cached ? J.rc_tab + J.rc[c].toff : J.reftab + (size_t)c * kTabStride;
+    const K* rbase = cached ? (const K*)J.rc_keys + J.rc[c].koff
+                            : (const K*)J.keys + gl(J.indptr)[c];
     // a reference beyond the on-chip sort: its sorted keys stay in HBM (read
     // through L2 by the lookups), its bucket index comes to LDS; with no spill
     // space the reference kernel raised the status
     const bool big = !hist && na > cap;
-    if (big && uniform_i64(J.spill_off ? gl(J.spill_off)[c] : -1) < 0) return;
+    if (big && !cached && uniform_i64(J.spill_off ? gl(J.spill_off)[c] : -1) < 0) return;
     RefView<K> rv;
     rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = hist;
     rv.big = big ? 1u : 0u; rv.nb = pl.nb; rv.iters = 0; rv.lkeys = nullptr;
@@ -1400,11 +1458,11 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     } else if (big) {
         uint32_t* bidx = (uint32_t*)((K*)smem + cap);
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = gl(tab)[b];
-        rv.cum = nullptr; rv.keys = (const K*)J.keys + gl(J.indptr)[c]; rv.bidx = bidx;
+        rv.cum = nullptr; rv.keys = rbase; rv.bidx = bidx;
     } else {
         K* rk = (K*)smem;
         uint32_t* bidx = (uint32_t*)(rk + cap);
-        const K* base = (const K*)J.keys + gl(J.indptr)[c];
+        const K* base = rbase;
         for (uint32_t x = tid; x < na; x += nt) rk[x] = gl(base)[x];
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = gl(tab)[b];
         rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
@@ -2403,6 +2461,18 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
         if (L.f64) sort_path(TypeTag<double>{});
         else sort_path(TypeTag<float>{});
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_refcache_fill(const JobDesc* job, bool f64, const int32_t* cols, uint32_t n,
+                                RefCacheCol* rc, void* rc_keys, uint32_t* rc_tab, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (f64)
+        refcache_fill_kernel<uint64_t><<<n, kColBlock, 0, s>>>(job, cols, n, rc,
+                                                                (uint64_t*)rc_keys, rc_tab);
+    else
+        refcache_fill_kernel<uint32_t><<<n, kColBlock, 0, s>>>(job, cols, n, rc,
+                                                                (uint32_t*)rc_keys, rc_tab);
     return hipGetLastError();
 }
 

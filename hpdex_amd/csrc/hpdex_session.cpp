@@ -199,6 +199,7 @@ void run_job(hpdex_session* S, JobRec& R, hipStream_t s, char* block, size_t cap
             if (!m) {
                 own.reset(new hpdex_matrix());
                 own->pack = false;   // the job's own matrix serves this job only
+                own->resident = false;
                 const hpdex_matrix_src src = source_of(j);
                 rc = upload_matrix_from(own.get(), &src, s, err, err_len);
                 if (rc) break;
@@ -244,6 +245,10 @@ void run_job(hpdex_session* S, JobRec& R, hipStream_t s, char* block, size_t cap
             rc = job_errors(js, s, err, err_len);
             if (rc) break;
             rc = copy_out(js, j.U1_out, j.P_out, nullptr, nullptr, nullptr, s, err, err_len);
+            if (rc) break;
+            // a resident matrix keeps this job's sorted reference for later
+            // jobs with the same reference rows
+            rc = refcache_fill(js, *m, s, err, err_len);
         } while (0);
         // nothing of this job may still run when its block goes back to the arena
         const hipError_t se = hipStreamSynchronize(s);

@@ -212,8 +212,25 @@ int hpdex_pden_matrix(hpdex_matrix* m, const int32_t* group_id, size_t n_targets
                       const hpdex_pden_opts* o, double* U_out, double* P_out, double* fc_out,
                       double* log2fc_out, double* fdr_out, char* err, size_t err_len);
 void hpdex_matrix_destroy(hpdex_matrix* m);
-/* Bytes of HBM the resident matrix occupies. */
+/* Bytes of HBM the resident matrix occupies (its reference tables below not
+ * included). */
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
+/* Reference tables kept by a resident matrix.  No counterpart in the
+ * reference, which sorts each gene's reference slice once per call
+ * (src/cpp/mannwhitneyu.cpp:928-929).  After the first run of a plan (or a
+ * session job) on the matrix whose reference rows -- the rows labelled 0 --
+ * had no tables yet, the matrix keeps the sorted reference values and bucket
+ * index of every column the sort path ranked (not the histogram branch), and
+ * later jobs with the same reference rows skip the reference sort for those
+ * columns: a new target set against the same reference.  One set of
+ * reference rows is kept per matrix (the latest replaces the one before; a
+ * job still reading the old one keeps it alive), in device memory of its own
+ * (not a session's arena), until the matrix is destroyed or the tables are
+ * cleared.  hpdex_tuning.ref_cache = 0 turns keeping and using them off.
+ * Results are the same with and without them, bit for bit.
+ * columns / bytes: columns with tables and their device bytes (0 if none). */
+int hpdex_matrix_ref_cache_info(const hpdex_matrix* m, int64_t* columns, size_t* bytes);
+void hpdex_matrix_ref_cache_clear(hpdex_matrix* m);
 
 /* A job on a resident matrix: group labels and options fixed, outputs held on
  * the device.  form: HPDEX_PLAN_PLAIN (the kernel sequence launched eagerly
@@ -255,6 +272,8 @@ typedef struct hpdex_plan_info {
     int64_t handed_over;
     int64_t count_blocks;
     int64_t count_packed_columns;
+    int64_t ref_cached_columns;   /* sort-path columns whose reference tables come
+                                     from the matrix's kept tables (no sort) */
 } hpdex_plan_info;
 int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t err_len);
 /* Per-kernel timing: runs the plain launch sequence `iters` times with HIP
@@ -308,6 +327,10 @@ typedef struct hpdex_tuning {
                                   reference slice of at most this many values is
                                   sorted in LDS, a larger one in HBM (a power of two
                                   in [64, 8192]; 8-byte values: at most 4096 on chip) */
+    int32_t ref_cache;         /* 1 = resident matrices keep the sorted reference of
+                                  sort-path jobs for later jobs with the same
+                                  reference rows (hpdex_matrix_ref_cache_info); 0 =
+                                  every job sorts its reference                    */
 } hpdex_tuning;
 int hpdex_get_tuning(hpdex_tuning* out);
 int hpdex_set_tuning(const hpdex_tuning* t, char* err, size_t err_len);

@@ -45,15 +45,21 @@ def lib(path=None):
         L.sim_set_ref_cap.argtypes = [ci]
         L.sim_set_ref_cap.restype = None
         L.sim_packed_blocks.restype = ci
+        L.sim_set_refcache.argtypes = [ci]
+        L.sim_set_refcache.restype = None
+        L.sim_cached_columns.restype = ci
         _lib = L
     return _lib
 
 
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
         count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0,
-        shrink_range=False, pack=True, split_arrival=True, ref_cap=8192):
+        shrink_range=False, pack=True, split_arrival=True, ref_cap=8192, refcache=False):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
-    count kernel (0 = every column takes the split/reference/target kernels)."""
+    count kernel (0 = every column takes the split/reference/target kernels).
+    refcache: run the job, keep its reference tables with the fill kernel,
+    then run it again with the job's own tables wiped, ranking every kept
+    column from the cache (mwu.last_cached: the kept columns)."""
     # laid out like a resident matrix: PAD padding entries (value 0, row -1)
     vals = np.concatenate([np.asarray(data, dtype=np.float64 if f64 else np.float32),
                            np.zeros(PAD, dtype=np.float64 if f64 else np.float32)])
@@ -70,6 +76,7 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     lib().sim_set_pack(int(pack))   # packed entries for the count kernel where they fit
     lib().sim_set_split_arrival(int(split_arrival))
     lib().sim_set_ref_cap(int(ref_cap))   # the LDS/global sort cutover of the reference
+    lib().sim_set_refcache(int(refcache))
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,
@@ -77,4 +84,5 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     mwu.last_counted = counted   # columns routed to the count kernel
     mwu.last_blocks = lib().sim_count_blocks()   # its workgroups (column x target slice)
     mwu.last_packed = lib().sim_packed_blocks()   # of them, those that read packed entries
+    mwu.last_cached = lib().sim_cached_columns()
     return tuple(o[:n] for o in outs) + (int(err[0]), int(status[0]))

@@ -32,6 +32,10 @@ int sim_shrink_range = 0;  // test hook: record each column's range one value sh
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
 int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
 uint32_t sim_ref_cap = 8192; // the LDS/global sort cutover (tuning ref_lds_keys)
+int sim_refcache = 0;      // run the job, keep its reference tables (refcache_fill_kernel),
+                           // then run it again ranking from the kept tables
+int sim_last_cached = 0;   // columns the second run took from the kept tables
+std::vector<int32_t> sim_gen_cols;   // sort-path columns of the last run_pipeline
 int sim_last_packed = 0;   // count-kernel workgroups of the last call that read packed entries
 
 struct ThreadInfo { std::atomic<const char*>* last; };
@@ -169,6 +173,8 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                 count_kernel<V, 512>(J, items.data(), count_lds);
         }, timeout_s);
     const WorkItem* it = gen_items.data();
+    sim_gen_cols.clear();
+    for (uint32_t k = 0; k < n_static; ++k) sim_gen_cols.push_back(gen_items[k].col);
     // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most,
     // the reference and target kernels' table
     auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
@@ -243,11 +249,58 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     J.colrange = range.data();
     std::vector<int32_t> order(C);
     for (int c = 0; c < C; ++c) order[c] = C - 1 - c;   // reversed order
-    if (f64)
-        return run_pipeline<double>(J, range.data(), order, (uint32_t)count_lds, count_block,
-                                    timeout_s);
-    return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, count_block,
-                               timeout_s);
+    auto run = [&] {
+        if (f64)
+            return run_pipeline<double>(J, range.data(), order, (uint32_t)count_lds,
+                                        count_block, timeout_s);
+        return run_pipeline<float>(J, range.data(), order, (uint32_t)count_lds, count_block,
+                                   timeout_s);
+    };
+    sim_last_cached = 0;
+    if (!sim_refcache) return run();
+    // the host's refcache_fill: offsets from the column plans, then the fill
+    // kernel copies the sorted keys and bucket indexes
+    run();
+    if (*err_key != ~0ull || *status != 0u) return -1;
+    std::vector<RefCacheCol> rc(C);
+    std::memset(rc.data(), 0, rc.size() * sizeof(RefCacheCol));
+    std::vector<int32_t> list;
+    int64_t koff = 0, toff = 0;
+    for (int32_t c : sim_gen_cols) {
+        if (plan[c].hist) continue;
+        rc[c].koff = koff; rc[c].toff = toff; rc[c].TA = plan[c].TA;
+        rc[c].na = plan[c].na; rc[c].nb = plan[c].nb;
+        koff += plan[c].na;
+        toff += (int64_t)plan[c].nb + 1;
+        list.push_back(c);
+    }
+    std::vector<uint64_t> rkeys((size_t)koff + 1);
+    std::vector<uint32_t> rtab((size_t)toff + 1);
+    const unsigned nl = (unsigned)list.size();
+    std::vector<unsigned char> lds(4096);   // the fill kernel takes no dynamic LDS
+    sim::g_lds = lds.data();
+    sim::g_lds_bytes = lds.size();
+    for (unsigned b = 0; b < nl; ++b)
+        run_block(b, kColBlock, nl, [&] {
+            if (f64) refcache_fill_kernel<uint64_t>(&J, list.data(), nl, rc.data(),
+                                                    (uint64_t*)rkeys.data(), rtab.data());
+            else refcache_fill_kernel<uint32_t>(&J, list.data(), nl, rc.data(),
+                                                (uint32_t*)rkeys.data(), rtab.data());
+        }, timeout_s);
+    // the job's own tables and plans are garbage now: the second run must
+    // take every kept column's reference from the cache
+    std::memset(reftab.data(), 0xAB, reftab.size() * 4);
+    std::memset(plan.data(), 0xCD, plan.size() * sizeof(ColPlan));
+    std::memset(keys.data(), 0xEF, keys.size() * 8);
+    J.rc = rc.data();
+    J.rc_keys = rkeys.data();
+    J.rc_tab = rtab.data();
+    *err_key = ~0ull;
+    *status = 0u;
+    status_col = ~0ull;
+    const int r = run();
+    sim_last_cached = (int)nl;
+    return r;
 }
 
 // Count-kernel workgroups (column x target slice) of the last sim_mwu call.
@@ -257,3 +310,5 @@ extern "C" void sim_set_pack(int on) { sim_pack = on; }
 extern "C" int sim_packed_blocks() { return sim_last_packed; }
 extern "C" void sim_set_split_arrival(int on) { sim_split_arrival = on; }
 extern "C" void sim_set_ref_cap(int keys) { sim_ref_cap = (uint32_t)keys; }
+extern "C" void sim_set_refcache(int on) { sim_refcache = on; }
+extern "C" int sim_cached_columns() { return sim_last_cached; }

@@ -181,3 +181,21 @@ def test_autotune_times_one_candidate_per_layout(fake_device, monkeypatch):
             src = res["trials"][t["same_layout_as"]]
             assert t["ms"] == src["ms"] and t["split_arrival"] == src["split_arrival"]
     assert {r["split_arrival"] for r in runs} == {0, 1}
+
+
+def test_autotune_sweeps_without_kept_reference_tables(fake_device, monkeypatch):
+    """Every candidate sorts its own reference (a matrix's kept tables, filled
+    by the first candidate's run, would time the later ones without that
+    sort); the tuning applied at the end keeps the caller's ref_cache."""
+    fake_device["ref_cache"] = 1
+    seen = []
+    orig = _FakePlan.__init__
+
+    def init(self, *a, **k):
+        orig(self, *a, **k)
+        seen.append(self.t.get("ref_cache"))
+
+    monkeypatch.setattr(_FakePlan, "__init__", init)
+    res = autotune.autotune(None, None, 1, None, space={"count_block": (256, 512)})
+    assert seen == [0, 0]
+    assert res["best"]["ref_cache"] == 1 and fake_device["ref_cache"] == 1

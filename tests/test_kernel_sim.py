@@ -460,3 +460,28 @@ def test_sim_reference_cutover(cap, f64):
     gid[perm[700:]] = rng.integers(1, T + 1, R - 700)
     for mode in ("min", "none"):
         run_both(data, rows, indptr, gid, T, R, mode, f64=f64, ref_cap=cap)
+
+
+@pytest.mark.parametrize("cap,f64,modes", [(8192, False, ("min",)), (128, False, ("mix", "none")),
+                                           (128, True, ("min",))],
+                         ids=["onchip-f32", "cap128-f32", "cap128-f64"])
+def test_sim_kept_reference_tables(cap, f64, modes):
+    # a job run, its reference tables kept by the fill kernel, then the job
+    # run again with its own tables wiped: every sort-path column ranks from
+    # the kept sorted reference (on chip and beyond the cutover), bit-identical
+    # to the oracle; integer columns on the histogram branch are not kept
+    rng = np.random.default_rng(47)
+    R, T = 1500, 10
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.5, 0.2, 0.05, 0.4]))
+    data = np.round(rng.normal(0, 2, len(rows)), 1)
+    last = slice(indptr[4], indptr[5])
+    data[last] = rng.integers(1, 30, indptr[5] - indptr[4])   # an integer column
+    data = data.astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:500]] = 0
+    gid[perm[500:]] = rng.integers(1, T + 1, R - 500)
+    for mode in modes:
+        run_both(data, rows, indptr, gid, T, R, mode, f64=f64, ref_cap=cap, count_lds=0,
+                 refcache=True)
+        assert sim.mwu.last_cached == 4
