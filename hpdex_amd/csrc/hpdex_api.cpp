@@ -746,7 +746,7 @@ int job_errors(const JobState& js, hipStream_t s, char* err, size_t err_len) {
     if (status) {
         const std::string why =
             (status & kStatusGroups)
-                ? "more target groups than the on-chip group counters hold (5,428)"
+                ? "more target groups than the split workgroup's LDS holds counters for"
             : (status & kStatusRange)
                 ? "a stored value outside the column's recorded value range (the resident "
                   "matrix's metadata does not match its values)"
@@ -1156,8 +1156,6 @@ int hpdex_group_mean_csc(int dtype, const void* data, const int64_t* indices,
     if (n_groups == 0 || n_cols == 0) return 0;
     if (ensure_init(err, err_len)) return 1;
     const Tuning tu = tuning();
-    if (16 * n_groups > (size_t)kLdsBudget)
-        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
     Stream st;
     HIPCHK(st.create());
     hpdex_matrix m;
@@ -1260,9 +1258,6 @@ static int de_run(hpdex_matrix& m, const int32_t* group_id, size_t n_targets, in
 static int de_checks(size_t n_targets, size_t n_cols, char* err, size_t err_len) {
     if (n_targets < 1)
         return set_err(err, err_len, "[mannwhitney] n_targets must be >= 1 (>=1 target groups)");
-    const size_t G = n_targets + 1;
-    if (16 * G > (size_t)kLdsBudget)
-        return set_err(err, err_len, "too many groups for the on-chip group_mean accumulators (4,096)");
     if (n_cols * n_targets > (size_t)UINT32_MAX) return set_err(err, err_len, "more than 2^32-1 tests");
     return 0;
 }

@@ -217,14 +217,18 @@ __global__ __launch_bounds__(256) void pden_fold_change_kernel(
 template <typename V, typename A>
 __global__ __launch_bounds__(64) void pden_group_sum_kernel(
     const V* __restrict__ vals, const int32_t* __restrict__ rows, const int64_t* __restrict__ indptr,
-    const int32_t* __restrict__ gid, int32_t R, int32_t G, double* __restrict__ out) {
+    const int32_t* __restrict__ gid, int32_t R, int32_t G, int32_t S, double* __restrict__ out) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem[];
+    // workgroup (c, y): column c, groups [y*S, y*S + S) (each slice of groups
+    // reads the column once)
+    const int32_t g0 = (int32_t)blockIdx.y * S;
+    const int32_t gs = min(S, G - g0);
     A* sum = (A*)smem;
-    int32_t* owner = (int32_t*)(sum + G);
+    int32_t* owner = (int32_t*)(sum + gs);
     const int64_t c = blockIdx.x;
     const int lane = (int)threadIdx.x;
-    for (int g = lane; g < G; g += 64) { sum[g] = A(0); owner[g] = 64; }
+    for (int g = lane; g < gs; g += 64) { sum[g] = A(0); owner[g] = 64; }
     __syncthreads();
     const int64_t p0 = indptr[c], p1 = indptr[c + 1];
     for (int64_t pb = p0; pb < p1; pb += 64) {
@@ -234,8 +238,8 @@ __global__ __launch_bounds__(64) void pden_group_sum_kernel(
         if (p < p1) {
             const int32_t r = rows[p];
             if (r >= 0 && r < R) {
-                const int32_t gg = gid[r];
-                if (gg >= 0 && gg < G) { g = gg; v = (A)vals[p]; }
+                const int32_t gg = gid[r] - g0;   // the slice's own groups only
+                if (gg >= 0 && gg < gs) { g = gg; v = (A)vals[p]; }
             }
         }
         bool pending = g >= 0;
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(64) void pden_group_sum_kernel(
             __syncthreads();
         }
     }
-    for (int g = lane; g < G; g += 64) out[c * G + g] = (double)sum[g];
+    for (int g = lane; g < gs; g += 64) out[c * G + g0 + g] = (double)sum[g];
 }
 
 // Order-preserving 64-bit key of a double, for atomic min / max.
@@ -431,18 +435,21 @@ hipError_t launch_pden_group_sums(const void* vals, bool f64, bool acc_f32, cons
                                   const int64_t* indptr, const int32_t* gid, int64_t C, int32_t R,
                                   int32_t G, double* sums, hipStream_t s) {
     if (C <= 0 || G <= 0) return hipSuccess;
-    const size_t lds = (size_t)G * ((acc_f32 ? 4 : 8) + 4);
-    if (lds > (size_t)kLdsBudget) return hipErrorInvalidValue;
+    // a sum and an owner word per group: slices of as many groups as fit
+    const size_t per = (acc_f32 ? 4 : 8) + 4;
+    const int32_t S = (size_t)G * per <= (size_t)kLdsBudget ? G : (int32_t)(kLdsBudget / per);
+    const dim3 grid((unsigned)C, (unsigned)((G + S - 1) / S));
+    const size_t lds = (size_t)S * per;
     if (f64) {
         if (acc_f32) return hipErrorInvalidValue;   // a float32 matrix is stored as float
-        pden_group_sum_kernel<double, double><<<(unsigned)C, 64, lds, s>>>(
-            (const double*)vals, rows, indptr, gid, R, G, sums);
+        pden_group_sum_kernel<double, double><<<grid, 64, lds, s>>>(
+            (const double*)vals, rows, indptr, gid, R, G, S, sums);
     } else if (acc_f32) {
-        pden_group_sum_kernel<float, float><<<(unsigned)C, 64, lds, s>>>(
-            (const float*)vals, rows, indptr, gid, R, G, sums);
+        pden_group_sum_kernel<float, float><<<grid, 64, lds, s>>>(
+            (const float*)vals, rows, indptr, gid, R, G, S, sums);
     } else {
-        pden_group_sum_kernel<float, double><<<(unsigned)C, 64, lds, s>>>(
-            (const float*)vals, rows, indptr, gid, R, G, sums);
+        pden_group_sum_kernel<float, double><<<grid, 64, lds, s>>>(
+            (const float*)vals, rows, indptr, gid, R, G, S, sums);
     }
     return hipGetLastError();
 }

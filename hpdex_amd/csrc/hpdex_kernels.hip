@@ -403,6 +403,41 @@ struct SplitState {
     uint32_t scan[20];
 };
 
+// The column's plan from its split statistics (na: its reference's stored
+// nonzero finite values): the histogram branch when every value is an integer
+// of a narrow enough range, else the reference's key range and bucket shift.
+// Leaves the plan's hist and vmin in st.
+__device__ inline void write_column_plan(const JobDesc& J, int64_t c, uint32_t na, SplitState& st) {
+    ColPlan pl;
+    pl.na = na;
+    pl.refneg = st.refneg;
+    pl.kmin = st.kmin;
+    pl.kmax = st.kmax;
+    pl.TA = 0;
+    pl.maxb = 0;
+    pl.bigt = 0;
+    const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
+    if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
+        pl.hist = 1;
+        pl.nb = (uint32_t)range;
+        pl.vmin = (st.vmax >= st.vmin) ? st.vmin : 0;
+        pl.shift = 0;
+    } else {
+        pl.hist = 0;
+        uint32_t nbk = 64;
+        while (nbk < pl.na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
+        const uint64_t rng = (pl.na > 0) ? pl.kmax - pl.kmin : 0ull;
+        const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
+        const uint32_t lg = 31u - (uint32_t)__clz(nbk);
+        pl.nb = nbk;
+        pl.shift = bits > lg ? bits - lg : 0u;
+        pl.vmin = 0;
+    }
+    J.plan[c] = pl;
+    st.hist = pl.hist;
+    st.vmin = pl.vmin;
+}
+
 template <typename V, int NT>
 __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* items, uint32_t k,
                                            uint32_t lds_bytes) {
@@ -421,8 +456,8 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t R = (uint32_t)J.R;
     const V* vals = (const V*)J.vals;
     const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
-    if (split_counter_bytes((uint64_t)G) > (uint64_t)kLdsBudget) {
-        if (tid == 0) raise_status(J, kStatusGroups, c);
+    if (split_counter_bytes((uint64_t)G) > (uint64_t)lds_bytes) {   // the launch takes
+        if (tid == 0) raise_status(J, kStatusGroups, c);               // split_global_kernel
         return;
     }
     // per group: explicit zeros (high half) and nonzero values (low half),
@@ -588,36 +623,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     for (int g = tid; g < G; g += nt) J.goff[c * G + g] = cur[g];
 
     // plan of the column
-    if (tid == 0) {
-        ColPlan pl;
-        pl.na = (uint32_t)cnt[0];
-        pl.refneg = st.refneg;
-        pl.kmin = st.kmin;
-        pl.kmax = st.kmax;
-        pl.TA = 0;
-        pl.maxb = 0;
-        pl.bigt = 0;
-        const int64_t range = (st.vmax >= st.vmin) ? (int64_t)st.vmax - st.vmin + 1 : 1;
-        if (J.hist_max > 0 && st.all_int && range <= J.hist_max) {
-            pl.hist = 1;
-            pl.nb = (uint32_t)range;
-            pl.vmin = (st.vmax >= st.vmin) ? st.vmin : 0;
-            pl.shift = 0;
-        } else {
-            pl.hist = 0;
-            uint32_t nbk = 64;
-            while (nbk < pl.na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
-            const uint64_t rng = (pl.na > 0) ? pl.kmax - pl.kmin : 0ull;
-            const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
-            const uint32_t lg = 31u - (uint32_t)__clz(nbk);
-            pl.nb = nbk;
-            pl.shift = bits > lg ? bits - lg : 0u;
-            pl.vmin = 0;
-        }
-        J.plan[c] = pl;
-        st.hist = pl.hist;
-        st.vmin = pl.vmin;
-    }
+    if (tid == 0) write_column_plan(J, c, (uint32_t)cnt[0], st);
     __syncthreads();
 
     const uint32_t hist = st.hist;
@@ -694,6 +700,144 @@ __global__ __launch_bounds__(kSplitBlock) void split_kernel(const JobDesc* jobs,
                                                             uint32_t n_items,
                                                             uint32_t lds_bytes) {
     if (blockIdx.x < n_items) split_item<V, kSplitBlock>(jobs, items, blockIdx.x, lds_bytes);
+}
+
+// A device-scope load of a counter that device-scope atomics update (it
+// bypasses the CU's L1, which may hold an older copy of the line).
+__device__ __forceinline__ uint32_t load_counter(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The split of a job with more groups than the workgroup's LDS holds
+// counters for: the outputs of split_item, with the per-group counts kept in
+// the job's cnt / zcnt arrays in HBM (device-scope atomics) and the slot
+// starts or offsets scanned in its goff array; keys go straight to their
+// slots or offsets in HBM.  A scatter counts cnt up from zero again as each
+// group's cursor, which leaves the same counts behind.
+template <typename V, int NT>
+__device__ __forceinline__ void split_global_item(const JobDesc* jobs, const WorkItem* items,
+                                                  uint32_t k) {
+    using KT = KeyOf<V>;
+    using K = typename KT::K;
+    __shared__ SplitState st;
+    int job;
+    int64_t c;
+    work_item(items, k, job, c);
+    const JobDesc J = jobs[job];
+    const int tid = tid_x(), nt = blockDim.x, lane = lane_id();
+    const int G = J.G;
+    const uint32_t R = (uint32_t)J.R;
+    const int64_t p0 = J.indptr[c], p1 = J.indptr[c + 1];
+    const uint32_t n_col = (uint32_t)(p1 - p0);
+    uint32_t* cnt = J.cnt + c * G;
+    uint32_t* zcnt = J.zcnt + c * G;
+    uint32_t* off = J.goff + c * G;
+    const bool packed = __builtin_amdgcn_readfirstlane(
+        (J.gid_bits > 0u && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
+    const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
+    auto grouped = [&](auto&& f) {
+        if (packed) for_each_grouped_pipelined<V, NT, true, false, 4>(J, p0, p1 - p0, f);
+        else if (in_order) for_each_grouped<V, NT, false, true, false, kUnroll>(J, p0, p1 - p0, f);
+        else for_each_grouped_pipelined<V, NT, false, false, 4>(J, p0, p1 - p0, f);
+    };
+    for (int g = tid; g < G; g += nt) {
+        atomicExch(&cnt[g], 0u);
+        atomicExch(&zcnt[g], 0u);
+        off[g] = J.group_rows[g];
+    }
+    if (tid == 0) {
+        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
+        st.kmin = ~0ull; st.kmax = 0;
+    }
+    __syncthreads();
+    // slots: group g's keys at the prefix of the group sizes when the column's
+    // key range holds every group's rows
+    const uint32_t slots = block_excl_scan(off, G, st.scan);
+    const bool fixed = __builtin_amdgcn_readfirstlane((int64_t)slots <= p1 - p0 ? 1 : 0) != 0;
+    K* keys = (K*)J.keys + p0;
+    {
+        bool li = true;
+        int lmin = INT_MAX, lmax = INT_MIN;
+        K kmn = KT::kSentinel, kmx = 0;
+        uint32_t neg = 0;
+        grouped([&](uint32_t g, V v, uint32_t) {
+            if (!KT::finite(v)) return;
+            if (v == V(0)) {
+                atomicAdd(&zcnt[g], 1u);
+                return;
+            }
+            const uint32_t pos = atomicAdd(&cnt[g], 1u);
+            const K key = KT::key(v);
+            // a group holding more entries than rows (repeated row indices)
+            // is an error of the column; its keys stay inside the column's range
+            if (fixed && off[g] + pos < n_col) keys[off[g] + pos] = key;
+            int iv;
+            if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+            else li = false;
+            if (g == 0) {
+                kmn = key < kmn ? key : kmn;
+                kmx = key > kmx ? key : kmx;
+                neg += (v < V(0)) ? 1u : 0u;
+            }
+        });
+        const bool wall = __all(li ? 1 : 0);
+        lmin = wave_min_i(lmin);
+        lmax = wave_max_i(lmax);
+        kmn = wave_min_k<K>(kmn);
+        kmx = wave_max_k<K>(kmx);
+        neg = wave_sum_u32(neg);
+        if (lane == 0) {
+            if (!wall) atomicAnd(&st.all_int, 0u);
+            atomicMin(&st.vmin, lmin);
+            atomicMax(&st.vmax, lmax);
+            atomicAdd(&st.refneg, neg);
+            atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
+            atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
+        }
+    }
+    __syncthreads();
+    // errors of this column (mannwhitneyu.cpp:936-1038); offsets from the counts
+    const uint64_t ebase = (uint64_t)c * (uint64_t)(2 * G + 1);
+    for (int g = tid; g < G; g += nt) {
+        const uint32_t nz = load_counter(&cnt[g]), ez = load_counter(&zcnt[g]),
+                       gr = J.group_rows[g];
+        if ((uint64_t)nz + ez > gr) atomicMin(J.err_key, (unsigned long long)(ebase + g));
+        const uint32_t n = (J.mode == kZeroNone) ? nz + ez : gr;
+        if (n < J.min_n) atomicMin(J.err_key, (unsigned long long)(ebase + (g == 0 ? G : G + g)));
+        if (!fixed) off[g] = nz;
+    }
+    __syncthreads();
+    if (!fixed) block_excl_scan(off, G, st.scan);
+    if (tid == 0) write_column_plan(J, c, load_counter(&cnt[0]), st);
+    __syncthreads();
+    const uint32_t hist = st.hist;
+    const int32_t vmin = st.vmin;
+    if (!fixed) {
+        for (int g = tid; g < G; g += nt) atomicExch(&cnt[g], 0u);
+        __syncthreads();
+        grouped([&](uint32_t g, V v, uint32_t) {
+            if (!KT::finite(v) || v == V(0)) return;
+            const uint32_t pos = off[g] + atomicAdd(&cnt[g], 1u);
+            keys[pos] = hist ? (K)(uint32_t)((int)v - vmin) : KT::key(v);
+        });
+    } else if (hist) {
+        // the slots hold keys; a histogram plan reads bins v - vmin
+        const int w = tid / kWave, nw = nt / kWave;
+        for (int g = w; g < G; g += nw) {
+            const uint32_t o = off[g], m = min(load_counter(&cnt[g]), J.group_rows[g]);
+            for (uint32_t i = (uint32_t)lane; i < m; i += kWave)
+                keys[o + i] = (K)(uint32_t)((int)KT::value(keys[o + i]) - vmin);
+        }
+    }
+}
+
+// workgroup b takes queue item b: the split of jobs whose group counters do
+// not fit LDS (kColBlock threads, several workgroups per CU)
+template <typename V>
+__global__ __launch_bounds__(kColBlock) void split_global_kernel(const JobDesc* jobs,
+                                                                 const WorkItem* items,
+                                                                 uint32_t n_items) {
+    if (blockIdx.x < n_items) split_global_item<V, kColBlock>(jobs, items, blockIdx.x);
 }
 
 // Search of a sorted run a[base, base + n) in LDS with a fixed number of
@@ -2247,32 +2391,40 @@ __global__ void exact_p_kernel(ExactJob E) {
 // group_mean (src/cpp/mannwhitneyu.cpp:1118-1192): one workgroup per column,
 // per-group FP64 sums and counts in LDS.
 // ---------------------------------------------------------------------------
+// Workgroup (c, y) takes column c and groups [y*S, y*S + S): with more groups
+// than one workgroup's LDS holds accumulators for, each slice of S groups
+// reads the column once.
 template <typename V>
 __global__ __launch_bounds__(1024) void group_mean_kernel(const V* vals, const int32_t* rows,
                                                           const int64_t* indptr,
                                                           const int32_t* gid,
                                                           const uint32_t* group_rows,
-                                                          int32_t R, int32_t G, int include_zeros,
-                                                          double* out) {
+                                                          int32_t R, int32_t G, int32_t S,
+                                                          int include_zeros, double* out) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const int32_t g0 = (int32_t)blockIdx.y * S;
+    const uint32_t gs = (uint32_t)min(S, G - g0);
     double* sum = (double*)smem;
-    uint32_t* vc = (uint32_t*)(sum + G);
-    uint32_t* ic = vc + G;
+    uint32_t* vc = (uint32_t*)(sum + gs);
+    uint32_t* ic = vc + gs;
     const int64_t c = blockIdx.x;
     const int tid = tid_x(), nt = blockDim.x;
-    for (int g = tid; g < G; g += nt) { sum[g] = 0.0; vc[g] = 0u; ic[g] = 0u; }
+    for (uint32_t j = tid; j < gs; j += nt) { sum[j] = 0.0; vc[j] = 0u; ic[j] = 0u; }
     __syncthreads();
     const int64_t p0 = indptr[c], p1 = indptr[c + 1];
     for_each_entry<V>(rows, vals, gid, (uint32_t)R, (uint32_t)G, p0, p1, [&](uint32_t g, V x) {
+        const uint32_t j = g - (uint32_t)g0;
+        if (j >= gs) return;
         const double v = (double)x;
-        if (__builtin_isfinite(v)) { atomicAdd(&sum[g], v); atomicAdd(&vc[g], 1u); }
-        else atomicAdd(&ic[g], 1u);
+        if (__builtin_isfinite(v)) { atomicAdd(&sum[j], v); atomicAdd(&vc[j], 1u); }
+        else atomicAdd(&ic[j], 1u);
     });
     __syncthreads();
-    for (int g = tid; g < G; g += nt) {
-        const uint64_t denom = include_zeros ? ((uint64_t)group_rows[g] - (uint64_t)ic[g])
-                                             : (uint64_t)vc[g];
-        out[c * G + g] = denom > 0 ? sum[g] / (double)denom : 0.0;
+    for (uint32_t j = tid; j < gs; j += nt) {
+        const uint32_t g = (uint32_t)g0 + j;
+        const uint64_t denom = include_zeros ? ((uint64_t)group_rows[g] - (uint64_t)ic[j])
+                                             : (uint64_t)vc[j];
+        out[c * G + g] = denom > 0 ? sum[j] / (double)denom : 0.0;
     }
 }
 
@@ -2404,17 +2556,21 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     }
     mark(1);
     if (L.n_gen > 0) {
-        size_t split = (size_t)split_counter_bytes((uint64_t)L.G);
-        if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
         // the most LDS the split kernel may take: one 1024-thread workgroup
         // per CU fits in any case, and a column whose key range fits in what
-        // is left gathers its keys there and writes them out coalesced
-        const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
+        // is left gathers its keys there and writes them out coalesced; with
+        // more groups than that LDS holds counters for, the split counts in HBM
+        const unsigned split_lds = (unsigned)L.split_lds_max;
+        const bool global_split = split_counter_bytes((uint64_t)L.G) > (uint64_t)split_lds;
         const unsigned tab_lds = table_lds_bytes(L.f64);
         const unsigned grid = (unsigned)L.n_gen, n = (uint32_t)L.n_gen;
         auto sort_path = [&](auto v_tag) {
             using V = typename decltype(v_tag)::type;
-            split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
+            if (global_split)
+                split_global_kernel<V><<<grid, kColBlock, 0, s>>>(L.jobs, L.gen_items, n);
+            else
+                split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
+                                                                     split_lds);
             mark(2);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             mark(3);
@@ -2447,14 +2603,17 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
 #undef HPDEX_COUNTQ
     }
     if (L.n_gen > 0) {
-        size_t split = (size_t)split_counter_bytes((uint64_t)L.G);
-        if (split > (size_t)kLdsBudget) split = 16;   // the kernel reports kStatusGroups
-        const unsigned split_lds = split > 16u ? (unsigned)L.split_lds_max : align16(split);
+        const unsigned split_lds = (unsigned)L.split_lds_max;
+        const bool global_split = split_counter_bytes((uint64_t)L.G) > (uint64_t)split_lds;
         const unsigned tab_lds = table_lds_bytes(L.f64);
         const unsigned grid = (unsigned)L.n_gen, n = (uint32_t)L.n_gen;
         auto sort_path = [&](auto v_tag) {
             using V = typename decltype(v_tag)::type;
-            split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n, split_lds);
+            if (global_split)
+                split_global_kernel<V><<<grid, kColBlock, 0, s>>>(L.jobs, L.gen_items, n);
+            else
+                split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
+                                                                     split_lds);
             ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         };
@@ -2535,14 +2694,16 @@ hipError_t launch_group_mean(const void* vals, bool f64, const int32_t* rows,
                              const uint32_t* group_rows, int64_t C, int32_t R, int32_t G,
                              int include_zeros, double* out, hipStream_t s) {
     if (C <= 0 || G <= 0) return hipSuccess;
-    const size_t need = 16u * (size_t)G;
-    if (need > (size_t)kLdsBudget) return hipErrorInvalidValue;
+    // 16 B of accumulators per group: slices of at most 4,096 groups
+    const int32_t S = G < (int32_t)(kLdsBudget / 16) ? G : (int32_t)(kLdsBudget / 16);
+    const dim3 grid((unsigned)C, (unsigned)((G + S - 1) / S));
+    const size_t need = 16u * (size_t)S;
     if (f64)
-        group_mean_kernel<double><<<(unsigned)C, kColBlock, align16(need), s>>>(
-            (const double*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
+        group_mean_kernel<double><<<grid, kColBlock, align16(need), s>>>(
+            (const double*)vals, rows, indptr, gid, group_rows, R, G, S, include_zeros, out);
     else
-        group_mean_kernel<float><<<(unsigned)C, kColBlock, align16(need), s>>>(
-            (const float*)vals, rows, indptr, gid, group_rows, R, G, include_zeros, out);
+        group_mean_kernel<float><<<grid, kColBlock, align16(need), s>>>(
+            (const float*)vals, rows, indptr, gid, group_rows, R, G, S, include_zeros, out);
     return hipGetLastError();
 }
 

@@ -32,6 +32,8 @@ int sim_shrink_range = 0;  // test hook: record each column's range one value sh
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
 int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
 uint32_t sim_ref_cap = 8192; // the LDS/global sort cutover (tuning ref_lds_keys)
+int sim_global_split = 0;  // test hook: the split with its group counters in memory
+                           // (split_global_kernel) whatever the group count
 int sim_refcache = 0;      // run the job, keep its reference tables (refcache_fill_kernel),
                            // then run it again ranking from the kept tables
 int sim_last_cached = 0;   // columns the second run took from the kept tables
@@ -177,13 +179,17 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
     for (uint32_t k = 0; k < n_static; ++k) sim_gen_cols.push_back(gen_items[k].col);
     // the launch's LDS sizes (launch_mwu_pipeline): the split kernel's most,
     // the reference and target kernels' table
-    auto a16 = [](size_t b) { return (uint32_t)((b + 15) & ~(size_t)15); };
-    size_t split = (size_t)split_counter_bytes((uint64_t)J.G);
-    if (split > (size_t)kLdsBudget) split = 16;
-    const uint32_t split_lds = split > 16u ? (uint32_t)kSplitLdsMax : a16(split);
+    const uint32_t split_lds = (uint32_t)kSplitLdsMax;
+    const bool global_split =
+        sim_global_split || split_counter_bytes((uint64_t)J.G) > (uint64_t)split_lds;
     const unsigned ng = n_static;
-    for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, ng, split_lds); }, timeout_s);
+    for (unsigned b = 0; b < ng; ++b) {
+        if (global_split)
+            run_block(b, kColBlock, ng, [&] { split_global_kernel<V>(jobs, it, ng); }, timeout_s);
+        else
+            run_block(b, kSplitBlock, ng, [&] { split_kernel<V>(jobs, it, ng, split_lds); },
+                      timeout_s);
+    }
     for (unsigned b = 0; b < ng; ++b)
         run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, ng); }, timeout_s);
     if constexpr (sizeof(V) == 4)
@@ -311,4 +317,5 @@ extern "C" int sim_packed_blocks() { return sim_last_packed; }
 extern "C" void sim_set_split_arrival(int on) { sim_split_arrival = on; }
 extern "C" void sim_set_ref_cap(int keys) { sim_ref_cap = (uint32_t)keys; }
 extern "C" void sim_set_refcache(int on) { sim_refcache = on; }
+extern "C" void sim_set_global_split(int on) { sim_global_split = on; }
 extern "C" int sim_cached_columns() { return sim_last_cached; }

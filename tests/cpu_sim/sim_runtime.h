@@ -238,3 +238,7 @@ inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
 inline unsigned atomicAnd(unsigned* p, unsigned v) { return std::atomic_ref<unsigned>(*p).fetch_and(v); }
 inline unsigned atomicOr(unsigned* p, unsigned v) { return std::atomic_ref<unsigned>(*p).fetch_or(v); }
 inline unsigned atomicExch(unsigned* p, unsigned v) { return std::atomic_ref<unsigned>(*p).exchange(v); }
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <typename T> inline T __hip_atomic_load(const T* p, int, int) {
+    return std::atomic_ref<T>(*const_cast<T*>(p)).load();
+}

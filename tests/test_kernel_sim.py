@@ -485,3 +485,24 @@ def test_sim_kept_reference_tables(cap, f64, modes):
         run_both(data, rows, indptr, gid, T, R, mode, f64=f64, ref_cap=cap, count_lds=0,
                  refcache=True)
         assert sim.mwu.last_cached == 4
+
+
+@pytest.mark.parametrize("f64,modes", [(False, ("min", "none")), (True, ("mix",))],
+                         ids=["f32", "f64"])
+def test_sim_split_with_counters_in_memory(f64, modes):
+    # the split of a job with more groups than LDS holds counters for: counts
+    # by device-scope atomics in the job's arrays, offsets scanned there, keys
+    # to their slots (a dense column) or offsets (sparse ones), an integer
+    # column on the histogram branch; bit-identical to the oracle
+    rng = np.random.default_rng(53)
+    R, T = 1200, 20
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([1.0, 0.6, 0.35, 0.5]))
+    data = np.round(rng.normal(0, 2, len(rows)), 1)
+    data[indptr[3]:indptr[4]] = rng.integers(-3, 30, indptr[4] - indptr[3])
+    data = data.astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:400]] = 0
+    gid[perm[400:1100]] = rng.integers(1, T + 1, 700)
+    for mode in modes:
+        run_both(data, rows, indptr, gid, T, R, mode, f64=f64, count_lds=0, global_split=True)
