@@ -34,7 +34,7 @@ EXPORTED = (
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
     "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
     "hpdex_session_batches", "hpdex_pden_matrix", "hpdex_matrix_ref_cache_info",
-    "hpdex_matrix_ref_cache_clear",
+    "hpdex_matrix_ref_cache_clear", "hpdex_matrix_upload_form",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -124,6 +124,8 @@ def lib() -> ctypes.CDLL:
                                                   ctypes.POINTER(sz)]
         L.hpdex_matrix_ref_cache_clear.argtypes = [vp]
         L.hpdex_matrix_ref_cache_clear.restype = None
+        L.hpdex_matrix_upload_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.hpdex_matrix_upload_form.restype = ctypes.c_int
         L.hpdex_plan_create.argtypes = [vp, vp, sz, ctypes.POINTER(MwuOpts), ci,
                                         ctypes.POINTER(vp), cp, sz]
         L.hpdex_plan_destroy.argtypes = [vp]
@@ -451,6 +453,13 @@ class DeviceMatrix:
 
     def ref_cache_clear(self):
         lib().hpdex_matrix_ref_cache_clear(self._h)
+
+    def compact_upload(self):
+        """(whether the entries crossed PCIe in the 4-byte compact form, the
+        columns written as they were because an entry did not qualify)."""
+        n = ctypes.c_int64()
+        form = lib().hpdex_matrix_upload_form(self._h, ctypes.byref(n))
+        return bool(form), int(n.value)
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -98,6 +98,53 @@ int check_indptr(const int64_t* indptr, size_t C, size_t nnz, char* err, size_t 
 // ---------------------------------------------------------------------------
 // Resident matrix upload, per-job state
 // ---------------------------------------------------------------------------
+// A float32 CSC source in 4 bytes per entry (upload_compact) when its values
+// are small whole numbers and its columns' rows ascend with small gaps,
+// decoded in place into the matrix's rows and values; the few columns that do
+// not qualify are then written as they are.  `done` says whether the compact
+// form was taken.  m->R, C, nnz, rows, vals and indptr are set up.
+template <typename IDX>
+static int upload_compact_csc(hpdex_matrix* m, const float* vals, const IDX* rows,
+                              const int64_t* indptr, bool& done, hipStream_t s, char* err,
+                              size_t err_len) {
+    done = false;
+    DevBuf hi;
+    HIPCHK(hi.alloc(m->C * 4));
+    std::vector<int32_t> patch;
+    const hipError_t e = upload_compact<IDX>(m->rows.as<uint32_t>(), hi.as<int32_t>(), vals, rows,
+                                             indptr, m->C, m->nnz, (int64_t)m->R, s, patch);
+    if (e == hipErrorNotSupported) return 0;
+    HIPCHK(e);
+    HIPCHK(hipMemcpyAsync(m->indptr.p, indptr, (m->C + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_decode_compact(m->rows.as<uint32_t>(), (float*)m->vals.p,
+                                 m->indptr.as<int64_t>(), hi.as<int32_t>(), m->C, s));
+    // the columns that did not qualify, as they are (rows outside [0, R) -> -1)
+    std::vector<int32_t> prow;
+    size_t total = 0;
+    for (int32_t c : patch) total += (size_t)(indptr[c + 1] - indptr[c]);
+    prow.reserve(total);
+    for (int32_t c : patch)
+        for (int64_t p = indptr[c]; p < indptr[c + 1]; ++p) {
+            const int64_t r = (int64_t)rows[p];
+            prow.push_back((r >= 0 && r < (int64_t)m->R) ? (int32_t)r : -1);
+        }
+    size_t at = 0;
+    for (int32_t c : patch) {
+        const int64_t p0 = indptr[c], n = indptr[c + 1] - p0;
+        if (n <= 0) continue;
+        HIPCHK(hipMemcpyAsync(m->rows.as<int32_t>() + p0, prow.data() + at, (size_t)n * 4,
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync((float*)m->vals.p + p0, vals + p0, (size_t)n * 4,
+                              hipMemcpyHostToDevice, s));
+        at += (size_t)n;
+    }
+    HIPCHK(hipStreamSynchronize(s));   // hi and prow are freed on return
+    done = true;
+    m->compact = true;
+    m->compact_patched = (int64_t)patch.size();
+    return 0;
+}
+
 int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* indices,
                   const int64_t* indptr, size_t R, size_t C, size_t nnz, hipStream_t s,
                   char* err, size_t err_len) {
@@ -117,7 +164,12 @@ int upload_matrix(hpdex_matrix* m, int dtype, const void* data, const int64_t* i
     HIPCHK(hipMemsetAsync((char*)m->vals.p + nnz * vsz, 0, kMatrixPad * vsz, s));
     HIPCHK(hipMemsetAsync((char*)m->rows.p + nnz * 4, 0xFF, kMatrixPad * 4, s));
     HIPCHK(m->indptr.alloc((C + 1) * 8));
-    if (nnz) {
+    bool compact = false;
+    if (nnz && dtype == HPDEX_FLOAT32)
+        if (upload_compact_csc<int64_t>(m, (const float*)data, indices, indptr, compact, s, err,
+                                        err_len))
+            return 1;
+    if (nnz && !compact) {
         if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
             HIPCHK(upload_bytes(m->vals.p, data, nnz * vsz, s));
         } else {
@@ -263,7 +315,13 @@ int upload_matrix_from(hpdex_matrix* m, const hpdex_matrix_src* src, hipStream_t
         const size_t nnz = src->nnz;
         if (widen_indptr(src->indptr, 32, C, nnz, m->h_indptr, err, err_len)) return 1;
         HIPCHK(alloc_out(nnz));
-        if (nnz) {
+        bool compact = false;
+        if (nnz && dtype == HPDEX_FLOAT32)
+            if (upload_compact_csc<int32_t>(m, (const float*)src->data,
+                                            (const int32_t*)src->indices, m->h_indptr.data(),
+                                            compact, s, err, err_len))
+                return 1;
+        if (nnz && !compact) {
             if (dtype == HPDEX_FLOAT32 || dtype == HPDEX_FLOAT64) {
                 HIPCHK(upload_bytes(m->vals.p, src->data, nnz * vsz, s));
             } else {
@@ -801,6 +859,7 @@ size_t matrix_footprint(int dtype, size_t R, size_t C, size_t nnz, int layout, i
     // in-order flags and the classification's staging flags
     size_t f = up256((nnz + kMatrixPad) * vsz) + up256((nnz + kMatrixPad) * 4) + up256((C + 1) * 8);
     f += up256(2 * C * 4) + 2 * up256(C);
+    if (layout == HPDEX_LAYOUT_CSC && dtype == HPDEX_FLOAT32) f += up256(C * 4);   // compact upload
     if (layout == HPDEX_LAYOUT_CSR) {
         // raw values, column indices and row pointers, the transpose workspace
         f += up256(nnz * dtype_size(dtype)) + up256(nnz * ib) + up256((R + 1) * ib);
@@ -1521,6 +1580,11 @@ size_t hpdex_de_csc_device_bytes(int dtype, size_t n_rows, size_t n_cols, size_t
                                  size_t n_targets) {
     if (!valid_dtype(dtype) || n_targets < 1) return 0;
     return de_footprint(dtype, n_rows, n_cols, nnz, n_targets);
+}
+
+int hpdex_matrix_upload_form(const hpdex_matrix* m, int64_t* patched_columns) {
+    if (patched_columns) *patched_columns = m && m->compact ? m->compact_patched : 0;
+    return m && m->compact ? 1 : 0;
 }
 
 int hpdex_matrix_ref_cache_info(const hpdex_matrix* m, int64_t* columns, size_t* bytes) {

@@ -85,6 +85,22 @@ hipError_t upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s)
 hipError_t download_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
 // int64 row indices to int32 on the way (rows outside [0, R) become -1)
 hipError_t upload_rows_i64(int32_t* dst, const int64_t* src, size_t n, int64_t R, hipStream_t s);
+// A float32 CSC matrix in 4 bytes per entry instead of 8: one word (row &
+// 0xFFFF) << 16 | value per entry into `words` (nnz of them; the matrix's rows
+// array, which launch_decode_compact turns into rows and values) and the high
+// bits of each column's first row into hi_dev (C int32).  An entry qualifies
+// when its value is a whole number in [0, 65535] (sign bit clear) and its row
+// lies in [0, R), above the column's previous row by less than 2^16; the
+// columns holding one that does not are listed in `patch` (their words are
+// not valid: the caller writes their rows and values as they are after the
+// decode).  With more than an eighth of the entries in such columns, for
+// copies too small to stage, or when copies are plain, it returns
+// hipErrorNotSupported and the caller uploads the matrix as it is.  Returns
+// when the copies are done.
+template <typename IDX>
+hipError_t upload_compact(uint32_t* words, int32_t* hi_dev, const float* vals, const IDX* rows,
+                          const int64_t* indptr, size_t C, size_t nnz, int64_t R, hipStream_t s,
+                          std::vector<int32_t>& patch);
 
 struct DevBuf {
     void* p = nullptr;
@@ -167,6 +183,8 @@ struct hpdex_matrix {
     // the sorted reference of the last sort-path job whose reference rows had
     // no tables yet (resident matrices only: those created through the C-ABI)
     bool resident = true;
+    bool compact = false;   // uploaded in 4 bytes per entry (upload_compact)
+    int64_t compact_patched = 0;   // of its columns, those written as they were
     mutable std::mutex rc_mu;
     mutable std::shared_ptr<hpdex_host::RefCacheEntry> rc;
 };

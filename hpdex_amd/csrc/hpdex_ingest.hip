@@ -40,6 +40,77 @@ __global__ void rows_i32_check_kernel(int32_t* rows, size_t n, int32_t R) {
     if (r < 0 || r >= R) rows[i] = -1;
 }
 
+// A CSC matrix uploaded compactly: one 32-bit word per stored entry, the
+// row's low 16 bits above the value as a 16-bit unsigned integer (float32
+// values that are whole numbers in [0, 65535]), in place of the rows array.
+// Rows of a column ascend with gaps below 2^16, so their high bits start at
+// hi[c] (the column's first row's) and go up by one wherever the low bits
+// fall.  One workgroup per column decodes it in place: chunks of 1,024
+// entries, 4 per thread, the low bits through LDS (each entry's predecessor),
+// a scan of the wrap flags, then rows and values written.
+constexpr uint32_t kDecodeChunk = 1024u;
+__global__ __launch_bounds__(256) void decode_compact_kernel(uint32_t* words, float* vals,
+                                                             const int64_t* indptr,
+                                                             const int32_t* hi) {
+    __shared__ uint32_t los[kDecodeChunk];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry_hi, carry_lo;
+    const int64_t c = blockIdx.x;
+    const int64_t p0 = indptr[c], p1 = indptr[c + 1];
+    if (p0 >= p1) return;   // the same in every thread
+    const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id(), w = tid / kWave;
+    if (tid == 0) {
+        carry_hi = (uint32_t)hi[c];
+        carry_lo = 0xFFFFFFFFu;   // the column's first entry has no predecessor
+    }
+    for (int64_t base = p0; base < p1; base += kDecodeChunk) {
+        __syncthreads();   // the carry of the chunk before; los free
+        uint32_t wd[4];
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t p = base + tid * 4 + k;
+            in[k] = p < p1;
+            wd[k] = in[k] ? words[p] : 0u;
+            los[tid * 4 + k] = wd[k] >> 16;
+        }
+        const uint32_t hi0 = carry_hi, prev0 = carry_lo;
+        __syncthreads();
+        uint32_t f[4], cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = tid * 4 + k;
+            const uint32_t prev = i ? los[i - 1] : prev0;
+            f[k] = (in[k] && prev != 0xFFFFFFFFu && (wd[k] >> 16) < prev) ? 1u : 0u;
+            cnt += f[k];
+        }
+        const uint32_t incl = wave_incl_scan_u32(cnt);
+        if (lane == (uint32_t)kWave - 1u) wsum[w] = incl;
+        __syncthreads();
+        uint32_t woff = 0, total = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < 4u; ++x) {
+            woff += x < w ? wsum[x] : 0u;
+            total += wsum[x];
+        }
+        uint32_t run = hi0 + woff + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            run += f[k];
+            if (!in[k]) continue;
+            const int64_t p = base + tid * 4 + k;
+            words[p] = (run << 16) | (wd[k] >> 16);
+            vals[p] = (float)(wd[k] & 0xFFFFu);
+        }
+        __syncthreads();   // every thread has read the carry and los
+        if (tid == 0) {
+            const int64_t last = (p1 - base < (int64_t)kDecodeChunk ? p1 - base : kDecodeChunk) - 1;
+            carry_hi = hi0 + total;
+            carry_lo = los[last];
+        }
+    }
+}
+
 // row of every CSR entry: one wave per row, lanes over its entries
 template <typename I>
 __global__ __launch_bounds__(256) void csr_rows_kernel(const I* indptr, size_t R, int32_t* row_of) {
@@ -223,6 +294,13 @@ size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 hipError_t launch_rows_i32_check(int32_t* rows, size_t n, int64_t R, hipStream_t s) {
     if (n == 0) return hipSuccess;
     rows_i32_check_kernel<<<grid_of(n, 256), 256, 0, s>>>(rows, n, (int32_t)R);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_compact(uint32_t* words, float* vals, const int64_t* indptr,
+                                 const int32_t* hi, size_t C, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    decode_compact_kernel<<<(unsigned)C, 256, 0, s>>>(words, vals, indptr, hi);
     return hipGetLastError();
 }
 

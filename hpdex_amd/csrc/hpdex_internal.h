@@ -326,6 +326,11 @@ bool dtype_uses_double(int dtype);
 
 // ---- ingest (hpdex_ingest.hip) ----
 hipError_t launch_rows_i32_check(int32_t* rows, size_t n, int64_t R, hipStream_t s);
+// Decode a compact upload in place (hpdex_ingest.hip): words[p] = (row & 0xFFFF)
+// << 16 | value becomes the int32 row, vals[p] the float value; hi[c] holds
+// the high bits of column c's first row.
+hipError_t launch_decode_compact(uint32_t* words, float* vals, const int64_t* indptr,
+                                 const int32_t* hi, size_t C, hipStream_t s);
 size_t csr_workspace_bytes(size_t nnz);
 // CSR (values of `dtype`, column indices and row pointers of index_bits 32 or
 // 64, all on the device) -> CSC rows, values (float, or double when f64) and

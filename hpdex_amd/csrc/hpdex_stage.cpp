@@ -16,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -72,7 +73,9 @@ private:
     };
     CopyPool() {
         const unsigned hc = std::thread::hardware_concurrency();
-        const int nw = (int)std::min<unsigned>(7u, hc > 2u ? hc / 2u : 1u);
+        // at most 15 (a GPU's share of a host's cores); the compact uploads'
+        // fills take all of them
+        const int nw = (int)std::min<unsigned>(15u, hc > 2u ? hc - 1u : 1u);
         for (int i = 0; i < nw; ++i) workers_.emplace_back([this] { loop(); });
     }
     ~CopyPool() {
@@ -167,9 +170,12 @@ void release_ring(Ring* r) {
 // off, n) writes bytes [off, off + n) of the source image into `stage`
 // (called from several threads on disjoint ranges).  Returns after the last
 // DMA has completed, so the ring is free and the caller's memory may change.
+// With `abort`: a fill may set it to stop the copy (no later chunk is filled
+// or sent); staged then returns hipErrorNotSupported once the chunks already
+// sent have landed.
 hipError_t staged(void* dst, size_t bytes, hipStream_t s,
                   const std::function<void(char* stage, size_t off, size_t n)>& fill,
-                  size_t align) {
+                  size_t align, const std::atomic<bool>* abort = nullptr) {
     Ring* ring = acquire_ring();
     if (!ring) return hipErrorOutOfMemory;
     CopyPool& pool = CopyPool::get();
@@ -178,6 +184,10 @@ hipError_t staged(void* dst, size_t bytes, hipStream_t s,
     bool used[kStageDepth] = {};
     size_t k = 0;
     for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageChunk, ++k) {
+        if (abort && abort->load()) {
+            e = hipErrorNotSupported;
+            break;
+        }
         const int slot = (int)(k % kStageDepth);
         if (used[slot] && (e = hipEventSynchronize(ring->ev[slot])) != hipSuccess) break;
         const size_t n = std::min(kStageChunk, bytes - off);
@@ -188,6 +198,10 @@ hipError_t staged(void* dst, size_t bytes, hipStream_t s,
             const size_t a = std::min(elems, (size_t)i * per), b = std::min(elems, a + per);
             if (b > a) fill(stage + a * align, off + a * align, (b - a) * align);
         });
+        if (abort && abort->load()) {
+            e = hipErrorNotSupported;
+            break;
+        }
         if ((e = hipMemcpyAsync((char*)dst + off, stage, n, hipMemcpyHostToDevice, s)) != hipSuccess)
             break;
         if ((e = hipEventRecord(ring->ev[slot], s)) != hipSuccess) break;
@@ -213,7 +227,9 @@ hipError_t download_bytes(void* dst, const void* src, size_t bytes, hipStream_t 
     Ring* ring = acquire_ring();
     if (!ring) return plain();   // no pinned memory to stage through
     CopyPool& pool = CopyPool::get();
-    const int parts = pool.width();
+    // copies into the caller's memory fault its fresh pages in: more than 8
+    // threads at once slowed them (38 MB of results: 2.0 -> 8.5 ms with 16)
+    const int parts = std::min(pool.width(), 8);
     const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
     hipError_t e = hipSuccess;
     // chunk k: its DMA lands in buffer k % kStageDepth, then host threads copy
@@ -277,5 +293,136 @@ hipError_t upload_rows_i64(int32_t* dst, const int64_t* src, size_t n, int64_t R
         }
     }, 4);
 }
+
+namespace {
+
+// Compact words (row & 0xFFFF) << 16 | value of entries a .. b-1 of one
+// column, every one of which has a predecessor in the column (entry a - 1);
+// returns 0 when one of them does not qualify: its row outside [0, R), not
+// above its predecessor's or 2^16 or more above it, or its value not a whole
+// number in [0, 65535] with the sign bit clear (-0.0 and NaN do not
+// qualify).  Written without branches so that it vectorizes: the checks of a
+// segment are or-ed together and tested once.
+template <typename IDX>
+static inline uint32_t compact_run(uint32_t* __restrict out, const float* __restrict vals,
+                                   const IDX* __restrict rows, int64_t a, int64_t b, int64_t R) {
+    uint32_t bad = 0u;
+    const int64_t n = b - a;
+    const IDX* rp = rows + a;
+    const float* vp = vals + a;
+#pragma clang loop vectorize(enable) interleave(enable)
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t r = (int64_t)rp[i], pr = (int64_t)rp[i - 1];
+        const float v = vp[i];
+        const float vc = v > 0.0f ? (v < 65535.0f ? v : 65535.0f) : 0.0f;
+        const int32_t u = (int32_t)vc;
+        bad |= (uint32_t)((uint64_t)(r - pr - 1) >= 65535u) |
+               (uint32_t)((uint64_t)r >= (uint64_t)R) | (uint32_t)((float)u != v) |
+               (__builtin_bit_cast(uint32_t, v) >> 31);
+        out[i] = ((uint32_t)r << 16) | (uint32_t)u;
+    }
+    return bad ? 0u : 1u;
+}
+
+// Entries [e0, e0 + m) of a CSC matrix as compact words.  A column with an
+// entry that does not qualify (for its first entry: its row in [0, R) and its
+// value) is reported through bad(c, entries of the column in the range); its
+// words are left as they come.
+template <typename IDX, uint32_t (*RUN)(uint32_t*, const float*, const IDX*, int64_t, int64_t,
+                                        int64_t), typename BAD>
+static void compact_fill_with(uint32_t* out, const float* vals, const IDX* rows,
+                              const int64_t* indptr, size_t C, size_t e0, size_t m, int64_t R,
+                              BAD&& bad) {
+    size_t c = (size_t)(std::upper_bound(indptr, indptr + C + 1, (int64_t)e0) - indptr) - 1;
+    int64_t e = (int64_t)e0;
+    const int64_t end = (int64_t)(e0 + m);
+    while (e < end) {
+        const int64_t b = std::min(end, indptr[c + 1]);
+        const int64_t s0 = e;
+        uint32_t ok = 1u;
+        if (e == indptr[c] && e < b) {   // the column's first entry
+            const int64_t r = (int64_t)rows[e];
+            const float v = vals[e];
+            const float vc = v > 0.0f ? (v < 65535.0f ? v : 65535.0f) : 0.0f;
+            const int32_t u = (int32_t)vc;
+            ok &= (uint32_t)((uint64_t)r < (uint64_t)R) & (uint32_t)((float)u == v) &
+                  (uint32_t)((__builtin_bit_cast(uint32_t, v) >> 31) == 0u);
+            out[e - (int64_t)e0] = ((uint32_t)r << 16) | (uint32_t)u;
+            ++e;
+        }
+        if (e < b) ok &= RUN(out + (e - (int64_t)e0), vals, rows, e, b, R);
+        if (!ok) bad(c, (size_t)(b - s0));
+        e = b;
+        ++c;
+    }
+}
+
+template <typename IDX>
+static uint32_t compact_run_base(uint32_t* out, const float* vals, const IDX* rows, int64_t a,
+                                 int64_t b, int64_t R) {
+    return compact_run<IDX>(out, vals, rows, a, b, R);
+}
+template <typename IDX>
+__attribute__((target("avx2"))) static uint32_t compact_run_avx2(uint32_t* out, const float* vals,
+                                                                 const IDX* rows, int64_t a,
+                                                                 int64_t b, int64_t R) {
+    return compact_run<IDX>(out, vals, rows, a, b, R);
+}
+
+template <typename IDX, typename BAD>
+void compact_fill(uint32_t* out, const float* vals, const IDX* rows, const int64_t* indptr,
+                  size_t C, size_t e0, size_t m, int64_t R, BAD&& bad) {
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2)
+        compact_fill_with<IDX, compact_run_avx2<IDX>>(out, vals, rows, indptr, C, e0, m, R, bad);
+    else
+        compact_fill_with<IDX, compact_run_base<IDX>>(out, vals, rows, indptr, C, e0, m, R, bad);
+}
+
+}  // namespace
+
+template <typename IDX>
+hipError_t upload_compact(uint32_t* words, int32_t* hi_dev, const float* vals, const IDX* rows,
+                          const int64_t* indptr, size_t C, size_t nnz, int64_t R, hipStream_t s,
+                          std::vector<int32_t>& patch) {
+    patch.clear();
+    if (t_plain_copies || nnz * 4 < kStageMin) return hipErrorNotSupported;
+    // the high bits of each column's first row (a row out of range makes the
+    // column one to patch)
+    std::vector<int32_t> hi(C, 0);
+    for (size_t c = 0; c < C; ++c)
+        if (indptr[c] < indptr[c + 1]) {
+            const int64_t r = (int64_t)rows[indptr[c]];
+            hi[c] = (r >= 0 && r < R) ? (int32_t)(r >> 16) : 0;
+        }
+    // columns with an entry that does not qualify; more than an eighth of the
+    // entries in them ends the compact copy
+    std::unique_ptr<std::atomic<uint8_t>[]> bad_col(new std::atomic<uint8_t>[C]);
+    for (size_t c = 0; c < C; ++c) bad_col[c].store(0, std::memory_order_relaxed);
+    std::atomic<size_t> bad_entries{0};
+    std::atomic<bool> stop{false};
+    const size_t limit = nnz / 8;
+    hipError_t e = staged(words, nnz * 4, s, [&](char* stage, size_t off, size_t n) {
+        if (stop.load()) return;
+        compact_fill<IDX>((uint32_t*)stage, vals, rows, indptr, C, off / 4, n / 4, R,
+                          [&](size_t c, size_t k) {
+                              bad_col[c].store(1, std::memory_order_relaxed);
+                              if (bad_entries.fetch_add(k) + k > limit) stop.store(true);
+                          });
+    }, 4, &stop);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? hipErrorNotSupported : e;
+    for (size_t c = 0; c < C; ++c)
+        if (bad_col[c].load(std::memory_order_relaxed)) patch.push_back((int32_t)c);
+    if ((e = hipMemcpyAsync(hi_dev, hi.data(), C * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    return hipStreamSynchronize(s);   // hi is a host temporary
+}
+
+template hipError_t upload_compact<int32_t>(uint32_t*, int32_t*, const float*, const int32_t*,
+                                            const int64_t*, size_t, size_t, int64_t, hipStream_t,
+                                            std::vector<int32_t>&);
+template hipError_t upload_compact<int64_t>(uint32_t*, int32_t*, const float*, const int64_t*,
+                                            const int64_t*, size_t, size_t, int64_t, hipStream_t,
+                                            std::vector<int32_t>&);
 
 }  // namespace hpdex_host

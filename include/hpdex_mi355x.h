@@ -215,6 +215,17 @@ void hpdex_matrix_destroy(hpdex_matrix* m);
 /* Bytes of HBM the resident matrix occupies (its reference tables below not
  * included). */
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m);
+/* How the matrix's entries crossed PCIe: 1 = compactly (a float32 CSC source:
+ * one 32-bit word per entry, the row's low 16 bits and the value as a 16-bit
+ * integer, decoded on the device; an entry qualifies when its value is a
+ * whole number in [0, 65535] with the sign bit clear and its row lies in
+ * [0, n_rows), less than 2^16 above the previous row of its column, and the
+ * columns holding one that does not -- *patched_columns of them -- are then
+ * written as they are), 0 = as given (8 bytes per entry after the int64 ->
+ * int32 row conversion: other dtypes, copies under 4 MB, or more than an
+ * eighth of the entries in columns that do not qualify).  The resident
+ * matrix is the same either way.  No counterpart in the reference.          */
+int hpdex_matrix_upload_form(const hpdex_matrix* m, int64_t* patched_columns);
 /* Reference tables kept by a resident matrix.  No counterpart in the
  * reference, which sorts each gene's reference slice once per call
  * (src/cpp/mannwhitneyu.cpp:928-929).  After the first run of a plan (or a
