@@ -385,17 +385,19 @@ def measure(name, w, steps, warmup, plain=True, cpu=True, cpu_target_s=12.0, m=N
     value = tests / (ms_step / 1e3)
     vbytes = 4 if w["data"].dtype == np.float32 else 8
     info = pb.info()
-    # roofline of the dominant kernel: its own mean duration from HIP events
-    # recorded around it on the plan's stream (plain launch sequence)
+    # roofline of the dominant kernel (the largest share of the plain launch
+    # sequence, HIP events between its kernels): its mean launch duration
+    # from HIP events around launches of it alone, back to back on the
+    # plan's stream
     stages = pb.stage_times(max(3, steps // 2))
     if stages["count"] >= max(stages["split"], stages["reference"], stages["target"]):
-        dom, dom_ms = "count_kernel", stages["count"]
+        dom = "count"
         B = synthetic.pass_bytes(info["count_nnz"], info["count_columns"], R, T, vbytes)
     else:
         dom = max(("split", "reference", "target"), key=lambda k: stages[k])
-        dom_ms = stages[dom]
-        dom = dom + "_kernel"
         B = synthetic.algorithmic_bytes(w, vbytes)
+    dom_ms = pb.kernel_ms(dom, iters=max(10, steps))
+    dom = dom + "_kernel"
     achieved = B / (dom_ms / 1e3) / 1e9
     step_B = synthetic.algorithmic_bytes(w, vbytes)
     log(f"{name}: {ms_step:.3f} ms/step, {value:.3e} tests/s; kernels {stages}; "

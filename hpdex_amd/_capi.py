@@ -34,7 +34,7 @@ EXPORTED = (
     "hpdex_de_csc_device_bytes", "hpdex_mwu_csc_noalloc",
     "hpdex_batch_create", "hpdex_batch_run", "hpdex_batch_get_info", "hpdex_batch_destroy",
     "hpdex_session_batches", "hpdex_pden_matrix", "hpdex_matrix_ref_cache_info",
-    "hpdex_matrix_ref_cache_clear", "hpdex_matrix_upload_form",
+    "hpdex_matrix_ref_cache_clear", "hpdex_matrix_upload_form", "hpdex_plan_kernel_ms",
 )
 
 LAYOUT_CSC, LAYOUT_CSR, LAYOUT_DENSE = 0, 1, 2
@@ -151,6 +151,8 @@ def lib() -> ctypes.CDLL:
         L.hpdex_session_destroy.argtypes = [vp]
         L.hpdex_session_destroy.restype = None
         L.hpdex_plan_stage_times.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_plan_kernel_ms.argtypes = [vp, ci, ci, ctypes.POINTER(ctypes.c_double), cp, sz]
+        L.hpdex_plan_kernel_ms.restype = ci
         L.hpdex_matrix_create_from.argtypes = [ctypes.POINTER(MatrixSrc), ctypes.POINTER(vp), cp, sz]
         L.hpdex_matrix_shape.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
                                          ctypes.POINTER(sz)]
@@ -530,6 +532,18 @@ class Plan:
         if lib().hpdex_plan_stage_times(self._h, int(iters), ms, err, len(err)) != 0:
             raise HpdexError(err.value.decode())
         return dict(zip(("count", "split", "reference", "target"), list(ms)))
+
+    KERNELS = ("count", "split", "reference", "target")
+
+    def kernel_ms(self, kernel, iters=20) -> float:
+        """Mean ms of one kernel ("count", "split", "reference" or "target")
+        over `iters` launches of it back to back between two HIP events."""
+        ms = ctypes.c_double()
+        err = ctypes.create_string_buffer(1024)
+        if lib().hpdex_plan_kernel_ms(self._h, self.KERNELS.index(kernel), int(iters),
+                                      ctypes.byref(ms), err, len(err)) != 0:
+            raise HpdexError(err.value.decode())
+        return ms.value
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -1738,6 +1738,40 @@ int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size
     return rc;
 }
 
+int hpdex_plan_kernel_ms(hpdex_plan* p, int kernel, int iters, double* ms, char* err,
+                         size_t err_len) {
+    if (!p || !ms) return set_err(err, err_len, "null handle");
+    if (kernel < 0 || kernel > 3) return set_err(err, err_len, "kernel must be 0, 1, 2 or 3");
+    if (iters < 1) iters = 1;
+    *ms = 0.0;
+    if (p->m->C == 0) return 0;
+    const hipStream_t s = p->st.s;
+    const PipelineLaunch L = pipeline_of(p->js, p->m->f64);
+    // the whole sequence first, so that every stage finds its inputs
+    HIPCHK(launch_mwu_pipeline(L, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    hipError_t e = hipEventRecord(e0, s);
+    for (int it = 0; it < iters && e == hipSuccess; ++it) e = launch_mwu_stage(L, kernel, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    // the whole sequence again: a split run alone leaves the reference keys
+    // unsorted until the reference kernel follows
+    if (e == hipSuccess) e = launch_mwu_pipeline(L, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        rc = set_err(err, err_len, hipGetErrorString(e));
+    } else {
+        float x = 0.f;
+        hipEventElapsedTime(&x, e0, e1);
+        *ms = (double)x / iters;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
 int hpdex_plan_run(hpdex_plan* p, int iters, double* ms_total, double* kernel_ms, char* err,
                    size_t err_len) {
     if (!p) return set_err(err, err_len, "null plan");

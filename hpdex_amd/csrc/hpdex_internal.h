@@ -394,6 +394,8 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s,
 // descriptor on the device; it must have run to completion on s.
 hipError_t launch_refcache_fill(const JobDesc* job, bool f64, const int32_t* cols, uint32_t n,
                                 RefCacheCol* rc, void* rc_keys, uint32_t* rc_tab, hipStream_t s);
+// One stage of the sequence (0 count, 1 split, 2 reference, 3 target) alone.
+hipError_t launch_mwu_stage(const PipelineLaunch& L, int stage, hipStream_t s);
 // Allow the count kernel `bytes` of dynamic LDS and the split kernel
 // kSplitLdsMax (once per process).
 hipError_t prepare_count_kernel(uint32_t bytes);

@@ -2586,6 +2586,51 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
     return hipGetLastError();
 }
 
+// One stage of the launch sequence alone (0 count, 1 split, 2 reference, 3
+// target: the window and sorted passes), for timing back-to-back launches of
+// it.  Every stage may run again on the outputs of the sequence before it:
+// each writes what it wrote the first time.
+hipError_t launch_mwu_stage(const PipelineLaunch& L, int stage, hipStream_t s) {
+    if (stage == 0) {
+        if (L.n_count <= 0) return hipSuccess;
+#define HPDEX_COUNT(V, NT) \
+    count_kernel<V, NT><<<L.n_count, NT, L.count_lds, s>>>(L.job, L.count_items, L.count_lds)
+        if (L.f64) {
+            if (L.count_block == 256) HPDEX_COUNT(double, 256);
+            else if (L.count_block == 1024) HPDEX_COUNT(double, 1024);
+            else HPDEX_COUNT(double, 512);
+        } else {
+            if (L.count_block == 256) HPDEX_COUNT(float, 256);
+            else if (L.count_block == 1024) HPDEX_COUNT(float, 1024);
+            else HPDEX_COUNT(float, 512);
+        }
+#undef HPDEX_COUNT
+        return hipGetLastError();
+    }
+    if (L.n_gen <= 0) return hipSuccess;
+    const unsigned split_lds = (unsigned)L.split_lds_max;
+    const bool global_split = split_counter_bytes((uint64_t)L.G) > (uint64_t)split_lds;
+    const unsigned tab_lds = table_lds_bytes(L.f64);
+    const unsigned grid = (unsigned)L.n_gen, n = (uint32_t)L.n_gen;
+    auto one = [&](auto v_tag) {
+        using V = typename decltype(v_tag)::type;
+        if (stage == 1) {
+            if (global_split)
+                split_global_kernel<V><<<grid, kColBlock, 0, s>>>(L.jobs, L.gen_items, n);
+            else
+                split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
+                                                                     split_lds);
+        } else if (stage == 2) {
+            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+        } else {
+            launch_target_pass<V>(L.jobs, L.gen_items, n, s);
+        }
+    };
+    if (L.f64) one(TypeTag<double>{});
+    else one(TypeTag<float>{});
+    return hipGetLastError();
+}
+
 hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
     if (L.n_count > 0) {
 #define HPDEX_COUNTQ(V, NT)                                                                  \

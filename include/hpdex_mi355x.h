@@ -293,6 +293,13 @@ int hpdex_plan_get_info(hpdex_plan* p, hpdex_plan_info* out, char* err, size_t e
  * (a kernel with no columns times as an empty launch).  The results are
  * those of a normal run.                                                    */
 int hpdex_plan_stage_times(hpdex_plan* p, int iters, double* ms, char* err, size_t err_len);
+/* Mean duration of one kernel of the plan (0 count, 1 split, 2 reference, 3
+ * target: its window and sorted passes) over `iters` launches of it back to
+ * back on the plan's stream, between two HIP events, with one run of the
+ * whole launch sequence before and one after (so the plan's results and
+ * state are those of a normal run).  No counterpart in the reference.       */
+int hpdex_plan_kernel_ms(hpdex_plan* p, int kernel, int iters, double* ms, char* err,
+                         size_t err_len);
 
 /* ---------------- batches: several plans in one launch sequence ----------------
  * No counterpart in the reference.  A batch runs several plans (jobs on

@@ -688,3 +688,35 @@ def test_exact_workload_full_size():
     o = oracle_stats(sub, "min", method="exact", cc=False)
     g = tuple(a.reshape(-1, T)[cols].reshape(-1) for a in (U1, P, tie, n1, n2))
     assert_parity(g, o)
+
+
+def test_plan_kernel_ms_keeps_results():
+    """Timing each kernel alone (back-to-back launches between HIP events)
+    leaves the plan's results and state those of a normal run, on a job with
+    count-path and sort-path columns."""
+    say("plan kernel_ms")
+    rng = np.random.default_rng(29)
+    R, T = 6000, 12
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.6, 0.3, 0.5, 0.2]))
+    data = rng.integers(1, 20, len(rows)).astype(np.float32)
+    data[indptr[2]:] += np.float32(0.5)   # two float columns: the sort path
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:1500]] = 0
+    gid[perm[1500:]] = rng.integers(1, T + 1, R - 1500)
+    w = dict(data=data, indices=rows, indptr=indptr, group_id=gid, n_targets=T, n_rows=R)
+    m, p = _plan_of(w, "min", form=1)
+    p.run(1)
+    want = p.results(with_stats=True)
+    for k in ("count", "split", "reference", "target"):
+        assert p.kernel_ms(k, iters=5) > 0.0
+        for a, b in zip(p.results(with_stats=True), want):
+            assert np.array_equal(a, b)
+    p.run(1)
+    for a, b in zip(p.results(with_stats=True), want):
+        assert np.array_equal(a, b)
+    assert_parity(want, oracle_stats(w, "min"))
+    with pytest.raises(ValueError):
+        p.kernel_ms("nothing")
+    p.close()
+    m.close()
