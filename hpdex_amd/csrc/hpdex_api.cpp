@@ -209,14 +209,13 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
     m->h_range.assign(2 * C, 0);
     HIPCHK(m->colrange.alloc(2 * C * 4));
     HIPCHK(m->colident.alloc(C ? C : 1));
+    HIPCHK(m->colint.alloc(C ? C : 1));
     if (C) {
-        DevBuf flags;
-        HIPCHK(flags.alloc(C));
         HIPCHK(launch_classify_columns(m->vals.p, m->f64, m->indptr.as<int64_t>(), (int64_t)C,
-                                       flags.as<uint8_t>(), m->colrange.as<int32_t>(),
+                                       m->colint.as<uint8_t>(), m->colrange.as<int32_t>(),
                                        m->rows.as<int32_t>(), (uint32_t)m->R,
                                        m->colident.as<uint8_t>(), s));
-        HIPCHK(hipMemcpyAsync(m->col_int.data(), flags.p, C, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(m->col_int.data(), m->colint.p, C, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(m->h_range.data(), m->colrange.p, 2 * C * 4, hipMemcpyDeviceToHost,
                               s));
         HIPCHK(hipStreamSynchronize(s));
@@ -424,6 +423,10 @@ int refcache_fill(JobState& js, const hpdex_matrix& m, hipStream_t s, char* err,
         rc.koff = koff;
         rc.toff = toff;
         rc.TA = pl.TA;
+        rc.kmin = pl.kmin;
+        rc.kmax = pl.kmax;
+        rc.shift = pl.shift;
+        rc.refneg = pl.refneg;
         rc.na = pl.na;
         rc.nb = pl.nb;
         koff += pl.na;
@@ -524,6 +527,7 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.group_rows = js.grows.as<uint32_t>();
     J.colrange = m->colrange.as<int32_t>();
     J.colident = m->colident.as<uint8_t>();
+    J.colint = m->colint.as<uint8_t>();
     J.packed = m->packed.p ? m->packed.as<uint32_t>() : nullptr;
     J.pack_vb = m->pack_vb;
     J.keys = js.keys.p;
@@ -1607,7 +1611,8 @@ void hpdex_matrix_ref_cache_clear(hpdex_matrix* m) {
 
 size_t hpdex_matrix_device_bytes(const hpdex_matrix* m) {
     if (!m) return 0;
-    return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n + m->colident.n + m->packed.n;
+    return m->vals.n + m->rows.n + m->indptr.n + m->colrange.n + m->colident.n + m->colint.n +
+           m->packed.n;
 }
 
 }  // extern "C"

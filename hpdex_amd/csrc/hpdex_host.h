@@ -168,6 +168,7 @@ struct hpdex_matrix {
     hpdex_host::DevBuf vals, rows, indptr;
     hpdex_host::DevBuf colrange;    // 2C int32: value range of integer-valued columns
     hpdex_host::DevBuf colident;    // C bytes: 1 = the column stores rows 0 .. R-1 in order
+    hpdex_host::DevBuf colint;      // C bytes: device copy of col_int
     std::vector<int64_t> h_indptr;
     std::vector<int32_t> order;     // columns, largest first
     std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or

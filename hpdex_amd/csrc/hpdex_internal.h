@@ -93,10 +93,13 @@ struct RefCacheCol {
     int64_t koff;
     int64_t toff;
     uint64_t TA;
+    uint64_t kmin, kmax;   // the reference's key range
     uint32_t na;
     uint32_t nb;
     uint32_t maxb;
-    uint32_t valid;   // 1: the entries above hold this column's tables
+    uint32_t shift;        // bucket = (key - kmin) >> shift
+    uint32_t refneg;       // reference values below zero
+    uint32_t valid;        // 1: the entries above hold this column's tables
 };
 
 // One Mann-Whitney job: a device-resident CSC matrix, its group labels and
@@ -112,6 +115,8 @@ struct JobDesc {
     const uint32_t* group_rows;  // G rows per group
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
     const uint8_t* colident;     // C: 1 = the column stores rows 0 .. R-1 in order
+    const uint8_t* colint;       // C: 1 = every stored value is a small integer (or non-
+                                 // finite); 0 = the column cannot take a histogram branch
     const uint32_t* packed;      // nnz packed entries (pack_entry) or null; CountItem flags
     uint32_t pack_vb;            // value bits of a packed entry
     // workspace

@@ -395,24 +395,42 @@ __device__ inline void pair_sizes(const JobDesc& J, int64_t c, uint32_t g, uint3
 // of nonzero values into the column's key range (mannwhitneyu.cpp:913-926).
 // ---------------------------------------------------------------------------
 struct SplitState {
-    uint64_t kmin, kmax;
     uint32_t all_int;
     int32_t vmin, vmax;
-    uint32_t refneg;
     uint32_t hist;
     uint32_t scan[20];
 };
 
+// Buckets of a sort-path column's reference index: the power of two of at
+// least na, 64 to kMaxBuckets.
+__device__ inline uint32_t ref_buckets(uint32_t na) {
+    uint32_t nbk = 64;
+    while (nbk < na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
+    return nbk;
+}
+
+// bucket = (key - kmin) >> shift: the shift that maps the key range onto nb
+// buckets
+__device__ inline uint32_t ref_shift(uint64_t kmin, uint64_t kmax, uint32_t na, uint32_t nb) {
+    const uint64_t rng = na > 0 ? kmax - kmin : 0ull;
+    const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
+    const uint32_t lg = 31u - (uint32_t)__clz(nb);
+    return bits > lg ? bits - lg : 0u;
+}
+
 // The column's plan from its split statistics (na: its reference's stored
 // nonzero finite values): the histogram branch when every value is an integer
-// of a narrow enough range, else the reference's key range and bucket shift.
+// of a narrow enough range, else the sort path with a bucket index of nb
+// buckets.  The reference kernel adds the reference's key range, bucket shift
+// and values below zero (it reads every reference value anyway, sorted).
 // Leaves the plan's hist and vmin in st.
 __device__ inline void write_column_plan(const JobDesc& J, int64_t c, uint32_t na, SplitState& st) {
     ColPlan pl;
     pl.na = na;
-    pl.refneg = st.refneg;
-    pl.kmin = st.kmin;
-    pl.kmax = st.kmax;
+    pl.refneg = 0;
+    pl.kmin = ~0ull;
+    pl.kmax = 0;
+    pl.shift = 0;
     pl.TA = 0;
     pl.maxb = 0;
     pl.bigt = 0;
@@ -421,16 +439,9 @@ __device__ inline void write_column_plan(const JobDesc& J, int64_t c, uint32_t n
         pl.hist = 1;
         pl.nb = (uint32_t)range;
         pl.vmin = (st.vmax >= st.vmin) ? st.vmin : 0;
-        pl.shift = 0;
     } else {
         pl.hist = 0;
-        uint32_t nbk = 64;
-        while (nbk < pl.na && nbk < (uint32_t)kMaxBuckets) nbk <<= 1;
-        const uint64_t rng = (pl.na > 0) ? pl.kmax - pl.kmin : 0ull;
-        const uint32_t bits = rng ? 64u - (uint32_t)__clzll(rng) : 0u;
-        const uint32_t lg = 31u - (uint32_t)__clz(nbk);
-        pl.nb = nbk;
-        pl.shift = bits > lg ? bits - lg : 0u;
+        pl.nb = ref_buckets(na);
         pl.vmin = 0;
     }
     J.plan[c] = pl;
@@ -475,6 +486,9 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // a column storing rows 0 .. R-1 in order (a dense matrix held as CSC):
     // rows from positions, so only its values are read
     const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
+    // a column whose values are not all small integers (the matrix's metadata)
+    // cannot take the histogram branch: its values' statistics are not taken
+    const bool int_col = __builtin_amdgcn_readfirstlane(!J.colint || J.colint[c] ? 1 : 0) != 0;
     // sparse columns through the two-stage reader (the label gathers of one
     // batch overlap the visits of the one before: 2.06 -> 1.98 ms on a
     // 400,000 x 18,000 split at 5 %), columns in row order through the
@@ -501,8 +515,7 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         cur[g] = g < G ? J.group_rows[g] : 0u;
     }
     if (tid == 0) {
-        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
-        st.kmin = ~0ull; st.kmax = 0;
+        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN;
     }
     __syncthreads();
     const uint32_t slots = block_excl_scan(cur, G, st.scan);
@@ -537,10 +550,8 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     // fixed_tag: 0 scatter later, 1 keys to their slots in HBM, 2 in LDS
     auto count_pass = [&](auto fixed_tag) {
         constexpr int kFixed = decltype(fixed_tag)::value;
-        bool li = true;
+        bool li = int_col;
         int lmin = INT_MAX, lmax = INT_MIN;
-        K kmn = KT::kSentinel, kmx = 0;
-        uint32_t neg = 0;
         grouped_batch([&](const auto& g, const auto& v, uint32_t e0) {
             constexpr int U = (int)(sizeof(g) / sizeof(g[0]));
             bool nz[U];
@@ -572,29 +583,20 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
                 } else {
                     if (pos[u] < slots) stage[pos[u]] = k;
                 }
-                int iv;
-                if (KT::small_int(v[u], iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
-                else li = false;
-                if (g[u] == 0) {
-                    kmn = k < kmn ? k : kmn;
-                    kmx = k > kmx ? k : kmx;
-                    neg += (v[u] < V(0)) ? 1u : 0u;
+                if (int_col) {
+                    int iv;
+                    if (KT::small_int(v[u], iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+                    else li = false;
                 }
             }
         });
         const bool wall = __all(li ? 1 : 0);
         lmin = wave_min_i(lmin);
         lmax = wave_max_i(lmax);
-        kmn = wave_min_k<K>(kmn);
-        kmx = wave_max_k<K>(kmx);
-        neg = wave_sum_u32(neg);
         if (lane == 0) {
             if (!wall) atomicAnd(&st.all_int, 0u);
             atomicMin(&st.vmin, lmin);
             atomicMax(&st.vmax, lmax);
-            atomicAdd(&st.refneg, neg);
-            atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
-            atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
     };
     if (arrival) __syncthreads();   // the cleared groups before the pass writes them
@@ -735,6 +737,7 @@ __device__ __forceinline__ void split_global_item(const JobDesc* jobs, const Wor
     const bool packed = __builtin_amdgcn_readfirstlane(
         (J.gid_bits > 0u && 4 * (p1 - p0) < (int64_t)R) ? 1 : 0) != 0;
     const bool in_order = __builtin_amdgcn_readfirstlane(J.colident && J.colident[c] ? 1 : 0) != 0;
+    const bool int_col = __builtin_amdgcn_readfirstlane(!J.colint || J.colint[c] ? 1 : 0) != 0;
     auto grouped = [&](auto&& f) {
         if (packed) for_each_grouped_pipelined<V, NT, true, false, 4>(J, p0, p1 - p0, f);
         else if (in_order) for_each_grouped<V, NT, false, true, false, kUnroll>(J, p0, p1 - p0, f);
@@ -746,8 +749,7 @@ __device__ __forceinline__ void split_global_item(const JobDesc* jobs, const Wor
         off[g] = J.group_rows[g];
     }
     if (tid == 0) {
-        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN; st.refneg = 0;
-        st.kmin = ~0ull; st.kmax = 0;
+        st.all_int = 1u; st.vmin = INT_MAX; st.vmax = INT_MIN;
     }
     __syncthreads();
     // slots: group g's keys at the prefix of the group sizes when the column's
@@ -756,10 +758,8 @@ __device__ __forceinline__ void split_global_item(const JobDesc* jobs, const Wor
     const bool fixed = __builtin_amdgcn_readfirstlane((int64_t)slots <= p1 - p0 ? 1 : 0) != 0;
     K* keys = (K*)J.keys + p0;
     {
-        bool li = true;
+        bool li = int_col;
         int lmin = INT_MAX, lmax = INT_MIN;
-        K kmn = KT::kSentinel, kmx = 0;
-        uint32_t neg = 0;
         grouped([&](uint32_t g, V v, uint32_t) {
             if (!KT::finite(v)) return;
             if (v == V(0)) {
@@ -771,28 +771,19 @@ __device__ __forceinline__ void split_global_item(const JobDesc* jobs, const Wor
             // a group holding more entries than rows (repeated row indices)
             // is an error of the column; its keys stay inside the column's range
             if (fixed && off[g] + pos < n_col) keys[off[g] + pos] = key;
-            int iv;
-            if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
-            else li = false;
-            if (g == 0) {
-                kmn = key < kmn ? key : kmn;
-                kmx = key > kmx ? key : kmx;
-                neg += (v < V(0)) ? 1u : 0u;
+            if (int_col) {
+                int iv;
+                if (KT::small_int(v, iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+                else li = false;
             }
         });
         const bool wall = __all(li ? 1 : 0);
         lmin = wave_min_i(lmin);
         lmax = wave_max_i(lmax);
-        kmn = wave_min_k<K>(kmn);
-        kmx = wave_max_k<K>(kmx);
-        neg = wave_sum_u32(neg);
         if (lane == 0) {
             if (!wall) atomicAnd(&st.all_int, 0u);
             atomicMin(&st.vmin, lmin);
             atomicMax(&st.vmax, lmax);
-            atomicAdd(&st.refneg, neg);
-            atomicMin((unsigned long long*)&st.kmin, (unsigned long long)kmn);
-            atomicMax((unsigned long long*)&st.kmax, (unsigned long long)kmx);
         }
     }
     __syncthreads();
@@ -1080,13 +1071,23 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
     // tables the matrix keeps for these reference rows: nothing to sort
     if (!hist && ref_cached(J, c, na, pl.nb)) {
         if (tid == 0) {
-            J.plan[c].TA = J.rc[c].TA;
-            J.plan[c].maxb = J.rc[c].maxb;
+            const RefCacheCol& e = J.rc[c];
+            ColPlan& q = J.plan[c];
+            q.TA = e.TA;
+            q.maxb = e.maxb;
+            q.kmin = e.kmin;
+            q.kmax = e.kmax;
+            q.shift = e.shift;
+            q.refneg = e.refneg;
         }
         return;
     }
     uint64_t ta = 0;
     uint32_t maxb = 0;   // keys in the fullest bucket of an on-chip reference
+    // the reference's key range, bucket shift and values below zero (sort
+    // path: read off the sorted keys; histogram branch: off its bins)
+    uint64_t kmin = ~0ull, kmax = 0;
+    uint32_t shift = 0, refneg = 0;
     // a reference slice beyond the on-chip sort: sorted in HBM with the
     // column's spill space, its bucket index built from the sorted keys there
     const int64_t spill_at =
@@ -1097,16 +1098,24 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         K* rk = (K*)smem;
         uint32_t* bidx = (uint32_t*)(rk + cap);
         sort_ref_global<K>(base, (K*)J.spill + spill_at, na, rk, cap, KT::kSentinel);
+        kmin = (uint64_t)base[0];   // na > cap > 0
+        kmax = (uint64_t)base[na - 1u];
+        shift = ref_shift(kmin, kmax, na, pl.nb);
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
         __syncthreads();
-        for (uint32_t x = tid; x < na; x += nt)
-            atomicAdd(&bidx[(uint32_t)(((uint64_t)base[x] - pl.kmin) >> pl.shift)], 1u);
+        uint64_t neg = 0;
+        for (uint32_t x = tid; x < na; x += nt) {
+            const K key = base[x];
+            atomicAdd(&bidx[(uint32_t)(((uint64_t)key - kmin) >> shift)], 1u);
+            neg += key < KT::kPosBit ? 1u : 0u;
+        }
         __syncthreads();
+        refneg = (uint32_t)block_sum_u64(neg, st.red);
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
         RefView<K> rv;
         rv.cum = nullptr; rv.keys = base; rv.lkeys = nullptr; rv.bidx = bidx;
-        rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
         rv.big = 1; rv.nb = pl.nb; rv.iters = 0;
         for (uint32_t x = tid; x < na; x += nt) {
             if (x > 0 && base[x - 1] == base[x]) continue;   // not the first of its run
@@ -1125,6 +1134,8 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         ta = block_sum_u64(ta, st.red);
         block_excl_scan(cum, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = cum[b];
+        // values below zero: the bins below bin -vmin
+        if (pl.vmin < 0) refneg = cum[min((uint32_t)(-pl.vmin), pl.nb)];
     } else {
         uint32_t P = 64;
         while (P < na) P <<= 1;
@@ -1134,17 +1145,26 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
         __syncthreads();
         sort_ref<K>(rk, P);
+        if (na > 0) {
+            kmin = (uint64_t)rk[0];
+            kmax = (uint64_t)rk[na - 1u];
+        }
+        shift = ref_shift(kmin, kmax, na, pl.nb);
+        uint64_t neg = 0;
         for (uint32_t x = tid; x < na; x += nt) {
-            base[x] = rk[x];
-            atomicAdd(&bidx[(uint32_t)(((uint64_t)rk[x] - pl.kmin) >> pl.shift)], 1u);
+            const K key = rk[x];
+            base[x] = key;
+            atomicAdd(&bidx[(uint32_t)(((uint64_t)key - kmin) >> shift)], 1u);
+            neg += key < KT::kPosBit ? 1u : 0u;
         }
         __syncthreads();
+        refneg = (uint32_t)block_sum_u64(neg, st.red);
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
         maxb = fullest_bucket(bidx, pl.nb, st);
         RefView<K> rv;
         rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
-        rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = 0;
+        rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
         rv.big = 0; rv.nb = pl.nb; rv.iters = halvings(maxb);
         for (uint32_t x = tid; x < na; x += nt) {
             uint32_t lt, eq;
@@ -1154,8 +1174,13 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         ta = block_sum_u64(ta, st.red);
     }
     if (tid == 0 && (hist || na <= cap || spill_at >= 0)) {
-        J.plan[c].TA = ta;
-        J.plan[c].maxb = maxb;
+        ColPlan& q = J.plan[c];
+        q.TA = ta;
+        q.maxb = maxb;
+        q.kmin = kmin;
+        q.kmax = kmax;
+        q.shift = shift;
+        q.refneg = refneg;
     }
 }
 

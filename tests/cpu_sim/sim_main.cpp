@@ -103,6 +103,7 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                                                               (uint32_t)J.R, col_ident.data()); },
                   timeout_s);
     J.colident = col_ident.data();
+    J.colint = col_int.data();
     // the matrix's packed entries (finish_matrix), made from the recorded ranges
     const uint32_t vb = 32u - pack_row_bits((uint32_t)J.R);
     std::vector<int32_t> lo(C, INT_MIN);
@@ -275,6 +276,8 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     for (int32_t c : sim_gen_cols) {
         if (plan[c].hist) continue;
         rc[c].koff = koff; rc[c].toff = toff; rc[c].TA = plan[c].TA;
+        rc[c].kmin = plan[c].kmin; rc[c].kmax = plan[c].kmax;
+        rc[c].shift = plan[c].shift; rc[c].refneg = plan[c].refneg;
         rc[c].na = plan[c].na; rc[c].nb = plan[c].nb;
         koff += plan[c].na;
         toff += (int64_t)plan[c].nb + 1;
