@@ -216,6 +216,10 @@ int finish_matrix(hpdex_matrix* m, hipStream_t s, char* err, size_t err_len) {
                                        m->rows.as<int32_t>(), (uint32_t)m->R,
                                        m->colident.as<uint8_t>(), s));
         HIPCHK(hipMemcpyAsync(m->col_int.data(), m->colint.p, C, hipMemcpyDeviceToHost, s));
+        std::vector<uint8_t> ident(C);
+        HIPCHK(hipMemcpyAsync(ident.data(), m->colident.p, C, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        m->any_ident = std::find(ident.begin(), ident.end(), (uint8_t)1) != ident.end();
         HIPCHK(hipMemcpyAsync(m->h_range.data(), m->colrange.p, 2 * C * 4, hipMemcpyDeviceToHost,
                               s));
         HIPCHK(hipStreamSynchronize(s));
@@ -498,6 +502,33 @@ int job_setup(JobState& js, const hpdex_matrix* m, const int32_t* group_id, size
     }
     HIPCHK(js.grows.alloc(js.G * 4));
     HIPCHK(hipMemcpyAsync(js.grows.p, js.group_rows.data(), js.G * 4, hipMemcpyHostToDevice, s));
+    // the rows in a group, ascending, with their groups: the split of a
+    // column in row order gathers their values (a wave's members span a few
+    // hundred rows, so its gathers fall in a few cache lines)
+    js.n_members = 0;
+    if (m->any_ident) {
+        uint32_t M = 0;
+        for (size_t g = 0; g < js.G; ++g) M += js.group_rows[g];
+        std::vector<uint32_t> mem(M ? M : 1), mgr(M ? M : 1);
+        uint32_t at = 0;
+        for (size_t r = 0; r < m->R; ++r) {
+            const int g = group_id[r];
+            if (g >= 0 && (size_t)g < js.G) {
+                mem[at] = (uint32_t)r;
+                mgr[at] = (uint32_t)g;
+                ++at;
+            }
+        }
+        HIPCHK(js.members.alloc((size_t)M * 4));
+        HIPCHK(js.member_group.alloc((size_t)M * 4));
+        if (M) {
+            HIPCHK(hipMemcpyAsync(js.members.p, mem.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(js.member_group.p, mgr.data(), (size_t)M * 4,
+                                  hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));   // mem and mgr are host temporaries
+        js.n_members = M;
+    }
     HIPCHK(js.U1.alloc(np * 8));
     HIPCHK(js.P.alloc(np * 8));
     // sample sizes are also needed by the exact p of zero_handling none
@@ -525,6 +556,9 @@ JobDesc make_desc(const JobState& js, const hpdex_matrix* m, const Tuning& tu) {
     J.gid_bits = js.gid_bits;
     J.gid_words = (uint32_t)js.gid_words;
     J.group_rows = js.grows.as<uint32_t>();
+    J.members = js.n_members ? js.members.as<uint32_t>() : nullptr;
+    J.member_group = js.n_members ? js.member_group.as<uint32_t>() : nullptr;
+    J.n_members = js.n_members;
     J.colrange = m->colrange.as<int32_t>();
     J.colident = m->colident.as<uint8_t>();
     J.colint = m->colint.as<uint8_t>();
@@ -903,6 +937,7 @@ size_t job_footprint(size_t R, size_t C, size_t nnz, bool f64, size_t T, const h
     // error block, the tie term and sample sizes when kept, sort-path workspace
     size_t f = up256(R * 4) + up256(label_words(R, 16) * 4) + up256(G * 4) + 2 * up256(np * 8) +
                up256(32);
+    f += 2 * up256(R * 4);   // the groups' rows and groups (matrices with columns in row order)
     if (stats) f += up256(np * 8);
     if (stats || (o.method == 1 && o.zero_handling == 0)) f += 2 * up256(np * 8);
     f += up256(nnz * vsz) + 3 * up256(C * G * 4) + up256(C * sizeof(ColPlan));

@@ -169,6 +169,7 @@ struct hpdex_matrix {
     hpdex_host::DevBuf colrange;    // 2C int32: value range of integer-valued columns
     hpdex_host::DevBuf colident;    // C bytes: 1 = the column stores rows 0 .. R-1 in order
     hpdex_host::DevBuf colint;      // C bytes: device copy of col_int
+    bool any_ident = false;         // some column stores rows 0 .. R-1 in order
     std::vector<int64_t> h_indptr;
     std::vector<int32_t> order;     // columns, largest first
     std::vector<uint8_t> col_int;   // 1: every stored value is an integer in 16 bits (or
@@ -199,6 +200,8 @@ struct JobState {
     hpdex_mwu_opts opts{};
     std::vector<uint32_t> group_rows;
     DevBuf gid, gidp, grows, U1, P, tie, n1, n2, flags;  // flags: err_key, status_col, status
+    DevBuf members, member_group;   // the groups' rows, group by group (matrices with
+    uint32_t n_members = 0;         // columns in row order)
     uint32_t gid_bits = 0;
     size_t gid_words = 0;
     DevBuf keys, cnt, zcnt, goff, plan, reftab;     // pipeline workspace

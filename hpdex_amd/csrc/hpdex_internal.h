@@ -113,6 +113,13 @@ struct JobDesc {
     uint32_t gid_bits;
     uint32_t gid_words;          // 32-bit words of gidp
     const uint32_t* group_rows;  // G rows per group
+    // the rows in groups 0 .. G-1, ascending, and their groups: n_members =
+    // the sum of group_rows.  Set when the matrix has columns storing rows
+    // 0 .. R-1 in order, whose split gathers these rows' values instead of
+    // reading every entry (or null)
+    const uint32_t* members;
+    const uint32_t* member_group;
+    uint32_t n_members;
     const int32_t* colrange;     // 2C: value range of integer-valued columns (matrix metadata)
     const uint8_t* colident;     // C: 1 = the column stores rows 0 .. R-1 in order
     const uint8_t* colint;       // C: 1 = every stored value is a small integer (or non-

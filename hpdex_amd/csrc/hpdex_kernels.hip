@@ -599,8 +599,73 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
             atomicMax(&st.vmax, lmax);
         }
     };
+    // A column storing rows 0 .. R-1 in order (a dense matrix held as CSC):
+    // the value of row r is entry r, so only the group members' values are
+    // read, gathered through the job's member list (rows ascending: a wave's
+    // 64 members span a few hundred rows, a few cache lines); the rows in no
+    // group -- most of a dense column when the groups are small -- take no
+    // label gather, counter or key.  Counts, slot keys and statistics as the
+    // count pass gives them.
+    auto member_pass = [&](auto fixed_tag) {
+        constexpr int kFixed = decltype(fixed_tag)::value;   // 1 slots in HBM, 2 in LDS
+        constexpr int U = 4;
+        const uint32_t M = J.n_members;
+        const V* col = vals + p0;
+        bool li = int_col;
+        int lmin = INT_MAX, lmax = INT_MIN;
+        for (uint32_t s0 = (uint32_t)tid; s0 < M; s0 += (uint32_t)nt * U) {
+            uint32_t gi[U], pos[U];
+            V v[U];
+            bool nz[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t sl = s0 + (uint32_t)(u * nt);
+                const bool ok = sl < M;
+                const uint32_t r = ok ? gl(J.members)[sl] : 0u;
+                gi[u] = ok ? gl(J.member_group)[sl] : dummy;
+                v[u] = ok ? gl(col)[r] : V(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = gi[u] < (uint32_t)G && KT::finite(v[u]);
+                nz[u] = in && !(v[u] == V(0));
+                if (!in) gi[u] = dummy;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                pos[u] = (uint32_t)atomicAdd(&cnt[gi[u]], nz[u] ? 1ull : (1ull << 32));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!nz[u]) continue;
+                const uint32_t at = pos[u] + cur[gi[u]];
+                const K k = KT::key(v[u]);
+                if (kFixed == 1) {
+                    if ((int64_t)at < p1 - p0) keys[at] = k;
+                } else {
+                    if (at < slots) stage[at] = k;
+                }
+                if (int_col) {
+                    int iv;
+                    if (KT::small_int(v[u], iv)) { lmin = min(lmin, iv); lmax = max(lmax, iv); }
+                    else li = false;
+                }
+            }
+        }
+        const bool wall = __all(li ? 1 : 0);
+        lmin = wave_min_i(lmin);
+        lmax = wave_max_i(lmax);
+        if (lane == 0) {
+            if (!wall) atomicAnd(&st.all_int, 0u);
+            atomicMin(&st.vmin, lmin);
+            atomicMax(&st.vmax, lmax);
+        }
+    };
+    const bool by_members =
+        __builtin_amdgcn_readfirstlane(in_order && fixed && J.members ? 1 : 0) != 0;
     if (arrival) __syncthreads();   // the cleared groups before the pass writes them
-    if (fixed && staged) count_pass(std::integral_constant<int, 2>{});
+    if (by_members && staged) member_pass(std::integral_constant<int, 2>{});
+    else if (by_members) member_pass(std::integral_constant<int, 1>{});
+    else if (fixed && staged) count_pass(std::integral_constant<int, 2>{});
     else if (fixed) count_pass(std::integral_constant<int, 1>{});
     else count_pass(std::integral_constant<int, 0>{});
     __syncthreads();

@@ -50,6 +50,8 @@ def lib(path=None):
         L.sim_cached_columns.restype = ci
         L.sim_set_global_split.argtypes = [ci]
         L.sim_set_global_split.restype = None
+        L.sim_set_members.argtypes = [ci]
+        L.sim_set_members.restype = None
         _lib = L
     return _lib
 
@@ -57,14 +59,16 @@ def lib(path=None):
 def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=True, cc=True,
         count_lds=40448, block=256, hist_max=8192, f64=False, timeout_s=30.0,
         shrink_range=False, pack=True, split_arrival=True, ref_cap=8192, refcache=False,
-        global_split=False):
+        global_split=False, members=True):
     """Run the launch sequence on the host.  count_lds: dynamic LDS of the
     count kernel (0 = every column takes the split/reference/target kernels).
     refcache: run the job, keep its reference tables with the fill kernel,
     then run it again with the job's own tables wiped, ranking every kept
     column from the cache (mwu.last_cached: the kept columns).  global_split:
     the split with its group counters in memory (the form taken when the
-    groups' counters do not fit LDS), whatever the group count."""
+    groups' counters do not fit LDS), whatever the group count.  members: the
+    job's member list (columns in row order gather their groups' rows through
+    it; without it they are read entry by entry)."""
     # laid out like a resident matrix: PAD padding entries (value 0, row -1)
     vals = np.concatenate([np.asarray(data, dtype=np.float64 if f64 else np.float32),
                            np.zeros(PAD, dtype=np.float64 if f64 else np.float32)])
@@ -83,6 +87,7 @@ def mwu(data, indices, indptr, group_id, n_targets, n_rows, mode=1, alt=2, tie=T
     lib().sim_set_ref_cap(int(ref_cap))   # the LDS/global sort cutover of the reference
     lib().sim_set_refcache(int(refcache))
     lib().sim_set_global_split(int(global_split))
+    lib().sim_set_members(int(members))
     counted = lib().sim_mwu(int(f64), vals.ctypes.data, rows.ctypes.data, indptr.ctypes.data, n_rows, C,
                   gid.ctypes.data, n_targets, mode, alt, int(tie), int(cc), count_lds, block, hist_max,
                   *[o.ctypes.data for o in outs], err.ctypes.data, status.ctypes.data,

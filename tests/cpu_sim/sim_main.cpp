@@ -32,6 +32,7 @@ int sim_shrink_range = 0;  // test hook: record each column's range one value sh
 int sim_pack = 1;          // the count kernel reads packed entries where a column has them
 int sim_split_arrival = 1; // the split keeps entry groups in LDS (tuning split_arrival)
 uint32_t sim_ref_cap = 8192; // the LDS/global sort cutover (tuning ref_lds_keys)
+int sim_members = 1;       // the member list for columns in row order (job_setup's)
 int sim_global_split = 0;  // test hook: the split with its group counters in memory
                            // (split_global_kernel) whatever the group count
 int sim_refcache = 0;      // run the job, keep its reference tables (refcache_fill_kernel),
@@ -227,6 +228,20 @@ extern "C" int sim_mwu(int f64, const void* vals, const int32_t* rows, const int
     const uint32_t bits = label_bits((uint32_t)G);
     std::vector<uint32_t> gidp(label_words((size_t)R, bits));
     pack_labels(gid, (size_t)R, (uint32_t)G, bits, gidp.data());
+    // the rows in a group, ascending, with their groups (job_setup builds
+    // them for matrices with columns in row order; here always: the split
+    // uses them for those columns only)
+    std::vector<uint32_t> members, mgroup;
+    for (int r = 0; r < R; ++r)
+        if (gid[r] >= 0 && gid[r] < G) {
+            members.push_back((uint32_t)r);
+            mgroup.push_back((uint32_t)gid[r]);
+        }
+    members.push_back(0u);
+    mgroup.push_back(0u);
+    J.members = sim_members ? members.data() : nullptr;
+    J.member_group = sim_members ? mgroup.data() : nullptr;
+    J.n_members = sim_members ? (uint32_t)(members.size() - 1) : 0u;
     J.vals = vals; J.rows = rows; J.indptr = indptr; J.gid = gid; J.group_rows = grows.data();
     J.gidp = gidp.data();
     J.gid_bits = bits;
@@ -321,4 +336,5 @@ extern "C" void sim_set_split_arrival(int on) { sim_split_arrival = on; }
 extern "C" void sim_set_ref_cap(int keys) { sim_ref_cap = (uint32_t)keys; }
 extern "C" void sim_set_refcache(int on) { sim_refcache = on; }
 extern "C" void sim_set_global_split(int on) { sim_global_split = on; }
+extern "C" void sim_set_members(int on) { sim_members = on; }
 extern "C" int sim_cached_columns() { return sim_last_cached; }

@@ -506,3 +506,36 @@ def test_sim_split_with_counters_in_memory(f64, modes):
     gid[perm[400:1100]] = rng.integers(1, T + 1, 700)
     for mode in modes:
         run_both(data, rows, indptr, gid, T, R, mode, f64=f64, count_lds=0, global_split=True)
+
+
+@pytest.mark.parametrize("f64,mode", [(False, "min"), (False, "none"), (True, "mix")],
+                         ids=["f32-min", "f32-none", "f64-mix"])
+def test_sim_dense_columns_through_the_member_list(f64, mode):
+    # columns storing rows 0 .. R-1 in order gather their groups' rows through
+    # the job's member list instead of reading every entry: with explicit
+    # zeros, NaN and an integer column (histogram branch), slots staged in LDS
+    # and -- for the wide groups -- written to HBM, the results equal the
+    # oracle's and those of the entry-by-entry read
+    rng = np.random.default_rng(61)
+    R, C, T = 3000, 3, 60
+    rows, indptr = _dense(R, C)
+    data = np.round(rng.normal(0, 1.5, R * C), 1)
+    z = rng.random(R * C)
+    data[z < 0.08] = 0.0
+    data[(z >= 0.08) & (z < 0.1)] = np.nan
+    data[2 * R:] = rng.integers(0, 12, R)   # an integer column
+    data = data.astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:400]] = 0
+    at = 400
+    for g, n in enumerate(rng.integers(3, 30, T), start=1):
+        gid[perm[at:at + n]] = g
+        at += n
+    run_both(data, rows, indptr, gid, T, R, mode, f64=f64, count_lds=0)
+    a = sim.mwu(data, rows, indptr, gid, T, R, mode=ZH[mode], f64=f64, count_lds=0,
+                timeout_s=120)
+    b = sim.mwu(data, rows, indptr, gid, T, R, mode=ZH[mode], f64=f64, count_lds=0,
+                timeout_s=120, members=False)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
