@@ -613,18 +613,29 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
         const V* col = vals + p0;
         bool li = int_col;
         int lmin = INT_MAX, lmax = INT_MIN;
+        // the member list of the next batch loads while this batch's values
+        // are gathered
+        uint32_t rn[U], gn[U];
+        auto members_of = [&](uint32_t s0, uint32_t (&r)[U], uint32_t (&g)[U]) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t sl = s0 + (uint32_t)(u * nt);
+                const bool ok = sl < M;
+                r[u] = ok ? gl(J.members)[sl] : 0u;
+                g[u] = ok ? gl(J.member_group)[sl] : dummy;
+            }
+        };
+        members_of((uint32_t)tid, rn, gn);
         for (uint32_t s0 = (uint32_t)tid; s0 < M; s0 += (uint32_t)nt * U) {
             uint32_t gi[U], pos[U];
             V v[U];
             bool nz[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t sl = s0 + (uint32_t)(u * nt);
-                const bool ok = sl < M;
-                const uint32_t r = ok ? gl(J.members)[sl] : 0u;
-                gi[u] = ok ? gl(J.member_group)[sl] : dummy;
-                v[u] = ok ? gl(col)[r] : V(0);
+                gi[u] = gn[u];
+                v[u] = s0 + (uint32_t)(u * nt) < M ? gl(col)[rn[u]] : V(0);
             }
+            members_of(s0 + (uint32_t)nt * U, rn, gn);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const bool in = gi[u] < (uint32_t)G && KT::finite(v[u]);
