@@ -1100,21 +1100,13 @@ struct RefState {
     uint32_t scan[20];
     uint32_t maxb;
     uint64_t red[16];
+    unsigned long long kmin, kmax;
 };
 
-// Keys in the fullest bucket of a bucket index bidx[0 .. nb] (LDS), for
-// every thread of the workgroup.
-__device__ inline uint32_t fullest_bucket(const uint32_t* bidx, uint32_t nb, RefState& st) {
-    const uint32_t tid = (uint32_t)tid_x(), nt = blockDim.x;
-    if (tid == 0) st.maxb = 0u;
-    __syncthreads();
-    uint32_t m = 0;
-    for (uint32_t b = tid; b < nb; b += nt) m = max(m, bidx[b + 1] - bidx[b]);
-    m = wave_max_u32(m);
-    if (lane_id() == 0) atomicMax(&st.maxb, m);
-    __syncthreads();
-    return st.maxb;
-}
+// Bucket sort of an on-chip reference: taken while its fullest bucket holds at
+// most this many keys (each key is ranked by a pass over its bucket), else the
+// bitonic sort
+constexpr uint32_t kBucketSortMax = 64;
 
 // Whether column c's reference tables come from the matrix's cache (the same
 // reference rows give the same count and bucket count: checked, so a stale
@@ -1213,39 +1205,95 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         // values below zero: the bins below bin -vmin
         if (pl.vmin < 0) refneg = cum[min((uint32_t)(-pl.vmin), pl.nb)];
     } else {
+        // On chip: the key range (a block min and max), the bucket index
+        // (counts, then their exclusive scan), then the sort -- by buckets
+        // when the fullest bucket is small and a second key array fits:
+        // each key goes to its bucket's range (the index as cursors), then
+        // to its bucket's start plus the keys below it in the bucket plus
+        // the equal keys before it there; the equal keys before it also give
+        // its run's tie term, 3j(j - 1) for the j-th key of a run (summing
+        // to t^3 - t over a run of t).  Otherwise the bitonic sort, and the
+        // tie sum from lookups of the sorted keys.
         uint32_t P = 64;
         while (P < na) P <<= 1;
         K* rk = (K*)smem;
         uint32_t* bidx = (uint32_t*)(rk + P);
-        for (uint32_t x = tid; x < P; x += nt) rk[x] = x < na ? base[x] : KT::kSentinel;
+        const uint32_t idx_bytes = ((pl.nb + 1u) * 4u + 15u) & ~15u;
+        K* tmp = (K*)((unsigned char*)bidx + idx_bytes);
+        const bool room = (uint64_t)2u * P * sizeof(K) + idx_bytes <=
+                          (uint64_t)ref_table_lds(sizeof(K) == 8);
+        if (tid == 0) { st.kmin = ~0ull; st.kmax = 0ull; st.maxb = 0u; }
+        K lo = KT::kSentinel, hi = K(0);
+        for (uint32_t x = tid; x < P; x += nt) {
+            const K key = x < na ? base[x] : KT::kSentinel;
+            rk[x] = key;
+            if (x < na) { lo = key < lo ? key : lo; hi = key > hi ? key : hi; }
+        }
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
+        lo = wave_min_k<K>(lo);
+        hi = wave_max_k<K>(hi);
+        __syncthreads();   // st cleared
+        if (lane_id() == 0 && na > 0) {
+            atomicMin(&st.kmin, (unsigned long long)lo);
+            atomicMax(&st.kmax, (unsigned long long)hi);
+        }
         __syncthreads();
-        sort_ref<K>(rk, P);
         if (na > 0) {
-            kmin = (uint64_t)rk[0];
-            kmax = (uint64_t)rk[na - 1u];
+            kmin = st.kmin;
+            kmax = st.kmax;
         }
         shift = ref_shift(kmin, kmax, na, pl.nb);
         uint64_t neg = 0;
         for (uint32_t x = tid; x < na; x += nt) {
             const K key = rk[x];
-            base[x] = key;
             atomicAdd(&bidx[(uint32_t)(((uint64_t)key - kmin) >> shift)], 1u);
             neg += key < KT::kPosBit ? 1u : 0u;
         }
         __syncthreads();
-        refneg = (uint32_t)block_sum_u64(neg, st.red);
+        uint32_t m = 0;
+        for (uint32_t b = tid; b < pl.nb; b += nt) m = max(m, bidx[b]);
+        m = wave_max_u32(m);
+        if (lane_id() == 0) atomicMax(&st.maxb, m);
+        refneg = (uint32_t)block_sum_u64(neg, st.red);   // its barriers publish st.maxb
+        maxb = uniform_u32(st.maxb);
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
-        for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
-        maxb = fullest_bucket(bidx, pl.nb, st);
-        RefView<K> rv;
-        rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
-        rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
-        rv.big = 0; rv.nb = pl.nb; rv.iters = halvings(maxb);
-        for (uint32_t x = tid; x < na; x += nt) {
-            uint32_t lt, eq;
-            rv.lookup(rk[x], lt, eq);
-            if (lt == x) ta += tie_f(eq);
+        if (room && maxb <= kBucketSortMax) {
+            for (uint32_t x = tid; x < na; x += nt) {
+                const K key = rk[x];
+                const uint32_t b = (uint32_t)(((uint64_t)key - kmin) >> shift);
+                tmp[atomicAdd(&bidx[b], 1u)] = key;
+            }
+            __syncthreads();
+            // bidx[b] now ends bucket b (the cursors moved one bucket on)
+            for (uint32_t i = tid; i < na; i += nt) {
+                const K key = tmp[i];
+                const uint32_t b = (uint32_t)(((uint64_t)key - kmin) >> shift);
+                const uint32_t s0 = b ? bidx[b - 1u] : 0u, s1 = bidx[b];
+                uint32_t below = 0, before = 0;
+                for (uint32_t j = s0; j < s1; ++j) {
+                    const K y = tmp[j];
+                    below += y < key ? 1u : 0u;
+                    before += (y == key && j < i) ? 1u : 0u;
+                }
+                rk[s0 + below + before] = key;
+                ta += 3ull * (uint64_t)(before + 1u) * (uint64_t)before;
+            }
+            __syncthreads();
+            for (uint32_t x = tid; x < na; x += nt) base[x] = rk[x];
+            for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = b ? bidx[b - 1u] : 0u;
+        } else {
+            sort_ref<K>(rk, P);
+            for (uint32_t x = tid; x < na; x += nt) base[x] = rk[x];
+            for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
+            RefView<K> rv;
+            rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
+            rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
+            rv.big = 0; rv.nb = pl.nb; rv.iters = halvings(maxb);
+            for (uint32_t x = tid; x < na; x += nt) {
+                uint32_t lt, eq;
+                rv.lookup(rk[x], lt, eq);
+                if (lt == x) ta += tie_f(eq);
+            }
         }
         ta = block_sum_u64(ta, st.red);
     }

@@ -539,3 +539,56 @@ def test_sim_dense_columns_through_the_member_list(f64, mode):
                 timeout_s=120, members=False)
     for x, y in zip(a, b):
         assert np.array_equal(x, y, equal_nan=True)
+
+
+def _fullest_bucket(keys):
+    """Keys in the fullest bucket of an on-chip reference's index: buckets
+    the power of two >= its size (64 to 4,096), the key range shifted onto
+    them -- computed here only to show which sort a column takes."""
+    na = len(keys)
+    if na == 0:
+        return 0
+    nb = 64
+    while nb < na and nb < 4096:
+        nb *= 2
+    k = keys.astype(np.uint64)
+    lo, hi = int(k.min()), int(k.max())
+    shift = max((hi - lo).bit_length() - (nb.bit_length() - 1), 0)
+    return int(np.bincount(((k - np.uint64(lo)) >> np.uint64(shift)).astype(np.int64)).max())
+
+
+def _sort_keys(v):
+    """Order-preserving unsigned keys of floats (negatives flipped)."""
+    b = v.view(np.uint64 if v.dtype == np.float64 else np.uint32)
+    top = np.array(1, b.dtype) << np.array(8 * b.itemsize - 1, b.dtype)
+    return np.where(b & top, ~b, b | top)
+
+
+@pytest.mark.parametrize("f64", [False, True], ids=["f32", "f64"])
+def test_sim_reference_sort_by_buckets_and_bitonic(f64):
+    # on-chip references sorted by buckets (the fullest bucket holds at most
+    # 64 keys: distinct values, and values at 0.01 with runs of equal keys)
+    # and by the bitonic sort (a handful of distinct values: full buckets),
+    # in every zero mode; the bucket sort's tie terms come from the runs
+    rng = np.random.default_rng(47)
+    R, T = 5000, 6
+    dens = np.array([0.6, 0.6, 0.6, 0.2])
+    rows, indptr = synthetic._bernoulli_rows(rng, R, dens)
+    n = np.diff(indptr)
+    # (8-byte keys of values on both sides of zero span some 2,000 binades:
+    # one bucket each, too coarse for the bucket sort, so positive values there)
+    sign = (lambda x: np.abs(x) + 0.01) if f64 else (lambda x: x)
+    parts = [sign(rng.normal(0, 2, n[0])), sign(np.round(rng.normal(0, 2, n[1]), 2)),
+             rng.choice([-1.5, 0.25, 0.75, 2.5], n[2]), np.log1p(rng.lognormal(0, 1.5, n[3]))]
+    data = np.concatenate(parts).astype(np.float64 if f64 else np.float32)
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:1800]] = 0
+    gid[perm[1800:]] = rng.integers(1, T + 1, R - 1800)
+    fullest = []
+    for c in range(len(n)):
+        r, v = rows[indptr[c]:indptr[c + 1]], data[indptr[c]:indptr[c + 1]]
+        fullest.append(_fullest_bucket(_sort_keys(v[(gid[r] == 0) & (v != 0)])))
+    assert max(fullest[0], fullest[1], fullest[3]) <= 64 < fullest[2]
+    for mode in ("min", "max", "mix", "none"):
+        run_both(data, rows, indptr, gid, T, R, mode, f64=f64)
