@@ -53,8 +53,14 @@ constexpr uint32_t kTieSlots = 128;
 constexpr uint64_t kTieEmpty = ~0ull;
 constexpr uint32_t kTieCountBits = 26;
 constexpr uint64_t kTieCountMask = (1ull << kTieCountBits) - 1ull;
+// A target of 65 to kHashTargetKeys 4-byte keys is ranked by the window
+// kernel in chunks of 64 keys with its tie positions from the wave's table,
+// then of kBigTieSlots words (the windows use its first kTieSlots); larger
+// ones, and 8-byte keys, are sorted (the sorted pass).
+constexpr uint32_t kHashTargetKeys = 128;
+constexpr uint32_t kBigTieSlots = 256;
 __host__ __device__ constexpr uint32_t target_lds(bool f64) {
-    return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kTieSlots * 8u;
+    return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kBigTieSlots * 8u;
 }
 
 // Count kernel (integer-valued columns in one pass).

@@ -1487,6 +1487,82 @@ __device__ __forceinline__ TargetResult process_target(const JobDesc& J, int64_t
     return r;
 }
 
+// process_target for a target of at most kHashTargetKeys 4-byte keys, without
+// a sort: its keys in chunks of 64, one per lane, each ranked against the
+// reference; a key's position among the target's equal keys comes from the
+// wave's table htab (kBigTieSlots words, all empty on entry and on return):
+// the key is placed by linear probing (compare and swap), then a counter
+// beside it is incremented, the count before the increment being the
+// position.  At most kHashTargetKeys keys in kBigTieSlots > kHashTargetKeys
+// slots: every lane finds its key's slot or an empty one.
+template <typename V>
+__device__ __forceinline__ TargetResult process_target_hash(
+    const JobDesc& J, int64_t c, uint32_t g, const ColPlan& pl,
+    const RefView<typename KeyOf<V>::K>& ref, unsigned long long* htab) {
+    using K = typename KeyOf<V>::K;
+    static_assert(sizeof(K) == 4, "the tie table holds 32-bit keys");
+    const uint32_t lane = (uint32_t)lane_id();
+    const int64_t b = c * J.G;
+    const uint32_t nzb = uniform_u32(gl(J.cnt)[b + g]);
+    const K* seg = (const K*)J.keys + gl(J.indptr)[c] + gl(J.goff)[b + g];
+    uint32_t za, zb, n1, n2;
+    pair_sizes(J, c, g, za, zb, n1, n2);
+    TargetCtx<K> t;
+    t.ref = ref;
+    t.mode = (int32_t)uniform_u32((uint32_t)J.mode);
+    t.vmin = pl.vmin;
+    t.za = za;
+    uint64_t S = 0, TT = 0;
+    constexpr uint32_t kChunks = kHashTargetKeys / (uint32_t)kWave;
+#pragma unroll 1
+    for (uint32_t q = 0; q < kChunks; ++q) {
+        const uint32_t e = q * (uint32_t)kWave + lane;
+        const bool valid = e < nzb;
+        const K key = valid ? gl(seg)[e] : K(0);
+        uint32_t lt, eq;
+        t.ref.template lookup<true>(key, lt, eq);
+        const uint64_t tag = (uint64_t)(uint32_t)key << 32;
+        uint32_t sl = ((uint32_t)key * 0x9E3779B1u) >> 24;   // kBigTieSlots = 2^8
+        bool placed = !valid;
+        for (uint32_t probe = 0; probe < kBigTieSlots; ++probe) {
+            if (__ballot(!placed) == 0ull) break;
+            if (!placed) {
+                const uint64_t old = atomicCAS(&htab[sl], (unsigned long long)kTieEmpty,
+                                               (unsigned long long)tag);
+                if (old == kTieEmpty || (old & ~kTieCountMask) == tag) placed = true;
+                else sl = (sl + 1u) & (kBigTieSlots - 1u);
+            }
+        }
+        if (valid) {
+            const uint32_t jj = 1u + (uint32_t)(atomicAdd(&htab[sl], 1ull) & kTieCountMask);
+            const uint64_t ltz = (uint64_t)lt + (zero_below<V>(t, key) ? za : 0u);
+            S += 2ull * ltz + eq;
+            const uint64_t x = (uint64_t)eq + jj;
+            TT += 3ull * x * (x - 1ull);
+        }
+    }
+    S = wave_sum_u64(S);
+    TT = wave_sum_u64(TT);
+    // the whole table empty again, after every lane's increments (the wave's
+    // LDS operations complete in order)
+    for (uint32_t i = lane; i < kBigTieSlots; i += (uint32_t)kWave) htab[i] = kTieEmpty;
+    uint32_t ltz;
+    if (J.mode == kZeroMin) ltz = 0;
+    else if (J.mode == kZeroMax) ltz = pl.na;
+    else ltz = pl.refneg;
+    const uint64_t S2 = S + (uint64_t)zb * (2ull * ltz + za);
+    const uint64_t tie = TT + pl.TA + tie_f((uint64_t)za + zb);
+    const uint64_t twoN = 2ull * (uint64_t)n1 * (uint64_t)n2;
+    const uint64_t u1x2 = twoN >= S2 ? twoN - S2 : 0ull;
+    TargetResult r;
+    r.idx = c * J.T + (int64_t)(g - 1);
+    r.U1 = (double)u1x2 * 0.5;
+    r.tie = J.tie_corr ? (double)tie : 0.0;
+    r.n1 = n1;
+    r.n2 = n2;
+    return r;
+}
+
 // Outputs of up to 64 targets held one per lane: written, and their FP64
 // asymptotic p computed, by the lanes in parallel.
 __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResult& r, int cnt) {
@@ -1508,14 +1584,14 @@ __device__ __forceinline__ void flush_targets(const JobDesc& J, const TargetResu
 // segment, so nothing is sorted; inclusive scans of the per-key terms, read at
 // each target's first and last position, give every target's sums.  Every
 // cross-lane step but the key offset and those reads is a DPP move.  A target
-// with more than 64 keys alone goes through process_target.  Each lane
-// returns its own target's result.
+// with more than 64 keys is left out (the window takes it as empty) and
+// ranked apart.  Each lane returns its own target's result.
 template <typename V>
 __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t c, uint32_t g0,
                                                        uint32_t ng, const ColPlan& pl,
                                                        const RefView<typename KeyOf<V>::K>& ref,
                                                        const typename KeyOf<V>::K* colkeys,
-                                                       unsigned long long* htab, bool& any_big) {
+                                                       unsigned long long* htab) {
     using K = typename KeyOf<V>::K;
     static_assert(sizeof(K) == 4, "target windows compare 32-bit keys across lanes");
     const uint32_t lane = (uint32_t)lane_id();
@@ -1679,9 +1755,9 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         if (valid) htab[slot] = kTieEmpty;
         w = nx;
     }
-    // targets of more than 64 keys: ranked by the sorted pass
+    // targets of more than 64 keys: ranked apart (process_target_hash or
+    // the sorted pass)
     big = own && nzj > (uint32_t)kWave;
-    any_big = __ballot(big) != 0ull;
     if (own && !big) {
         uint32_t ltz;
         if (J.mode == kZeroMin) ltz = 0;
@@ -1702,16 +1778,18 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
 
 struct TargetState {
     uint32_t next_t;
+    uint32_t next_h;
     uint32_t any_big;
 };
 
 // The target phase of one column.  kWindows (4-byte keys): a wave claims 64
-// consecutive targets, one per lane, and ranks those of at most 64 keys
-// several at a time (target_windows); a column with larger targets is
+// consecutive targets, one per lane, ranks those of at most 64 keys several
+// at a time (target_windows) and those of 65 to kHashTargetKeys one at a time
+// without a sort (process_target_hash); a column with larger targets is
 // flagged for the sorted pass.  Otherwise (the sorted pass): a wave claims
 // 64 targets and ranks each of them alone (process_target) -- every target
-// for 8-byte keys, the targets of more than 64 keys of a flagged column for
-// 4-byte keys.
+// for 8-byte keys, the targets of more than kHashTargetKeys keys of a flagged
+// column for 4-byte keys.
 template <typename V, bool kWindows>
 __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem* items, uint32_t k) {
     using K = typename KeyOf<V>::K;
@@ -1761,13 +1839,15 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
         rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
         rv.iters = halvings(uniform_u32(pl.maxb));
     }
-    if (tid == 0) { st.next_t = 1u; st.any_big = 0u; }
+    if (tid == 0) { st.next_t = 1u; st.next_h = 1u; st.any_big = 0u; }
     if constexpr (kWindows) {
         static_assert(sizeof(K) == 4, "target windows take 4-byte keys");
-        // each wave's tie-position table (target_windows), all slots empty
+        // each wave's tie-position table (target_windows, process_target_hash),
+        // all slots empty
         unsigned long long* htab = (unsigned long long*)(smem + ref_table_lds(false)) +
-                                   (size_t)(tid / kWave) * kTieSlots;
-        for (uint32_t i = (uint32_t)lane; i < kTieSlots; i += (uint32_t)kWave) htab[i] = kTieEmpty;
+                                   (size_t)(tid / kWave) * kBigTieSlots;
+        for (uint32_t i = (uint32_t)lane; i < kBigTieSlots; i += (uint32_t)kWave)
+            htab[i] = kTieEmpty;
         __syncthreads();
         const K* colkeys = (const K*)J.keys + gl(J.indptr)[c];
         bool any_big = false;
@@ -1777,11 +1857,28 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
             g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
             if (g0 >= (uint32_t)J.G) break;
             const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
-            bool claim_big = false;
-            const TargetResult r =
-                target_windows<V>(J, c, g0, ng, pl, rv, colkeys, htab, claim_big);
-            any_big = any_big || claim_big;
-            flush_targets(J, r, (int)ng);
+            flush_targets(J, target_windows<V>(J, c, g0, ng, pl, rv, colkeys, htab), (int)ng);
+        }
+        // then the targets of 65 to kHashTargetKeys keys, ranked without a
+        // sort and each written by its lane (a second claim loop: nothing of
+        // the windows is live here); larger ones are flagged for the sorted
+        // pass
+        for (;;) {
+            uint32_t g0 = 0;
+            if (lane == 0) g0 = atomicAdd(&st.next_h, (uint32_t)kWave);
+            g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)shfl_u32(g0, 0));
+            if (g0 >= (uint32_t)J.G) break;
+            const uint32_t ng = min((uint32_t)kWave, (uint32_t)J.G - g0);
+            const uint32_t nzj =
+                (uint32_t)lane < ng ? gl(J.cnt)[c * J.G + g0 + (uint32_t)lane] : 0u;
+            for (uint64_t m = __ballot(nzj > (uint32_t)kWave && nzj <= kHashTargetKeys); m;
+                 m &= m - 1ull) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(m);
+                TargetResult h = process_target_hash<V>(J, c, g0 + j, pl, rv, htab);
+                if ((uint32_t)lane != j) h.idx = -1;
+                flush_targets(J, h, (int)ng);
+            }
+            any_big = any_big || __ballot(nzj > kHashTargetKeys) != 0ull;
         }
         if (any_big && lane == 0) st.any_big = 1u;
         __syncthreads();
@@ -1799,7 +1896,7 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
             const uint32_t nzj = own ? gl(J.cnt)[c * J.G + g0 + (uint32_t)lane] : 0u;
             TargetResult mine{};
             mine.idx = -1;
-            for (uint64_t m = __ballot(own && (sizeof(K) == 8 || nzj > (uint32_t)kWave)); m;
+            for (uint64_t m = __ballot(own && (sizeof(K) == 8 || nzj > kHashTargetKeys)); m;
                  m &= m - 1ull) {
                 const uint32_t j = (uint32_t)__builtin_ctzll(m);
                 const TargetResult r = process_target<V>(J, c, g0 + j, pl, rv);
@@ -1812,7 +1909,7 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
 
 // The window pass of 4-byte keys: workgroup b takes queue item b (the grid
 // is sized for the queue's capacity).  512 threads at four waves per SIMD
-// (at most 128 VGPRs): the reference table and eight tie tables (57 KB of
+// (at most 128 VGPRs): the reference table and eight tie tables (65 KB of
 // LDS) fit two workgroups per CU, 16 waves to hide the latency of the
 // LDS searches and key loads.
 template <typename V>
@@ -1821,8 +1918,8 @@ void target_window_kernel(const JobDesc* jobs, const WorkItem* items, uint32_t n
     if (blockIdx.x < n_items) target_item<V, true>(jobs, items, blockIdx.x);
 }
 
-// The sorted pass (every target of 8-byte keys; the targets of more than 64
-// keys of 4-byte keys): workgroup b takes queue item b.  At most 168 VGPRs,
+// The sorted pass (every target of 8-byte keys; the targets of more than
+// kHashTargetKeys keys of 4-byte keys): workgroup b takes queue item b.  At most 168 VGPRs,
 // so that three waves share a SIMD: the table LDS (49 KB) fits three
 // workgroups per CU.
 template <typename V>

@@ -720,3 +720,33 @@ def test_plan_kernel_ms_keeps_results():
         p.kernel_ms("nothing")
     p.close()
     m.close()
+
+
+@pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
+def test_float_targets_of_65_to_128_keys(mode):
+    """Float targets of 65 to 128 keys (ranked by the window kernel without a
+    sort, tie positions from the wave's table), beside targets of more keys
+    (the sorted pass) and of at most 64 (windows); many equal keys inside a
+    target.  U1, tie and sizes bit-exact, p within 1e-12 of the oracle."""
+    say(f"targets of 65-128 keys, {mode}")
+    rng = np.random.default_rng(61)
+    R = 20000
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.6, 0.35, 0.2, 0.05]))
+    data = np.round(rng.normal(0, 1.5, len(rows)), 1).astype(np.float32)
+    sizes = [90, 120, 140, 180, 260, 330, 30, 10, 400, 700, 2000]
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:3000]] = 0
+    k = 3000
+    for t, n in enumerate(sizes, start=1):
+        gid[perm[k:k + n]] = t
+        k += n
+    T = len(sizes)
+    opts = _capi.make_opts(tie_correction=True, use_continuity=True,
+                           zero_handling={"none": 0, "min": 1, "max": 2, "mix": 3}[mode])
+    g = _capi.mwu_csc(data, rows, indptr, gid, T, opts, n_rows=R, with_stats=True)
+    o = oracle.mwu(data, rows, indptr, gid, T, n_rows=R, tie_correction=True,
+                   use_continuity=True, zero_handling=mode, with_stats=True)
+    for a, b in zip((g[0], g[2], g[3], g[4]), (o[0], o[2], o[3], o[4])):
+        assert np.array_equal(a, b)
+    assert rel_close(g[1], o[1], rtol=1e-12)

@@ -592,3 +592,28 @@ def test_sim_reference_sort_by_buckets_and_bitonic(f64):
     assert max(fullest[0], fullest[1], fullest[3]) <= 64 < fullest[2]
     for mode in ("min", "max", "mix", "none"):
         run_both(data, rows, indptr, gid, T, R, mode, f64=f64)
+
+
+@pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
+def test_sim_sorted_pass_hashed_and_sorted_targets(mode):
+    # targets of 65 to 128 keys (ranked in two chunks with a tie table, no
+    # sort) beside targets of more (sorted) and of at most 64 (windows):
+    # many equal keys inside a target (values at 0.1), negatives, zeros
+    rng = np.random.default_rng(53)
+    R = 4000
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.6, 0.35]))
+    data = np.round(rng.normal(0, 1.5, len(rows)), 1).astype(np.float32)
+    sizes = [90, 120, 140, 180, 260, 30, 10, 400]
+    gid = np.full(R, -1, np.int32)
+    perm = rng.permutation(R)
+    gid[perm[:1200]] = 0
+    k = 1200
+    for t, n in enumerate(sizes, start=1):
+        gid[perm[k:k + n]] = t
+        k += n
+    keys = [[int(((gid[rows[indptr[c]:indptr[c + 1]]] == t) &
+                  (data[indptr[c]:indptr[c + 1]] != 0)).sum()) for t in range(1, len(sizes) + 1)]
+            for c in range(3)]
+    flat = [x for col in keys for x in col]
+    assert any(64 < x <= 128 for x in flat) and any(x > 128 for x in flat)
+    run_both(data, rows, indptr, gid, len(sizes), R, mode)
