@@ -172,14 +172,6 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t l) {
     return ((uint64_t)readlane_u32((uint32_t)(v >> 32), l) << 32) | readlane_u32((uint32_t)v, l);
 }
-// Index of the k-th (1-based) set bit of m; 63 when m has fewer than k.
-__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t s = 32; s >= 1; s >>= 1)   // pos + s stays below 64
-        if ((uint32_t)__builtin_popcountll(m & ((1ull << (pos + s)) - 1ull)) < k) pos += s;
-    return pos;
-}
 
 __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
     const int l = lane_id();

@@ -1635,6 +1635,10 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         uint32_t j0, j1, m, first, owner;
         K key;
     };
+    // the start map: the last 64 bytes of the wave's tie table, which the
+    // windows' probes (its first kTieSlots words) never reach; 0xFF bytes are
+    // the table's empty words
+    uint8_t* smap = (uint8_t*)(htab + kBigTieSlots) - kWave;
     auto plan = [&](uint32_t j0) -> Win {
         Win w;
         w.j0 = j0; w.j1 = j0; w.m = 0; w.first = 0; w.owner = 0; w.key = K(0);
@@ -1648,24 +1652,23 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         // target j's keys take positions [excl_j, excl_j + sz_j) of the window
         const uint32_t excl = inc - sz - base;
         const bool starts = lane >= j0 && lane < j1 && sz > 0u;
-        // the window's segment starts as bits (an OR over the wave) and the
-        // targets that start them; a key's target is the k-th of those, k =
-        // the starts at or before the key
-        const uint64_t smask = readlane_u64(
-            wave_incl_scan_dpp(starts ? (1ull << excl) : 0ull,
-                               [](uint64_t a, uint64_t x) { return a | x; }),
-            (uint32_t)kWave - 1u);
-        const uint64_t tmask = __ballot(starts);
+        // the window's segment starts: each starting target writes its lane
+        // at its first position in the wave's start map (64 bytes, 0xFF where
+        // no target starts); the positions that hold a target, and the last
+        // of them at or before a key's position, give the key's target.  The
+        // wave's LDS operations complete in order (the ballots between them
+        // order the lanes of a host model of the wave)
+        if (starts) smap[excl] = (uint8_t)lane;
+        (void)__ballot(starts);
+        const uint64_t smask = __ballot(smap[lane] != 0xFFu);
         const uint64_t upto = (lane == (uint32_t)kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
         const uint64_t sm = smask & upto;
         w.first = sm ? (uint32_t)(63 - __clzll(sm)) : 0u;   // its first key
-        const uint32_t k = (uint32_t)__builtin_popcountll(sm);
-        // with no empty target in the window the k-th starting target is j0 + k - 1
-        const uint64_t wmask = (j1 == 64u ? ~0ull : ((1ull << j1) - 1ull)) & ~((1ull << j0) - 1ull);
-        w.owner = tmask == wmask ? j0 + k - 1u : nth_set_bit(tmask, k);
+        w.owner = min((uint32_t)smap[w.first], (uint32_t)kWave - 1u);
         // the lane's key: position lane - first of the owner's keys (a
         // group's keys need not end where the next group's start)
         const uint32_t at = shfl_u32(off, (int)w.owner) + lane - w.first;
+        if (starts) smap[excl] = 0xFFu;   // the map empty again
         w.key = lane < w.m ? gl(colkeys)[at] : K(0);
         return w;
     };

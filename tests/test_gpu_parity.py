@@ -731,9 +731,9 @@ def test_float_targets_of_65_to_128_keys(mode):
     say(f"targets of 65-128 keys, {mode}")
     rng = np.random.default_rng(61)
     R = 20000
-    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.6, 0.35, 0.2, 0.05]))
+    rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.6, 0.35, 0.2, 0.15]))
     data = np.round(rng.normal(0, 1.5, len(rows)), 1).astype(np.float32)
-    sizes = [90, 120, 140, 180, 260, 330, 30, 10, 400, 700, 2000]
+    sizes = [90, 120, 140, 180, 260, 330, 60, 75, 400, 700, 2000]
     gid = np.full(R, -1, np.int32)
     perm = rng.permutation(R)
     gid[perm[:3000]] = 0
