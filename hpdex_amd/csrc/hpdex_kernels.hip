@@ -708,17 +708,22 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const int32_t vmin = st.vmin;
     if (arrival) {
         // scatter from the recorded groups, the values read again (coalesced,
-        // four entries per thread in flight)
+        // eight entries per thread in flight; every entry's value is loaded,
+        // the cache lines being the same, so no load waits for its group)
         const V* col_vals = vals + p0;
-        constexpr uint32_t kA = sizeof(V) == 8 ? 2u : 4u;
+        constexpr uint32_t kA = sizeof(V) == 8 ? 4u : 8u;
         for (uint32_t e0 = (uint32_t)tid; e0 < n_col; e0 += (uint32_t)nt * kA) {
             uint32_t g[kA];
             V v[kA];
 #pragma unroll
             for (uint32_t u = 0; u < kA; ++u) {
                 const uint32_t e = e0 + u * (uint32_t)nt;
+                v[u] = e < n_col ? gl(col_vals)[e] : V(0);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kA; ++u) {
+                const uint32_t e = e0 + u * (uint32_t)nt;
                 g[u] = e < n_col ? (uint32_t)agrp[e] : 0xFFFFu;
-                v[u] = g[u] != 0xFFFFu ? gl(col_vals)[e] : V(0);
             }
             // the offsets taken back to back (entries with no key take a
             // dummy word), then the keys stored
