@@ -939,6 +939,9 @@ struct RefView {
     const K* keys;           // sort path, reference beyond the on-chip sort: sorted keys in HBM
     const HPDEX_LDS K* lkeys;  // sort path: sorted keys in LDS
     const uint32_t* bidx;    // sort path: bucket -> first key index (LDS), nb + 1 entries
+    // sort path, on chip: at the first of each run of equal keys, the run's
+    // length (LDS), or null (the upper bound is searched instead)
+    const HPDEX_LDS uint32_t* lrun;
     uint64_t kmin, kmax;
     uint32_t shift, na, hist, big, nb;
     uint32_t iters;          // halvings that cover the fullest bucket
@@ -970,8 +973,13 @@ struct RefView {
         const uint32_t lo = bidx[bkt], hi = bidx[above ? bkt : bkt + 1u];
         const uint32_t last = na ? na - 1u : 0u;
         const uint32_t l = bound_fixed<false, K>(lkeys, lo, hi - lo, v, iters, last);
-        // the upper bound only where the key is present (rare for float data)
         const bool hit = l < hi && lkeys[min(l, last)] == v;
+        if (lrun) {   // l is the first of the run of keys equal to v
+            lt = l;
+            eq = hit ? lrun[l] : 0u;
+            return;
+        }
+        // the upper bound only where some lane's key is present
         uint32_t u = l;
         if (!kWave || __ballot(hit)) {
             const uint32_t uu = bound_fixed<true, K>(lkeys, l, hi - l, v, iters, last);
@@ -1187,7 +1195,7 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
         for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
         RefView<K> rv;
-        rv.cum = nullptr; rv.keys = base; rv.lkeys = nullptr; rv.bidx = bidx;
+        rv.cum = nullptr; rv.keys = base; rv.lkeys = nullptr; rv.bidx = bidx; rv.lrun = nullptr;
         rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
         rv.big = 1; rv.nb = pl.nb; rv.iters = 0;
         for (uint32_t x = tid; x < na; x += nt) {
@@ -1292,6 +1300,7 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
             for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = bidx[b];
             RefView<K> rv;
             rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
+            rv.lrun = nullptr;
             rv.kmin = kmin; rv.kmax = kmax; rv.shift = shift; rv.na = na; rv.hist = 0;
             rv.big = 0; rv.nb = pl.nb; rv.iters = halvings(maxb);
             for (uint32_t x = tid; x < na; x += nt) {
@@ -1829,7 +1838,13 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     if (big && !cached && uniform_i64(J.spill_off ? gl(J.spill_off)[c] : -1) < 0) return;
     RefView<K> rv;
     rv.kmin = pl.kmin; rv.kmax = pl.kmax; rv.shift = pl.shift; rv.na = na; rv.hist = hist;
-    rv.big = big ? 1u : 0u; rv.nb = pl.nb; rv.iters = 0; rv.lkeys = nullptr;
+    rv.big = big ? 1u : 0u; rv.nb = pl.nb; rv.iters = 0; rv.lkeys = nullptr; rv.lrun = nullptr;
+    // an on-chip reference filling at most half the key space, whose runs of
+    // equal keys (each inside one bucket) hold at most kBucketSortMax keys:
+    // the run lengths go to the free half, so a present key's equal count is
+    // one read instead of a second search
+    uint32_t* runs = nullptr;
+    K* rkeys = nullptr;
     if (hist) {
         uint32_t* cum = (uint32_t*)smem;
         for (uint32_t b = tid; b <= pl.nb; b += nt) cum[b] = gl(tab)[b];
@@ -1846,7 +1861,24 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = gl(tab)[b];
         rv.cum = nullptr; rv.keys = nullptr; rv.lkeys = lds(rk); rv.bidx = bidx;
         rv.iters = halvings(uniform_u32(pl.maxb));
+        if (2u * na <= cap && uniform_u32(pl.maxb) <= kBucketSortMax) {
+            runs = (uint32_t*)(rk + na);
+            rkeys = rk;
+            rv.lrun = lds(runs);
+        }
     }
+    // after the barrier that publishes the keys: the run lengths, then a
+    // barrier of their own (runs is uniform)
+    auto fill_runs = [&]() {
+        if (!runs) return;
+        for (uint32_t x = tid; x < na; x += nt) {
+            if (x > 0 && rkeys[x - 1u] == rkeys[x]) continue;   // not the first of its run
+            uint32_t e = x + 1u;
+            while (e < na && rkeys[e] == rkeys[x]) ++e;
+            runs[x] = e - x;
+        }
+        __syncthreads();
+    };
     if (tid == 0) { st.next_t = 1u; st.next_h = 1u; st.any_big = 0u; }
     if constexpr (kWindows) {
         static_assert(sizeof(K) == 4, "target windows take 4-byte keys");
@@ -1857,6 +1889,7 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
         for (uint32_t i = (uint32_t)lane; i < kBigTieSlots; i += (uint32_t)kWave)
             htab[i] = kTieEmpty;
         __syncthreads();
+        fill_runs();
         const K* colkeys = (const K*)J.keys + gl(J.indptr)[c];
         bool any_big = false;
         for (;;) {
@@ -1893,6 +1926,7 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
         if (tid == 0 && st.any_big) J.plan[c].bigt = 1u;
     } else {
         __syncthreads();
+        fill_runs();
         // lane j of the wave keeps target g0 + j's result
         for (;;) {
             uint32_t g0 = 0;
