@@ -57,8 +57,10 @@ constexpr uint64_t kTieCountMask = (1ull << kTieCountBits) - 1ull;
 // kernel in chunks of 64 keys with its tie positions from the wave's table,
 // then of kBigTieSlots words (the windows use its first kTieSlots); larger
 // ones, and 8-byte keys, are sorted (the sorted pass).
-constexpr uint32_t kHashTargetKeys = 128;
+constexpr uint32_t kHashTargetKeys = 192;   // at most 3/4 of the table's slots
 constexpr uint32_t kBigTieSlots = 256;
+static_assert(kHashTargetKeys % 64u == 0u && kHashTargetKeys < kBigTieSlots,
+              "whole chunks of 64 keys, and an empty slot for every probe sequence");
 __host__ __device__ constexpr uint32_t target_lds(bool f64) {
     return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kBigTieSlots * 8u;
 }

@@ -723,12 +723,12 @@ def test_plan_kernel_ms_keeps_results():
 
 
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
-def test_float_targets_of_65_to_128_keys(mode):
-    """Float targets of 65 to 128 keys (ranked by the window kernel without a
+def test_float_targets_of_65_to_192_keys(mode):
+    """Float targets of 65 to 192 keys (ranked by the window kernel without a
     sort, tie positions from the wave's table), beside targets of more keys
     (the sorted pass) and of at most 64 (windows); many equal keys inside a
     target.  U1, tie and sizes bit-exact, p within 1e-12 of the oracle."""
-    say(f"targets of 65-128 keys, {mode}")
+    say(f"targets of 65-192 keys, {mode}")
     rng = np.random.default_rng(61)
     R = 20000
     rows, indptr = synthetic._bernoulli_rows(rng, R, np.array([0.9, 0.6, 0.35, 0.2, 0.15]))

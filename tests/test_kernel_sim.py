@@ -596,7 +596,7 @@ def test_sim_reference_sort_by_buckets_and_bitonic(f64):
 
 @pytest.mark.parametrize("mode", ["min", "max", "mix", "none"])
 def test_sim_sorted_pass_hashed_and_sorted_targets(mode):
-    # targets of 65 to 128 keys (ranked in two chunks with a tie table, no
+    # targets of 65 to 192 keys (ranked in chunks of 64 with a tie table, no
     # sort) beside targets of more (sorted) and of at most 64 (windows):
     # many equal keys inside a target (values at 0.1), negatives, zeros
     rng = np.random.default_rng(53)
@@ -615,5 +615,6 @@ def test_sim_sorted_pass_hashed_and_sorted_targets(mode):
                   (data[indptr[c]:indptr[c + 1]] != 0)).sum()) for t in range(1, len(sizes) + 1)]
             for c in range(3)]
     flat = [x for col in keys for x in col]
-    assert any(64 < x <= 128 for x in flat) and any(x > 128 for x in flat)
+    assert any(64 < x <= 128 for x in flat) and any(128 < x <= 192 for x in flat)
+    assert any(x > 192 for x in flat)
     run_both(data, rows, indptr, gid, len(sizes), R, mode)
