@@ -20,6 +20,7 @@ enum : uint32_t {
 // Launch geometry shared by the per-column kernels.
 constexpr int kColBlock = 256;              // threads per workgroup (4 waves)
 constexpr int kTargetBlock = 512;           // threads per target-window workgroup (8 waves)
+constexpr int kRefBlock = 512;              // threads per reference-kernel workgroup (8 waves)
 // Split kernel counters: per group a 64-bit word (explicit zeros in the high
 // half, nonzero values in the low) and a 32-bit offset, plus kSplitDummy
 // words that entries outside every group hit (one per lane of a half-wave,

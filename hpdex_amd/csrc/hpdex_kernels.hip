@@ -1324,7 +1324,7 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
 
 // workgroup b takes queue item b (the grid is sized for the queue's capacity)
 template <typename V>
-__global__ __launch_bounds__(kColBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
+__global__ __launch_bounds__(kRefBlock) void ref_kernel(const JobDesc* jobs, const WorkItem* items,
                                                    uint32_t n_items) {
     if (blockIdx.x < n_items) ref_item<V>(jobs, items, blockIdx.x);
 }
@@ -2860,7 +2860,7 @@ hipError_t launch_mwu_pipeline(const PipelineLaunch& L, hipStream_t s, hipEvent_
                 split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
                                                                      split_lds);
             mark(2);
-            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            ref_kernel<V><<<grid, kRefBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             mark(3);
             launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         };
@@ -2909,7 +2909,7 @@ hipError_t launch_mwu_stage(const PipelineLaunch& L, int stage, hipStream_t s) {
                 split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
                                                                      split_lds);
         } else if (stage == 2) {
-            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            ref_kernel<V><<<grid, kRefBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
         } else {
             launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         }
@@ -2947,7 +2947,7 @@ hipError_t launch_mwu_batch(const BatchLaunch& L, hipStream_t s) {
             else
                 split_kernel<V><<<grid, kSplitBlock, split_lds, s>>>(L.jobs, L.gen_items, n,
                                                                      split_lds);
-            ref_kernel<V><<<grid, kColBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
+            ref_kernel<V><<<grid, kRefBlock, tab_lds, s>>>(L.jobs, L.gen_items, n);
             launch_target_pass<V>(L.jobs, L.gen_items, n, s);
         };
         if (L.f64) sort_path(TypeTag<double>{});

@@ -193,7 +193,7 @@ static int run_pipeline(JobDesc& J, int32_t* range, const std::vector<int32_t>& 
                       timeout_s);
     }
     for (unsigned b = 0; b < ng; ++b)
-        run_block(b, kColBlock, ng, [&] { ref_kernel<V>(jobs, it, ng); }, timeout_s);
+        run_block(b, kRefBlock, ng, [&] { ref_kernel<V>(jobs, it, ng); }, timeout_s);
     if constexpr (sizeof(V) == 4)
         for (unsigned b = 0; b < ng; ++b)
             run_block(b, kTargetBlock, ng, [&] { target_window_kernel<V>(jobs, it, ng); },
