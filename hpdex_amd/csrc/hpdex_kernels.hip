@@ -530,10 +530,11 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const bool staged = __builtin_amdgcn_readfirstlane(
         (uint64_t)cnt_bytes + (uint64_t)sizeof(K) * span <= lds_bytes ? 1 : 0) != 0;
     K* stage = (K*)(smem + cnt_bytes);
-    // A column scattered by offsets whose entries' groups also fit LDS (16 bits
-    // each, after the staged keys) keeps them from its counting pass, so the
-    // scatter reads only the values again: no second pass over the rows and
-    // no second gather of the labels.  Entries that give no key keep 0xFFFF.
+    // A column scattered by offsets whose entries fit LDS keeps each entry's
+    // group (16 bits, after the key space) and key (in the key space, in
+    // arrival order) from its counting pass, and the scatter reads them
+    // there: no second pass over the rows, values or labels.  Entries that
+    // give no key keep group 0xFFFF.
     const uint32_t n_col = (uint32_t)(p1 - p0);
     const uint32_t stage_bytes = ((uint32_t)sizeof(K) * n_col + 15u) & ~15u;
     const bool arrival = __builtin_amdgcn_readfirstlane(
@@ -574,7 +575,10 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
                 if (!nz[u]) continue;
                 const K k = KT::key(v[u]);
                 if (kFixed == 0) {
-                    if (arrival) agrp[e0 + (uint32_t)(u * NT)] = (uint16_t)g[u];
+                    if (arrival) {
+                        agrp[e0 + (uint32_t)(u * NT)] = (uint16_t)g[u];
+                        stage[e0 + (uint32_t)(u * NT)] = k;
+                    }
                 } else if (kFixed == 1) {
                     // a group holding more entries than rows (repeated row
                     // indices) is an error of the column; its keys stay
@@ -707,38 +711,27 @@ __device__ __forceinline__ void split_item(const JobDesc* jobs, const WorkItem* 
     const uint32_t hist = st.hist;
     const int32_t vmin = st.vmin;
     if (arrival) {
-        // scatter from the recorded groups, the values read again (coalesced,
-        // eight entries per thread in flight; every entry's value is loaded,
-        // the cache lines being the same, so no load waits for its group)
-        const V* col_vals = vals + p0;
-        constexpr uint32_t kA = sizeof(V) == 8 ? 4u : 8u;
-        for (uint32_t e0 = (uint32_t)tid; e0 < n_col; e0 += (uint32_t)nt * kA) {
-            uint32_t g[kA];
-            V v[kA];
+        // scatter from the recorded groups and keys (LDS, in arrival order)
+        // straight to the keys' places in HBM: one workgroup per CU keeps
+        // few columns' key ranges in flight, so their lines fill in L2
+        for (uint32_t e0 = (uint32_t)tid; e0 < n_col; e0 += (uint32_t)nt * 4u) {
+            uint32_t g[4], pos[4];
+            K kk[4];
 #pragma unroll
-            for (uint32_t u = 0; u < kA; ++u) {
-                const uint32_t e = e0 + u * (uint32_t)nt;
-                v[u] = e < n_col ? gl(col_vals)[e] : V(0);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kA; ++u) {
+            for (uint32_t u = 0; u < 4u; ++u) {
                 const uint32_t e = e0 + u * (uint32_t)nt;
                 g[u] = e < n_col ? (uint32_t)agrp[e] : 0xFFFFu;
+                kk[u] = g[u] != 0xFFFFu ? stage[e] : K(0);
             }
-            // the offsets taken back to back (entries with no key take a
-            // dummy word), then the keys stored
-            uint32_t pos[kA];
 #pragma unroll
-            for (uint32_t u = 0; u < kA; ++u)
+            for (uint32_t u = 0; u < 4u; ++u)
                 pos[u] = atomicAdd(&cur[g[u] != 0xFFFFu ? g[u] : dummy], 1u);
 #pragma unroll
-            for (uint32_t u = 0; u < kA; ++u) {
+            for (uint32_t u = 0; u < 4u; ++u) {
                 if (g[u] == 0xFFFFu) continue;
-                stage[pos[u]] = hist ? (K)(uint32_t)((int)v[u] - vmin) : KT::key(v[u]);
+                keys[pos[u]] = hist ? (K)(uint32_t)((int)KT::value(kk[u]) - vmin) : kk[u];
             }
         }
-        __syncthreads();   // the column's key range out, coalesced
-        for (uint32_t i = (uint32_t)tid; i < n_col; i += (uint32_t)nt) keys[i] = stage[i];
     } else if (!fixed) {
         // scatter nonzero values into the groups' key ranges
         K* dst = staged ? stage : keys;

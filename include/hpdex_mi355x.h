@@ -337,9 +337,9 @@ typedef struct hpdex_tuning {
                                   32-bit word per entry, made when the matrix is
                                   created) where it has them; 0 = rows and values  */
     int32_t split_arrival;     /* sort path: 1 = the split kernel keeps each entry's
-                                  group from its counting pass in LDS (columns whose
-                                  entries fit) and reads only the values again to
-                                  scatter them; 0 = counting and scattering passes
+                                  group and key from its counting pass in LDS
+                                  (columns whose entries fit) and scatters the keys
+                                  from there; 0 = counting and scattering passes
                                   both read rows, values and group labels           */
     int32_t ref_lds_keys;      /* sort path, the LDS/global sort cutover: a column's
                                   reference slice of at most this many values is
