@@ -1104,9 +1104,11 @@ __device__ void sort_ref_global(K* a, K* b, uint32_t n, K* rk, uint32_t cap, K s
 // ---------------------------------------------------------------------------
 struct RefState {
     uint32_t scan[20];
-    uint32_t maxb;
     uint64_t red[16];
-    unsigned long long kmin, kmax;
+    // per-wave partial results of the on-chip branch, one slot per wave:
+    // written by each wave's lane 0, read by every thread after a barrier
+    uint64_t wlo[16], whi[16], wta[16];
+    uint32_t wneg[16], wmaxb[16];
 };
 
 // Bucket sort of an on-chip reference: taken while its fullest bucket holds at
@@ -1228,41 +1230,43 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
         K* tmp = (K*)((unsigned char*)bidx + idx_bytes);
         const bool room = (uint64_t)2u * P * sizeof(K) + idx_bytes <=
                           (uint64_t)ref_table_lds(sizeof(K) == 8);
-        if (tid == 0) { st.kmin = ~0ull; st.kmax = 0ull; st.maxb = 0u; }
+        const uint32_t wid = (uint32_t)tid / (uint32_t)kWave, nw = (uint32_t)nt / (uint32_t)kWave;
+        const bool lead = lane_id() == 0;
         K lo = KT::kSentinel, hi = K(0);
+        uint32_t neg = 0;
         for (uint32_t x = tid; x < P; x += nt) {
             const K key = x < na ? base[x] : KT::kSentinel;
             rk[x] = key;
-            if (x < na) { lo = key < lo ? key : lo; hi = key > hi ? key : hi; }
+            if (x < na) {
+                lo = key < lo ? key : lo;
+                hi = key > hi ? key : hi;
+                neg += key < KT::kPosBit ? 1u : 0u;
+            }
         }
         for (uint32_t b = tid; b <= pl.nb; b += nt) bidx[b] = 0u;
         lo = wave_min_k<K>(lo);
         hi = wave_max_k<K>(hi);
-        __syncthreads();   // st cleared
-        if (lane_id() == 0 && na > 0) {
-            atomicMin(&st.kmin, (unsigned long long)lo);
-            atomicMax(&st.kmax, (unsigned long long)hi);
-        }
+        neg = (uint32_t)wave_sum_u64(neg);
+        if (lead) { st.wlo[wid] = (uint64_t)lo; st.whi[wid] = (uint64_t)hi; st.wneg[wid] = neg; }
         __syncthreads();
-        if (na > 0) {
-            kmin = st.kmin;
-            kmax = st.kmax;
+        refneg = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            kmin = st.wlo[w] < kmin ? st.wlo[w] : kmin;
+            kmax = st.whi[w] > kmax ? st.whi[w] : kmax;
+            refneg += st.wneg[w];
         }
+        if (na == 0) { kmin = ~0ull; kmax = 0ull; }
         shift = ref_shift(kmin, kmax, na, pl.nb);
-        uint64_t neg = 0;
-        for (uint32_t x = tid; x < na; x += nt) {
-            const K key = rk[x];
-            atomicAdd(&bidx[(uint32_t)(((uint64_t)key - kmin) >> shift)], 1u);
-            neg += key < KT::kPosBit ? 1u : 0u;
-        }
+        for (uint32_t x = tid; x < na; x += nt)
+            atomicAdd(&bidx[(uint32_t)(((uint64_t)rk[x] - kmin) >> shift)], 1u);
         __syncthreads();
         uint32_t m = 0;
         for (uint32_t b = tid; b < pl.nb; b += nt) m = max(m, bidx[b]);
         m = wave_max_u32(m);
-        if (lane_id() == 0) atomicMax(&st.maxb, m);
-        refneg = (uint32_t)block_sum_u64(neg, st.red);   // its barriers publish st.maxb
-        maxb = uniform_u32(st.maxb);
-        block_excl_scan(bidx, (int)pl.nb + 1, st.scan);
+        if (lead) st.wmaxb[wid] = m;
+        block_excl_scan(bidx, (int)pl.nb + 1, st.scan);   // its barriers publish wmaxb
+        for (uint32_t w = 0; w < nw; ++w) maxb = max(maxb, st.wmaxb[w]);
+        maxb = uniform_u32(maxb);
         if (room && maxb <= kBucketSortMax) {
             for (uint32_t x = tid; x < na; x += nt) {
                 const K key = rk[x];
@@ -1284,9 +1288,13 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
                 rk[s0 + below + before] = key;
                 ta += 3ull * (uint64_t)(before + 1u) * (uint64_t)before;
             }
+            ta = wave_sum_u64(ta);
+            if (lead) st.wta[wid] = ta;
             __syncthreads();
             for (uint32_t x = tid; x < na; x += nt) base[x] = rk[x];
             for (uint32_t b = tid; b <= pl.nb; b += nt) tab[b] = b ? bidx[b - 1u] : 0u;
+            ta = 0;
+            for (uint32_t w = 0; w < nw; ++w) ta += st.wta[w];
         } else {
             sort_ref<K>(rk, P);
             for (uint32_t x = tid; x < na; x += nt) base[x] = rk[x];
@@ -1301,8 +1309,8 @@ __device__ __forceinline__ void ref_item(const JobDesc* jobs, const WorkItem* it
                 rv.lookup(rk[x], lt, eq);
                 if (lt == x) ta += tie_f(eq);
             }
+            ta = block_sum_u64(ta, st.red);
         }
-        ta = block_sum_u64(ta, st.red);
     }
     if (tid == 0 && (hist || na <= cap || spill_at >= 0)) {
         ColPlan& q = J.plan[c];
