@@ -15,12 +15,17 @@
 //              count stored values per group, exclusive scan, scatter each
 //              group's nonzero values as order-preserving keys (or histogram
 //              bins) into a per-column key range in HBM    (group split)
-//   reference  sort the reference slice once per gene (register bitonic
-//              runs + LDS merges) and build its rank table -- a bucket index
-//              over the sorted keys, or the histogram prefix -- in HBM (sort)
-//   target     load the column's rank table into LDS; each wave sorts one
-//              target slice in registers, ranks it against the table and
-//              writes U1, sizes and the asymptotic p   (ranks, U, p-value)
+//   reference  sort the reference slice once per gene (on chip by buckets,
+//              or register bitonic runs + LDS merges when buckets are full;
+//              in HBM by merge passes beyond the on-chip size) and build its
+//              rank table -- a bucket index over the sorted keys, or the
+//              histogram prefix -- in HBM                            (sort)
+//   target     load the column's rank table into LDS; a wave ranks windows
+//              of small targets (at most 64 keys together) and targets of up
+//              to 192 keys without sorting them, tie positions from an LDS
+//              hash table; larger targets (and 8-byte keys) are sorted in
+//              registers by a second pass; U1, sizes and the asymptotic p
+//              are written                             (ranks, U, p-value)
 //
 // Ranking never materialises a merge: a target value v needs only
 // lt(v) = #reference values below v and eq(v) = #reference values equal to v:
