@@ -48,22 +48,24 @@ __host__ __device__ constexpr uint32_t ref_table_lds(bool f64) {
     return ((hist > sort ? hist : sort) + 15u) & ~15u;
 }
 
-// Target kernel: per wave, a table of kTieSlots (key, target, count) words
-// that finds a key's position among the equal keys of its target.
-constexpr uint32_t kTieSlots = 128;
+// Target window kernel: per wave, a table of kBigTieSlots (key, target,
+// count) words that finds a key's position among the equal keys of its
+// target (a window's at most 64 keys fill a quarter of it), followed by the
+// wave's 64-byte start map (kStartMapWords words).  A target of 65 to
+// kHashTargetKeys 4-byte keys is ranked by the window kernel in chunks of 64
+// keys with its tie positions from the same table; larger ones, and 8-byte
+// keys, are sorted (the sorted pass).
 constexpr uint64_t kTieEmpty = ~0ull;
 constexpr uint32_t kTieCountBits = 26;
 constexpr uint64_t kTieCountMask = (1ull << kTieCountBits) - 1ull;
-// A target of 65 to kHashTargetKeys 4-byte keys is ranked by the window
-// kernel in chunks of 64 keys with its tie positions from the wave's table,
-// then of kBigTieSlots words (the windows use its first kTieSlots); larger
-// ones, and 8-byte keys, are sorted (the sorted pass).
 constexpr uint32_t kHashTargetKeys = 192;   // at most 3/4 of the table's slots
 constexpr uint32_t kBigTieSlots = 256;
+constexpr uint32_t kStartMapWords = 8;
+constexpr uint32_t kWaveTableWords = kBigTieSlots + kStartMapWords;
 static_assert(kHashTargetKeys % 64u == 0u && kHashTargetKeys < kBigTieSlots,
               "whole chunks of 64 keys, and an empty slot for every probe sequence");
 __host__ __device__ constexpr uint32_t target_lds(bool f64) {
-    return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kBigTieSlots * 8u;
+    return ref_table_lds(f64) + (uint32_t)(kTargetBlock / 64) * kWaveTableWords * 8u;
 }
 
 // Count kernel (integer-valued columns in one pass).

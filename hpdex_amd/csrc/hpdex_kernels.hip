@@ -1655,10 +1655,9 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         uint32_t j0, j1, m, first, owner;
         K key;
     };
-    // the start map: the last 64 bytes of the wave's tie table, which the
-    // windows' probes (its first kTieSlots words) never reach; 0xFF bytes are
-    // the table's empty words
-    uint8_t* smap = (uint8_t*)(htab + kBigTieSlots) - kWave;
+    // the start map: the 64 bytes after the wave's tie table (0xFF: no
+    // target starts there)
+    uint8_t* smap = (uint8_t*)(htab + kBigTieSlots);
     auto plan = [&](uint32_t j0) -> Win {
         Win w;
         w.j0 = j0; w.j1 = j0; w.m = 0; w.first = 0; w.owner = 0; w.key = K(0);
@@ -1717,18 +1716,19 @@ __device__ __forceinline__ TargetResult target_windows(const JobDesc& J, int64_t
         // pair is placed in the wave's hash table (linear probing, compare
         // and swap), then a counter beside the pair is incremented; the
         // count before the increment is the position.  At most 64 pairs in
-        // kTieSlots > 64 slots: every lane finds its pair's slot or an empty
-        // one within kTieSlots probes.
+        // kBigTieSlots slots (a quarter full: short probe runs, and the loop
+        // runs as long as the wave's longest): every lane finds its pair's
+        // slot or an empty one within kBigTieSlots probes.
         const uint64_t tag = ((uint64_t)(uint32_t)key << 32) | ((uint64_t)owner << kTieCountBits);
-        uint32_t slot = (((uint32_t)key * 0x9E3779B1u) ^ (owner * 0x85EBCA77u)) >> 25;
+        uint32_t slot = (((uint32_t)key * 0x9E3779B1u) ^ (owner * 0x85EBCA77u)) >> 24;
         bool placed = !valid;
-        for (uint32_t probe = 0; probe < kTieSlots; ++probe) {
+        for (uint32_t probe = 0; probe < kBigTieSlots; ++probe) {
             if (__ballot(!placed) == 0ull) break;
             if (!placed) {
                 const uint64_t old = atomicCAS(&htab[slot], (unsigned long long)kTieEmpty,
                                                (unsigned long long)tag);
                 if (old == kTieEmpty || (old & ~kTieCountMask) == tag) placed = true;
-                else slot = (slot + 1u) & (kTieSlots - 1u);
+                else slot = (slot + 1u) & (kBigTieSlots - 1u);
             }
         }
         uint32_t jj = 1u;
@@ -1888,11 +1888,11 @@ __device__ __forceinline__ void target_item(const JobDesc* jobs, const WorkItem*
     if (tid == 0) { st.next_t = 1u; st.next_h = 1u; st.any_big = 0u; }
     if constexpr (kWindows) {
         static_assert(sizeof(K) == 4, "target windows take 4-byte keys");
-        // each wave's tie-position table (target_windows, process_target_hash),
-        // all slots empty
+        // each wave's tie-position table (target_windows, process_target_hash)
+        // and start map, all slots empty (all-ones words)
         unsigned long long* htab = (unsigned long long*)(smem + ref_table_lds(false)) +
-                                   (size_t)(tid / kWave) * kBigTieSlots;
-        for (uint32_t i = (uint32_t)lane; i < kBigTieSlots; i += (uint32_t)kWave)
+                                   (size_t)(tid / kWave) * kWaveTableWords;
+        for (uint32_t i = (uint32_t)lane; i < kWaveTableWords; i += (uint32_t)kWave)
             htab[i] = kTieEmpty;
         __syncthreads();
         fill_runs();
